@@ -1,0 +1,350 @@
+"""hastyrenderer_amd: MI355X path tracer behind HastyRenderer's render entry points.
+
+The compute path is ``libhasty_amd.so`` (hand-written HIP for gfx950, C ABI in
+``include/hasty_amd.h``).  This module binds it with ctypes and mirrors the
+reference's host-side render interface:
+
+* :class:`RenderJob` -- ``RenderJob::loadJSON`` / ``fetchSample`` / ``stopFlag``
+  (core/include/Hasty/RenderJob.h:42-64, core/src/RenderJob.cpp:21-28);
+* :class:`AccumulationBuffer` / :class:`Image3fAccDoubleBuffer` -- the film
+  sums and their double buffer (core/include/Hasty/Image.h:18-45, 302-316);
+* :func:`renderThread` -- the pass loop of ``renderThread``
+  (core/src/PathTracing.cpp:98-160), issuing GPU batches of sample indices;
+* :func:`computeEstimate` -- ``computeEstimate`` (apps/ViewerUtils.cpp:95-121).
+
+There is no CPU fallback: if the HIP library is missing every entry point
+raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import threading
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libhasty_amd.so")
+_lib = None
+
+# Every symbol declared in include/hasty_amd.h.
+EXPORTS = (
+    "hasty_scene_load",
+    "hasty_scene_destroy",
+    "hasty_scene_info",
+    "hasty_scene_set_settings",
+    "hasty_render_pt",
+    "hasty_render_pt_device",
+    "hasty_trace_closest",
+    "hasty_set_profiling",
+    "hasty_kernel_stats",
+    "hasty_reset_stats",
+    "hasty_counters",
+    "hasty_set_launch",
+    "hasty_last_error",
+)
+
+
+class HastyError(RuntimeError):
+    pass
+
+
+class RenderSettingsC(C.Structure):
+    _fields_ = [
+        ("roulette_start_depth", C.c_int32),
+        ("roulette_q", C.c_float),
+        ("width", C.c_uint32),
+        ("height", C.c_uint32),
+        ("num_threads", C.c_uint32),
+        ("max_depth", C.c_int32),
+        ("num_samples", C.c_int32),
+        ("use_mis", C.c_int32),
+    ]
+
+
+class SceneInfoC(C.Structure):
+    _fields_ = [
+        ("settings", RenderSettingsC),
+        ("num_triangles", C.c_uint32),
+        ("num_shapes", C.c_uint32),
+        ("num_materials", C.c_uint32),
+        ("num_light_triangles", C.c_uint32),
+        ("has_environment", C.c_int32),
+        ("bvh_nodes", C.c_uint32),
+        ("bvh_depth", C.c_int32),
+        ("device_bytes", C.c_uint64),
+    ]
+
+
+class KernelStatC(C.Structure):
+    _fields_ = [("name", C.c_char * 32), ("launches", C.c_uint64), ("total_ms", C.c_double)]
+
+
+def lib():
+    """Loads libhasty_amd.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise HastyError("libhasty_amd.so is not built (run __graft_entry__.build() or make -C hastyrenderer_amd)")
+    L = C.CDLL(LIB_PATH)
+    P = C.c_void_p
+    fp = C.POINTER(C.c_float)
+    L.hasty_scene_load.argtypes = [C.c_char_p, C.POINTER(P)]
+    L.hasty_scene_destroy.argtypes = [P]
+    L.hasty_scene_destroy.restype = None
+    L.hasty_scene_info.argtypes = [P, C.POINTER(SceneInfoC)]
+    L.hasty_scene_set_settings.argtypes = [P, C.POINTER(RenderSettingsC)]
+    L.hasty_render_pt.argtypes = [P, C.c_uint32, C.c_uint32, C.c_uint32, fp, fp, fp]
+    L.hasty_render_pt_device.argtypes = [P, C.c_uint32, C.c_uint32, C.c_uint32, P, P, P]
+    L.hasty_trace_closest.argtypes = [P, fp, C.c_uint32, C.POINTER(C.c_uint32), C.POINTER(C.c_int32), fp]
+    L.hasty_set_profiling.argtypes = [P, C.c_int]
+    L.hasty_kernel_stats.argtypes = [P, C.POINTER(KernelStatC), C.c_int]
+    L.hasty_reset_stats.argtypes = [P]
+    L.hasty_counters.argtypes = [P, C.POINTER(C.c_uint64)]
+    L.hasty_set_launch.argtypes = [P, C.c_uint32, C.c_uint32, C.c_uint32]
+    L.hasty_last_error.restype = C.c_char_p
+    _lib = L
+    return L
+
+
+def _err(rc):
+    if rc != 0:
+        raise HastyError(lib().hasty_last_error().decode())
+
+
+def _fptr(a: np.ndarray):
+    assert a.dtype == np.float32 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+class Scene:
+    """A loaded render job on the GPU (HastyScene*)."""
+
+    def __init__(self, render_job_path: str):
+        h = C.c_void_p()
+        _err(lib().hasty_scene_load(os.fsencode(render_job_path), C.byref(h)))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and _lib is not None:
+            _lib.hasty_scene_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    @property
+    def handle(self):
+        return self._h
+
+    def info(self) -> dict:
+        i = SceneInfoC()
+        _err(lib().hasty_scene_info(self._h, C.byref(i)))
+        s = i.settings
+        return {
+            "roulette_start_depth": s.roulette_start_depth,
+            "roulette_q": s.roulette_q,
+            "width": s.width,
+            "height": s.height,
+            "num_threads": s.num_threads,
+            "max_depth": s.max_depth,
+            "num_samples": s.num_samples,
+            "use_mis": bool(s.use_mis),
+            "num_triangles": i.num_triangles,
+            "num_shapes": i.num_shapes,
+            "num_materials": i.num_materials,
+            "num_light_triangles": i.num_light_triangles,
+            "has_environment": bool(i.has_environment),
+            "bvh_nodes": i.bvh_nodes,
+            "bvh_depth": i.bvh_depth,
+            "device_bytes": i.device_bytes,
+        }
+
+    def set_settings(self, **kw):
+        i = SceneInfoC()
+        _err(lib().hasty_scene_info(self._h, C.byref(i)))
+        s = i.settings
+        for k, v in kw.items():
+            setattr(s, k, int(v) if k != "roulette_q" else float(v))
+        _err(lib().hasty_scene_set_settings(self._h, C.byref(s)))
+
+    def set_launch(self, samples_per_batch=0, trace_blocks=0, shade_blocks=0):
+        _err(lib().hasty_set_launch(self._h, samples_per_batch, trace_blocks, shade_blocks))
+
+    def render_pt(self, seed: int, first_sample: int, num_samples: int, films=None):
+        """Adds samples [first, first+n) into host films; returns (color, normal, albedo) sums."""
+        info = self.info()
+        h, w = info["height"], info["width"]
+        if films is None:
+            films = tuple(np.zeros((h, w, 3), np.float32) for _ in range(3))
+        for f in films:
+            assert f.shape == (h, w, 3)
+        _err(lib().hasty_render_pt(self._h, seed, first_sample, num_samples, *(_fptr(f) for f in films)))
+        return films
+
+    def render_pt_device(self, seed: int, first_sample: int, num_samples: int, d_color, d_normal, d_albedo):
+        """Films are device pointers (ints), e.g. torch tensors' data_ptr()."""
+        _err(lib().hasty_render_pt_device(self._h, seed, first_sample, num_samples, C.c_void_p(d_color),
+                                          C.c_void_p(d_normal), C.c_void_p(d_albedo)))
+
+    def trace_closest(self, rays: np.ndarray):
+        rays = np.ascontiguousarray(rays, dtype=np.float32).reshape(-1, 6)
+        n = rays.shape[0]
+        gp = np.zeros((n, 2), np.uint32)
+        tri = np.zeros(n, np.int32)
+        tuv = np.zeros((n, 3), np.float32)
+        _err(lib().hasty_trace_closest(self._h, _fptr(rays), n, gp.ctypes.data_as(C.POINTER(C.c_uint32)),
+                                       tri.ctypes.data_as(C.POINTER(C.c_int32)), _fptr(tuv)))
+        return gp, tri, tuv
+
+    def set_profiling(self, on=True):
+        _err(lib().hasty_set_profiling(self._h, 1 if on else 0))
+
+    def reset_stats(self):
+        _err(lib().hasty_reset_stats(self._h))
+
+    def kernel_stats(self) -> dict:
+        arr = (KernelStatC * 16)()
+        n = lib().hasty_kernel_stats(self._h, arr, 16)
+        if n < 0:
+            raise HastyError(lib().hasty_last_error().decode())
+        return {arr[i].name.decode(): (int(arr[i].launches), float(arr[i].total_ms)) for i in range(n)}
+
+    def counters(self) -> dict:
+        out = (C.c_uint64 * 5)()
+        _err(lib().hasty_counters(self._h, out))
+        keys = ("extension_rays", "shadow_rays", "node_visits", "triangle_tests", "media_overflow")
+        return dict(zip(keys, (int(x) for x in out)))
+
+
+# ------------------------------------------------------------------------------
+# Host mirror of the reference's render-thread interface.
+
+
+class RenderSettings:
+    """core/include/Hasty/RenderJob.h:15-30 (defaults) filled from the job file."""
+
+    def __init__(self, info: dict):
+        self.rouletteStartDepth = info["roulette_start_depth"]
+        self.rouletteQ = info["roulette_q"]
+        self.width = info["width"]
+        self.height = info["height"]
+        self.numThreads = info["num_threads"]
+        self.maxDepth = info["max_depth"]
+        self.numSamples = info["num_samples"]
+        self.useMIS = info["use_mis"]
+
+
+class RenderJobSample:
+    def __init__(self, idx: int):
+        self.idx = idx
+
+
+class RenderJob:
+    """RenderJob (RenderJob.h:42-64): settings, the GPU scene, a shared
+    sample-index counter and the stop flag."""
+
+    def __init__(self):
+        self.scene = None
+        self.renderSettings = None
+        self.stopFlag = False
+        self._next = 0
+        self._lock = threading.Lock()
+
+    def loadJSON(self, path: str):
+        self.scene = Scene(path)
+        self.renderSettings = RenderSettings(self.scene.info())
+
+    def fetchSample(self) -> RenderJobSample:
+        with self._lock:
+            s = RenderJobSample(self._next)
+            self._next += 1
+            return s
+
+    def fetchSamples(self, count: int) -> tuple:
+        """A contiguous batch of sample indices (one GPU launch)."""
+        with self._lock:
+            first = self._next
+            self._next += count
+            return first, count
+
+
+class RenderThreadData:
+    def __init__(self, seed: int):
+        self.seed = seed
+        self.stoppedFlag = False
+
+
+class AccumulationBuffer:
+    def __init__(self, width: int, height: int):
+        self.data = np.zeros((height, width, 3), np.float32)
+        self.numSamples = 0
+
+
+class Image3fAccDoubleBuffer:
+    """DoubleBuffer<AccumulationBuffer<Vec3f>> (Image.h:18-45)."""
+
+    def __init__(self, width: int, height: int):
+        self._write = AccumulationBuffer(width, height)
+        self._read = AccumulationBuffer(width, height)
+        self._mutex = threading.Lock()
+
+    def getWriteBuffer(self):
+        return self._write
+
+    def getReadBuffer(self):
+        return self._read
+
+    def lock(self):
+        self._mutex.acquire()
+
+    def unlock(self):
+        self._mutex.release()
+
+    def copyBuffer(self):
+        with self._mutex:
+            self._read.data = self._write.data.copy()
+            self._read.numSamples = self._write.numSamples
+
+
+def renderThread(colorBuffer, normalBuffer, albedoBuffer, job: RenderJob, threadData: RenderThreadData,
+                 batch: int = 0):
+    """PathTracing.cpp:98-160 with GPU batches: pulls `batch` sample indices at a
+    time (default: all remaining), renders them in one call, publishes the sums
+    and honours stopFlag between batches."""
+    st = job.renderSettings
+    for b in (colorBuffer, normalBuffer, albedoBuffer):
+        b.getWriteBuffer().data[...] = 0.0
+        b.getWriteBuffer().numSamples = 0
+    while not job.stopFlag:
+        remaining = st.numSamples - job._next
+        if remaining <= 0:
+            break
+        count = min(batch, remaining) if batch > 0 else remaining
+        first, count = job.fetchSamples(count)
+        count = min(count, st.numSamples - first)
+        if count <= 0:
+            break
+        job.scene.render_pt(threadData.seed, first, count, films=(colorBuffer.getWriteBuffer().data,
+                                                                  normalBuffer.getWriteBuffer().data,
+                                                                  albedoBuffer.getWriteBuffer().data))
+        for b in (colorBuffer, normalBuffer, albedoBuffer):
+            b.getWriteBuffer().numSamples += count
+            b.copyBuffer()
+    threadData.stoppedFlag = True
+
+
+def computeEstimate(buffers) -> np.ndarray:
+    """apps/ViewerUtils.cpp:95-121: sum of read buffers / (1e-7 + total samples)."""
+    total = 0
+    image = None
+    for b in buffers:
+        b.lock()
+        try:
+            rb = b.getReadBuffer()
+            image = rb.data.copy() if image is None else image + rb.data
+            total += rb.numSamples
+        finally:
+            b.unlock()
+    return image / np.float32(1e-7 + float(total))

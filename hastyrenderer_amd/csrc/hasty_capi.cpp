@@ -1,0 +1,561 @@
+// C ABI (include/hasty_amd.h): scene loading and upload, BVH build, device
+// memory and stream ownership, render and trace entry points, per-kernel
+// HIP-event timing.
+#include "../../include/hasty_amd.h"
+
+#include <hip/hip_runtime.h>
+#include <string.h>
+
+#include <algorithm>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "bvh.h"
+#include "pt_kernels.h"
+#include "rng.h"
+#include "scene_io.h"
+
+namespace {
+
+thread_local std::string g_lastError;
+
+void check(hipError_t e, const char* what) {
+  if (e != hipSuccess) throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
+}
+#define HIPCHECK(x) check((x), #x)
+
+// Reference Halton point generator (core/include/Hasty/Random.h:14-63).
+struct Halton {
+  uint32_t b, n = 0, d = 1;
+  explicit Halton(uint32_t base) : b(base) {}
+  float next() {
+    uint32_t x = d - n;
+    if (x == 1) {
+      n = 1;
+      d *= b;
+    } else {
+      uint32_t y = d / b;
+      while (x <= y) y /= b;
+      n = (b + 1) * y - x;
+    }
+    return (float)n / (float)d;
+  }
+};
+
+struct EventTimer : hasty::KernelTimer {
+  struct Rec {
+    int kernel;
+    hipEvent_t a, b;
+  };
+  std::vector<Rec> pending;
+  std::vector<hipEvent_t> pool;
+  std::vector<hipEvent_t> open;  // one open start event per kernel id
+  uint64_t launches[hasty::kNumKernels] = {0};
+  double ms[hasty::kNumKernels] = {0};
+  EventTimer() : open(hasty::kNumKernels, nullptr) {}
+  hipEvent_t get() {
+    if (!pool.empty()) {
+      hipEvent_t e = pool.back();
+      pool.pop_back();
+      return e;
+    }
+    hipEvent_t e;
+    HIPCHECK(hipEventCreate(&e));
+    return e;
+  }
+  void begin(int k, hipStream_t s) override {
+    hipEvent_t e = get();
+    HIPCHECK(hipEventRecord(e, s));
+    open[k] = e;
+  }
+  void end(int k, hipStream_t s) override {
+    hipEvent_t e = get();
+    HIPCHECK(hipEventRecord(e, s));
+    pending.push_back({k, open[k], e});
+    open[k] = nullptr;
+  }
+  void collect() {
+    for (Rec& r : pending) {
+      HIPCHECK(hipEventSynchronize(r.b));
+      float t = 0.0f;
+      HIPCHECK(hipEventElapsedTime(&t, r.a, r.b));
+      launches[r.kernel] += 1;
+      ms[r.kernel] += t;
+      pool.push_back(r.a);
+      pool.push_back(r.b);
+    }
+    pending.clear();
+  }
+  void reset() {
+    collect();
+    for (int k = 0; k < hasty::kNumKernels; ++k) {
+      launches[k] = 0;
+      ms[k] = 0.0;
+    }
+  }
+  ~EventTimer() override {
+    for (Rec& r : pending) {
+      (void)hipEventDestroy(r.a);
+      (void)hipEventDestroy(r.b);
+    }
+    for (hipEvent_t e : pool) (void)hipEventDestroy(e);
+  }
+};
+
+}  // namespace
+
+struct HastyScene {
+  hasty::Job job;
+  hasty::Bvh4 bvh;
+  hdev::DevScene dev{};
+  std::vector<void*> sceneAllocs;
+  uint64_t sceneBytes = 0;
+  hipStream_t stream = nullptr;
+  int device = 0;
+
+  // path buffers
+  hasty::PathBuffers pb{};
+  std::vector<void*> pathAllocs;
+  uint32_t pathCapacity = 0;
+  uint32_t samplesPerBatch = 0;  // 0 = automatic
+  hasty::LaunchConfig launch;
+  float2* dOffsets = nullptr;
+  uint32_t offsetsCapacity = 0;
+
+  // films for the host-pointer entry point
+  float* dFilm = nullptr;
+  size_t filmFloats = 0;
+
+  // trace entry point
+  uint2* traceOverflow = nullptr;
+  uint32_t traceOverflowThreads = 0;
+
+  bool profiling = false;
+  std::unique_ptr<EventTimer> timer;
+  uint64_t stats[hasty::kNumStats] = {0};
+
+  template <class T>
+  T* upload(const T* host, size_t count) {
+    if (count == 0) return nullptr;
+    void* p = nullptr;
+    HIPCHECK(hipMalloc(&p, count * sizeof(T)));
+    sceneAllocs.push_back(p);
+    sceneBytes += count * sizeof(T);
+    HIPCHECK(hipMemcpy(p, host, count * sizeof(T), hipMemcpyHostToDevice));
+    return (T*)p;
+  }
+  template <class T>
+  T* allocPath(size_t count) {
+    void* p = nullptr;
+    HIPCHECK(hipMalloc(&p, count * sizeof(T)));
+    pathAllocs.push_back(p);
+    return (T*)p;
+  }
+  void freePath() {
+    for (void* p : pathAllocs) (void)hipFree(p);
+    pathAllocs.clear();
+    pathCapacity = 0;
+  }
+  ~HastyScene() {
+    if (timer) timer->collect();
+    freePath();
+    for (void* p : sceneAllocs) (void)hipFree(p);
+    if (dOffsets) (void)hipFree(dOffsets);
+    if (dFilm) (void)hipFree(dFilm);
+    if (traceOverflow) (void)hipFree(traceOverflow);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+
+  void applySettings() {
+    const hasty::RenderSettings& st = job.settings;
+    dev.maxDepth = st.maxDepth;
+    dev.rrStart = st.rouletteStartDepth;
+    dev.rrQ = st.rouletteQ;
+    dev.useMIS = st.useMIS ? 1 : 0;
+    dev.width = st.width;
+    dev.height = st.height;
+  }
+
+  void ensurePathBuffers(uint32_t K) {
+    uint32_t Pp = hasty::paddedPixels(dev.width, dev.height);
+    uint64_t need = (uint64_t)K * Pp;
+    if (need > 0xFFFFFFFFull / 2) throw std::runtime_error("too many paths in flight");
+    uint32_t traceThreads = launch.traceBlocks * hasty::kBlock;
+    if (need <= pathCapacity && pb.traceThreads == traceThreads) return;
+    freePath();
+    size_t n = (size_t)need;
+    pb.rayO = allocPath<float4>(n);
+    pb.rayD = allocPath<float4>(n);
+    pb.hit = allocPath<float4>(n);
+    pb.beta = allocPath<float4>(n);
+    pb.L = allocPath<float4>(n);
+    pb.state = allocPath<uint4>(n);
+    pb.media = allocPath<uint4>(2 * n);
+    pb.auxN = allocPath<float4>(n);
+    pb.auxA = allocPath<float4>(n);
+    pb.shO = allocPath<float4>(n);
+    pb.shD = allocPath<float4>(n);
+    pb.shFD = allocPath<float4>(n);
+    pb.shFC = allocPath<float4>(n);
+    pb.shThr = allocPath<float4>(n);
+    pb.shBeta = allocPath<float4>(n);
+    pb.shT = allocPath<float4>(n);
+    pb.qA = allocPath<uint32_t>(n);
+    pb.qB = allocPath<uint32_t>(n);
+    pb.qS = allocPath<uint32_t>(n);
+    pb.counters = allocPath<uint32_t>(hasty::kNumCounters);
+    pb.stats = allocPath<unsigned long long>(hasty::kNumStats);
+    pb.stackOverflow = allocPath<uint2>((size_t)traceThreads * hasty::kStackOvf);
+    pb.traceThreads = traceThreads;
+    pb.capacity = (uint32_t)need;
+    pathCapacity = (uint32_t)need;
+    HIPCHECK(hipMemset(pb.stats, 0, sizeof(unsigned long long) * hasty::kNumStats));
+  }
+
+  uint32_t chooseK(uint32_t nSamples) const {
+    uint32_t Pp = hasty::paddedPixels(dev.width, dev.height);
+    uint32_t K = samplesPerBatch;
+    if (K == 0) {
+      const uint32_t target = 1u << 21;  // paths in flight per batch
+      K = std::max<uint32_t>(1, target / std::max<uint32_t>(1, Pp));
+    }
+    return std::max<uint32_t>(1, std::min(K, nSamples));
+  }
+
+  void renderDevice(uint32_t seed, uint32_t first, uint32_t nSamples, float* dC, float* dN, float* dA) {
+    if (dev.width == 0 || dev.height == 0) throw std::runtime_error("empty film");
+    dev.seed = seed;
+    uint32_t K = chooseK(nSamples);
+    ensurePathBuffers(K);
+    if (offsetsCapacity < K) {
+      if (dOffsets) (void)hipFree(dOffsets);
+      HIPCHECK(hipMalloc(&dOffsets, sizeof(float2) * K));
+      offsetsCapacity = K;
+    }
+    // Halton offsets of samples first .. first+n-1 (point index k+1)
+    Halton h2(2), h3(3);
+    std::vector<float2> offs(nSamples);
+    for (uint32_t k = 0; k < first + nSamples; ++k) {
+      float a = h2.next(), b = h3.next();
+      if (k >= first) offs[k - first] = make_float2(a, b);
+    }
+    launch.countStats = profiling;
+    if (profiling && !timer) timer = std::make_unique<EventTimer>();
+    for (uint32_t done = 0; done < nSamples; done += K) {
+      uint32_t k = std::min(K, nSamples - done);
+      HIPCHECK(hipMemcpyAsync(dOffsets, offs.data() + done, sizeof(float2) * k, hipMemcpyHostToDevice, stream));
+      hasty::launchPTBatch(dev, pb, first + done, k, dOffsets, dC, dN, dA, launch, stream,
+                           profiling ? timer.get() : nullptr);
+      HIPCHECK(hipGetLastError());
+    }
+    HIPCHECK(hipStreamSynchronize(stream));
+    if (profiling) {
+      unsigned long long s[hasty::kNumStats];
+      HIPCHECK(hipMemcpy(s, pb.stats, sizeof(s), hipMemcpyDeviceToHost));
+      HIPCHECK(hipMemset(pb.stats, 0, sizeof(s)));
+      for (int i = 0; i < hasty::kNumStats; ++i) stats[i] += s[i];
+      timer->collect();
+    }
+  }
+};
+
+static void buildScene(HastyScene& S, const char* path) {
+  hasty::loadJob(path, S.job);
+  hasty::Scene& sc = S.job.scene;
+  const hasty::RenderSettings& st = S.job.settings;
+  if (sc.shapeNames.size() >= 65536) throw std::runtime_error("more than 65535 shapes");
+  if (sc.materials.size() >= 65536) throw std::runtime_error("more than 65535 materials");
+  std::vector<hdev::DevMaterial> mats(sc.materials.size());
+  for (size_t i = 0; i < sc.materials.size(); ++i) {
+    const hasty::Material& m = sc.materials[i];
+    hdev::DevMaterial& d = mats[i];
+    for (int c = 0; c < 3; ++c) {
+      d.albedo[c] = m.albedo[c];
+      d.emission[c] = m.emission[c];
+    }
+    d.roughness = m.roughness;
+    float alpha = m.roughness * m.roughness;  // PrincipledBRDF::computeAlpha
+    if (alpha < 1e-2f && alpha != 0.0f) alpha = 1e-2f;
+    d.alpha = alpha;
+    d.metallic = m.metallic;
+    d.specular = m.specular;
+    d.ior = m.ior;
+    d.anisotropy = m.anisotropy;
+    d.transmission = m.transmission;
+    d.varyIOR = m.varyIOR ? 1 : 0;
+    d.isLight = m.isLight() ? 1 : 0;
+    d.hasDiffuseLobe = ((1.0f - 0.5f * m.specular) * (1.0 - m.transmission) * (1.0 - m.metallic) > 0.0f) ? 1 : 0;
+    // Rough transmission throws "not implemented" when shaded (BRDF.cpp:386-390);
+    // the GPU path rejects such a scene when it is loaded.
+    if (m.transmission > 0.0f && alpha != 0.0f) throw std::runtime_error("not implemented");
+  }
+  // Scene.cpp:508-512 exits the process when NEE needs a light triangle and
+  // there is none; reject the scene instead.
+  if (sc.hasLight() && st.useMIS && !sc.hasEnvTexture && sc.lightTris.empty())
+    throw std::runtime_error("error: no light triangle found ");
+
+  hasty::BvhBuildOptions opt;
+  S.bvh = hasty::buildBvh4(sc.triPos.data(), sc.numTris, opt);
+  if (3 * S.bvh.depth + 1 > hasty::kStackLds + hasty::kStackOvf)
+    throw std::runtime_error("BVH too deep for the traversal stack (depth " + std::to_string(S.bvh.depth) + ")");
+
+  HIPCHECK(hipGetDevice(&S.device));
+  HIPCHECK(hipStreamCreate(&S.stream));
+  hdev::DevScene& d = S.dev;
+  d.nodes = S.upload((const float4*)S.bvh.nodes.data(), S.bvh.nodes.size() / 4);
+  d.tris = S.upload((const float4*)S.bvh.tris.data(), S.bvh.tris.size() / 4);
+  d.triPos = S.upload(sc.triPos.data(), sc.triPos.size());
+  d.triNrm = S.upload(sc.triNrm.data(), sc.triNrm.size());
+  d.triMat = S.upload(sc.triMaterial.data(), sc.triMaterial.size());
+  d.triGeom = S.upload(sc.triGeomID.data(), sc.triGeomID.size());
+  d.triSmooth = S.upload(sc.triSmooth.data(), sc.triSmooth.size());
+  d.mats = S.upload(mats.data(), mats.size());
+  d.lightTris = S.upload(sc.lightTris.data(), sc.lightTris.size());
+  d.lightCdf = S.upload(sc.lightCdf.data(), sc.lightCdf.size());
+  d.numLights = (uint32_t)sc.lightTris.size();
+  d.totalLightArea = sc.totalLightArea;
+  d.hasEnv = sc.hasEnvTexture ? 1 : 0;
+  if (sc.hasEnvTexture) {
+    d.envRGB = S.upload(sc.env.rgb.data(), sc.env.rgb.size());
+    d.envPdf = S.upload(sc.envPdf.data(), sc.envPdf.size());
+    d.envCdf = S.upload(sc.envCdf.data(), sc.envCdf.size());
+    std::vector<uint32_t> guide(1025);
+    uint32_t n = (uint32_t)sc.envCdf.size();
+    for (uint32_t g = 0; g <= 1024; ++g) guide[g] = hrng::upper_bound(sc.envCdf.data(), n, g / 1024.0);
+    d.envCdfGuide = S.upload(guide.data(), guide.size());
+    d.envW = sc.env.width;
+    d.envH = sc.env.height;
+    d.envValueSum = sc.envValueSum;
+  }
+  for (int c = 0; c < 3; ++c) {
+    d.bg[c] = sc.backgroundColor[c];
+    d.camPos[c] = sc.camera.position[c];
+    d.camFwd[c] = sc.camera.forward[c];
+    d.camRight[c] = sc.camera.right[c];
+    d.camUp[c] = sc.camera.up[c];
+  }
+  d.hasLight = sc.hasLight() ? 1 : 0;
+  d.hasSurfaceLight = sc.hasSurfaceLight() ? 1 : 0;
+  d.fovDegree = sc.camera.fovDegree;
+  d.aperture = sc.camera.aperture;
+  d.focalDistance = sc.camera.focalDistance;
+  d.bladeRotation = sc.camera.bladeRotation;
+  d.numBlades = sc.camera.numBlades;
+  S.applySettings();
+}
+
+static int fail(const std::exception& e, int code) {
+  g_lastError = e.what();
+  return code;
+}
+
+extern "C" {
+
+const char* hasty_last_error(void) { return g_lastError.c_str(); }
+
+int hasty_scene_load(const char* path, HastyScene** out) {
+  if (!path || !out) {
+    g_lastError = "null argument";
+    return 1;
+  }
+  try {
+    auto S = std::make_unique<HastyScene>();
+    buildScene(*S, path);
+    *out = S.release();
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e, 2);
+  }
+}
+
+void hasty_scene_destroy(HastyScene* s) { delete s; }
+
+int hasty_scene_info(const HastyScene* s, HastySceneInfo* info) {
+  if (!s || !info) {
+    g_lastError = "null argument";
+    return 1;
+  }
+  const hasty::RenderSettings& st = s->job.settings;
+  info->settings.roulette_start_depth = st.rouletteStartDepth;
+  info->settings.roulette_q = st.rouletteQ;
+  info->settings.width = st.width;
+  info->settings.height = st.height;
+  info->settings.num_threads = st.numThreads;
+  info->settings.max_depth = st.maxDepth;
+  info->settings.num_samples = st.numSamples;
+  info->settings.use_mis = st.useMIS ? 1 : 0;
+  info->num_triangles = s->job.scene.numTris;
+  info->num_shapes = (uint32_t)s->job.scene.shapeNames.size();
+  info->num_materials = (uint32_t)s->job.scene.materials.size();
+  info->num_light_triangles = (uint32_t)s->job.scene.lightTris.size();
+  info->has_environment = s->job.scene.hasEnvTexture ? 1 : 0;
+  info->bvh_nodes = s->bvh.numNodes;
+  info->bvh_depth = s->bvh.depth;
+  info->device_bytes = s->sceneBytes;
+  return 0;
+}
+
+int hasty_scene_set_settings(HastyScene* s, const HastyRenderSettings* st) {
+  if (!s || !st) {
+    g_lastError = "null argument";
+    return 1;
+  }
+  hasty::RenderSettings& r = s->job.settings;
+  r.rouletteStartDepth = st->roulette_start_depth;
+  r.rouletteQ = st->roulette_q;
+  r.width = st->width;
+  r.height = st->height;
+  r.numThreads = st->num_threads;
+  r.maxDepth = st->max_depth;
+  r.numSamples = st->num_samples;
+  r.useMIS = st->use_mis != 0;
+  if (r.maxDepth > 255) {
+    g_lastError = "max_depth above 255 is not supported";
+    return 1;
+  }
+  s->applySettings();
+  return 0;
+}
+
+int hasty_render_pt_device(HastyScene* s, uint32_t seed, uint32_t first, uint32_t n, float* dC, float* dN,
+                           float* dA) {
+  if (!s || !dC || !dN || !dA) {
+    g_lastError = "null argument";
+    return 1;
+  }
+  try {
+    HIPCHECK(hipSetDevice(s->device));
+    if (n > 0) s->renderDevice(seed, first, n, dC, dN, dA);
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e, 3);
+  }
+}
+
+int hasty_render_pt(HastyScene* s, uint32_t seed, uint32_t first, uint32_t n, float* color, float* normal,
+                    float* albedo) {
+  if (!s || !color || !normal || !albedo) {
+    g_lastError = "null argument";
+    return 1;
+  }
+  try {
+    HIPCHECK(hipSetDevice(s->device));
+    size_t P = (size_t)s->dev.width * s->dev.height * 3;
+    if (s->filmFloats < 3 * P) {
+      if (s->dFilm) (void)hipFree(s->dFilm);
+      HIPCHECK(hipMalloc(&s->dFilm, 3 * P * sizeof(float)));
+      s->filmFloats = 3 * P;
+    }
+    float* dC = s->dFilm;
+    float* dN = dC + P;
+    float* dA = dN + P;
+    HIPCHECK(hipMemcpy(dC, color, P * sizeof(float), hipMemcpyHostToDevice));
+    HIPCHECK(hipMemcpy(dN, normal, P * sizeof(float), hipMemcpyHostToDevice));
+    HIPCHECK(hipMemcpy(dA, albedo, P * sizeof(float), hipMemcpyHostToDevice));
+    if (n > 0) s->renderDevice(seed, first, n, dC, dN, dA);
+    HIPCHECK(hipMemcpy(color, dC, P * sizeof(float), hipMemcpyDeviceToHost));
+    HIPCHECK(hipMemcpy(normal, dN, P * sizeof(float), hipMemcpyDeviceToHost));
+    HIPCHECK(hipMemcpy(albedo, dA, P * sizeof(float), hipMemcpyDeviceToHost));
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e, 3);
+  }
+}
+
+int hasty_trace_closest(HastyScene* s, const float* rays, uint32_t n, uint32_t* geomPrim, int32_t* tri, float* tuv) {
+  if (!s || (n > 0 && (!rays || !geomPrim || !tri || !tuv))) {
+    g_lastError = "null argument";
+    return 1;
+  }
+  if (n == 0) return 0;
+  try {
+    HIPCHECK(hipSetDevice(s->device));
+    const uint32_t threads = 256 * hasty::kBlock;
+    if (!s->traceOverflow) {
+      HIPCHECK(hipMalloc(&s->traceOverflow, sizeof(uint2) * threads * hasty::kStackOvf));
+      s->traceOverflowThreads = threads;
+    }
+    float* dRays = nullptr;
+    int32_t* dTri = nullptr;
+    float* dTuv = nullptr;
+    HIPCHECK(hipMalloc(&dRays, sizeof(float) * 6 * n));
+    HIPCHECK(hipMalloc(&dTri, sizeof(int32_t) * n));
+    HIPCHECK(hipMalloc(&dTuv, sizeof(float) * 3 * n));
+    HIPCHECK(hipMemcpy(dRays, rays, sizeof(float) * 6 * n, hipMemcpyHostToDevice));
+    hasty::launchTraceClosest(s->dev, dRays, n, dTri, dTuv, s->traceOverflow, s->traceOverflowThreads, s->stream);
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipStreamSynchronize(s->stream));
+    HIPCHECK(hipMemcpy(tri, dTri, sizeof(int32_t) * n, hipMemcpyDeviceToHost));
+    HIPCHECK(hipMemcpy(tuv, dTuv, sizeof(float) * 3 * n, hipMemcpyDeviceToHost));
+    (void)hipFree(dRays);
+    (void)hipFree(dTri);
+    (void)hipFree(dTuv);
+    for (uint32_t i = 0; i < n; ++i) {
+      if (tri[i] < 0) {
+        geomPrim[2 * i] = geomPrim[2 * i + 1] = 0xFFFFFFFFu;
+      } else {
+        geomPrim[2 * i] = s->job.scene.triGeomID[tri[i]];
+        geomPrim[2 * i + 1] = s->job.scene.triPrimID[tri[i]];
+      }
+    }
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e, 3);
+  }
+}
+
+int hasty_set_profiling(HastyScene* s, int enable) {
+  if (!s) return 1;
+  s->profiling = enable != 0;
+  return 0;
+}
+
+int hasty_kernel_stats(HastyScene* s, HastyKernelStat* out, int maxStats) {
+  if (!s || !out) return -1;
+  if (!s->timer) return 0;
+  try {
+    s->timer->collect();
+  } catch (const std::exception& e) {
+    g_lastError = e.what();
+    return -1;
+  }
+  int n = 0;
+  for (int k = 0; k < hasty::kNumKernels && n < maxStats; ++k) {
+    if (s->timer->launches[k] == 0) continue;
+    memset(out[n].name, 0, sizeof(out[n].name));
+    strncpy(out[n].name, hasty::kernelName(k), sizeof(out[n].name) - 1);
+    out[n].launches = s->timer->launches[k];
+    out[n].total_ms = s->timer->ms[k];
+    ++n;
+  }
+  return n;
+}
+
+int hasty_reset_stats(HastyScene* s) {
+  if (!s) return 1;
+  if (s->timer) s->timer->reset();
+  for (int i = 0; i < hasty::kNumStats; ++i) s->stats[i] = 0;
+  return 0;
+}
+
+int hasty_counters(HastyScene* s, uint64_t* out) {
+  if (!s || !out) return 1;
+  out[0] = s->stats[hasty::kStatExtRays];
+  out[1] = s->stats[hasty::kStatShadowRays];
+  out[2] = s->stats[hasty::kStatNodes];
+  out[3] = s->stats[hasty::kStatTris];
+  out[4] = s->stats[hasty::kStatMediaOverflow];
+  return 0;
+}
+
+int hasty_set_launch(HastyScene* s, uint32_t spb, uint32_t traceBlocks, uint32_t shadeBlocks) {
+  if (!s) return 1;
+  if (spb) s->samplesPerBatch = spb;
+  if (traceBlocks) s->launch.traceBlocks = traceBlocks;
+  if (shadeBlocks) s->launch.shadeBlocks = shadeBlocks;
+  return 0;
+}
+
+}  // extern "C"

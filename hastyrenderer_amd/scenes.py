@@ -1,0 +1,325 @@
+"""Procedural scenes written in the reference's own input formats.
+
+Each writer produces ``renderJob.json`` -> scene JSON -> ``.obj``/``.mtl`` (+ a
+PFM lat-long environment map where needed), i.e. exactly what
+``RenderJob::loadJSON`` (core/src/RenderJob.cpp:21-28) and ``Scene::Scene``
+(core/src/Scene.cpp:12-195) read.  Material parameters go through the MTL
+mapping of Scene.cpp:65-118:
+
+* ``Kd`` -> albedo, ``Ka[0]`` -> metallic, ``Ks[0]`` -> specular,
+* ``Ns`` -> roughness = 1 - sqrt(Ns / 1000), so GGX alpha = roughness^2 and
+  ``Ns = 1000 (1 - sqrt(alpha))^2``,
+* ``d`` -> transmission = 1 - d, ``Ni`` -> IOR, ``Ke`` -> emission with
+  radiance L = Ke / pi (Scene.cpp:460-477).
+
+The camera FOV is fixed at 45 degrees in the reference (Camera.h:41); the
+renderer accepts an optional ``camera.fov`` key, used here for the Cornell box.
+Coordinates are written with 9 significant digits so they parse back to the
+same float32 values.
+"""
+from __future__ import annotations
+
+import json
+import math
+import os
+
+import numpy as np
+
+F32 = np.float32
+
+
+def _fmt(v) -> str:
+    return "%.9g" % float(F32(v))
+
+
+def _write_job(folder, settings, scene_name="scene.json"):
+    with open(os.path.join(folder, "renderJob.json"), "w") as f:
+        json.dump({"settings": settings, "scene": scene_name}, f, indent=2)
+
+
+def write_pfm(path, rgb_top_first: np.ndarray):
+    """Colour PFM, rows bottom-to-top as the format specifies."""
+    h, w, _ = rgb_top_first.shape
+    with open(path, "wb") as f:
+        f.write(b"PF\n%d %d\n-1.0\n" % (w, h))
+        f.write(np.ascontiguousarray(rgb_top_first[::-1].astype("<f4")).tobytes())
+
+
+def read_pfm(path) -> np.ndarray:
+    """Returns an (H, W, 3) float32 image with row 0 = top."""
+    with open(path, "rb") as f:
+        data = f.read()
+    parts = data.split(b"\n", 3)
+    magic, dims, scale = parts[0].strip(), parts[1].split(), float(parts[2])
+    w, h = int(dims[0]), int(dims[1])
+    ch = 3 if magic == b"PF" else 1
+    arr = np.frombuffer(parts[3][: w * h * ch * 4], dtype="<f4" if scale < 0 else ">f4")
+    arr = arr.reshape(h, w, ch)[::-1]
+    if ch == 1:
+        arr = np.repeat(arr, 3, axis=2)
+    return np.ascontiguousarray(arr.astype(np.float32))
+
+
+class _Obj:
+    def __init__(self):
+        self.lines = []
+        self.nv = 0
+        self.nn = 0
+
+    def shape(self, name):
+        self.lines.append("o %s" % name)
+
+    def material(self, name):
+        self.lines.append("usemtl %s" % name)
+
+    def smooth(self, on):
+        self.lines.append("s %s" % ("1" if on else "off"))
+
+    def vertices(self, v):
+        base = self.nv
+        for p in np.asarray(v, dtype=F32):
+            self.lines.append("v %s %s %s" % (_fmt(p[0]), _fmt(p[1]), _fmt(p[2])))
+        self.nv += len(v)
+        return base
+
+    def normals(self, n):
+        base = self.nn
+        for p in np.asarray(n, dtype=F32):
+            self.lines.append("vn %s %s %s" % (_fmt(p[0]), _fmt(p[1]), _fmt(p[2])))
+        self.nn += len(n)
+        return base
+
+    def faces(self, tris, vbase, nbase=None):
+        for t in tris:
+            a, b, c = (int(i) + vbase + 1 for i in t)
+            if nbase is None:
+                self.lines.append("f %d %d %d" % (a, b, c))
+            else:
+                na, nb, nc = (int(i) + nbase + 1 for i in t)
+                self.lines.append("f %d//%d %d//%d %d//%d" % (a, na, b, nb, c, nc))
+
+    def write(self, path, mtllib):
+        with open(path, "w") as f:
+            f.write("mtllib %s\n" % mtllib)
+            f.write("\n".join(self.lines))
+            f.write("\n")
+
+
+def _mtl(name, kd=(0, 0, 0), ka=(0, 0, 0), ks=(0, 0, 0), ke=(0, 0, 0), ns=0.0, ni=1.0, d=1.0):
+    return (
+        "newmtl %s\nKa %s %s %s\nKd %s %s %s\nKs %s %s %s\nKe %s %s %s\nNs %s\nNi %s\nd %s\nillum 2\n"
+        % (name, *map(_fmt, ka), *map(_fmt, kd), *map(_fmt, ks), *map(_fmt, ke), _fmt(ns), _fmt(ni), _fmt(d))
+    )
+
+
+def _box(center, size, angle_deg):
+    """12 triangles of an axis-aligned box rotated about +y, outward winding."""
+    cx, cy, cz = center
+    sx, sy, sz = (s * 0.5 for s in size)
+    corners = np.array(
+        [[x, y, z] for x in (-sx, sx) for y in (-sy, sy) for z in (-sz, sz)], dtype=np.float64
+    )
+    a = math.radians(angle_deg)
+    rot = np.array([[math.cos(a), 0, math.sin(a)], [0, 1, 0], [-math.sin(a), 0, math.cos(a)]])
+    v = corners @ rot.T + np.array([cx, cy, cz])
+    # index = 4*ix + 2*iy + iz
+    quads = [
+        (0, 1, 3, 2),  # -x
+        (4, 6, 7, 5),  # +x
+        (0, 4, 5, 1),  # -y
+        (2, 3, 7, 6),  # +y
+        (0, 2, 6, 4),  # -z
+        (1, 5, 7, 3),  # +z
+    ]
+    tris = []
+    for q in quads:
+        tris.append((q[0], q[1], q[2]))
+        tris.append((q[0], q[2], q[3]))
+    return v, tris
+
+
+def write_cornell(folder, width=256, height=256, spp=64, max_depth=8):
+    """Procedural Cornell box: 5 walls (10 triangles), two boxes rotated
+    +-17 degrees (12 triangles each) and a 0.25 x 0.25 quad light of radiance
+    (17, 12, 4) just under the ceiling; 36 triangles, black background,
+    40 degree FOV pinhole camera, RR from depth 3, NEE + MIS."""
+    os.makedirs(folder, exist_ok=True)
+    obj = _Obj()
+    mtl = [
+        _mtl("white", kd=(0.73, 0.73, 0.73)),
+        _mtl("red", kd=(0.65, 0.05, 0.05)),
+        _mtl("green", kd=(0.12, 0.45, 0.15)),
+        _mtl("light", ke=tuple(math.pi * c for c in (17.0, 12.0, 4.0))),
+    ]
+    obj.shape("walls")
+    obj.smooth(False)
+    quad = [(0, 1, 2), (0, 2, 3)]
+    walls = {
+        "floor": ([[-1, 0, -1], [1, 0, -1], [1, 0, 1], [-1, 0, 1]], "white"),
+        "ceiling": ([[-1, 2, -1], [-1, 2, 1], [1, 2, 1], [1, 2, -1]], "white"),
+        "back": ([[-1, 0, -1], [-1, 2, -1], [1, 2, -1], [1, 0, -1]], "white"),
+        "left": ([[-1, 0, -1], [-1, 0, 1], [-1, 2, 1], [-1, 2, -1]], "red"),
+        "right": ([[1, 0, -1], [1, 2, -1], [1, 2, 1], [1, 0, 1]], "green"),
+    }
+    for _, (v, m) in walls.items():
+        obj.material(m)
+        base = obj.vertices(v)
+        obj.faces(quad, base)
+    obj.shape("boxes")
+    obj.smooth(False)
+    obj.material("white")
+    for center, size, ang in (((0.33, 0.3, 0.3), (0.6, 0.6, 0.6), -17.0), ((-0.33, 0.6, -0.35), (0.6, 1.2, 0.6), 17.0)):
+        v, t = _box(center, size, ang)
+        base = obj.vertices(v)
+        obj.faces(t, base)
+    obj.shape("light")
+    obj.smooth(False)
+    obj.material("light")
+    s = 0.125
+    base = obj.vertices([[-s, 1.98, -s], [s, 1.98, -s], [s, 1.98, s], [-s, 1.98, s]])
+    obj.faces(quad, base)  # normal -y: emits downwards
+    obj.write(os.path.join(folder, "scene.obj"), "scene.mtl")
+    with open(os.path.join(folder, "scene.mtl"), "w") as f:
+        f.write("\n".join(mtl))
+    scene = {
+        "camera": {"position": [0.0, 1.0, 3.9], "look_at": [0.0, 1.0, 0.0], "fov": 40.0},
+        "world": {"background_color": [0.0, 0.0, 0.0]},
+        "mesh": "scene.obj",
+    }
+    with open(os.path.join(folder, "scene.json"), "w") as f:
+        json.dump(scene, f, indent=2)
+    _write_job(
+        folder,
+        {
+            "width": width,
+            "height": height,
+            "num_samples": spp,
+            "max_depth": max_depth,
+            "roulette_start_depth": 3,
+            "roulette_q": 0.3,
+            "use_mis": True,
+            "num_threads": 1,
+        },
+    )
+    return os.path.join(folder, "renderJob.json")
+
+
+def icosphere(subdiv):
+    """Unit icosphere: returns (vertices float64 (N,3), triangles int (M,3))."""
+    t = (1.0 + math.sqrt(5.0)) / 2.0
+    verts = [
+        (-1, t, 0), (1, t, 0), (-1, -t, 0), (1, -t, 0),
+        (0, -1, t), (0, 1, t), (0, -1, -t), (0, 1, -t),
+        (t, 0, -1), (t, 0, 1), (-t, 0, -1), (-t, 0, 1),
+    ]
+    verts = [np.array(v, dtype=np.float64) / np.linalg.norm(v) for v in verts]
+    faces = [
+        (0, 11, 5), (0, 5, 1), (0, 1, 7), (0, 7, 10), (0, 10, 11),
+        (1, 5, 9), (5, 11, 4), (11, 10, 2), (10, 7, 6), (7, 1, 8),
+        (3, 9, 4), (3, 4, 2), (3, 2, 6), (3, 6, 8), (3, 8, 9),
+        (4, 9, 5), (2, 4, 11), (6, 2, 10), (8, 6, 7), (9, 8, 1),
+    ]
+    for _ in range(subdiv):
+        cache = {}
+        new_faces = []
+
+        def mid(a, b):
+            key = (min(a, b), max(a, b))
+            if key not in cache:
+                m = verts[a] + verts[b]
+                verts.append(m / np.linalg.norm(m))
+                cache[key] = len(verts) - 1
+            return cache[key]
+
+        for a, b, c in faces:
+            ab, bc, ca = mid(a, b), mid(b, c), mid(c, a)
+            new_faces += [(a, ab, ca), (b, bc, ab), (c, ca, bc), (ab, bc, ca)]
+        faces = new_faces
+    return np.array(verts), np.array(faces, dtype=np.int64)
+
+
+def sky_texture(width=2048, height=1024, sun_elev_deg=35.0, sun_azim_deg=60.0, sun_radiance=5e4,
+                sun_diameter_deg=0.5):
+    """Procedural lat-long sky (theta from +y, row 0 = zenith): a blue
+    gradient above the horizon, a dark ground below, and a sun disc."""
+    y = (np.arange(height) + 0.5) / height * math.pi
+    x = (np.arange(width) + 0.5) / width * 2.0 * math.pi
+    theta, phi = np.meshgrid(y, x, indexing="ij")
+    d = np.stack([np.sin(theta) * np.cos(phi), np.cos(theta), np.sin(theta) * np.sin(phi)], axis=-1)
+    up = np.clip(d[..., 1], 0.0, 1.0)
+    zenith = np.array([0.25, 0.45, 0.95])
+    horizon = np.array([0.9, 0.95, 1.0])
+    sky = horizon[None, None] * (1 - up[..., None]) ** 2 + zenith[None, None] * (1 - (1 - up[..., None]) ** 2)
+    ground = np.array([0.18, 0.16, 0.14])
+    img = np.where(d[..., 1:2] >= 0.0, sky, ground[None, None])
+    el, az = math.radians(sun_elev_deg), math.radians(sun_azim_deg)
+    sun = np.array([math.cos(el) * math.cos(az), math.sin(el), math.cos(el) * math.sin(az)])
+    cosang = d @ sun
+    disc = cosang >= math.cos(math.radians(sun_diameter_deg * 0.5))
+    img[disc] = sun_radiance
+    return img.astype(np.float32), int(disc.sum())
+
+
+def write_ggx_grid(folder, width=1024, height=1024, spp=256, max_depth=16, seed=1, env_size=(2048, 1024),
+                   subdiv=4):
+    """7 x 7 icospheres (subdivision 4: 5,120 triangles each) on a
+    2-triangle ground plane; GGX alpha linear in [0.02, 1] across columns;
+    rows alternate conductor (metallic, F0 ~ U[0.5, 1]^3) and opaque
+    dielectric (IOR 1.5, specular 0.08 -> F0 = 0.04); importance-sampled
+    2048 x 1024 sky with a 5e4 sun of 0.5 degrees; thin lens f/2.8
+    (36 mm film at the fixed 45 degree FOV -> f = 43.5 mm, aperture radius
+    7.8 mm, 1 unit = 1 m), 8 blades."""
+    os.makedirs(folder, exist_ok=True)
+    rng = np.random.default_rng(seed)
+    verts, faces = icosphere(subdiv)
+    obj = _Obj()
+    mtl = [_mtl("ground", kd=(0.5, 0.5, 0.5), ns=0.0)]
+    obj.shape("spheres")
+    obj.smooth(True)
+    radius = 0.4
+    for r in range(7):
+        for c in range(7):
+            alpha = 0.02 + (1.0 - 0.02) * c / 6.0
+            ns = 1000.0 * (1.0 - math.sqrt(alpha)) ** 2
+            name = "sphere_%d_%d" % (r, c)
+            if r % 2 == 0:
+                f0 = rng.uniform(0.5, 1.0, 3)
+                mtl.append(_mtl(name, kd=tuple(f0), ka=(1, 1, 1), ns=ns))
+            else:
+                kd = rng.uniform(0.2, 0.8, 3)
+                mtl.append(_mtl(name, kd=tuple(kd), ks=(0.08, 0.08, 0.08), ns=ns, ni=1.5))
+            center = np.array([(c - 3) * 1.0, radius, (r - 3) * 1.0])
+            obj.material(name)
+            vb = obj.vertices(verts * radius + center)
+            nb = obj.normals(verts)
+            obj.faces(faces, vb, nb)
+    obj.shape("ground")
+    obj.smooth(False)
+    obj.material("ground")
+    g = 12.0
+    vb = obj.vertices([[-g, 0, -g], [-g, 0, g], [g, 0, g], [g, 0, -g]])
+    obj.faces([(0, 1, 2), (0, 2, 3)], vb)  # normal +y
+    obj.write(os.path.join(folder, "scene.obj"), "scene.mtl")
+    with open(os.path.join(folder, "scene.mtl"), "w") as f:
+        f.write("\n".join(mtl))
+    img, _ = sky_texture(*env_size)
+    write_pfm(os.path.join(folder, "sky.pfm"), img)
+    pos = np.array([0.0, 4.0, 8.5])
+    look = np.array([0.0, 0.4, 0.0])
+    focal_mm = 18.0 / math.tan(math.radians(22.5))
+    scene = {
+        "camera": {
+            "position": pos.tolist(),
+            "look_at": look.tolist(),
+            "aperture": focal_mm / (2.0 * 2.8) / 1000.0,
+            "focal_distance": float(np.linalg.norm(look - pos)),
+            "num_blades": 8,
+        },
+        "world": {"environment_texture": "sky.pfm"},
+        "mesh": "scene.obj",
+    }
+    with open(os.path.join(folder, "scene.json"), "w") as f:
+        json.dump(scene, f, indent=2)
+    _write_job(folder, {"width": width, "height": height, "num_samples": spp, "max_depth": max_depth,
+                        "use_mis": True, "num_threads": 1})
+    return os.path.join(folder, "renderJob.json")

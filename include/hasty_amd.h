@@ -1,0 +1,120 @@
+/* C ABI of the MI355X path tracer.
+ *
+ * Plain C: opaque handles, plain pointers and sizes, int status codes
+ * (0 = OK, nonzero = error; hasty_last_error() returns the message of the
+ * calling thread's last failure).  No C++ exception crosses this boundary.
+ *
+ * Films are row-major RGB float32 arrays of width*height*3 floats holding
+ * SUMS over samples, exactly like the reference's accumulation buffers
+ * (core/include/Hasty/Image.h:302-313); estimate = sum / number of samples
+ * (apps/ViewerUtils.cpp:95-121).
+ *
+ * Calls on one scene must be serialised by the caller (the reference's
+ * threads only share the scene read-only and hand out sample indices under a
+ * mutex, core/include/Hasty/RenderJob.h:53-60).  All device memory belongs to
+ * the scene; every call is synchronous on the scene's own HIP stream.
+ */
+#ifndef HASTY_AMD_H
+#define HASTY_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct HastyScene HastyScene;
+
+/* RenderSettings (core/include/Hasty/RenderJob.h:15-30; JSON keys
+ * core/src/RenderJob.cpp:10-20). */
+typedef struct HastyRenderSettings {
+  int32_t roulette_start_depth;
+  float roulette_q;
+  uint32_t width;
+  uint32_t height;
+  uint32_t num_threads;
+  int32_t max_depth;
+  int32_t num_samples;
+  int32_t use_mis;
+} HastyRenderSettings;
+
+typedef struct HastySceneInfo {
+  HastyRenderSettings settings;
+  uint32_t num_triangles;
+  uint32_t num_shapes;
+  uint32_t num_materials;
+  uint32_t num_light_triangles;
+  int32_t has_environment;
+  uint32_t bvh_nodes;      /* 4-wide nodes, 128 bytes each */
+  int32_t bvh_depth;       /* inner levels of the 4-wide tree */
+  uint64_t device_bytes;   /* scene + BVH bytes resident in HBM */
+} HastySceneInfo;
+
+/* Replaces RenderJob::loadJSON (core/src/RenderJob.cpp:21-28) together with
+ * Scene::Scene and its rtcCommitScene (core/src/Scene.cpp:12-195): parses
+ * renderJob.json -> scene JSON -> .obj/.mtl (+ PFM environment map), builds
+ * the 4-wide BVH on the host and uploads everything to the current device.
+ * Errors use the reference's messages (malformed input, a face without a
+ * material, an emissive face with a non-black Kd/Ks, ...). */
+int hasty_scene_load(const char* render_job_path, HastyScene** out_scene);
+
+/* Frees the scene and its device memory (Scene::~Scene, Scene.cpp:196-200). */
+void hasty_scene_destroy(HastyScene* scene);
+
+int hasty_scene_info(const HastyScene* scene, HastySceneInfo* out_info);
+
+/* Overrides the job's settings (e.g. a smaller film for a check render).
+ * width/height/max_depth/roulette/use_mis take effect on the next render. */
+int hasty_scene_set_settings(HastyScene* scene, const HastyRenderSettings* settings);
+
+/* Path tracing: one call renders samples [first_sample, first_sample +
+ * num_samples) of every pixel and ADDS them, in sample order, into the three
+ * films -- what renderThread's pass loop does (core/src/PathTracing.cpp:
+ * 98-160, declared core/include/Hasty/PathTracing.h:63).  Sample k uses the
+ * Halton (2,3) point of index k+1 as its shared sub-pixel offset
+ * (RenderJob.h:53-60).  Random numbers come from one counter-based stream per
+ * (seed, pixel, sample).  Samples whose radiance is NaN are skipped in all
+ * three films (PathTracing.cpp:132-135).  Host pointers. */
+int hasty_render_pt(HastyScene* scene, uint32_t seed, uint32_t first_sample, uint32_t num_samples, float* color_sum,
+                    float* normal_sum, float* albedo_sum);
+
+/* Same, with the films already resident in device memory (HBM) of the
+ * scene's device. */
+int hasty_render_pt_device(HastyScene* scene, uint32_t seed, uint32_t first_sample, uint32_t num_samples,
+                           float* d_color_sum, float* d_normal_sum, float* d_albedo_sum);
+
+/* Batched Scene::rayHit (core/src/Scene.cpp:281-303): closest hit of n rays
+ * (origin xyz, direction xyz; tnear 1e-6, tfar +inf).  Outputs per ray:
+ * geom_prim[2i] = geomID (OBJ shape index) and geom_prim[2i+1] = primID, or
+ * 0xFFFFFFFF twice for a miss; tri[i] = global triangle index (-1 miss);
+ * tuv[3i..] = t, u, v with barycentrics (1-u-v, u, v).  Host pointers. */
+int hasty_trace_closest(HastyScene* scene, const float* rays, uint32_t n, uint32_t* geom_prim, int32_t* tri,
+                        float* tuv);
+
+/* Per-kernel timing with HIP events on the scene's stream. */
+typedef struct HastyKernelStat {
+  char name[32];
+  uint64_t launches;
+  double total_ms;
+} HastyKernelStat;
+int hasty_set_profiling(HastyScene* scene, int enable);
+/* Writes up to max_stats entries, returns the number written (negative on error). */
+int hasty_kernel_stats(HastyScene* scene, HastyKernelStat* stats, int max_stats);
+int hasty_reset_stats(HastyScene* scene);
+
+/* Device counters of the last renders (enabled together with profiling):
+ * out[0] extension rays, out[1] shadow rays, out[2] BVH node visits,
+ * out[3] triangle tests, out[4] paths whose media stack overflowed. */
+int hasty_counters(HastyScene* scene, uint64_t* out5);
+
+/* Launch-shape knobs: paths in flight per batch (samples per pixel rendered
+ * together) and grid sizes.  0 keeps the current value. */
+int hasty_set_launch(HastyScene* scene, uint32_t samples_per_batch, uint32_t trace_blocks, uint32_t shade_blocks);
+
+const char* hasty_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HASTY_AMD_H */

@@ -1,0 +1,223 @@
+// ORACLE (test infrastructure only).  CPU restatement of HastyRenderer's
+// unidirectional path tracer: sampling routines, PrincipledBRDF, media state,
+// scene queries, light and environment sampling, MIS and the recursive
+// estimateRadiance.  Only tests/, the smoke check and bench.py's CPU baseline
+// use it, always as the checker; the renderer itself never links it.
+//
+// Each function names the reference lines it restates.  Three documented
+// substitutions: Embree -> oracle_bvh.h, shishua -> rng.h (one counter-based
+// stream per path), std::discrete_distribution -> explicit CDF + upper_bound.
+#pragma once
+#include <stdint.h>
+
+#include <atomic>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../hastyrenderer_amd/csrc/rng.h"
+#include "../hastyrenderer_amd/csrc/scene_io.h"
+#include "oracle_bvh.h"
+#include "oracle_math.h"
+
+namespace oracle {
+
+using RNG = hrng::Stream;
+inline float uniform01f(RNG& rng) { return rng.uniform01f(); }
+
+// Random.h:14-63
+struct HaltonSequence {
+  uint32_t b, n = 0, d = 1;
+  explicit HaltonSequence(uint32_t base) : b(base) {}
+  float next() {
+    uint32_t x = d - n;
+    if (x == 1) {
+      n = 1;
+      d *= b;
+    } else {
+      uint32_t y = d / b;
+      while (x <= y) y /= b;
+      n = (b + 1) * y - x;
+    }
+    return (float)n / (float)d;
+  }
+};
+// Offset for sample index k: the (k+1)-th point of HaltonSequence2
+// (RenderJob.h:53-60).
+Vec2f haltonOffset(uint32_t k);
+
+// ---------------------------------------------------------- Sample.cpp ----
+Vec2f sampleDiskUniformly(RNG& rng, float& pDensity);
+Vec2f sampleCircularNGonUniformly(RNG& rng, int numBlades, float bladeRotation, float& pDensity);
+Vec3f sampleHemisphereCosImportance(RNG& rng, const Vec3f& normal, float& pDensity);
+float evaluateHemisphereCosImportancePDF(const Vec3f& normal, const Vec3f& direction);
+Vec3f sampleTriangleUniformly(RNG& rng);
+Vec3f sampleSphereSurfaceUniformly(RNG& rng, float& pDensity);
+Vec3f sampleHemisphereSurfaceUniformly(RNG& rng, const Vec3f& normal, float& pDensity);
+
+// ------------------------------------------------------------ BRDF.cpp ----
+bool computeRefractionDirection(const Vec3f& wi, const Vec3f& normal, float iorI, float iorT, Vec3f& wt);
+float fresnelDielectric(float cos_i, float cos_t, float iorI, float iorT);
+float DGGX(float nh, float th, float bh, float alpha_t, float alpha_b);
+float smithGGX(float nv, float tv, float bv, float alpha_t, float alpha_b);
+Vec3f sampleGGXVNDF(Vec3f wOut, float alphaX, float alphaY, float xi1, float xi2);
+Vec3f sampleGGXVNDFGlobal(RNG& rng, const Vec3f& normal, float at, float ab, const Vec3f& tangent,
+                          const Vec3f& bitangent, const Vec3f& dir1, float& pDensity);
+float sampleGGXVNDFGlobalDensity(const Vec3f& normal, float at, float ab, const Vec3f& tangent, const Vec3f& bitangent,
+                                 const Vec3f& dir1, const Vec3f& dir2);
+
+struct SurfaceInteraction {  // Scene.h:20-33
+  uint32_t geomID = 0, primID = 0;
+  int32_t tri = -1;
+  float h = 0.0f;
+  Vec3f x, normalGeometric, normalShadingDefault, tangent, bitangent;
+  int32_t materialId = -1;
+  Vec3f barycentricCoordinates;
+};
+struct RayHit {  // Scene.h:35-54
+  HitRecord rtc;
+  Vec3f rayDir;
+  SurfaceInteraction interaction;
+  Vec3f wFrom() const { return -rayDir; }
+};
+inline bool hasHitSurface(const RayHit& h) { return h.rtc.tri >= 0; }
+inline bool hasHitSurface(const RayHit& h, int32_t tri) { return h.rtc.tri == tri; }
+inline bool isSelfIntersection(const RayHit& a, const RayHit& b) {  // PathTracing.h:41-45
+  return a.rtc.tri == b.rtc.tri;
+}
+
+struct MaterialEvalResult {
+  Vec3f fDiffuse, fConcentrated, normalShading;
+};
+struct SampleResult {
+  bool outside = true;
+  Vec3f direction, throughputDiffuse, throughputConcentrated;
+  float pdfOmega = 0.0f;
+  Vec3f throughput() const { return throughputDiffuse + throughputConcentrated; }
+};
+enum EvalFlag { kDiffuse = 1, kConcentrated = 2, kAll = 3 };
+
+class Scene;
+
+// LightRayInfo.h / LightRayInfo.cpp:8-125
+struct Medium {
+  Vec3f transmittance = Vec3f::Ones();
+  float indexOfRefraction = 1.0f;
+  int geomID = -1;
+};
+struct LightRayInfo {
+  int wavelength = -1;
+  std::vector<Medium> media;  // SmallVector<.., 4> semantics (unbounded)
+  Vec3f transmittance = Vec3f::Ones();
+  const Vec3f& getTransmittance() const { return transmittance; }
+  void updateMedia(const Scene& scene, const RayHit& hit, const Vec3f& wTo);
+  int sampleOrGetWavelength(RNG& rng, bool sample, float& sampleProbability);
+  float getOutsideIOR(const RayHit& hit) const;
+  void recompute();
+};
+
+struct RenderSettings {
+  int rouletteStartDepth = 1;
+  float rouletteQ = 0.3f;
+  uint32_t width = 0, height = 0;
+  int maxDepth = 30;
+  int numSamples = 32;
+  bool useMIS = true;
+};
+
+class Scene {
+ public:
+  hasty::Scene data;
+  Bvh2 bvh;
+  mutable Bvh2::Stats* stats = nullptr;  // per-thread counters when set
+
+  void init();  // builds the BVH and derived tables
+  SurfaceInteraction getInteraction(int32_t tri, const Vec3f& bary, const Vec3f& geomNormal) const;
+  Vec3f constructRayX(const SurfaceInteraction& it, bool outside) const;
+  Ray constructRay(const SurfaceInteraction& it, const Vec3f& dir, bool outside) const;
+  Ray constructRayEnd(const SurfaceInteraction& it, const Vec3f& end, bool outside) const;
+  void rayHit(const Ray& ray, RayHit& hit) const;
+
+  // material access
+  const hasty::Material& mat(const SurfaceInteraction& it) const { return data.materials[it.materialId]; }
+  Vec3f getAlbedo(const SurfaceInteraction& it) const;
+  bool hasBRDFDiffuseLobe(const RayHit& hit) const;
+  float getIORInside(const RayHit& hit, int wavelength) const;
+  MaterialEvalResult evaluteBRDF(const RayHit& hit, const Vec3f& wo, const Vec3f& wi, float iorOutside, bool adjoint,
+                                 int flags) const;
+  SampleResult sampleBRDF(RNG& rng, LightRayInfo& lightRay, const RayHit& hit, bool adjoint, int flags) const;
+  float evaluateSamplePDF(const RayHit& hit, const LightRayInfo& lightRay, const Vec3f& dir2) const;
+
+  // lights / background
+  bool hasLight() const { return data.hasLight(); }
+  bool hasSurfaceLight() const { return data.hasSurfaceLight(); }
+  bool isSurfaceLight(const RayHit& hit) const;
+  bool isInfiniteAreaLight(const RayHit& hit) const { return !hasHitSurface(hit); }
+  Vec3f getEmissionRadiance(const Vec3f& wo, const RayHit& hit) const;
+  Vec3f getEmissionRadianceOfTri(int32_t tri) const;
+  Vec3f evalEnvironment(const Ray& ray) const;
+  SurfaceInteraction sampleSurfaceLightPosition(RNG& rng, float& pDensity) const;
+  Ray sampleLightRay(RNG& rng, Vec3f& flux) const;
+  Ray sampleLightRayFromStartPoint(RNG& rng, const SurfaceInteraction& point, float& pDensity, RayHit& hit,
+                                   bool& lightVisible) const;
+  float evalLightRayFromStartPointDensity(const SurfaceInteraction& point, const Ray& ray2, const RayHit& hit2) const;
+
+  // environment (Environment.h:47-171)
+  Vec3f envEvaluateUV(const Vec2f& uv) const;
+  Vec2f envSampleUV(RNG& rng, float& density) const;
+  float envEvalDensityUV(const Vec2f& uv) const;
+  Vec3f envSample(RNG& rng, float& density) const;
+  float envEvalSamplingDensity(const Vec3f& dir) const;
+
+  // camera (Camera.cpp:41-98)
+  Ray computeRay(RNG& rng, const Vec2f& p, float w, float h) const;
+  float computeRayAngle(const Vec2f& p, float w, float h) const;
+};
+
+// PrincipledBRDF (BRDF.cpp:253-750) on a scene material.
+struct Principled {
+  static float computeAlpha(float roughness);
+  struct Prob {
+    bool noStrategy, refractionPossible;
+    float pDiffuse, pSpecular, pRefractive, FDaccurate, cosT;
+  };
+  static Prob computeProbability(float metallic, float specular, float transmission, float cos_o, float ior_o,
+                                 float ior_refr_o, int flags);
+  static Vec3f getShadingNormal(const SurfaceInteraction& it, const Vec3f& wo, float dotNgWo);
+  static float getIndexOfRefraction(const hasty::Material& m, int wavelength);
+  static MaterialEvalResult evaluate(const hasty::Material& m, const SurfaceInteraction& it, const Vec3f& wo,
+                                     const Vec3f& wi, float iorOutside, bool adjoint, int flags);
+  static SampleResult sample(const hasty::Material& m, RNG& rng, const SurfaceInteraction& it, const Vec3f& wo,
+                             int wavelength, float iorOutside, bool adjoint, int flags);
+  static float evaluateSamplePDF(const hasty::Material& m, const SurfaceInteraction& it, const Vec3f& wo,
+                                 const Vec3f& wi, float outsideIOR);
+  static bool hasDiffuseLobe(const hasty::Material& m);
+};
+
+// PathTracing.h:46-59
+Vec3f beersLaw(const Vec3f& transmittance, float d);
+
+struct RenderContext {
+  const Scene& scene;
+  RNG& rng;
+  const RenderSettings& settings;
+};
+
+// PathTracing.cpp:21-95
+Vec3f estimateRadiance(RenderContext ctx, LightRayInfo& lightRay, const Ray& ray, Vec3f& normal, Vec3f& albedo,
+                       int depth, RayHit* rayhitPtr = nullptr);
+
+struct PTJob {
+  Scene scene;
+  RenderSettings settings;
+  uint32_t seed = 0;
+};
+
+// One renderThread pass (PathTracing.cpp:113-152) for samples
+// [firstSample, firstSample + nSamples), adding into the three film sums
+// (row-major RGB float32).  Pixels are split over `threads` workers; each
+// pixel's samples are still added in sample order.
+void renderPT(const PTJob& job, uint32_t firstSample, uint32_t nSamples, int threads, float* color, float* normal,
+              float* albedo, uint64_t* nanCount, Bvh2::Stats* stats);
+
+}  // namespace oracle

@@ -1,0 +1,192 @@
+"""ORACLE (test infrastructure only): ctypes binding of ``liboracle.so``.
+
+Imported by tests/, ``__graft_entry__.smoke()`` and bench.py's CPU-baseline
+leg, always as the checker.  The renderer never imports it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            raise RuntimeError("oracle library not built: run `make -C oracle` (or __graft_entry__.build())")
+        L = C.CDLL(path)
+        P = C.c_void_p
+        fp = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
+        ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+        up = np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS")
+        u64 = np.ctypeslib.ndpointer(dtype=np.uint64, flags="C_CONTIGUOUS")
+        L.oracle_last_error.restype = C.c_char_p
+        L.oracle_job_load.argtypes = [C.c_char_p, C.c_uint32, C.POINTER(P)]
+        L.oracle_job_free.argtypes = [P]
+        L.oracle_job_set.argtypes = [P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_float, C.c_int]
+        L.oracle_job_info.argtypes = [P, ip]
+        L.oracle_render_pt.argtypes = [P, C.c_uint32, C.c_uint32, C.c_int, fp, fp, fp, u64]
+        L.oracle_render_pt_counted.argtypes = [P, C.c_uint32, C.c_uint32, fp, fp, fp, u64]
+        L.oracle_trace.argtypes = [P, fp, C.c_uint32, C.c_int, ip, fp, ip]
+        L.oracle_tri_ids.argtypes = [P, C.c_int32, up]
+        L.oracle_camera_rays.argtypes = [P, C.c_uint32, fp]
+        L.oracle_halton.argtypes = [C.c_uint32, fp]
+        L.oracle_philox.argtypes = [up, up, up]
+        L.oracle_sample_vndf.argtypes = [fp, C.c_uint32, fp]
+        for name in ("oracle_dggx", "oracle_smith"):
+            getattr(L, name).argtypes = [C.c_float] * 5
+            getattr(L, name).restype = C.c_float
+        L.oracle_fresnel.argtypes = [C.c_float] * 4
+        L.oracle_fresnel.restype = C.c_float
+        L.oracle_refract.argtypes = [fp, fp, C.c_float, C.c_float, fp]
+        L.oracle_sampler.argtypes = [C.c_int, C.c_uint32, C.c_uint32, fp, fp, C.c_float, fp]
+        L.oracle_env_sample.argtypes = [P, C.c_uint32, C.c_uint32, fp]
+        L.oracle_env_pdf.argtypes = [P, fp]
+        L.oracle_env_pdf.restype = C.c_float
+        L.oracle_bsdf_eval.argtypes = [fp, fp, fp, C.c_uint32, fp]
+        L.oracle_bsdf_sample.argtypes = [fp, fp, fp, C.c_uint32, C.c_uint32, fp]
+        L.oracle_rotation.argtypes = [fp, fp, fp, fp]
+        L.oracle_spherical.argtypes = [fp, fp]
+        L.oracle_hm.argtypes = [C.c_int, C.c_float, C.c_float]
+        L.oracle_hm.restype = C.c_float
+        L.oracle_hm_vec.argtypes = [C.c_int, fp, fp, C.c_uint32, fp]
+        _LIB = L
+    return _LIB
+
+
+def f32(a):
+    return np.ascontiguousarray(np.asarray(a, dtype=np.float32))
+
+
+class OracleJob:
+    """A loaded renderJob.json with the oracle's BVH2 built."""
+
+    def __init__(self, job_path: str, seed: int = 0):
+        L = lib()
+        h = C.c_void_p()
+        rc = L.oracle_job_load(job_path.encode(), seed, C.byref(h))
+        if rc != 0:
+            raise RuntimeError(L.oracle_last_error().decode())
+        self.h = h
+        self.seed = seed
+
+    def __del__(self):
+        if getattr(self, "h", None) is not None and _LIB is not None:
+            _LIB.oracle_job_free(self.h)
+            self.h = None
+
+    def set(self, width=0, height=0, max_depth=-1, rr_start=-1, rr_q=-1.0, use_mis=-1):
+        lib().oracle_job_set(self.h, width, height, max_depth, rr_start, rr_q, use_mis)
+
+    def info(self):
+        a = np.zeros(8, np.int32)
+        lib().oracle_job_info(self.h, a)
+        keys = ("width", "height", "spp", "max_depth", "num_tris", "num_lights", "has_env", "rr_start")
+        return dict(zip(keys, (int(x) for x in a)))
+
+    def render(self, first_sample: int, n_samples: int, threads: int = 8, films=None):
+        info = self.info()
+        w, h = info["width"], info["height"]
+        if films is None:
+            films = tuple(np.zeros((h, w, 3), np.float32) for _ in range(3))
+        nan = np.zeros(1, np.uint64)
+        rc = lib().oracle_render_pt(self.h, first_sample, n_samples, threads, *films, nan)
+        if rc != 0:
+            raise RuntimeError(lib().oracle_last_error().decode())
+        return films, int(nan[0])
+
+    def render_counted(self, first_sample: int, n_samples: int):
+        info = self.info()
+        w, h = info["width"], info["height"]
+        films = tuple(np.zeros((h, w, 3), np.float32) for _ in range(3))
+        out = np.zeros(3, np.uint64)
+        rc = lib().oracle_render_pt_counted(self.h, first_sample, n_samples, *films, out)
+        if rc != 0:
+            raise RuntimeError(lib().oracle_last_error().decode())
+        return films, {"rays": int(out[0]), "node_visits": int(out[1]), "tri_tests": int(out[2])}
+
+    def trace(self, rays, brute=False):
+        rays = f32(rays).reshape(-1, 6)
+        n = rays.shape[0]
+        tri = np.zeros(n, np.int32)
+        tuv = np.zeros((n, 3), np.float32)
+        edge = np.zeros(n, np.int32)
+        lib().oracle_trace(self.h, rays, n, 1 if brute else 0, tri, tuv, edge)
+        return tri, tuv, edge
+
+    def tri_ids(self, tri):
+        out = np.zeros(2, np.uint32)
+        lib().oracle_tri_ids(self.h, int(tri), out)
+        return int(out[0]), int(out[1])
+
+    def camera_rays(self, sample: int):
+        info = self.info()
+        out = np.zeros((info["height"], info["width"], 6), np.float32)
+        lib().oracle_camera_rays(self.h, sample, out)
+        return out
+
+    def env_sample(self, seed, count):
+        out = np.zeros((count, 5), np.float32)
+        lib().oracle_env_sample(self.h, seed, count, out)
+        return out
+
+    def env_pdf(self, d):
+        return float(lib().oracle_env_pdf(self.h, f32(d)))
+
+
+def sampler(kind, seed, count, normal=(0, 0, 1), wo=(0, 0, 1), param=0.0):
+    out = np.zeros((count, 5), np.float32)
+    lib().oracle_sampler(kind, seed, count, f32(normal), f32(wo), float(param), out)
+    return out
+
+
+def bsdf_eval(params, n, dirs6):
+    """params: albedo rgb, roughness, metallic, specular, ior, anisotropy, transmission."""
+    dirs6 = f32(dirs6).reshape(-1, 6)
+    out = np.zeros((dirs6.shape[0], 7), np.float32)
+    if lib().oracle_bsdf_eval(f32(params), f32(n), dirs6, dirs6.shape[0], out) != 0:
+        raise RuntimeError(lib().oracle_last_error().decode())
+    return out
+
+
+def bsdf_sample(params, n, wo, seed, count):
+    out = np.zeros((count, 7), np.float32)
+    if lib().oracle_bsdf_sample(f32(params), f32(n), f32(wo), seed, count, out) != 0:
+        raise RuntimeError(lib().oracle_last_error().decode())
+    return out
+
+
+def sample_vndf(rows7):
+    rows7 = f32(rows7).reshape(-1, 7)
+    out = np.zeros((rows7.shape[0], 3), np.float32)
+    lib().oracle_sample_vndf(rows7, rows7.shape[0], out)
+    return out
+
+
+def halton(k):
+    out = np.zeros(2, np.float32)
+    lib().oracle_halton(k, out)
+    return out
+
+
+def philox(ctr, key):
+    out = np.zeros(4, np.uint32)
+    lib().oracle_philox(np.asarray(ctr, np.uint32), np.asarray(key, np.uint32), out)
+    return out
+
+
+def hm_vec(fn, a, b):
+    a, b = f32(a), f32(b)
+    out = np.zeros_like(a)
+    lib().oracle_hm_vec(fn, a, b, a.size, out)
+    return out
+
+
+def hm(fn, a, b=0.0):
+    return float(lib().oracle_hm(fn, float(a), float(b)))

@@ -1,0 +1,226 @@
+// Test-only HIP entry points: run the renderer's device primitives (the
+// deterministic math, the RNG, the GGX/BSDF and sampling routines of
+// pt_device.h) on the GPU over host-supplied inputs, so the GPU tests can
+// compare them bit for bit with the CPU oracle.  Built into
+// tests/libhasty_testkernels.so; the renderer does not link it.
+#include <hip/hip_runtime.h>
+#include <string.h>
+
+#include <vector>
+
+#include "../../hastyrenderer_amd/csrc/pt_device.h"
+
+using namespace hdev;
+
+namespace {
+
+template <class T>
+T* toDevice(const T* h, size_t n) {
+  T* d = nullptr;
+  if (hipMalloc(&d, sizeof(T) * (n ? n : 1)) != hipSuccess) return nullptr;
+  if (n) (void)hipMemcpy(d, h, sizeof(T) * n, hipMemcpyHostToDevice);
+  return d;
+}
+template <class T>
+T* devAlloc(size_t n) {
+  T* d = nullptr;
+  if (hipMalloc(&d, sizeof(T) * (n ? n : 1)) != hipSuccess) return nullptr;
+  return d;
+}
+
+__global__ void kMath(int fn, const float* a, const float* b, uint32_t n, float* out) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float x = a[i], y = b[i], r;
+  switch (fn) {
+    case 0: r = hm::sinf(x); break;
+    case 1: r = hm::cosf(x); break;
+    case 2: r = hm::tanf(x); break;
+    case 3: r = hm::atan2f(x, y); break;
+    case 4: r = hm::acosf(x); break;
+    case 5: r = hm::powf_pos(x, y); break;
+    case 6: r = hm::fmodf(x, y); break;
+    case 7: r = hm::atanf(x); break;
+    case 8: r = hm::fsqrt(x); break;
+    case 9: r = x / y; break;
+    default: r = (float)((double)x * (double)y + 1.0); break;
+  }
+  out[i] = r;
+}
+
+__global__ void kPhilox(const uint32_t* in, uint32_t n, uint32_t* out) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t* c = in + i * 6;
+  hrng::U32x4 r = hrng::philox4x32_10(c[0], c[1], c[2], c[3], c[4], c[5]);
+  for (int k = 0; k < 4; ++k) out[i * 4 + k] = r.v[k];
+}
+
+__global__ void kVndf(const float* in, uint32_t n, float* out) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float* a = in + i * 7;
+  V3 r = sampleVNDF(v3(a[4], a[5], a[6]), a[2], a[3], a[0], a[1]);
+  out[i * 3] = r.x, out[i * 3 + 1] = r.y, out[i * 3 + 2] = r.z;
+}
+
+// Same draw sequence as oracle_sampler (stream (seed, 0, 0, 99)), one lane.
+__global__ void kSampler(int kind, uint32_t seed, uint32_t count, V3 n, V3 wo, float param, float* out) {
+  if (blockIdx.x * blockDim.x + threadIdx.x != 0) return;
+  hrng::Stream rng = hrng::make_stream(seed, 0, 0, 99);
+  for (uint32_t i = 0; i < count; ++i) {
+    float* o = out + (size_t)i * 5;
+    if (kind == 0) {
+      float pdf;
+      V3 d = sampleCosHemisphere(rng, n, &pdf);
+      o[0] = d.x, o[1] = d.y, o[2] = d.z, o[3] = pdf, o[4] = cosHemispherePdf(n, d);
+    } else if (kind == 1) {
+      V3 t = anyOrthonormal(n);
+      V3 b = cross(n, t);
+      float pdf;
+      V3 d = sampleVNDFGlobal(rng, n, param, param, t, b, wo, &pdf);
+      o[0] = d.x, o[1] = d.y, o[2] = d.z, o[3] = pdf, o[4] = vndfGlobalPdf(n, param, param, t, b, wo, d);
+    } else if (kind == 3) {
+      float x, y;
+      sampleNGon(rng, (int)param, 0.0f, &x, &y);
+      float anglePerSector = (float)(2.0 * kPi / (int)param);
+      float areaSector = 0.5 * hm::sinf(anglePerSector);
+      float areaNGon = (int)param * areaSector;
+      float pdf = 1.0 / areaNGon;
+      o[0] = x, o[1] = y, o[2] = 0.0f, o[3] = pdf, o[4] = pdf;
+    } else {
+      float s = rng.uniform01f();
+      float t = hm::fsqrt(rng.uniform01f());
+      float b1 = 1.0f - t, b3 = t * s, b2 = fmaxf(0.0f, 1.0f - b1 - b3);
+      o[0] = b1, o[1] = b2, o[2] = b3, o[3] = 0.0f, o[4] = 0.0f;
+    }
+  }
+}
+
+// Flat interaction with normal n at x = (0.3, 0.2, 0.1), as oracle_bsdf_eval.
+__device__ Interaction flatInteraction(V3 n) {
+  Interaction it;
+  it.tri = 0;
+  it.geomID = 0;
+  it.mat = 0;
+  it.h = 0.0f;
+  it.x = v3(0.3f, 0.2f, 0.1f);
+  it.ng = n;
+  it.ns = n;
+  it.tangent = orthonormalizedOrAny(n, v3(-it.x.z, 0.0f, it.x.x));
+  it.bitangent = cross(n, it.tangent);
+  return it;
+}
+__global__ void kBsdfEval(DevMaterial m, V3 n, const float* dirs, uint32_t count, float* out) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  Interaction it = flatInteraction(n);
+  V3 wo = load3(dirs + i * 6), wi = load3(dirs + i * 6 + 3);
+  BsdfEval e = bsdfEvaluate(m, it, wo, wi);
+  float* o = out + i * 7;
+  o[0] = e.fD.x, o[1] = e.fD.y, o[2] = e.fD.z, o[3] = e.fC.x, o[4] = e.fC.y, o[5] = e.fC.z;
+  o[6] = bsdfPdf(m, it, wo, wi, 1.0f);
+}
+__global__ void kBsdfSample(DevMaterial m, V3 n, V3 wo, uint32_t seed, uint32_t count, float* out) {
+  if (blockIdx.x * blockDim.x + threadIdx.x != 0) return;
+  Interaction it = flatInteraction(n);
+  hrng::Stream rng = hrng::make_stream(seed, 0, 0, 97);
+  for (uint32_t i = 0; i < count; ++i) {
+    BsdfSample s = bsdfSample(m, rng, it, wo, -1, 1.0f, false, 3);
+    float* o = out + (size_t)i * 7;
+    V3 t = s.tD + s.tC;
+    o[0] = s.dir.x, o[1] = s.dir.y, o[2] = s.dir.z, o[3] = t.x, o[4] = t.y, o[5] = t.z, o[6] = s.pdf;
+  }
+}
+
+DevMaterial makeMaterial(const float* p) {
+  // albedo rgb, roughness, metallic, specular, ior, anisotropy, transmission
+  DevMaterial m{};
+  m.albedo[0] = p[0], m.albedo[1] = p[1], m.albedo[2] = p[2];
+  m.roughness = p[3];
+  float alpha = p[3] * p[3];
+  if (alpha < 1e-2f && alpha != 0.0f) alpha = 1e-2f;
+  m.alpha = alpha;
+  m.metallic = p[4];
+  m.specular = p[5];
+  m.ior = p[6];
+  m.anisotropy = p[7];
+  m.transmission = p[8];
+  m.varyIOR = 0;
+  m.isLight = 0;
+  m.hasDiffuseLobe = ((1.0f - 0.5f * m.specular) * (1.0 - m.transmission) * (1.0 - m.metallic) > 0.0f) ? 1 : 0;
+  return m;
+}
+
+int finish(float* dOut, float* hOut, size_t n) {
+  if (hipDeviceSynchronize() != hipSuccess) return 2;
+  if (hipMemcpy(hOut, dOut, sizeof(float) * n, hipMemcpyDeviceToHost) != hipSuccess) return 3;
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int tk_math(int fn, const float* a, const float* b, uint32_t n, float* out) {
+  float* da = toDevice(a, n);
+  float* db = toDevice(b, n);
+  float* dout = devAlloc<float>(n);
+  if (!da || !db || !dout) return 1;
+  kMath<<<(n + 255) / 256, 256>>>(fn, da, db, n, dout);
+  int rc = finish(dout, out, n);
+  (void)hipFree(da), (void)hipFree(db), (void)hipFree(dout);
+  return rc;
+}
+
+int tk_philox(const uint32_t* in6, uint32_t n, uint32_t* out4) {
+  uint32_t* din = toDevice(in6, (size_t)n * 6);
+  uint32_t* dout = devAlloc<uint32_t>((size_t)n * 4);
+  if (!din || !dout) return 1;
+  kPhilox<<<(n + 255) / 256, 256>>>(din, n, dout);
+  if (hipDeviceSynchronize() != hipSuccess) return 2;
+  (void)hipMemcpy(out4, dout, sizeof(uint32_t) * n * 4, hipMemcpyDeviceToHost);
+  (void)hipFree(din), (void)hipFree(dout);
+  return 0;
+}
+
+int tk_vndf(const float* in7, uint32_t n, float* out3) {
+  float* din = toDevice(in7, (size_t)n * 7);
+  float* dout = devAlloc<float>((size_t)n * 3);
+  if (!din || !dout) return 1;
+  kVndf<<<(n + 255) / 256, 256>>>(din, n, dout);
+  int rc = finish(dout, out3, (size_t)n * 3);
+  (void)hipFree(din), (void)hipFree(dout);
+  return rc;
+}
+
+int tk_sampler(int kind, uint32_t seed, uint32_t count, const float* n, const float* wo, float param, float* out5) {
+  float* dout = devAlloc<float>((size_t)count * 5);
+  if (!dout) return 1;
+  kSampler<<<1, 64>>>(kind, seed, count, v3(n[0], n[1], n[2]), v3(wo[0], wo[1], wo[2]), param, dout);
+  int rc = finish(dout, out5, (size_t)count * 5);
+  (void)hipFree(dout);
+  return rc;
+}
+
+int tk_bsdf_eval(const float* matParams, const float* n, const float* dirs6, uint32_t count, float* out7) {
+  float* dd = toDevice(dirs6, (size_t)count * 6);
+  float* dout = devAlloc<float>((size_t)count * 7);
+  if (!dd || !dout) return 1;
+  kBsdfEval<<<(count + 255) / 256, 256>>>(makeMaterial(matParams), v3(n[0], n[1], n[2]), dd, count, dout);
+  int rc = finish(dout, out7, (size_t)count * 7);
+  (void)hipFree(dd), (void)hipFree(dout);
+  return rc;
+}
+
+int tk_bsdf_sample(const float* matParams, const float* n, const float* wo, uint32_t seed, uint32_t count,
+                   float* out7) {
+  float* dout = devAlloc<float>((size_t)count * 7);
+  if (!dout) return 1;
+  kBsdfSample<<<1, 64>>>(makeMaterial(matParams), v3(n[0], n[1], n[2]), v3(wo[0], wo[1], wo[2]), seed, count, dout);
+  int rc = finish(dout, out7, (size_t)count * 7);
+  (void)hipFree(dout);
+  return rc;
+}
+
+}  // extern "C"

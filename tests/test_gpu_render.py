@@ -1,0 +1,163 @@
+"""Closest-hit queries and path-traced films on the GPU vs the CPU oracle.
+
+Parity bar (per the path tracer's contract): triangle ids of closest hits are
+identical (the watertight test with smallest-t / smallest-index tie-break is
+order independent), and per-pixel radiance at equal spp is within 1e-3
+relative on at least 99.9% of pixels.  Both sides use the same counter-based
+RNG keys, so paths take the same decisions; only float reassociation of the
+accumulated radiance differs.
+"""
+import numpy as np
+import pytest
+
+import hastyrenderer_amd as ha
+from conftest import rel_ok
+from oracle import pyoracle as po
+
+pytestmark = pytest.mark.gpu
+
+
+def _random_rays(n, lo, hi, rng):
+    o = rng.uniform(lo, hi, (n, 3))
+    d = rng.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    return np.concatenate([o, d], axis=1).astype(np.float32)
+
+
+@pytest.mark.parametrize("which", ["cornell", "ggx_small"])
+def test_closest_hit_matches_oracle(which, cornell_job, ggx_small_job):
+    job = cornell_job if which == "cornell" else ggx_small_job
+    rng = np.random.default_rng(5)
+    scene = ha.Scene(job)
+    oj = po.OracleJob(job)
+    rays = _random_rays(20000, -1.2, 2.2, rng)
+    cam = oj.camera_rays(0).reshape(-1, 6)
+    rays = np.concatenate([rays, cam[::7]])
+    gp, tri, tuv = scene.trace_closest(rays)
+    otri, otuv, edge = oj.trace(rays)
+    assert (tri == otri).all()
+    hit = tri >= 0
+    assert (tuv[hit].view(np.uint32) == otuv[hit].view(np.uint32)).all()
+    for i in np.nonzero(hit)[0][:50]:
+        assert tuple(gp[i]) == oj.tri_ids(tri[i])
+    assert (gp[~hit] == 0xFFFFFFFF).all()
+
+
+def test_edge_rays_watertight(cornell_job):
+    """Rays aimed exactly at shared vertices/edges of the walls never slip through."""
+    scene = ha.Scene(cornell_job)
+    oj = po.OracleJob(cornell_job)
+    targets = []
+    for y in (0.0, 1.0, 2.0):
+        for x in (-1.0, -0.5, 0.0, 0.5, 1.0):
+            targets.append((x, y, -1.0))  # back wall incl. its diagonal and corners
+    targets = np.array(targets, np.float32)
+    o = np.array([0.013, 1.0, 3.0], np.float32)
+    d = targets - o
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    rays = np.concatenate([np.repeat(o[None], len(d), 0), d], axis=1).astype(np.float32)
+    _, tri, _ = scene.trace_closest(rays)
+    otri, _, _ = oj.trace(rays)
+    btri, _, _ = oj.trace(rays, brute=True)
+    assert (tri == otri).all() and (otri == btri).all()
+    assert (tri >= 0).all()
+
+
+def test_cornell_parity_full_config(cornell_job):
+    """Procedural Cornell box, 256^2, 64 spp, max depth 8, RR from depth 3, NEE+MIS, seed 0."""
+    scene = ha.Scene(cornell_job)
+    info = scene.info()
+    spp = info["num_samples"]
+    assert (info["width"], info["height"], spp, info["max_depth"]) == (256, 256, 64, 8)
+    c, n, a = scene.render_pt(seed=0, first_sample=0, num_samples=spp)
+    oj = po.OracleJob(cornell_job, seed=0)
+    (rc, rn, ra), nan = oj.render(0, spp, threads=16)
+    ok = rel_ok(c, rc)
+    frac = ok.mean()
+    print("cornell: pixels within 1e-3: %.6f, mean gpu %s oracle %s" % (frac, c.mean((0, 1)) / spp,
+                                                                        rc.mean((0, 1)) / spp))
+    assert frac >= 0.999
+    assert rel_ok(a, ra, 1e-5).mean() >= 0.9999
+    assert rel_ok(n, rn, 1e-5).mean() >= 0.9999
+
+
+def test_ggx_grid_parity_reduced(ggx_small_job):
+    """GGX grid (env-map NEE, thin lens, conductors + dielectrics) at 96^2 x 4 spp."""
+    scene = ha.Scene(ggx_small_job)
+    spp = 4
+    c, n, a = scene.render_pt(seed=1, first_sample=0, num_samples=spp)
+    oj = po.OracleJob(ggx_small_job, seed=1)
+    (rc, rn, ra), nan = oj.render(0, spp, threads=16)
+    frac = rel_ok(c, rc).mean()
+    print("ggx small: pixels within 1e-3: %.6f" % frac)
+    assert frac >= 0.999
+
+
+def test_full_ggx_grid_parity_sampled(ggx_job):
+    """The benchmark GGX grid (250,882 triangles, 2048x1024 sky, f/2.8 lens) on a
+    160x160 film crop of the same camera, 2 spp."""
+    scene = ha.Scene(ggx_job)
+    scene.set_settings(width=160, height=160)
+    c, n, a = scene.render_pt(seed=1, first_sample=0, num_samples=2)
+    oj = po.OracleJob(ggx_job, seed=1)
+    oj.set(width=160, height=160)
+    (rc, rn, ra), nan = oj.render(0, 2, threads=16)
+    frac = rel_ok(c, rc).mean()
+    print("ggx full scene: pixels within 1e-3: %.6f" % frac)
+    assert frac >= 0.999
+
+
+def test_deterministic_across_launch_shapes(cornell_job):
+    scene = ha.Scene(cornell_job)
+    scene.set_settings(width=128, height=96)
+    ref = scene.render_pt(seed=3, first_sample=5, num_samples=6)
+    scene.set_launch(samples_per_batch=1, trace_blocks=64, shade_blocks=32)
+    alt = scene.render_pt(seed=3, first_sample=5, num_samples=6)
+    scene.set_launch(samples_per_batch=4, trace_blocks=1024, shade_blocks=4096)
+    alt2 = scene.render_pt(seed=3, first_sample=5, num_samples=6)
+    for x, y, z in zip(ref, alt, alt2):
+        assert (x.view(np.uint32) == y.view(np.uint32)).all()
+        assert (x.view(np.uint32) == z.view(np.uint32)).all()
+
+
+def test_incremental_passes_equal_one_call(cornell_job):
+    """Adding samples over several calls equals one call (film sums in sample order)."""
+    scene = ha.Scene(cornell_job)
+    scene.set_settings(width=64, height=64)
+    one = scene.render_pt(seed=0, first_sample=0, num_samples=8)
+    films = tuple(np.zeros((64, 64, 3), np.float32) for _ in range(3))
+    for k in range(0, 8, 3):
+        scene.render_pt(seed=0, first_sample=k, num_samples=min(3, 8 - k), films=films)
+    for x, y in zip(one, films):
+        assert (x.view(np.uint32) == y.view(np.uint32)).all()
+
+
+def test_render_thread_mirror(cornell_job):
+    """The renderThread host mirror (batches of sample indices, stop flag,
+    copyBuffer) yields the same sums as a direct call and computeEstimate
+    divides by the sample count."""
+    job = ha.RenderJob()
+    job.loadJSON(cornell_job)
+    job.scene.set_settings(width=48, height=32, num_samples=10)
+    job.renderSettings = ha.RenderSettings(job.scene.info())
+    bufs = [ha.Image3fAccDoubleBuffer(48, 32) for _ in range(3)]
+    td = ha.RenderThreadData(seed=0)
+    ha.renderThread(*bufs, job, td, batch=3)
+    assert td.stoppedFlag
+    assert bufs[0].getReadBuffer().numSamples == 10
+    direct = job.scene.render_pt(seed=0, first_sample=0, num_samples=10)
+    assert (bufs[0].getReadBuffer().data.view(np.uint32) == direct[0].view(np.uint32)).all()
+    est = ha.computeEstimate([bufs[0]])
+    assert np.allclose(est, direct[0] / np.float32(1e-7 + 10.0))
+
+
+def test_stop_flag_stops_between_batches(cornell_job):
+    job = ha.RenderJob()
+    job.loadJSON(cornell_job)
+    job.scene.set_settings(width=16, height=16, num_samples=12)
+    job.renderSettings = ha.RenderSettings(job.scene.info())
+    job.stopFlag = True
+    bufs = [ha.Image3fAccDoubleBuffer(16, 16) for _ in range(3)]
+    td = ha.RenderThreadData(seed=0)
+    ha.renderThread(*bufs, job, td, batch=4)
+    assert td.stoppedFlag and bufs[0].getWriteBuffer().numSamples == 0
