@@ -132,7 +132,8 @@ struct HastyScene {
   uint2* traceOverflow = nullptr;
   uint32_t traceOverflowThreads = 0;
 
-  bool profiling = false;
+  bool profiling = false;  // HIP-event timing of every launch
+  bool counting = false;   // device ray / node / triangle counters
   std::unique_ptr<EventTimer> timer;
   uint64_t stats[hasty::kNumStats] = {0};
 
@@ -241,7 +242,7 @@ struct HastyScene {
       float a = h2.next(), b = h3.next();
       if (k >= first) offs[k - first] = make_float2(a, b);
     }
-    launch.countStats = profiling;
+    launch.countStats = counting;
     if (profiling && !timer) timer = std::make_unique<EventTimer>();
     for (uint32_t done = 0; done < nSamples; done += K) {
       uint32_t k = std::min(K, nSamples - done);
@@ -251,13 +252,13 @@ struct HastyScene {
       HIPCHECK(hipGetLastError());
     }
     HIPCHECK(hipStreamSynchronize(stream));
-    if (profiling) {
+    if (counting) {
       unsigned long long s[hasty::kNumStats];
       HIPCHECK(hipMemcpy(s, pb.stats, sizeof(s), hipMemcpyDeviceToHost));
       HIPCHECK(hipMemset(pb.stats, 0, sizeof(s)));
       for (int i = 0; i < hasty::kNumStats; ++i) stats[i] += s[i];
-      timer->collect();
     }
+    if (profiling) timer->collect();
   }
 };
 
@@ -506,9 +507,10 @@ int hasty_trace_closest(HastyScene* s, const float* rays, uint32_t n, uint32_t* 
   }
 }
 
-int hasty_set_profiling(HastyScene* s, int enable) {
+int hasty_set_profiling(HastyScene* s, int flags) {
   if (!s) return 1;
-  s->profiling = enable != 0;
+  s->profiling = (flags & 1) != 0;
+  s->counting = (flags & 2) != 0;
   return 0;
 }
 

@@ -222,67 +222,82 @@ __device__ inline bool intersectTri(const RayPre& r, float4 p0, float4 p1, float
 constexpr float kTmaxScale = 1.0f + 2.0f * 3.0f * 0x1p-24f / (1.0f - 3.0f * 0x1p-24f);
 constexpr float kCullSlack = 1.001f;
 
-// Closest hit with tie-break on the smaller triangle index.  The stack keeps
-// (ref, entry distance): the first `ldsDepth` entries in LDS, interleaved by
-// lane (entry i of lane l at lds[i * laneStride + l]), deeper ones in the
-// lane's global overflow slice.
-template <int LDS_DEPTH, int OVF_DEPTH>
-__device__ inline Hit traceClosest(const DevScene& s, V3 o, V3 d, uint2* lds, int laneStride, uint2* ovf,
-                                   uint32_t* nodeCount, uint32_t* triCount) {
-  RayPre r = prepareRay(o, d);
+// Register compare-exchange of (entry distance, child ref) pairs.
+__device__ inline void cswap(float& ta, uint32_t& ra, float& tb, uint32_t& rb) {
+  bool sw = tb < ta;
+  float t = sw ? tb : ta;
+  uint32_t r = sw ? rb : ra;
+  tb = sw ? ta : tb;
+  rb = sw ? ra : rb;
+  ta = t;
+  ra = r;
+}
+
+// Shared BVH walk.  CLOSEST: smallest t, ties to the smaller triangle index.
+// !CLOSEST (occlusion): stops at the first triangle that beats the bound
+// (t < tBound, or t == tBound with index < idBound) and reports it.
+// The stack keeps (ref, entry distance): the first LDS_DEPTH entries in LDS,
+// interleaved by lane (entry i of a lane at lds[i * laneStride]), deeper ones
+// in the lane's global overflow slice.  Children are ordered near-to-far with
+// a 4-input sorting network held in registers.
+template <int LDS_DEPTH, int OVF_DEPTH, bool CLOSEST>
+__device__ inline Hit traverse(const DevScene& s, const RayPre& r, float tBound, int32_t idBound, uint2* lds,
+                               int laneStride, uint2* ovf, uint32_t* nodeCount, uint32_t* triCount) {
   Hit best;
-  best.t = __builtin_inff();
+  best.t = tBound;
   best.u = best.v = 0.0f;
   best.tri = -1;
   int sp = 0;
   uint32_t ref = 0;  // root (always inner)
   uint32_t nNodes = 0, nTris = 0;
-  while (true) {
+  bool done = false;
+  while (!done) {
     if ((ref & kLeafFlag) == 0) {
       const float4* n = s.nodes + (size_t)ref * 8;
       float4 lox = n[0], hix = n[1], loy = n[2], hiy = n[3], loz = n[4], hiz = n[5];
       int4 refs = *reinterpret_cast<const int4*>(n + 6);
       nNodes++;
-      float tb = best.tri < 0 ? __builtin_inff() : best.t * kCullSlack;
-      float te[4];
-      uint32_t cr[4];
-      int cnt = 0;
-      float lo_[4][3] = {{lox.x, loy.x, loz.x}, {lox.y, loy.y, loz.y}, {lox.z, loy.z, loz.z}, {lox.w, loy.w, loz.w}};
-      float hi_[4][3] = {{hix.x, hiy.x, hiz.x}, {hix.y, hiy.y, hiz.y}, {hix.z, hiy.z, hiz.z}, {hix.w, hiy.w, hiz.w}};
-      uint32_t rf[4] = {(uint32_t)refs.x, (uint32_t)refs.y, (uint32_t)refs.z, (uint32_t)refs.w};
-#pragma unroll
-      for (int k = 0; k < 4; ++k) {
-        float t0x = (lo_[k][0] - r.o.x) * r.inv.x, t1x = (hi_[k][0] - r.o.x) * r.inv.x;
-        float t0y = (lo_[k][1] - r.o.y) * r.inv.y, t1y = (hi_[k][1] - r.o.y) * r.inv.y;
-        float t0z = (lo_[k][2] - r.o.z) * r.inv.z, t1z = (hi_[k][2] - r.o.z) * r.inv.z;
-        float tmin = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fmaxf(fminf(t0z, t1z), -__builtin_inff()));
-        float tmax = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fminf(fmaxf(t0z, t1z), __builtin_inff()));
-        tmax *= kTmaxScale;
-        bool hit = rf[k] != kLeafFlag && tmin <= tmax && tmax > 0.0f && tmin <= tb;
-        if (hit) {
-          // insertion into the small sorted list (ascending entry distance)
-          int pos = cnt;
-          while (pos > 0 && te[pos - 1] > tmin) {
-            te[pos] = te[pos - 1];
-            cr[pos] = cr[pos - 1];
-            --pos;
-          }
-          te[pos] = tmin;
-          cr[pos] = rf[k];
-          ++cnt;
-        }
-      }
+      float tb = (CLOSEST && best.tri < 0) ? __builtin_inff() : best.t * kCullSlack;
+#define HDEV_SLAB(LX, HX, LY, HY, LZ, HZ, RF, TOUT)                                          \
+  {                                                                                          \
+    float t0x = (LX - r.o.x) * r.inv.x, t1x = (HX - r.o.x) * r.inv.x;                        \
+    float t0y = (LY - r.o.y) * r.inv.y, t1y = (HY - r.o.y) * r.inv.y;                        \
+    float t0z = (LZ - r.o.z) * r.inv.z, t1z = (HZ - r.o.z) * r.inv.z;                        \
+    float tmin = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fminf(t0z, t1z));           \
+    float tmax = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fmaxf(t0z, t1z)) * kTmaxScale; \
+    bool h = (uint32_t)(RF) != kLeafFlag && tmin <= tmax && tmax > 0.0f && tmin <= tb;      \
+    TOUT = h ? tmin : __builtin_inff();                                                      \
+  }
+      float e0, e1, e2, e3;
+      HDEV_SLAB(lox.x, hix.x, loy.x, hiy.x, loz.x, hiz.x, refs.x, e0)
+      HDEV_SLAB(lox.y, hix.y, loy.y, hiy.y, loz.y, hiz.y, refs.y, e1)
+      HDEV_SLAB(lox.z, hix.z, loy.z, hiy.z, loz.z, hiz.z, refs.z, e2)
+      HDEV_SLAB(lox.w, hix.w, loy.w, hiy.w, loz.w, hiz.w, refs.w, e3)
+#undef HDEV_SLAB
+      uint32_t r0 = (uint32_t)refs.x, r1 = (uint32_t)refs.y, r2 = (uint32_t)refs.z, r3 = (uint32_t)refs.w;
+      int cnt = (e0 != __builtin_inff()) + (e1 != __builtin_inff()) + (e2 != __builtin_inff()) +
+                (e3 != __builtin_inff());
       if (cnt > 0) {
-        for (int k = cnt - 1; k >= 1; --k) {
-          uint2 e = make_uint2(cr[k], __float_as_uint(te[k]));
-          // The BVH depth is checked at load so that 3 * depth fits; the
-          // guard only keeps every write inside this lane's slice.
-          if (sp >= LDS_DEPTH + OVF_DEPTH) break;
-          if (sp < LDS_DEPTH) lds[sp * laneStride] = e;
-          else ovf[sp - LDS_DEPTH] = e;
-          ++sp;
-        }
-        ref = cr[0];
+        cswap(e0, r0, e1, r1);
+        cswap(e2, r2, e3, r3);
+        cswap(e0, r0, e2, r2);
+        cswap(e1, r1, e3, r3);
+        cswap(e1, r1, e2, r2);
+        // push the farther children (farthest first), continue with the nearest
+        // The BVH depth is checked at load so that 3 * depth fits; the guard
+        // only keeps every write inside this lane's slice.
+#define HDEV_PUSH(RR, EE)                                       \
+  if (sp < LDS_DEPTH + OVF_DEPTH) {                            \
+    uint2 e_ = make_uint2(RR, __float_as_uint(EE));            \
+    if (sp < LDS_DEPTH) lds[sp * laneStride] = e_;             \
+    else ovf[sp - LDS_DEPTH] = e_;                             \
+    ++sp;                                                      \
+  }
+        if (cnt > 3) HDEV_PUSH(r3, e3)
+        if (cnt > 2) HDEV_PUSH(r2, e2)
+        if (cnt > 1) HDEV_PUSH(r1, e1)
+#undef HDEV_PUSH
+        ref = r0;
         continue;
       }
     } else if (ref != kLeafFlag) {
@@ -294,31 +309,59 @@ __device__ inline Hit traceClosest(const DevScene& s, V3 o, V3 d, uint2* lds, in
         int32_t id = (int32_t)__float_as_uint(p0.w);
         nTris++;
         float t, u, v;
-        if (intersectTri(r, p0, p1, p2, 1e-6f, &t, &u, &v) &&
-            (best.tri < 0 || t < best.t || (t == best.t && id < best.tri))) {
-          best.t = t;
-          best.u = u;
-          best.v = v;
-          best.tri = id;
+        if (intersectTri(r, p0, p1, p2, 1e-6f, &t, &u, &v)) {
+          if (CLOSEST) {
+            if (best.tri < 0 || t < best.t || (t == best.t && id < best.tri)) {
+              best.t = t;
+              best.u = u;
+              best.v = v;
+              best.tri = id;
+            }
+          } else if (t < tBound || (t == tBound && id < idBound)) {
+            best.t = t;
+            best.u = u;
+            best.v = v;
+            best.tri = id;
+            done = true;
+            break;
+          }
         }
       }
+      if (done) break;
     }
     // pop the nearest remaining entry that can still hold a closer hit
     bool found = false;
     while (sp > 0) {
       --sp;
       uint2 e = sp < LDS_DEPTH ? lds[sp * laneStride] : ovf[sp - LDS_DEPTH];
-      if (best.tri < 0 || __uint_as_float(e.y) <= best.t * kCullSlack) {
+      float lim = (CLOSEST && best.tri < 0) ? __builtin_inff() : best.t * kCullSlack;
+      if (__uint_as_float(e.y) <= lim) {
         ref = e.x;
         found = true;
         break;
       }
     }
-    if (!found) break;
+    if (!found) done = true;
   }
   if (nodeCount) *nodeCount += nNodes;
   if (triCount) *triCount += nTris;
+  if (CLOSEST && best.tri < 0) best.t = __builtin_inff();
   return best;
+}
+
+template <int LDS_DEPTH, int OVF_DEPTH>
+__device__ inline Hit traceClosest(const DevScene& s, V3 o, V3 d, uint2* lds, int laneStride, uint2* ovf,
+                                   uint32_t* nodeCount, uint32_t* triCount) {
+  RayPre r = prepareRay(o, d);
+  return traverse<LDS_DEPTH, OVF_DEPTH, true>(s, r, __builtin_inff(), 0, lds, laneStride, ovf, nodeCount, triCount);
+}
+
+// Is there a triangle whose hit beats (tBound, idBound)?  Returns its index or -1.
+template <int LDS_DEPTH, int OVF_DEPTH>
+__device__ inline int32_t findBlocker(const DevScene& s, const RayPre& r, float tBound, int32_t idBound, uint2* lds,
+                                      int laneStride, uint2* ovf, uint32_t* nodeCount, uint32_t* triCount) {
+  Hit h = traverse<LDS_DEPTH, OVF_DEPTH, false>(s, r, tBound, idBound, lds, laneStride, ovf, nodeCount, triCount);
+  return h.tri;
 }
 
 // ---------------------------------------------------------- interaction ----
@@ -330,16 +373,16 @@ struct Interaction {
   V3 x, ng, ns, tangent, bitangent;
 };
 
-// Scene::rayHit + getInteraction (Scene.cpp:202-255, 281-303).
-__device__ inline Interaction makeInteraction(const DevScene& s, int32_t tri, float u, float v) {
+// Scene::getInteraction (Scene.cpp:202-255) for explicit barycentrics and
+// geometric normal.
+__device__ inline Interaction makeInteractionB(const DevScene& s, int32_t tri, float b0, float b1, float b2, V3 ng) {
   Interaction it;
   const float* p = s.triPos + (size_t)tri * 9;
   V3 p0 = load3(p), p1 = load3(p + 3), p2 = load3(p + 6);
-  float b0 = 1.0f - u - v, b1 = u, b2 = v;
   it.tri = tri;
   it.geomID = s.triGeom[tri];
   it.mat = s.triMat[tri];
-  it.ng = normalize(cross(p1 - p0, p2 - p0));
+  it.ng = ng;
   it.x = p0 * b0 + p1 * b1 + p2 * b2;
   if (s.triSmooth[tri]) {
     const float* q = s.triNrm + (size_t)tri * 9;
@@ -363,6 +406,14 @@ __device__ inline Interaction makeInteraction(const DevScene& s, int32_t tri, fl
   it.tangent = orthonormalizedOrAny(it.ng, v3(-it.x.z, 0.0f, it.x.x));
   it.bitangent = cross(it.ng, it.tangent);
   return it;
+}
+// Scene::rayHit + getInteraction (Scene.cpp:281-303): barycentrics (1-u-v, u, v),
+// geometric normal normalize((v1-v0) x (v2-v0)).
+__device__ inline Interaction makeInteraction(const DevScene& s, int32_t tri, float u, float v) {
+  const float* p = s.triPos + (size_t)tri * 9;
+  V3 p0 = load3(p), p1 = load3(p + 3), p2 = load3(p + 6);
+  V3 ng = normalize(cross(p1 - p0, p2 - p0));
+  return makeInteractionB(s, tri, 1.0f - u - v, u, v, ng);
 }
 // Scene.cpp:257-270
 __device__ inline V3 offsetOrigin(const Interaction& it, bool outside) {

@@ -69,6 +69,15 @@ __device__ inline uint32_t blockAppend(uint32_t* counter, bool pred, uint32_t* l
   return r;
 }
 
+// Workgroups are dealt round-robin to the 8 XCDs; renumber them so that
+// consecutive queue ranges (neighbouring pixels, similar rays) run on the same
+// XCD and share its L2.
+__device__ inline uint32_t xcdBlock() {
+  uint32_t g = gridDim.x, b = blockIdx.x;
+  if (g % 8u != 0u) return b;
+  return (b % 8u) * (g / 8u) + b / 8u;
+}
+
 __device__ inline void waveAddStat(unsigned long long* c, uint32_t v) {
   // lane-sum within the wave, one atomic per wave
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
@@ -131,7 +140,7 @@ __global__ void __launch_bounds__(kBlock) kExtend(DevScene s, PathBuffers pb, co
                                                   const uint32_t* count, int countStats) {
   __shared__ uint2 stack[kStackLds * kBlock];
   uint32_t n = *count;
-  uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t gtid = xcdBlock() * blockDim.x + threadIdx.x;
   uint32_t stride = gridDim.x * blockDim.x;
   uint2* ovf = pb.stackOverflow + (size_t)gtid * kStackOvf;
   uint32_t nodes = 0, tris = 0, rays = 0;
@@ -332,7 +341,7 @@ __global__ void __launch_bounds__(kBlock) kShade(DevScene s, PathBuffers pb, con
 __global__ void __launch_bounds__(kBlock) kConnect(DevScene s, PathBuffers pb, const uint32_t* count, int countStats) {
   __shared__ uint2 stack[kStackLds * kBlock];
   uint32_t n = *count;
-  uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t gtid = xcdBlock() * blockDim.x + threadIdx.x;
   uint32_t stride = gridDim.x * blockDim.x;
   uint2* ovf = pb.stackOverflow + (size_t)gtid * kStackOvf;
   uint32_t nodes = 0, tris = 0, rays = 0;
@@ -340,12 +349,38 @@ __global__ void __launch_bounds__(kBlock) kConnect(DevScene s, PathBuffers pb, c
     uint32_t slot = pb.qS[q];
     float4 o4 = pb.shO[slot], d4 = pb.shD[slot];
     V3 so = f4xyz(o4), sd = f4xyz(d4);
-    Hit h = traceClosest<kStackLds, kStackOvf>(s, so, sd, stack + threadIdx.x, kBlock, ovf, countStats ? &nodes : nullptr,
-                                    countStats ? &tris : nullptr);
-    rays++;
     int32_t target = __float_as_int(d4.w);
-    bool hitLight = h.tri >= 0 && s.mats[s.triMat[h.tri]].isLight;
-    bool visible = target >= 0 ? h.tri == target : (h.tri < 0 || hitLight);
+    RayPre rp = prepareRay(so, sd);
+    rays++;
+    // Visibility exactly as the closest-hit test of Scene.cpp:558-559 / 572-577,
+    // answered with an early-exit search wherever that is equivalent.
+    Hit h;
+    bool visible;
+    if (target >= 0) {
+      // closest hit == target  <=>  the target is hit at (tT, target) and no
+      // other triangle hits before it (or at tT with a smaller index)
+      const float* p = s.triPos + (size_t)target * 9;
+      float4 p0 = make_float4(p[0], p[1], p[2], 0.0f), p1 = make_float4(p[3], p[4], p[5], 0.0f),
+             p2 = make_float4(p[6], p[7], p[8], 0.0f);
+      float tT, uT, vT;
+      visible = intersectTri(rp, p0, p1, p2, 1e-6f, &tT, &uT, &vT);
+      if (visible) {
+        visible = findBlocker<kStackLds, kStackOvf>(s, rp, tT, target, stack + threadIdx.x, kBlock, ovf,
+                                                    countStats ? &nodes : nullptr, countStats ? &tris : nullptr) < 0;
+        h.t = tT, h.u = uT, h.v = vT, h.tri = target;
+      }
+    } else if (!s.hasSurfaceLight) {
+      // no emissive triangle: visible <=> nothing is hit at all
+      visible = findBlocker<kStackLds, kStackOvf>(s, rp, __builtin_inff(), 0x7FFFFFFF, stack + threadIdx.x, kBlock,
+                                                  ovf, countStats ? &nodes : nullptr,
+                                                  countStats ? &tris : nullptr) < 0;
+      h.t = __builtin_inff(), h.u = h.v = 0.0f, h.tri = -1;
+    } else {
+      h = traceClosest<kStackLds, kStackOvf>(s, so, sd, stack + threadIdx.x, kBlock, ovf,
+                                             countStats ? &nodes : nullptr, countStats ? &tris : nullptr);
+      bool hitLight = h.tri >= 0 && s.mats[s.triMat[h.tri]].isLight;
+      visible = h.tri < 0 || hitLight;
+    }
     if (!visible) continue;
     V3 hx = v3(0.0f, 0.0f, 0.0f), hn = v3(0.0f, 0.0f, 0.0f);
     if (h.tri >= 0) hitPointNormal(s, h.tri, h.u, h.v, &hx, &hn);

@@ -138,7 +138,7 @@ def _box(center, size, angle_deg):
     return v, tris
 
 
-def write_cornell(folder, width=256, height=256, spp=64, max_depth=8):
+def write_cornell(folder, width=256, height=256, spp=64, max_depth=8, sky=False):
     """Procedural Cornell box: 5 walls (10 triangles), two boxes rotated
     +-17 degrees (12 triangles each) and a 0.25 x 0.25 quad light of radiance
     (17, 12, 4) just under the ceiling; 36 triangles, black background,
@@ -181,9 +181,14 @@ def write_cornell(folder, width=256, height=256, spp=64, max_depth=8):
     obj.write(os.path.join(folder, "scene.obj"), "scene.mtl")
     with open(os.path.join(folder, "scene.mtl"), "w") as f:
         f.write("\n".join(mtl))
+    world = {"background_color": [0.0, 0.0, 0.0]}
+    if sky:  # variant: the open front also sees a small sky map (env + surface-light NEE)
+        img, _ = sky_texture(64, 32, sun_radiance=50.0, sun_diameter_deg=8.0, sun_elev_deg=10.0, sun_azim_deg=90.0)
+        write_pfm(os.path.join(folder, "sky.pfm"), img)
+        world = {"environment_texture": "sky.pfm"}
     scene = {
         "camera": {"position": [0.0, 1.0, 3.9], "look_at": [0.0, 1.0, 0.0], "fov": 40.0},
-        "world": {"background_color": [0.0, 0.0, 0.0]},
+        "world": world,
         "mesh": "scene.obj",
     }
     with open(os.path.join(folder, "scene.json"), "w") as f:
@@ -321,5 +326,95 @@ def write_ggx_grid(folder, width=1024, height=1024, spp=256, max_depth=16, seed=
     with open(os.path.join(folder, "scene.json"), "w") as f:
         json.dump(scene, f, indent=2)
     _write_job(folder, {"width": width, "height": height, "num_samples": spp, "max_depth": max_depth,
+                        "use_mis": True, "num_threads": 1})
+    return os.path.join(folder, "renderJob.json")
+
+
+def torus(major_r, minor_r, n_major, n_minor):
+    """Torus around +y: returns (vertices, normals, triangles)."""
+    u = np.arange(n_major) / n_major * 2.0 * math.pi
+    v = np.arange(n_minor) / n_minor * 2.0 * math.pi
+    uu, vv = np.meshgrid(u, v, indexing="ij")
+    cx, cz = np.cos(uu), np.sin(uu)
+    nrm = np.stack([np.cos(vv) * cx, np.sin(vv), np.cos(vv) * cz], axis=-1)
+    pos = np.stack([major_r * cx, np.zeros_like(cx), major_r * cz], axis=-1) + minor_r * nrm
+    idx = np.arange(n_major * n_minor).reshape(n_major, n_minor)
+    a = idx
+    b = np.roll(idx, -1, axis=0)
+    c = np.roll(np.roll(idx, -1, axis=0), -1, axis=1)
+    d = np.roll(idx, -1, axis=1)
+    tris = np.concatenate([np.stack([a, d, c], -1).reshape(-1, 3), np.stack([a, c, b], -1).reshape(-1, 3)])
+    return pos.reshape(-1, 3), nrm.reshape(-1, 3), tris
+
+
+def write_caustic_scene(folder, width=1920, height=1080, passes=100, n_major=2000, n_minor=500, max_depth=16,
+                        light_radiance=40.0):
+    """Caustic box: a triangulated glass prism (IOR 1.5, RGB dispersion via
+    world.vary_glass_ior) and a GGX metal torus ring (R = 1, r = 0.08,
+    alpha = 0.05; 2 * n_major * n_minor triangles, 2M at the defaults) inside
+    a closed diffuse box lit by a square light covering 1% of the ceiling."""
+    os.makedirs(folder, exist_ok=True)
+    obj = _Obj()
+    alpha = 0.05
+    mtl = [
+        _mtl("wall", kd=(0.7, 0.7, 0.7)),
+        _mtl("floor", kd=(0.6, 0.55, 0.5)),
+        _mtl("glass", kd=(1.0, 1.0, 1.0), ns=1000.0, ni=1.5, d=0.0),
+        _mtl("gold", kd=(1.0, 0.78, 0.34), ka=(1, 1, 1), ns=1000.0 * (1.0 - math.sqrt(alpha)) ** 2),
+        _mtl("light", ke=tuple(math.pi * c for c in (light_radiance, light_radiance * 0.92, light_radiance * 0.8))),
+    ]
+    X, Y, Z = 2.0, 3.0, 2.0
+    quad = [(0, 1, 2), (0, 2, 3)]
+    obj.shape("box")
+    obj.smooth(False)
+    walls = [
+        ([[-X, 0, -Z], [X, 0, -Z], [X, 0, Z], [-X, 0, Z]], "floor"),
+        ([[-X, Y, -Z], [-X, Y, Z], [X, Y, Z], [X, Y, -Z]], "wall"),
+        ([[-X, 0, -Z], [-X, Y, -Z], [X, Y, -Z], [X, 0, -Z]], "wall"),
+        ([[-X, 0, Z], [X, 0, Z], [X, Y, Z], [-X, Y, Z]], "wall"),
+        ([[-X, 0, -Z], [-X, 0, Z], [-X, Y, Z], [-X, Y, -Z]], "wall"),
+        ([[X, 0, -Z], [X, Y, -Z], [X, Y, Z], [X, 0, Z]], "wall"),
+    ]
+    for v, m in walls:
+        obj.material(m)
+        obj.faces(quad, obj.vertices(v))
+    obj.shape("prism")
+    obj.smooth(False)
+    obj.material("glass")
+    h, s = 0.7, 0.45
+    cx, cz = -0.9, -0.3
+    tri = [(cx - s * 0.5, cz - s * 0.29), (cx + s * 0.5, cz - s * 0.29), (cx, cz + s * 0.58)]
+    v = [[x, 0.001, z] for x, z in tri] + [[x, 0.001 + h, z] for x, z in tri]
+    base = obj.vertices(v)
+    # outward geometric normals (v1-v0) x (v2-v0): bottom -y, top +y, sides out
+    obj.faces([(0, 1, 2)], base)
+    obj.faces([(3, 5, 4)], base)
+    for i in range(3):
+        j = (i + 1) % 3
+        obj.faces([(i, j + 3, j), (i, i + 3, j + 3)], base)
+    obj.shape("torus")
+    obj.smooth(True)
+    obj.material("gold")
+    pos, nrm, tris = torus(1.0, 0.08, n_major, n_minor)
+    pos = pos + np.array([0.6, 0.08, 0.1])
+    vb = obj.vertices(pos)
+    nb = obj.normals(nrm)
+    obj.faces(tris, vb, nb)
+    obj.shape("light")
+    obj.smooth(False)
+    obj.material("light")
+    ls = 0.5 * math.sqrt(0.01 * (2 * X) * (2 * Z))
+    obj.faces(quad, obj.vertices([[-ls, Y - 0.001, -ls], [ls, Y - 0.001, -ls], [ls, Y - 0.001, ls], [-ls, Y - 0.001, ls]]))
+    obj.write(os.path.join(folder, "scene.obj"), "scene.mtl")
+    with open(os.path.join(folder, "scene.mtl"), "w") as f:
+        f.write("\n".join(mtl))
+    scene = {
+        "camera": {"position": [0.0, 1.7, 1.95], "look_at": [0.0, 0.2, -0.2]},
+        "world": {"background_color": [0.0, 0.0, 0.0], "vary_glass_ior": True},
+        "mesh": "scene.obj",
+    }
+    with open(os.path.join(folder, "scene.json"), "w") as f:
+        json.dump(scene, f, indent=2)
+    _write_job(folder, {"width": width, "height": height, "num_samples": passes, "max_depth": max_depth,
                         "use_mis": True, "num_threads": 1})
     return os.path.join(folder, "renderJob.json")

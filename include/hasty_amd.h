@@ -91,18 +91,19 @@ int hasty_render_pt_device(HastyScene* scene, uint32_t seed, uint32_t first_samp
 int hasty_trace_closest(HastyScene* scene, const float* rays, uint32_t n, uint32_t* geom_prim, int32_t* tri,
                         float* tuv);
 
-/* Per-kernel timing with HIP events on the scene's stream. */
+/* Per-kernel timing with HIP events on the scene's stream (flags bit 0) and
+ * device counters (flags bit 1, see hasty_counters). */
 typedef struct HastyKernelStat {
   char name[32];
   uint64_t launches;
   double total_ms;
 } HastyKernelStat;
-int hasty_set_profiling(HastyScene* scene, int enable);
+int hasty_set_profiling(HastyScene* scene, int flags);
 /* Writes up to max_stats entries, returns the number written (negative on error). */
 int hasty_kernel_stats(HastyScene* scene, HastyKernelStat* stats, int max_stats);
 int hasty_reset_stats(HastyScene* scene);
 
-/* Device counters of the last renders (enabled together with profiling):
+/* Device counters accumulated since the last reset (profiling flag bit 1):
  * out[0] extension rays, out[1] shadow rays, out[2] BVH node visits,
  * out[3] triangle tests, out[4] paths whose media stack overflowed. */
 int hasty_counters(HastyScene* scene, uint64_t* out5);

@@ -8,6 +8,7 @@
 #include <string>
 
 #include "oracle_pt.h"
+#include "oracle_sppm.h"
 
 using namespace oracle;
 
@@ -106,6 +107,26 @@ int oracle_trace(void* jp, const float* rays, uint32_t n, int mode, int32_t* tri
     tuv[i * 3 + 2] = h.v;
     edge[i] = h.edgeTie ? 1 : 0;
   }
+  return 0;
+}
+// Material table entry: albedo rgb, roughness, metallic, specular, ior,
+// anisotropy, transmission, emission rgb, varyIOR.
+int oracle_material(void* jp, int idx, float* out) {
+  PTJob* job = (PTJob*)jp;
+  if (idx < 0 || idx >= (int)job->scene.data.materials.size()) return 1;
+  const hasty::Material& m = job->scene.data.materials[idx];
+  float v[14] = {m.albedo[0], m.albedo[1], m.albedo[2], m.roughness, m.metallic, m.specular, m.ior,
+                 m.anisotropy, m.transmission, m.emission[0], m.emission[1], m.emission[2], m.varyIOR ? 1.0f : 0.0f,
+                 0.0f};
+  memcpy(out, v, sizeof(v));
+  return 0;
+}
+// Light table: global triangle index and area of light i.
+int oracle_light(void* jp, int i, int32_t* tri, float* area) {
+  PTJob* job = (PTJob*)jp;
+  if (i < 0 || i >= (int)job->scene.data.lightTris.size()) return 1;
+  *tri = (int32_t)job->scene.data.lightTris[i];
+  *area = job->scene.data.lightAreas[i];
   return 0;
 }
 // Geometry ids of a global triangle index.
@@ -296,6 +317,111 @@ void oracle_spherical(const float* dir, float* out) {
   ToDirectionResult r = sphericalToDirection(c);
   out[0] = c[0], out[1] = c[1], out[2] = r.direction[0], out[3] = r.direction[1], out[4] = r.direction[2];
 }
+// ---- photon mapper ------------------------------------------------------
+int oracle_sppm_create(const char* path, uint32_t seed, const float* params, void** out) {
+  try {
+    auto job = std::make_unique<SppmJob>();
+    hasty::Job j;
+    hasty::loadJob(path, j);
+    job->pt.scene.data = std::move(j.scene);
+    job->pt.scene.init();
+    job->pt.settings.rouletteStartDepth = j.settings.rouletteStartDepth;
+    job->pt.settings.rouletteQ = j.settings.rouletteQ;
+    job->pt.settings.width = j.settings.width;
+    job->pt.settings.height = j.settings.height;
+    job->pt.settings.maxDepth = j.settings.maxDepth;
+    job->pt.settings.numSamples = j.settings.numSamples;
+    job->pt.settings.useMIS = j.settings.useMIS;
+    job->pt.seed = seed;
+    if (params) {  // alpha, footprint scale, store fraction, emit fraction (<= 0 keeps the default)
+      if (params[0] > 0) job->params.alpha = params[0];
+      if (params[1] > 0) job->params.footprintScale = params[1];
+      if (params[2] > 0) job->params.storeFraction = params[2];
+      if (params[3] > 0) job->params.emitFraction = params[3];
+    }
+    *out = job.release();
+    return 0;
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    return 2;
+  }
+}
+void oracle_sppm_free(void* h) { delete (SppmJob*)h; }
+int oracle_sppm_set_size(void* h, int width, int height) {
+  SppmJob* job = (SppmJob*)h;
+  if (job->initialised) return 1;
+  job->pt.settings.width = (uint32_t)width;
+  job->pt.settings.height = (uint32_t)height;
+  return 0;
+}
+int oracle_sppm_pass(void* h, uint32_t pass, float* color, float* normal, float* albedo) {
+  try {
+    sppmPass(*(SppmJob*)h, pass, color, normal, albedo);
+    return 0;
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    return 3;
+  }
+}
+// out: emitted, stored photons of the last pass; cell size; passes done
+void oracle_sppm_info(void* h, double* out) {
+  SppmJob* job = (SppmJob*)h;
+  out[0] = (double)job->lastEmitted;
+  out[1] = (double)job->lastStored;
+  out[2] = job->lastCellSize;
+  out[3] = job->passesDone;
+}
+// per pixel: radius, N, Ne, flux rgb, LSum rgb, Nd (10 floats)
+void oracle_sppm_pixels(void* h, float* out) {
+  SppmJob* job = (SppmJob*)h;
+  for (size_t p = 0; p < job->pixels.size(); ++p) {
+    const PixelStat& px = job->pixels[p];
+    float* o = out + p * 10;
+    o[0] = px.radius, o[1] = px.N, o[2] = px.Ne;
+    o[3] = px.flux[0], o[4] = px.flux[1], o[5] = px.flux[2];
+    o[6] = px.LSum[0], o[7] = px.LSum[1], o[8] = px.LSum[2];
+    o[9] = (float)px.Nd;
+  }
+}
+// initial radii (computeInitialRadii) of the job's film
+int oracle_sppm_initial_radii(void* h, float* out) {
+  SppmJob* job = (SppmJob*)h;
+  std::vector<float> r = computeInitialRadii(job->pt, job->params.footprintScale);
+  memcpy(out, r.data(), sizeof(float) * r.size());
+  return 0;
+}
+float oracle_cell_size(const float* radii, uint32_t W, uint32_t H) {
+  std::vector<float> r(radii, radii + (size_t)W * H);
+  return computeCellSize(r, W, H);
+}
+// Photons stored by emission indices [first, first+count) of pass `pass`, in
+// emission order: rows of x xyz, wi xyz, ngwi, flux rgb, emitter index.
+// Returns the number of photons written (at most maxOut).
+int64_t oracle_trace_photons(void* h, uint32_t pass, uint32_t first, uint32_t count, float* out, int64_t maxOut) {
+  SppmJob* job = (SppmJob*)h;
+  PhotonMap map;
+  map.clear(1.0f);
+  int64_t n = 0;
+  for (uint32_t i = first; i < first + count; ++i) {
+    size_t before = map.photons.size();
+    RNG rng = hrng::make_stream(job->pt.seed, i, pass, hrng::kDomainPhoton);
+    RenderContext ctx{job->pt.scene, rng, job->pt.settings};
+    Vec3f flux;
+    Ray ray = job->pt.scene.sampleLightRay(rng, flux);
+    tracePhoton(ctx, map, ray, flux);
+    for (size_t k = before; k < map.photons.size() && n < maxOut; ++k, ++n) {
+      const Photon& p = map.photons[k];
+      float* o = out + n * 11;
+      o[0] = p.x[0], o[1] = p.x[1], o[2] = p.x[2];
+      o[3] = p.wi[0], o[4] = p.wi[1], o[5] = p.wi[2];
+      o[6] = p.ngwi;
+      o[7] = p.flux[0], o[8] = p.flux[1], o[9] = p.flux[2];
+      o[10] = (float)i;
+    }
+  }
+  return n;
+}
+
 void oracle_hm_vec(int fn, const float* a, const float* b, uint32_t n, float* out) {
   for (uint32_t i = 0; i < n; ++i) {
     float x = a[i], y = b[i], r;

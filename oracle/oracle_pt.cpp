@@ -846,23 +846,14 @@ Vec3f beersLaw(const Vec3f& transmittance, float d) {
 }
 
 // MIS.h:10-21
-static float msiWeightPowerHeuristic2(int index, const float* pdfs, int n) {
+float msiWeightPowerHeuristic2(int index, const float* pdfs, int n) {
   float s = 0.0f;
   for (int i = 0; i < n; ++i) s += powci<2>(pdfs[i]);
   return powci<2>(pdfs[index]) / s;
 }
 
-struct MISSampleResult {  // MIS.h:24-38
-  Ray ray;
-  LightRayInfo lightRay;
-  RayHit rayhit;
-  Vec3f throughputDiffuse, throughputConcentrated;
-  float pdfOmega = 0.0f;
-  Vec3f throughput() const { return throughputDiffuse + throughputConcentrated; }
-};
-
 // MIS.cpp:6-30
-static MISSampleResult brdfStrategySample(RenderContext ctx, const LightRayInfo& lightRay, const RayHit& hit) {
+MISSampleResult brdfStrategySample(RenderContext ctx, const LightRayInfo& lightRay, const RayHit& hit) {
   MISSampleResult r;
   r.lightRay = lightRay;
   SampleResult s = ctx.scene.sampleBRDF(ctx.rng, r.lightRay, hit, false, kAll);
@@ -880,7 +871,7 @@ static MISSampleResult brdfStrategySample(RenderContext ctx, const LightRayInfo&
   return r;
 }
 // MIS.cpp:32-54
-static MISSampleResult lightStrategySample(RenderContext ctx, const LightRayInfo& lightRay, const Ray& ray,
+MISSampleResult lightStrategySample(RenderContext ctx, const LightRayInfo& lightRay, const Ray& ray,
                                            const RayHit& hit) {
   MISSampleResult r;
   r.lightRay = lightRay;

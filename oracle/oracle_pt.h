@@ -203,6 +203,19 @@ struct RenderContext {
   const RenderSettings& settings;
 };
 
+struct MISSampleResult {  // MIS.h:24-38
+  Ray ray;
+  LightRayInfo lightRay;
+  RayHit rayhit;
+  Vec3f throughputDiffuse, throughputConcentrated;
+  float pdfOmega = 0.0f;
+  Vec3f throughput() const { return throughputDiffuse + throughputConcentrated; }
+};
+MISSampleResult brdfStrategySample(RenderContext ctx, const LightRayInfo& lightRay, const RayHit& hit);
+MISSampleResult lightStrategySample(RenderContext ctx, const LightRayInfo& lightRay, const Ray& ray,
+                                    const RayHit& hit);
+float msiWeightPowerHeuristic2(int index, const float* pdfs, int n);
+
 // PathTracing.cpp:21-95
 Vec3f estimateRadiance(RenderContext ctx, LightRayInfo& lightRay, const Ray& ray, Vec3f& normal, Vec3f& albedo,
                        int depth, RayHit* rayhitPtr = nullptr);

@@ -35,6 +35,8 @@ def lib():
         L.oracle_render_pt_counted.argtypes = [P, C.c_uint32, C.c_uint32, fp, fp, fp, u64]
         L.oracle_trace.argtypes = [P, fp, C.c_uint32, C.c_int, ip, fp, ip]
         L.oracle_tri_ids.argtypes = [P, C.c_int32, up]
+        L.oracle_material.argtypes = [P, C.c_int, fp]
+        L.oracle_light.argtypes = [P, C.c_int, C.POINTER(C.c_int32), C.POINTER(C.c_float)]
         L.oracle_camera_rays.argtypes = [P, C.c_uint32, fp]
         L.oracle_halton.argtypes = [C.c_uint32, fp]
         L.oracle_philox.argtypes = [up, up, up]
@@ -56,6 +58,17 @@ def lib():
         L.oracle_hm.argtypes = [C.c_int, C.c_float, C.c_float]
         L.oracle_hm.restype = C.c_float
         L.oracle_hm_vec.argtypes = [C.c_int, fp, fp, C.c_uint32, fp]
+        L.oracle_sppm_create.argtypes = [C.c_char_p, C.c_uint32, fp, C.POINTER(P)]
+        L.oracle_sppm_free.argtypes = [P]
+        L.oracle_sppm_set_size.argtypes = [P, C.c_int, C.c_int]
+        L.oracle_sppm_pass.argtypes = [P, C.c_uint32, fp, fp, fp]
+        L.oracle_sppm_info.argtypes = [P, np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")]
+        L.oracle_sppm_pixels.argtypes = [P, fp]
+        L.oracle_sppm_initial_radii.argtypes = [P, fp]
+        L.oracle_cell_size.argtypes = [fp, C.c_uint32, C.c_uint32]
+        L.oracle_cell_size.restype = C.c_float
+        L.oracle_trace_photons.argtypes = [P, C.c_uint32, C.c_uint32, C.c_uint32, fp, C.c_int64]
+        L.oracle_trace_photons.restype = C.c_int64
         _LIB = L
     return _LIB
 
@@ -125,6 +138,21 @@ class OracleJob:
         lib().oracle_tri_ids(self.h, int(tri), out)
         return int(out[0]), int(out[1])
 
+    def material(self, idx):
+        out = np.zeros(14, np.float32)
+        if lib().oracle_material(self.h, idx, out) != 0:
+            raise IndexError(idx)
+        keys = ("albedo", "roughness", "metallic", "specular", "ior", "anisotropy", "transmission", "emission",
+                "vary_ior")
+        return {"albedo": out[0:3], "roughness": out[3], "metallic": out[4], "specular": out[5], "ior": out[6],
+                "anisotropy": out[7], "transmission": out[8], "emission": out[9:12], "vary_ior": bool(out[12])}
+
+    def light(self, i):
+        tri, area = C.c_int32(), C.c_float()
+        if lib().oracle_light(self.h, i, C.byref(tri), C.byref(area)) != 0:
+            raise IndexError(i)
+        return tri.value, area.value
+
     def camera_rays(self, sample: int):
         info = self.info()
         out = np.zeros((info["height"], info["width"], 6), np.float32)
@@ -190,3 +218,56 @@ def hm_vec(fn, a, b):
 
 def hm(fn, a, b=0.0):
     return float(lib().oracle_hm(fn, float(a), float(b)))
+
+
+class OracleSppm:
+    """The photon mapper's oracle: one job, passes rendered in order."""
+
+    def __init__(self, job_path: str, seed: int = 0, alpha=0.0, footprint_scale=0.0, store_fraction=0.0,
+                 emit_fraction=0.0, width=0, height=0):
+        h = C.c_void_p()
+        params = f32([alpha, footprint_scale, store_fraction, emit_fraction])
+        if lib().oracle_sppm_create(job_path.encode(), seed, params, C.byref(h)) != 0:
+            raise RuntimeError(lib().oracle_last_error().decode())
+        self.h = h
+        if width and height:
+            lib().oracle_sppm_set_size(self.h, width, height)
+        self.width = width
+        self.height = height
+
+    def __del__(self):
+        if getattr(self, "h", None) is not None and _LIB is not None:
+            _LIB.oracle_sppm_free(self.h)
+            self.h = None
+
+    def run_pass(self, k, w, h):
+        films = tuple(np.zeros((h, w, 3), np.float32) for _ in range(3))
+        if lib().oracle_sppm_pass(self.h, k, *films) != 0:
+            raise RuntimeError(lib().oracle_last_error().decode())
+        return films
+
+    def info(self):
+        out = np.zeros(4, np.float64)
+        lib().oracle_sppm_info(self.h, out)
+        return {"emitted": int(out[0]), "stored": int(out[1]), "cell_size": float(out[2]), "passes": int(out[3])}
+
+    def pixels(self, w, h):
+        out = np.zeros((h * w, 10), np.float32)
+        lib().oracle_sppm_pixels(self.h, out)
+        return out
+
+    def initial_radii(self, w, h):
+        out = np.zeros((h, w), np.float32)
+        lib().oracle_sppm_initial_radii(self.h, out)
+        return out
+
+    def trace_photons(self, k, first, count, max_out=1 << 22):
+        out = np.zeros((max_out, 11), np.float32)
+        n = lib().oracle_trace_photons(self.h, k, first, count, out, max_out)
+        return out[:n]
+
+
+def cell_size(radii):
+    radii = f32(radii)
+    h, w = radii.shape
+    return float(lib().oracle_cell_size(radii, w, h))

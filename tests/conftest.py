@@ -44,6 +44,26 @@ def ggx_small_job(scene_dir):
                                  env_size=(256, 128))
 
 
+@pytest.fixture(scope="session")
+def cornell_sky_job(scene_dir):
+    """Cornell box whose open front sees a sky map: NEE picks surface light or
+    environment with probability 1/2 each, environment visibility needs the
+    closest hit (a light may be in front)."""
+    from hastyrenderer_amd import scenes
+
+    return scenes.write_cornell(str(scene_dir / "cornell_sky"), width=128, height=128, spp=16, sky=True)
+
+
+@pytest.fixture(scope="session")
+def caustic_small_job(scene_dir):
+    """Caustic box with a 4,608-triangle torus: dispersive glass prism (media
+    stack, hero wavelength, refraction), GGX gold ring, 1% ceiling light."""
+    from hastyrenderer_amd import scenes
+
+    return scenes.write_caustic_scene(str(scene_dir / "caustic_small"), width=192, height=108, passes=4,
+                                      n_major=96, n_minor=24)
+
+
 _TK = None
 
 

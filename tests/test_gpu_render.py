@@ -107,6 +107,28 @@ def test_full_ggx_grid_parity_sampled(ggx_job):
     assert frac >= 0.999
 
 
+def test_cornell_sky_parity(cornell_sky_job):
+    scene = ha.Scene(cornell_sky_job)
+    c, n, a = scene.render_pt(seed=0, first_sample=0, num_samples=16)
+    oj = po.OracleJob(cornell_sky_job, seed=0)
+    (rc, rn, ra), nan = oj.render(0, 16, threads=16)
+    frac = rel_ok(c, rc).mean()
+    print("cornell sky: pixels within 1e-3: %.6f" % frac)
+    assert frac >= 0.999
+
+
+def test_caustic_scene_parity(caustic_small_job):
+    """Glass with dispersion and media tracking, metal torus, small area light."""
+    scene = ha.Scene(caustic_small_job)
+    c, n, a = scene.render_pt(seed=3, first_sample=0, num_samples=16)
+    oj = po.OracleJob(caustic_small_job, seed=3)
+    (rc, rn, ra), nan = oj.render(0, 16, threads=16)
+    frac = rel_ok(c, rc).mean()
+    print("caustic: pixels within 1e-3: %.6f" % frac)
+    assert frac >= 0.999
+    assert scene.info()["num_triangles"] == oj.info()["num_tris"]
+
+
 def test_deterministic_across_launch_shapes(cornell_job):
     scene = ha.Scene(cornell_job)
     scene.set_settings(width=128, height=96)
