@@ -41,6 +41,14 @@ EXPORTS = (
     "hasty_reset_stats",
     "hasty_counters",
     "hasty_set_launch",
+    "hasty_sppm_create",
+    "hasty_sppm_destroy",
+    "hasty_sppm_pass_device",
+    "hasty_sppm_pass",
+    "hasty_sppm_info",
+    "hasty_sppm_pixels",
+    "hasty_sppm_radii",
+    "hasty_sppm_photons",
     "hasty_last_error",
 )
 
@@ -76,6 +84,26 @@ class SceneInfoC(C.Structure):
     ]
 
 
+class SppmParamsC(C.Structure):
+    _fields_ = [
+        ("alpha", C.c_float),
+        ("footprint_scale", C.c_float),
+        ("store_fraction", C.c_double),
+        ("emit_fraction", C.c_double),
+    ]
+
+
+class SppmInfoC(C.Structure):
+    _fields_ = [
+        ("emitted", C.c_uint64),
+        ("stored", C.c_uint64),
+        ("cell_size", C.c_float),
+        ("passes_done", C.c_uint32),
+        ("emit_limit", C.c_uint32),
+        ("store_limit", C.c_uint32),
+    ]
+
+
 class KernelStatC(C.Structure):
     _fields_ = [("name", C.c_char * 32), ("launches", C.c_uint64), ("total_ms", C.c_double)]
 
@@ -103,6 +131,15 @@ def lib():
     L.hasty_reset_stats.argtypes = [P]
     L.hasty_counters.argtypes = [P, C.POINTER(C.c_uint64)]
     L.hasty_set_launch.argtypes = [P, C.c_uint32, C.c_uint32, C.c_uint32]
+    L.hasty_sppm_create.argtypes = [P, C.c_uint32, C.POINTER(SppmParamsC), C.POINTER(P)]
+    L.hasty_sppm_destroy.argtypes = [P]
+    L.hasty_sppm_destroy.restype = None
+    L.hasty_sppm_pass_device.argtypes = [P, C.c_uint32, P, P, P]
+    L.hasty_sppm_pass.argtypes = [P, C.c_uint32, fp, fp, fp]
+    L.hasty_sppm_info.argtypes = [P, C.POINTER(SppmInfoC)]
+    L.hasty_sppm_pixels.argtypes = [P, fp]
+    L.hasty_sppm_radii.argtypes = [P, fp]
+    L.hasty_sppm_photons.argtypes = [P, fp, C.c_uint64]
     L.hasty_last_error.restype = C.c_char_p
     _lib = L
     return L
@@ -216,6 +253,69 @@ class Scene:
         _err(lib().hasty_counters(self._h, out))
         keys = ("extension_rays", "shadow_rays", "node_visits", "triangle_tests", "media_overflow")
         return dict(zip(keys, (int(x) for x in out)))
+
+
+class Sppm:
+    """One progressive photon-mapping job on the GPU (HastySppm*): per-pixel
+    statistics and the radius image stay on the device between passes."""
+
+    def __init__(self, scene: Scene, seed: int, alpha=0.4, footprint_scale=10.0, store_fraction=0.5,
+                 emit_fraction=10.0):
+        self.scene = scene  # keeps the scene alive
+        p = SppmParamsC(alpha, footprint_scale, store_fraction, emit_fraction)
+        h = C.c_void_p()
+        _err(lib().hasty_sppm_create(scene.handle, seed, C.byref(p), C.byref(h)))
+        self._h = h
+        info = scene.info()
+        self.width, self.height = info["width"], info["height"]
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and _lib is not None:
+            _lib.hasty_sppm_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def pass_(self, index: int, films=None):
+        """One pass; returns (color, normal, albedo) holding the current estimate."""
+        h, w = self.height, self.width
+        if films is None:
+            films = tuple(np.zeros((h, w, 3), np.float32) for _ in range(3))
+        for f in films:
+            assert f.shape == (h, w, 3)
+        _err(lib().hasty_sppm_pass(self._h, index, *(_fptr(f) for f in films)))
+        return films
+
+    def pass_device(self, index: int, d_color, d_normal, d_albedo):
+        _err(lib().hasty_sppm_pass_device(self._h, index, C.c_void_p(d_color), C.c_void_p(d_normal),
+                                          C.c_void_p(d_albedo)))
+
+    def info(self) -> dict:
+        i = SppmInfoC()
+        _err(lib().hasty_sppm_info(self._h, C.byref(i)))
+        return {"emitted": i.emitted, "stored": i.stored, "cell_size": i.cell_size, "passes_done": i.passes_done,
+                "emit_limit": i.emit_limit, "store_limit": i.store_limit}
+
+    def pixels(self) -> dict:
+        """PMRenderJobPixel planes: radius, N, Ne, flux (h, w, 3), LSum (h, w, 3), Nd (int32)."""
+        h, w = self.height, self.width
+        a = np.zeros((10, h, w), np.float32)
+        _err(lib().hasty_sppm_pixels(self._h, _fptr(a)))
+        return {"radius": a[0], "N": a[1], "Ne": a[2], "flux": np.stack(a[3:6], -1), "LSum": np.stack(a[6:9], -1),
+                "Nd": a[9].view(np.int32)}
+
+    def radii(self) -> np.ndarray:
+        a = np.zeros((self.height, self.width), np.float32)
+        _err(lib().hasty_sppm_radii(self._h, _fptr(a)))
+        return a
+
+    def photons(self) -> np.ndarray:
+        """(stored, 10): position, dot(n_g, w_i), w_i, flux in emission order."""
+        n = self.info()["stored"]
+        a = np.zeros((max(n, 1), 10), np.float32)
+        _err(lib().hasty_sppm_photons(self._h, _fptr(a), n))
+        return a[:n]
 
 
 # ------------------------------------------------------------------------------
@@ -332,6 +432,37 @@ def renderThread(colorBuffer, normalBuffer, albedoBuffer, job: RenderJob, thread
         for b in (colorBuffer, normalBuffer, albedoBuffer):
             b.getWriteBuffer().numSamples += count
             b.copyBuffer()
+    threadData.stoppedFlag = True
+
+
+class PMRenderJob(RenderJob):
+    """PMRenderJob (PhotonMapping.h:33-121): a RenderJob whose per-pixel
+    progressive statistics live on the GPU (created by the first
+    renderPhotonThread)."""
+
+    def __init__(self, alpha: float = 0.4):
+        super().__init__()
+        self.alpha = alpha
+        self.sppm = None
+
+
+def renderPhotonThread(colorBuffer, normalBuffer, albedoBuffer, job: PMRenderJob, threadData: RenderThreadData):
+    """PhotonMapping.cpp:814-933: one photon-map pass per fetched sample index
+    until stopFlag or numSamples; each pass publishes the current estimate
+    with numSamples = 1."""
+    st = job.renderSettings
+    if job.sppm is None:
+        job.sppm = Sppm(job.scene, threadData.seed, alpha=job.alpha)
+    for b in (colorBuffer, normalBuffer, albedoBuffer):
+        b.getWriteBuffer().data[...] = 0.0
+    sample = job.fetchSample()
+    while not job.stopFlag and sample.idx < st.numSamples:
+        job.sppm.pass_(sample.idx, films=(colorBuffer.getWriteBuffer().data, normalBuffer.getWriteBuffer().data,
+                                          albedoBuffer.getWriteBuffer().data))
+        for b in (colorBuffer, normalBuffer, albedoBuffer):
+            b.getWriteBuffer().numSamples = 1
+            b.copyBuffer()
+        sample = job.fetchSample()
     threadData.stoppedFlag = True
 
 

@@ -16,6 +16,7 @@
 #include "pt_kernels.h"
 #include "rng.h"
 #include "scene_io.h"
+#include "sppm.h"
 
 namespace {
 
@@ -43,6 +44,19 @@ struct Halton {
     return (float)n / (float)d;
   }
 };
+
+// Offset of sample / pass index k: the (k+1)-th Halton(2, 3) point
+// (RenderJob.h:53-60).
+void haltonPoint(uint32_t k, float* x, float* y) {
+  Halton h2(2), h3(3);
+  float a = 0.0f, b = 0.0f;
+  for (uint32_t i = 0; i <= k; ++i) {
+    a = h2.next();
+    b = h3.next();
+  }
+  *x = a;
+  *y = b;
+}
 
 struct EventTimer : hasty::KernelTimer {
   struct Rec {
@@ -259,6 +273,16 @@ struct HastyScene {
       for (int i = 0; i < hasty::kNumStats; ++i) stats[i] += s[i];
     }
     if (profiling) timer->collect();
+  }
+};
+
+struct HastySppm {
+  HastyScene* scene = nullptr;
+  std::unique_ptr<hasty::SppmRenderer> r;
+  float* dFilm = nullptr;
+  ~HastySppm() {
+    r.reset();
+    if (dFilm) (void)hipFree(dFilm);
   }
 };
 
@@ -558,6 +582,133 @@ int hasty_set_launch(HastyScene* s, uint32_t spb, uint32_t traceBlocks, uint32_t
   if (traceBlocks) s->launch.traceBlocks = traceBlocks;
   if (shadeBlocks) s->launch.shadeBlocks = shadeBlocks;
   return 0;
+}
+
+int hasty_sppm_create(HastyScene* s, uint32_t seed, const HastySppmParams* params, HastySppm** out) {
+  if (!s || !out) {
+    g_lastError = "null argument";
+    return 1;
+  }
+  try {
+    HIPCHECK(hipSetDevice(s->device));
+    if (s->dev.width == 0 || s->dev.height == 0) throw std::runtime_error("empty film");
+    hasty::SppmParams p;
+    if (params) {
+      p.alpha = params->alpha;
+      p.footprintScale = params->footprint_scale;
+      p.storeFraction = params->store_fraction;
+      p.emitFraction = params->emit_fraction;
+    }
+    if (!(p.alpha > 0.0f && p.alpha <= 1.0f)) throw std::runtime_error("alpha must be in (0, 1]");
+    hdev::DevScene d = s->dev;
+    d.seed = seed;
+    auto h = std::make_unique<HastySppm>();
+    h->scene = s;
+    h->r = std::make_unique<hasty::SppmRenderer>(d, p, s->stream);
+    *out = h.release();
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e, 2);
+  }
+}
+
+void hasty_sppm_destroy(HastySppm* h) { delete h; }
+
+int hasty_sppm_pass_device(HastySppm* h, uint32_t pass, float* dC, float* dN, float* dA) {
+  if (!h || !dC || !dN || !dA) {
+    g_lastError = "null argument";
+    return 1;
+  }
+  try {
+    HastyScene* s = h->scene;
+    HIPCHECK(hipSetDevice(s->device));
+    float ox, oy;
+    haltonPoint(pass, &ox, &oy);
+    if (s->profiling && !s->timer) s->timer = std::make_unique<EventTimer>();
+    h->r->pass(pass, ox, oy, dC, dN, dA, s->profiling ? s->timer.get() : nullptr);
+    if (s->profiling) s->timer->collect();
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e, 3);
+  }
+}
+
+int hasty_sppm_pass(HastySppm* h, uint32_t pass, float* color, float* normal, float* albedo) {
+  if (!h || !color || !normal || !albedo) {
+    g_lastError = "null argument";
+    return 1;
+  }
+  try {
+    HastyScene* s = h->scene;
+    HIPCHECK(hipSetDevice(s->device));
+    size_t P = (size_t)s->dev.width * s->dev.height * 3;
+    if (!h->dFilm) HIPCHECK(hipMalloc(&h->dFilm, 3 * P * sizeof(float)));
+    int rc = hasty_sppm_pass_device(h, pass, h->dFilm, h->dFilm + P, h->dFilm + 2 * P);
+    if (rc != 0) return rc;
+    HIPCHECK(hipMemcpy(color, h->dFilm, P * sizeof(float), hipMemcpyDeviceToHost));
+    HIPCHECK(hipMemcpy(normal, h->dFilm + P, P * sizeof(float), hipMemcpyDeviceToHost));
+    HIPCHECK(hipMemcpy(albedo, h->dFilm + 2 * P, P * sizeof(float), hipMemcpyDeviceToHost));
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e, 3);
+  }
+}
+
+int hasty_sppm_info(const HastySppm* h, HastySppmInfo* info) {
+  if (!h || !info) {
+    g_lastError = "null argument";
+    return 1;
+  }
+  const hasty::SppmPassInfo& i = h->r->lastInfo();
+  info->emitted = i.emitted;
+  info->stored = i.stored;
+  info->cell_size = i.cellSize;
+  info->passes_done = h->r->passesDone();
+  info->emit_limit = h->r->emitLimit();
+  info->store_limit = h->r->storeLimit();
+  return 0;
+}
+
+int hasty_sppm_pixels(HastySppm* h, float* out) {
+  if (!h || !out) {
+    g_lastError = "null argument";
+    return 1;
+  }
+  try {
+    HIPCHECK(hipSetDevice(h->scene->device));
+    h->r->pixelStats(out);
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e, 3);
+  }
+}
+
+int hasty_sppm_radii(HastySppm* h, float* out) {
+  if (!h || !out) {
+    g_lastError = "null argument";
+    return 1;
+  }
+  try {
+    HIPCHECK(hipSetDevice(h->scene->device));
+    h->r->radii(out);
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e, 3);
+  }
+}
+
+int hasty_sppm_photons(HastySppm* h, float* out, uint64_t maxPhotons) {
+  if (!h || !out) {
+    g_lastError = "null argument";
+    return 1;
+  }
+  try {
+    HIPCHECK(hipSetDevice(h->scene->device));
+    h->r->photons(out, (size_t)maxPhotons);
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e, 3);
+  }
 }
 
 }  // extern "C"

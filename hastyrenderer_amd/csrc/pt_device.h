@@ -947,4 +947,18 @@ __device__ inline void cameraRay(const DevScene& s, hrng::Stream& rng, float px,
   *dOut = dir;
 }
 
+// Camera::computeRayAngle (Camera.cpp:83-98): angle subtended by one pixel.
+__device__ inline float cameraRayAngle(const DevScene& s, float px, float py) {
+  float W = (float)s.width, H = (float)s.height;
+  float fovRad = (float)kPi * s.fovDegree / 180.0f;
+  float q = hm::tanf(fovRad * 0.5f);
+  float lx1 = (px + 0.5f - 0.5f * W) / W * 2.0f;
+  float ly1 = (py + 0.5f - 0.5f * H) / W * 2.0f;
+  float lx2 = (px - 0.5f - 0.5f * W) / W * 2.0f;
+  float ly2 = (py - 0.5f - 0.5f * H) / W * 2.0f;
+  float ax = __builtin_fabsf(hm::atanf(q * lx1) - hm::atanf(q * lx2));
+  float ay = __builtin_fabsf(hm::atanf(q * ly1) - hm::atanf(q * ly2));
+  return 0.5f * ax + 0.5f * ay;
+}
+
 }  // namespace hdev

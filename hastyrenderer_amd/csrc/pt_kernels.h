@@ -50,7 +50,20 @@ struct LaunchConfig {
   bool countStats = false;
 };
 
-enum KernelId { kKGenerate = 0, kKExtend, kKShade, kKConnect, kKAccumulate, kKMisc, kNumKernels };
+enum KernelId {
+  kKGenerate = 0,
+  kKExtend,
+  kKShade,
+  kKConnect,
+  kKAccumulate,
+  kKMisc,
+  kKSppmRadii,
+  kKSppmCellSize,
+  kKSppmPhotons,
+  kKSppmBuild,
+  kKSppmCamera,
+  kNumKernels
+};
 const char* kernelName(int id);
 
 // Timing hook: called around each launch when profiling is on.

@@ -25,6 +25,11 @@ const char* kernelName(int id) {
     case kKShade: return "pt_shade";
     case kKConnect: return "pt_connect";
     case kKAccumulate: return "pt_accumulate";
+    case kKSppmRadii: return "sppm_init_radii";
+    case kKSppmCellSize: return "sppm_cell_size";
+    case kKSppmPhotons: return "sppm_photons";
+    case kKSppmBuild: return "sppm_build";
+    case kKSppmCamera: return "sppm_camera";
     default: return "pt_misc";
   }
 }

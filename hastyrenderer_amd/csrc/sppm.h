@@ -7,6 +7,7 @@
 #include <vector>
 
 #include "pt_device.h"
+#include "pt_kernels.h"
 
 namespace hasty {
 
@@ -15,70 +16,89 @@ struct SppmParams {
   float footprintScale = 10.0f;  // initial radius = 10 x pixel footprint (PhotonMapping.cpp:840)
   double storeFraction = 0.5;    // stop emitting once 0.5 W H photons are stored (PhotonMapping.cpp:834)
   double emitFraction = 10.0;    // ... or 10 W H photons were emitted (PhotonMapping.cpp:835)
-  uint32_t seed = 0;
 };
 
 struct SppmPassInfo {
-  uint64_t emitted = 0, stored = 0, cells = 0;
+  uint64_t emitted = 0, stored = 0;
   float cellSize = 0.0f;
-  double traceMs = 0, buildMs = 0, gatherMs = 0;
 };
 
+// Device state of one progressive photon-mapping job: per-pixel statistics
+// (PMRenderJobPixel), the radius image and the per-pass photon map.
 class SppmRenderer {
  public:
-  SppmRenderer(const hdev::DevScene& scene, const SppmParams& params, int bvhDepth, hipStream_t stream);
+  SppmRenderer(const hdev::DevScene& scene, const SppmParams& params, hipStream_t stream);
   ~SppmRenderer();
-  // One pass: photon map, camera paths with gathers, statistics update.
-  // Writes the current estimate (not a sum) and the depth-0 normal/albedo
-  // into the device films.
-  void pass(uint32_t passIndex, float* dColor, float* dNormal, float* dAlbedo);
-  void pixelStats(float* host10);  // radius, N, Ne, flux rgb, LSum rgb, Nd
+  SppmRenderer(const SppmRenderer&) = delete;
+  SppmRenderer& operator=(const SppmRenderer&) = delete;
+  // One pass with pixel offset (offX, offY): photon map, camera paths with
+  // gathers, statistics update.  Writes the current estimate (not a sum) and
+  // the depth-0 normal / albedo into the device films.
+  void pass(uint32_t passIndex, float offX, float offY, float* dColor, float* dNormal, float* dAlbedo,
+            KernelTimer* timer);
+  // radius, N, Ne, flux rgb, LSum rgb, Nd (int bits): 10 planes of W*H floats
+  void pixelStats(float* host10);
   void radii(float* host);
+  // photons of the last pass in emission order: x, ngwi, wi, flux (10 floats)
+  void photons(float* host, size_t maxPhotons);
   const SppmPassInfo& lastInfo() const { return info_; }
   uint32_t passesDone() const { return passes_; }
+  uint32_t emitLimit() const { return emitLimit_; }
+  uint32_t storeLimit() const { return storeLimit_; }
 
  private:
-  void init();
-  void ensurePhotonCapacity(size_t n);
+  void initRadii();
+  template <class T>
+  T* alloc(size_t n);
+
   hdev::DevScene s_;
   SppmParams p_;
   hipStream_t stream_;
-  uint32_t W_, H_;
+  uint32_t W_ = 0, H_ = 0;
   bool initialised_ = false;
   uint32_t passes_ = 0;
   SppmPassInfo info_;
   std::vector<void*> allocs_;
-  template <class T>
-  T* alloc(size_t n);
+  uint32_t traceBlocks_ = 0, partialBlocks_ = 0;
+  uint32_t emitLimit_ = 0, storeLimit_ = 0;
   // per pixel
+  float* dPixel_ = nullptr;  // 10 planes
   float* dRadii_ = nullptr;
-  float* dPixel_ = nullptr;  // 10 floats per pixel
-  // photon batches
-  uint32_t batch_ = 0;
-  float4* dSlots_ = nullptr;  // per-thread slots: batch x maxDepth x 3 float4
-  uint32_t* dSlotCount_ = nullptr;
-  uint32_t* dSlotOffset_ = nullptr;
+  // cell size reductions
+  float* dPartial_ = nullptr;
+  float* dCand_ = nullptr;
+  float* dState_ = nullptr;
+  float* dCellSize_ = nullptr;
+  int* dBox_ = nullptr;
+  uint32_t* dCounters_ = nullptr;
+  uint2* dOverflow_ = nullptr;
+  // photon batches: per-thread slots, batch x cap
+  uint32_t cap_ = 0, batchMax_ = 0;
+  float4* dSlX_ = nullptr;
+  float4* dSlWi_ = nullptr;
+  float4* dSlF_ = nullptr;
+  uint32_t* dSlCount_ = nullptr;
+  uint32_t* dSlOffset_ = nullptr;
   void* dScanTmp_ = nullptr;
   size_t scanTmpBytes_ = 0;
-  // photons of the pass (emission order), sorted copy by cell
+  // photons of the pass in emission order and in cell order
   size_t photonCap_ = 0;
-  float4* dPhotons_ = nullptr;        // 3 float4 per photon
-  float4* dSorted_ = nullptr;         // 3 float4 per photon, cell order
+  float4* dPhX_ = nullptr;
+  float4* dPhWi_ = nullptr;
+  float4* dPhF_ = nullptr;
+  float4* dSX_ = nullptr;
+  float4* dSWi_ = nullptr;
+  float4* dSF_ = nullptr;
   unsigned long long* dKeys_ = nullptr;
   unsigned long long* dKeysSorted_ = nullptr;
   uint32_t* dIdx_ = nullptr;
   uint32_t* dIdxSorted_ = nullptr;
   void* dSortTmp_ = nullptr;
   size_t sortTmpBytes_ = 0;
-  // hash table of cells
+  // cell table
   uint32_t tableSize_ = 0;
   unsigned long long* dTableKeys_ = nullptr;
   uint2* dTableVals_ = nullptr;
-  uint32_t* dCounters_ = nullptr;
-  float* dCellSize_ = nullptr;
-  uint2* dOverflow_ = nullptr;
-  uint32_t traceThreads_ = 0;
-  std::vector<void*> tmpAllocs_;
 };
 
 }  // namespace hasty

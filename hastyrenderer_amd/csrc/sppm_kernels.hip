@@ -1,0 +1,1008 @@
+// Stochastic progressive photon mapping on gfx950: caustic photon map,
+// uniform-grid cell table, camera paths with photon gathers and the
+// per-pixel progressive statistics (core/src/PhotonMapping.cpp:186-933,
+// core/include/Hasty/PhotonMapping.h:12-105, core/include/Hasty/HashCells.h).
+//
+// One pass:
+//   1. cell size: three deterministic reductions over the radius image
+//      (mean; eleven growing candidates; ten shrinking candidates);
+//   2. photons: batches of photon indices traced in parallel, each thread
+//      keeping its stored photons in private slots; a scan of the per-thread
+//      counts decides where the sequential emission loop of the reference
+//      stops (fewer than storeLimit stored, fewer than emitLimit emitted)
+//      and compacts the emitted photons in emission order;
+//   3. cells: 63-bit packed cell keys, stable radix sort (key, emission
+//      index), cell table (open addressing, key -> first, count);
+//   4. camera: one thread per pixel runs the whole MIS path with gathers in
+//      the reference's cell order (x, then y, then z; photons of a cell in
+//      emission order) and applies the pixel statistics update.
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <stdexcept>
+#include <string>
+
+#include "pt_kernels.h"
+#include "sppm.h"
+
+using namespace hdev;
+
+namespace hasty {
+namespace {
+
+constexpr int kB = 256;
+constexpr unsigned long long kEmptyKey = 0xFFFFFFFFFFFFFFFFull;
+constexpr int kKeyBits = 21;
+constexpr int kKeyBias = 1 << (kKeyBits - 1);
+
+void hipCheck(hipError_t e, const char* what) {
+  if (e != hipSuccess) throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
+}
+#define HC(x) hipCheck((x), #x)
+
+__device__ inline uint32_t xcdRemap() {
+  uint32_t g = gridDim.x, b = blockIdx.x;
+  if ((g & 7u) != 0u) return b;
+  return (b & 7u) * (g >> 3) + (b >> 3);
+}
+
+// PointHashCells::index (HashCells.h:33-36)
+__device__ inline int3 cellOf(V3 p, float cs) {
+  return make_int3(floorInt32(p.x / cs), floorInt32(p.y / cs), floorInt32(p.z / cs));
+}
+__device__ inline bool keyRange(int3 c) {
+  return c.x >= -kKeyBias && c.x < kKeyBias && c.y >= -kKeyBias && c.y < kKeyBias && c.z >= -kKeyBias &&
+         c.z < kKeyBias;
+}
+// Lexicographic (x, y, z) order of cells == numeric order of the keys.
+__device__ inline unsigned long long packKey(int3 c) {
+  return ((unsigned long long)(uint32_t)(c.x + kKeyBias) << (2 * kKeyBits)) |
+         ((unsigned long long)(uint32_t)(c.y + kKeyBias) << kKeyBits) | (unsigned long long)(uint32_t)(c.z + kKeyBias);
+}
+__device__ inline uint32_t hashKey(unsigned long long k) {
+  k ^= k >> 33;
+  k *= 0xff51afd7ed558ccdull;
+  k ^= k >> 33;
+  k *= 0xc4ceb9fe1a85ec53ull;
+  k ^= k >> 33;
+  return (uint32_t)k;
+}
+
+struct SppmDev {
+  // pixels (SoA, P entries each)
+  float* radius;
+  float* N;
+  float* Ne;
+  float* flux;  // 3P
+  float* LSum;  // 3P
+  int* Nd;
+  float* radii;  // the radius image used for the next pass
+  // photons in emission order and in cell order
+  float4* phX;   // x, ngwi
+  float4* phWi;  // wi
+  float4* phF;   // flux
+  float4* sX;
+  float4* sWi;
+  float4* sF;
+  unsigned long long* keys;
+  uint32_t* idx;
+  // cell table
+  unsigned long long* tKeys;
+  uint2* tVals;
+  uint32_t tMask;
+  int* bbox;  // min xyz, max xyz of occupied cells
+  const float* cellSize;
+  uint32_t* counters;
+  uint2* ovf;
+};
+
+enum Counter { kCtrStored = 0, kCtrEmitted = 1, kCtrBatchEmit = 2, kCtrKeyOverflow = 3, kNumCtr = 4 };
+
+// ------------------------------------------------------------ cell size ----
+// PhotonMapping.cpp:522-576 as three reductions.  Each block writes its
+// partial sums; a one-block kernel folds them in block order.
+constexpr int kCandidates = 11;
+__global__ void __launch_bounds__(kB) kCellPartial(const float* radii, uint32_t P, const float* cand, int nCand,
+                                                   float* partial) {
+  __shared__ float red[kB];
+  float acc[kCandidates];
+  float cs[kCandidates];
+#pragma unroll
+  for (int c = 0; c < kCandidates; ++c) {
+    acc[c] = 0.0f;
+    cs[c] = c < nCand ? cand[c] : 1.0f;
+  }
+  for (uint32_t p = blockIdx.x * kB + threadIdx.x; p < P; p += gridDim.x * kB) {
+    float r = radii[p];
+    if (nCand == 0) {
+      acc[0] += r;
+    } else {
+#pragma unroll
+      for (int c = 0; c < kCandidates; ++c)
+        if (c < nCand) acc[c] += 5.0f * powci<3>(r / cs[c]) + powci<3>(fmaxf(r, cs[c])) * 1e5f;
+    }
+  }
+  int n = nCand == 0 ? 1 : nCand;
+#pragma unroll
+  for (int c = 0; c < kCandidates; ++c) {
+    if (c < n) {  // n is uniform over the block
+      red[threadIdx.x] = acc[c];
+      __syncthreads();
+      for (int s = kB / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+      }
+      if (threadIdx.x == 0) partial[(size_t)c * gridDim.x + blockIdx.x] = red[0];
+      __syncthreads();
+    }
+  }
+}
+// stage 0: mean -> 11 growing candidates; stage 1: pick the grown size ->
+// 10 shrinking candidates; stage 2: pick the final size.
+__global__ void __launch_bounds__(kB) kCellFold(const float* partial, int nBlocks, int stage, uint32_t P, float* cand,
+                                                float* state, float* cellSizeOut) {
+  __shared__ float red[kB];
+  __shared__ float sums[kCandidates];
+  int n = stage == 0 ? 1 : (stage == 1 ? 11 : 10);
+  for (int c = 0; c < n; ++c) {
+    float a = 0.0f;
+    for (int b = threadIdx.x; b < nBlocks; b += kB) a += partial[(size_t)c * nBlocks + b];
+    red[threadIdx.x] = a;
+    __syncthreads();
+    for (int s = kB / 2; s > 0; s >>= 1) {
+      if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) sums[c] = red[0];
+    __syncthreads();
+  }
+  if (threadIdx.x != 0) return;
+  if (stage == 0) {
+    float cs = sums[0] / (float)P;
+    cand[0] = cs;
+    for (int j = 1; j < 11; ++j) cand[j] = cand[j - 1] * 1.5f;
+  } else if (stage == 1) {
+    // candidates[0] is the mean, [1..10] the growing trials
+    float cost = sums[0];
+    int a = 0;
+    for (int j = 1; j < 11; ++j) {
+      if (!(sums[j] < cost)) break;
+      cost = sums[j];
+      a = j;
+    }
+    float cs = cand[a];
+    state[0] = cs;
+    state[1] = cost;
+    cand[0] = cs * 0.6666f;
+    for (int j = 1; j < 10; ++j) cand[j] = cand[j - 1] * 0.6666f;
+  } else {
+    float cs = state[0], cost = state[1];
+    for (int j = 0; j < 10; ++j) {
+      if (!(sums[j] < cost)) break;
+      cost = sums[j];
+      cs = cand[j];
+    }
+    *cellSizeOut = cs;
+  }
+}
+
+// ------------------------------------------------------- initial radii ----
+// PhotonMapping.cpp:578-651 (ignoreCosineTerm) times the footprint scale;
+// -1 marks pixels without a usable surface (filled on the host).
+__global__ void __launch_bounds__(kB) kInitRadii(DevScene s, float footprintScale, float* radii, uint2* ovfAll) {
+  __shared__ uint2 stack[kStackLds * kB];
+  uint32_t P = s.width * s.height;
+  uint32_t gtid = xcdRemap() * kB + threadIdx.x;
+  uint2* ovf = ovfAll + (size_t)gtid * kStackOvf;
+  for (uint32_t p = gtid; p < P; p += gridDim.x * kB) {
+    uint32_t i = p % s.width, j = p / s.width;
+    hrng::Stream rng = hrng::make_stream(s.seed, p, 0u, hrng::kDomainRadius);
+    float px = (float)i + 0.5f, py = (float)j + 0.5f;
+    V3 o, d;
+    cameraRay(s, rng, px, py, &o, &d);
+    Hit h = traceClosest<kStackLds, kStackOvf>(s, o, d, stack + threadIdx.x, kB, ovf, nullptr, nullptr);
+    float result = -1.0f;
+    if (h.tri >= 0) {
+      float angle = cameraRayAngle(s, px, py);
+      Interaction it = makeInteraction(s, h.tri, h.u, h.v);
+      float dist = norm(load3(s.camPos) - it.x);
+      bool ok = true;
+      if (!s.mats[it.mat].hasDiffuseLobe) {
+        V3 rd = d;
+        for (int k = 0; k < s.maxDepth; ++k) {
+          const DevMaterial& m = s.mats[it.mat];
+          if (m.hasDiffuseLobe) break;
+          BsdfSample bs;
+          int tries = 0;
+          V3 wo = -rd;
+          do {
+            // a fresh LightRayInfo each try: no media, wavelength unset
+            int wavelength = -1;
+            float sp = 1.0f;
+            if (m.varyIOR) {
+              float p0 = 1.0f / 3.0f;
+              float xi = rng.uniform01f();
+              wavelength = xi < p0 ? 0 : (xi < p0 + p0 ? 1 : 2);
+              sp = p0;
+            }
+            Media fresh;
+            fresh.n = 0;
+            fresh.overflow = false;
+            float outsideIOR = mediaOutsideIOR(s, fresh, it, wo, wavelength);
+            bs = bsdfSample(m, rng, it, wo, wavelength, outsideIOR, false, 2);
+            bs.tD = bs.tD / sp;
+            bs.tC = bs.tC / sp;
+            tries += 1;
+          } while (isZero(bs.tC) && tries < 3);
+          if (isZero(bs.tC)) break;
+          V3 o2 = offsetOrigin(it, bs.outside);
+          Hit h2 = traceClosest<kStackLds, kStackOvf>(s, o2, bs.dir, stack + threadIdx.x, kB, ovf, nullptr, nullptr);
+          if (h2.tri < 0 || h2.tri == it.tri) {
+            ok = false;
+            break;
+          }
+          Interaction it2 = makeInteraction(s, h2.tri, h2.u, h2.v);
+          dist += norm(it.x - it2.x);
+          it = it2;
+          rd = bs.dir;
+        }
+      }
+      if (ok) result = 0.5f * dist * hm::tanf(angle);
+    }
+    radii[p] = footprintScale * result;
+  }
+}
+
+// -------------------------------------------------------------- photons ----
+// Scene::sampleLightRay (Scene.cpp:527-537) + tracePhoton
+// (PhotonMapping.cpp:191-265).  Thread t traces photon index first + t and
+// keeps its stores in slots [d * batch + t], d < cap.
+__global__ void __launch_bounds__(kB) kTracePhotons(DevScene s, SppmDev g, uint32_t first, uint32_t batch,
+                                                    uint32_t emitLimit, uint32_t pass, uint32_t cap, float4* slX,
+                                                    float4* slWi, float4* slF, uint32_t* slCount) {
+  __shared__ uint2 stack[kStackLds * kB];
+  uint32_t gtid = xcdRemap() * kB + threadIdx.x;
+  uint2* ovf = g.ovf + (size_t)gtid * kStackOvf;
+  const float minQ = 1.0f - hm::powf_pos(0.1f, 1.0f / (float)s.maxDepth);
+  for (uint32_t t = gtid; t < batch; t += gridDim.x * kB) {
+    uint32_t index = first + t;
+    uint32_t stored = 0;
+    if (index < emitLimit) {
+      hrng::Stream rng = hrng::make_stream(s.seed, index, pass, hrng::kDomainPhoton);
+      // sampleSurfaceLightPosition (Scene.cpp:506-526)
+      float pPos = 1.0f / s.totalLightArea;
+      uint32_t li = lightCdfSearch(s, rng.canonical_double());
+      int32_t tri = (int32_t)s.lightTris[li];
+      const float* pp = s.triPos + (size_t)tri * 9;
+      V3 p0 = load3(pp), p1 = load3(pp + 3), p2 = load3(pp + 6);
+      V3 ngL = normalize(cross(p1 - p0, p2 - p0));
+      float su = rng.uniform01f();
+      float tt = hm::fsqrt(rng.uniform01f());
+      float b0 = 1.0f - tt;
+      float b2 = tt * su;
+      float b1 = fmaxf(0.0f, 1.0f - b0 - b2);
+      Interaction itL = makeInteractionB(s, tri, b0, b1, b2, ngL);
+      float pDir;
+      V3 dir = sampleCosHemisphere(rng, itL.ng, &pDir);
+      const DevMaterial& lm = s.mats[itL.mat];
+      V3 Le = v3(lm.emission[0], lm.emission[1], lm.emission[2]) * (float)kInvPi;
+      V3 flux = Le * (__builtin_fabsf(dot(itL.ng, dir)) / (pPos * pDir));
+      V3 ro = offsetOrigin(itL, true), rd = dir;
+      Hit h = traceClosest<kStackLds, kStackOvf>(s, ro, rd, stack + threadIdx.x, kB, ovf, nullptr, nullptr);
+      if (h.tri >= 0) {
+        Media md;
+        md.n = 0;
+        md.overflow = false;
+        int wavelength = -1;
+        Interaction it = makeInteraction(s, h.tri, h.u, h.v);
+        // empty media: transmittance is exactly one
+        for (int depth = 0; depth < s.maxDepth; ++depth) {
+          if (!(__builtin_isfinite(flux.x) && __builtin_isfinite(flux.y) && __builtin_isfinite(flux.z))) break;
+          const DevMaterial& m = s.mats[it.mat];
+          V3 wo = -rd;
+          if (m.hasDiffuseLobe && depth > 0 && stored < cap) {
+            size_t sl = (size_t)stored * batch + t;
+            slX[sl] = f4(it.x, dot(it.ng, wo));
+            slWi[sl] = f4(wo, 0.0f);
+            slF[sl] = f4(flux, 0.0f);
+            stored++;
+          }
+          float sp = 1.0f;
+          if (wavelength == -1 && m.varyIOR) {
+            float q0 = 1.0f / 3.0f;
+            float xi = rng.uniform01f();
+            wavelength = xi < q0 ? 0 : (xi < q0 + q0 ? 1 : 2);
+            sp = q0;
+          }
+          float outsideIOR = mediaOutsideIOR(s, md, it, wo, wavelength);
+          BsdfSample bs = bsdfSample(m, rng, it, wo, wavelength, outsideIOR, true, 2);
+          bs.tD = bs.tD / sp;
+          bs.tC = bs.tC / sp;
+          V3 thrThis = bs.tD + bs.tC;
+          if (isZero(thrThis)) break;
+          V3 o2 = offsetOrigin(it, bs.outside);
+          Hit h2 = traceClosest<kStackLds, kStackOvf>(s, o2, bs.dir, stack + threadIdx.x, kB, ovf, nullptr, nullptr);
+          if (h2.tri < 0 || h2.tri == it.tri) break;
+          mediaUpdate(md, it, wo, bs.dir);
+          Interaction it2 = makeInteraction(s, h2.tri, h2.u, h2.v);
+          V3 t2 = beersLaw(mediaTransmittance(s, md), norm(it2.x - o2));
+          thrThis = mul(thrThis, t2);
+          float rq = fmaxf(minQ, 1.0f - norm(thrThis));
+          if (rng.uniform01f() <= rq) break;
+          flux = mul(flux, thrThis) / (1.0f - rq);
+          it = it2;
+          rd = bs.dir;
+        }
+      }
+    }
+    slCount[t] = stored;
+  }
+}
+
+// The reference emits photon i while fewer than storeLimit photons are
+// stored (PhotonMapping.cpp:861): with exclusive offsets o[t] of this batch,
+// photon t is emitted iff stored + o[t] < storeLimit (and i < emitLimit).
+// Emitted threads form a prefix; its length goes to kCtrBatchEmit.
+__global__ void __launch_bounds__(kB) kEmitCut(SppmDev g, const uint32_t* offs, uint32_t first, uint32_t batch,
+                                               uint32_t emitLimit, uint32_t storeLimit) {
+  uint32_t t = blockIdx.x * kB + threadIdx.x;
+  if (t >= batch) return;
+  uint32_t stored = g.counters[kCtrStored];
+  bool emitted = first + t < emitLimit && (unsigned long long)stored + offs[t] < storeLimit;
+  bool nextEmitted = t + 1 < batch && first + t + 1 < emitLimit &&
+                     (unsigned long long)stored + offs[t + 1] < storeLimit;
+  if (emitted && !nextEmitted) g.counters[kCtrBatchEmit] = t + 1;
+}
+__global__ void __launch_bounds__(kB) kEmitCompact(SppmDev g, const uint32_t* offs, const uint32_t* cnt,
+                                                   uint32_t batch, const float4* slX, const float4* slWi,
+                                                   const float4* slF) {
+  uint32_t n = g.counters[kCtrBatchEmit];
+  uint32_t base = g.counters[kCtrStored];
+  for (uint32_t t = blockIdx.x * kB + threadIdx.x; t < n; t += gridDim.x * kB) {
+    uint32_t c = cnt[t], o = base + offs[t];
+    for (uint32_t d = 0; d < c; ++d) {
+      size_t sl = (size_t)d * batch + t;
+      g.phX[o + d] = slX[sl];
+      g.phWi[o + d] = slWi[sl];
+      g.phF[o + d] = slF[sl];
+    }
+  }
+}
+__global__ void kEmitAdvance(SppmDev g, const uint32_t* offs, const uint32_t* cnt) {
+  uint32_t n = g.counters[kCtrBatchEmit];
+  if (n > 0) g.counters[kCtrStored] += offs[n - 1] + cnt[n - 1];
+  g.counters[kCtrEmitted] += n;
+  g.counters[kCtrBatchEmit] = 0;
+}
+
+// ---------------------------------------------------------------- cells ----
+__global__ void __launch_bounds__(kB) kCellKeys(SppmDev g, uint32_t n) {
+  __shared__ int box[6];
+  if (threadIdx.x < 6) box[threadIdx.x] = threadIdx.x < 3 ? 0x7FFFFFFF : -0x7FFFFFFF;
+  __syncthreads();
+  float cs = *g.cellSize;
+  int3 lo = make_int3(0x7FFFFFFF, 0x7FFFFFFF, 0x7FFFFFFF), hi = make_int3(-0x7FFFFFFF, -0x7FFFFFFF, -0x7FFFFFFF);
+  for (uint32_t k = blockIdx.x * kB + threadIdx.x; k < n; k += gridDim.x * kB) {
+    float4 x = g.phX[k];
+    int3 c = cellOf(v3(x.x, x.y, x.z), cs);
+    if (!keyRange(c)) {
+      atomicAdd(&g.counters[kCtrKeyOverflow], 1u);
+      c = make_int3(0, 0, 0);
+    }
+    g.keys[k] = packKey(c);
+    g.idx[k] = k;
+    lo = make_int3(min(lo.x, c.x), min(lo.y, c.y), min(lo.z, c.z));
+    hi = make_int3(max(hi.x, c.x), max(hi.y, c.y), max(hi.z, c.z));
+  }
+  atomicMin(&box[0], lo.x);
+  atomicMin(&box[1], lo.y);
+  atomicMin(&box[2], lo.z);
+  atomicMax(&box[3], hi.x);
+  atomicMax(&box[4], hi.y);
+  atomicMax(&box[5], hi.z);
+  __syncthreads();
+  if (threadIdx.x < 3) atomicMin(&g.bbox[threadIdx.x], box[threadIdx.x]);
+  else if (threadIdx.x < 6) atomicMax(&g.bbox[threadIdx.x], box[threadIdx.x]);
+}
+__global__ void __launch_bounds__(kB) kCellInit(SppmDev g, int clearBox) {
+  for (uint32_t k = blockIdx.x * kB + threadIdx.x; k <= g.tMask; k += gridDim.x * kB) g.tKeys[k] = kEmptyKey;
+  if (clearBox && blockIdx.x == 0 && threadIdx.x < 6) g.bbox[threadIdx.x] = threadIdx.x < 3 ? 0x7FFFFFFF : -0x7FFFFFFF;
+}
+// Gather the photon records into cell order and insert each cell's first
+// photon into the table; kCellCount then closes each cell at its last photon.
+__global__ void __launch_bounds__(kB) kCellBuild(SppmDev g, const unsigned long long* keysSorted,
+                                                 const uint32_t* idxSorted, uint32_t n) {
+  for (uint32_t k = blockIdx.x * kB + threadIdx.x; k < n; k += gridDim.x * kB) {
+    uint32_t src = idxSorted[k];
+    g.sX[k] = g.phX[src];
+    g.sWi[k] = g.phWi[src];
+    g.sF[k] = g.phF[src];
+    unsigned long long key = keysSorted[k];
+    if (k == 0 || keysSorted[k - 1] != key) {
+      uint32_t h = hashKey(key) & g.tMask;
+      while (true) {
+        unsigned long long prev = atomicCAS(&g.tKeys[h], kEmptyKey, key);
+        if (prev == kEmptyKey) break;
+        h = (h + 1) & g.tMask;
+      }
+      g.tVals[h] = make_uint2(k, 0u);
+    }
+  }
+}
+__global__ void __launch_bounds__(kB) kCellCount(SppmDev g, const unsigned long long* keysSorted, uint32_t n) {
+  for (uint32_t k = blockIdx.x * kB + threadIdx.x; k < n; k += gridDim.x * kB) {
+    unsigned long long key = keysSorted[k];
+    if (k + 1 == n || keysSorted[k + 1] != key) {
+      uint32_t h = hashKey(key) & g.tMask;
+      while (g.tKeys[h] != key) h = (h + 1) & g.tMask;
+      g.tVals[h].y = k + 1 - g.tVals[h].x;
+    }
+  }
+}
+
+__device__ inline uint2 cellLookup(const SppmDev& g, unsigned long long key) {
+  uint32_t h = hashKey(key) & g.tMask;
+  while (true) {
+    unsigned long long k = g.tKeys[h];
+    if (k == key) return g.tVals[h];
+    if (k == kEmptyKey) return make_uint2(0u, 0u);
+    h = (h + 1) & g.tMask;
+  }
+}
+
+// photonMapFluxEstimate (PhotonMapping.cpp:271-296) with
+// PointHashCells::radiusNeighbors (HashCells.h:71-98): cells x -> y -> z,
+// photons of a cell in emission order, |x - c|^2 <= r^2; the BSDF is the
+// diffuse lobe only with outside IOR 1.
+__device__ inline V3 gather(const SppmDev& g, const DevMaterial& m, const Interaction& it, V3 wo, float r, int* M) {
+  V3 est = v3(0.0f, 0.0f, 0.0f);
+  float cs = *g.cellSize;
+  float r2 = r * r;
+  V3 c = it.x;
+  int3 lo = cellOf(c - v3(r, r, r), cs), hi = cellOf(c + v3(r, r, r), cs);
+  lo = make_int3(max(lo.x, g.bbox[0]), max(lo.y, g.bbox[1]), max(lo.z, g.bbox[2]));
+  hi = make_int3(min(hi.x, g.bbox[3]), min(hi.y, g.bbox[4]), min(hi.z, g.bbox[5]));
+  V3 ns = shadingNormal(it, wo);
+  int found = 0;
+  for (int i = lo.x; i <= hi.x; ++i)
+    for (int j = lo.y; j <= hi.y; ++j)
+      for (int k = lo.z; k <= hi.z; ++k) {
+        uint2 cell = cellLookup(g, packKey(make_int3(i, j, k)));
+        for (uint32_t q = cell.x; q < cell.x + cell.y; ++q) {
+          float4 x4 = g.sX[q];
+          V3 dlt = v3(x4.x, x4.y, x4.z) - c;
+          if (!(dot(dlt, dlt) <= r2)) continue;
+          V3 wi = f4xyz(g.sWi[q]);
+          V3 fl = f4xyz(g.sF[q]);
+          float dotNsWi = dot(ns, wi);
+          V3 fD = evalDiffuse(dotNsWi, m, normalScale(wi, ns, it.ng));
+          float fc = __builtin_fabsf(dot(wi, it.ng)) / (1e-7f + __builtin_fabsf(x4.w));
+          est = est + mul(fl * fc, fD + v3(0.0f, 0.0f, 0.0f));
+          found++;
+        }
+      }
+  *M = found;
+  return est;
+}
+
+// --------------------------------------------------------------- camera ----
+// renderPhotonThread's pixel loop (PhotonMapping.cpp:874-925) with
+// pathTraceWithCausticsMap (PhotonMapping.cpp:404-503) unrolled into one
+// path.  A node's result (photon flux F, light LL, rest LR, specular LS)
+// enters the pixel through per-node weights: W_R for F and LR, W_S for LS,
+// and a light hit reached through an edge (tD, tC) counts W_S(parent) tC +
+// W_R(parent) tD.  Children: W_S' = W_S tC, W_R' = W_R (tD + tC).
+__global__ void __launch_bounds__(kB) kSppmCamera(DevScene s, SppmDev g, uint32_t pass, float offX, float offY,
+                                                  float alpha, int emittedInt, float* color, float* normalOut,
+                                                  float* albedoOut) {
+  __shared__ uint2 stack[kStackLds * kB];
+  uint32_t P = s.width * s.height;
+  uint32_t gtid = xcdRemap() * kB + threadIdx.x;
+  uint2* ovf = g.ovf + (size_t)gtid * kStackOvf;
+  const bool twoStrategies = s.hasLight && s.useMIS;
+  for (uint32_t p = gtid; p < P; p += gridDim.x * kB) {
+    uint32_t i = p % s.width, j = p / s.width;
+    hrng::Stream rng = hrng::make_stream(s.seed, p, pass, hrng::kDomainPath);
+    V3 ro, rd;
+    cameraRay(s, rng, (float)i + offX, (float)j + offY, &ro, &rd);
+    float pxRadius = g.radius[p];
+    if (pxRadius == -1.0f) pxRadius = g.radii[p];
+    float radius = pxRadius;
+    V3 nrm = v3(0.0f, 0.0f, 0.0f), alb = v3(0.0f, 0.0f, 0.0f);
+    V3 LL0 = v3(0.0f, 0.0f, 0.0f), LR0 = v3(0.0f, 0.0f, 0.0f), LS0 = v3(0.0f, 0.0f, 0.0f), F0 = v3(0.0f, 0.0f, 0.0f);
+    V3 WS = v3(1.0f, 1.0f, 1.0f), WR = v3(1.0f, 1.0f, 1.0f), WLR = v3(0.0f, 0.0f, 0.0f);
+    int Mtot = 0;
+    Media md;
+    md.n = 0;
+    md.overflow = false;
+    int wavelength = -1;
+    Hit h = traceClosest<kStackLds, kStackOvf>(s, ro, rd, stack + threadIdx.x, kB, ovf, nullptr, nullptr);
+    for (int depth = 0; depth < s.maxDepth; ++depth) {
+      if (h.tri < 0) {
+        V3 T = beersLaw(mediaTransmittance(s, md), 1e6f);
+        V3 LR = mul(T, evalEnvironment(s, rd));
+        if (depth == 0) {
+          LR0 = LR;
+          nrm = v3(0.0f, 0.0f, 0.0f);
+          alb = LR;
+        } else {
+          LR0 = LR0 + mul(WR, LR);
+        }
+        break;
+      }
+      Interaction it = makeInteraction(s, h.tri, h.u, h.v);
+      const DevMaterial& m = s.mats[it.mat];
+      if (depth == 0) {
+        nrm = it.ns;
+        alb = albedoOf(m);
+      }
+      V3 Tb = beersLaw(mediaTransmittance(s, md), norm(it.x - ro));
+      V3 wo = -rd;
+      if (m.isLight) {
+        V3 Le = dot(it.ng, wo) <= 0.0f ? v3(0.0f, 0.0f, 0.0f)
+                                        : v3(m.emission[0], m.emission[1], m.emission[2]) * (float)kInvPi;
+        V3 LL = mul(Tb, Le);
+        if (depth == 0) {
+          LL0 = LL;
+        } else {
+          LS0 = LS0 + mul(WS, LL);
+          LR0 = LR0 + mul(WLR, LL);
+        }
+        break;
+      }
+      if (m.hasDiffuseLobe) {
+        int M = 0;
+        V3 est = gather(g, m, it, wo, radius, &M);
+        Mtot += M;
+        F0 = F0 + mul(WR, mul(Tb, est));
+      }
+      float rrQ = depth >= s.rrStart ? s.rrQ : 0.0f;
+      if (rng.uniform01f() <= rrQ) break;
+      V3 thr = Tb / (1.0f - rrQ);
+      if (depth + 1 >= s.maxDepth) break;  // every child returns zero
+      if (twoStrategies) {
+        // lightStrategySample (MIS.cpp:32-54, Scene.cpp:538-586)
+        float pSurf = !s.hasEnv ? 1.0f : (s.hasSurfaceLight ? 0.5f : 0.0f);
+        float xi = rng.uniform01f();
+        V3 so, sd;
+        int32_t target = -1;
+        if (xi < pSurf) {
+          uint32_t li = lightCdfSearch(s, rng.canonical_double());
+          target = (int32_t)s.lightTris[li];
+          const float* pp = s.triPos + (size_t)target * 9;
+          V3 p0 = load3(pp), p1 = load3(pp + 3), p2 = load3(pp + 6);
+          float su = rng.uniform01f();
+          float tt = hm::fsqrt(rng.uniform01f());
+          float b1 = 1.0f - tt;
+          float b3 = tt * su;
+          float b2 = fmaxf(0.0f, 1.0f - b1 - b3);
+          V3 x2 = p0 * b1 + p1 * b2 + p2 * b3;
+          so = offsetOrigin(it, dot(it.ng, x2 - it.x) >= 0.0f);
+          sd = normalize(x2 - so);
+        } else {
+          float dens;
+          sd = envSample(s, rng, &dens);
+          so = offsetOrigin(it, dot(it.ng, sd) >= 0.0f);
+        }
+        Hit hl = traceClosest<kStackLds, kStackOvf>(s, so, sd, stack + threadIdx.x, kB, ovf, nullptr, nullptr);
+        bool visible = target >= 0 ? hl.tri == target : (hl.tri < 0 || s.mats[s.triMat[hl.tri]].isLight);
+        if (visible) {
+          V3 hx = v3(0.0f, 0.0f, 0.0f), hn = v3(0.0f, 0.0f, 0.0f);
+          if (hl.tri >= 0) hitPointNormal(s, hl.tri, hl.u, hl.v, &hx, &hn);
+          float pL = lightDensity(s, so, sd, hl.tri, hx, hn);
+          if (pL != 0.0f) {
+            BsdfEval ev = bsdfEvaluate(m, it, wo, sd);
+            float mm = __builtin_fabsf(dot(sd, it.ng)) / pL;
+            V3 sD = ev.fD * mm, sC = ev.fC * mm;
+            if (!isZero(sD + sC) && hl.tri != it.tri) {
+              float pB = bsdfPdf(m, it, wo, sd, mediaOutsideIOR(s, md, it, wo, wavelength));
+              float sum = 0.0f;
+              sum += powci<2>(pL);
+              sum += powci<2>(pB);
+              float w = powci<2>(pL) / sum;
+              Media mc = md;
+              mediaUpdate(mc, it, wo, sd);
+              V3 tD = mul(thr, sD) * w, tC = mul(thr, sC) * w;
+              V3 Tc = mediaTransmittance(s, mc);
+              if (hl.tri >= 0) {
+                const DevMaterial& lm = s.mats[s.triMat[hl.tri]];
+                V3 Le = dot(hn, -sd) <= 0.0f ? v3(0.0f, 0.0f, 0.0f)
+                                              : v3(lm.emission[0], lm.emission[1], lm.emission[2]) * (float)kInvPi;
+                V3 LL = mul(beersLaw(Tc, norm(hx - so)), Le);
+                LS0 = LS0 + mul(WS, mul(LL, tC));
+                LR0 = LR0 + mul(WR, mul(LL, tD));
+              } else {
+                V3 LR = mul(beersLaw(Tc, 1e6f), evalEnvironment(s, sd));
+                LR0 = LR0 + mul(WR, mul(LR, tD + tC));
+              }
+            }
+          }
+        }
+      }
+      // brdfStrategySample (MIS.cpp:6-26, Scene.cpp:328-338)
+      float sp = 1.0f;
+      int wl = wavelength;
+      if (wl == -1 && m.varyIOR) {
+        float q0 = 1.0f / 3.0f;
+        float xi = rng.uniform01f();
+        wl = xi < q0 ? 0 : (xi < q0 + q0 ? 1 : 2);
+        sp = q0;
+      }
+      float outsideIOR = mediaOutsideIOR(s, md, it, wo, wl);
+      BsdfSample bs = bsdfSample(m, rng, it, wo, wl, outsideIOR, false, 3);
+      bs.tD = bs.tD / sp;
+      bs.tC = bs.tC / sp;
+      // zero throughput (including pdf 0) ends the path before any trace
+      if (bs.pdf == 0.0f || isZero(bs.tD + bs.tC)) break;
+      V3 no = offsetOrigin(it, bs.outside);
+      Hit h2 = traceClosest<kStackLds, kStackOvf>(s, no, bs.dir, stack + threadIdx.x, kB, ovf, nullptr, nullptr);
+      if (h2.tri == it.tri) break;
+      float w;
+      {
+        float pB = bs.pdf;
+        float sum = 0.0f;
+        if (twoStrategies) {
+          V3 hx = v3(0.0f, 0.0f, 0.0f), hn = v3(0.0f, 0.0f, 0.0f);
+          if (h2.tri >= 0) hitPointNormal(s, h2.tri, h2.u, h2.v, &hx, &hn);
+          float pL = lightDensity(s, no, bs.dir, h2.tri, hx, hn);
+          sum += powci<2>(pL);
+        }
+        sum += powci<2>(pB);
+        w = powci<2>(pB) / sum;
+      }
+      mediaUpdate(md, it, wo, bs.dir);
+      wavelength = wl;
+      V3 tD = mul(thr, bs.tD) * w, tC = mul(thr, bs.tC) * w;
+      WLR = mul(WR, tD);
+      WS = mul(WS, tC);
+      WR = mul(WR, tD + tC);
+      ro = no;
+      rd = bs.dir;
+      h = h2;
+    }
+    // PMRenderJob::unlockAndUpdateStatistic (PhotonMapping.h:63-105)
+    V3 L = LL0 + LR0 + LS0;
+    bool bad = L.x != L.x || L.y != L.y || L.z != L.z || F0.x != F0.x || F0.y != F0.y || F0.z != F0.z;
+    V3 LSum = v3(g.LSum[p], g.LSum[P + p], g.LSum[2 * P + p]);
+    int Nd = g.Nd[p];
+    V3 Lnew;
+    if (bad) {
+      Lnew = LSum / (1e-7f + (float)Nd);
+      g.radius[p] = pxRadius;
+    } else {
+      LSum = LSum + L;
+      Nd += 1;
+      float N = g.N[p], Ne = g.Ne[p];
+      V3 flux = v3(g.flux[p], g.flux[P + p], g.flux[2 * P + p]);
+      V3 Lflux = v3(0.0f, 0.0f, 0.0f);
+      if (pxRadius < 0.0f) {
+        if (!(radius < 0.0f)) {
+          pxRadius = radius;
+          N += alpha * Mtot;
+          Ne += emittedInt;
+          flux = F0;
+          Lflux = Ne == 0 ? v3(0.0f, 0.0f, 0.0f) : flux / (pxRadius * pxRadius * (float)kPi * Ne);
+        }
+      } else {
+        float rold = pxRadius;
+        float Ni = N;
+        if (Ni + Mtot > 0) pxRadius *= hm::fsqrt((Ni + alpha * Mtot) / (Ni + Mtot));
+        N += alpha * Mtot;
+        Ne += emittedInt;
+        flux = (flux + F0) * (pxRadius * pxRadius) / (rold * rold);
+        Lflux = Ne == 0 ? v3(0.0f, 0.0f, 0.0f) : flux / (pxRadius * pxRadius * (float)kPi * Ne);
+        radius = pxRadius;
+      }
+      Lnew = Lflux + LSum / (float)Nd;
+      g.radius[p] = pxRadius;
+      g.N[p] = N;
+      g.Ne[p] = Ne;
+      g.flux[p] = flux.x, g.flux[P + p] = flux.y, g.flux[2 * P + p] = flux.z;
+      g.LSum[p] = LSum.x, g.LSum[P + p] = LSum.y, g.LSum[2 * P + p] = LSum.z;
+      g.Nd[p] = Nd;
+    }
+    g.radii[p] = radius;
+    color[p * 3] = Lnew.x, color[p * 3 + 1] = Lnew.y, color[p * 3 + 2] = Lnew.z;
+    normalOut[p * 3] = nrm.x, normalOut[p * 3 + 1] = nrm.y, normalOut[p * 3 + 2] = nrm.z;
+    albedoOut[p * 3] = alb.x, albedoOut[p * 3 + 1] = alb.y, albedoOut[p * 3 + 2] = alb.z;
+  }
+}
+
+__global__ void __launch_bounds__(kB) kPixelInit(SppmDev g, uint32_t P) {
+  for (uint32_t p = blockIdx.x * kB + threadIdx.x; p < P; p += gridDim.x * kB) {
+    g.radius[p] = -1.0f;
+    g.N[p] = 0.0f;
+    g.Ne[p] = 0.0f;
+    g.Nd[p] = 0;
+    for (int c = 0; c < 3; ++c) {
+      g.flux[c * P + p] = 0.0f;
+      g.LSum[c * P + p] = 0.0f;
+    }
+  }
+}
+__global__ void __launch_bounds__(kB) kFill(float* a, uint32_t n, float v) {
+  for (uint32_t k = blockIdx.x * kB + threadIdx.x; k < n; k += gridDim.x * kB) a[k] = v;
+}
+__global__ void kZeroU32(uint32_t* a, uint32_t n) {
+  if (threadIdx.x < n) a[threadIdx.x] = 0u;
+}
+
+}  // namespace
+
+// ===================================================== SppmRenderer ====
+template <class T>
+T* SppmRenderer::alloc(size_t n) {
+  void* p = nullptr;
+  HC(hipMalloc(&p, std::max<size_t>(n, 1) * sizeof(T)));
+  allocs_.push_back(p);
+  return (T*)p;
+}
+
+SppmRenderer::SppmRenderer(const DevScene& scene, const SppmParams& params, hipStream_t stream)
+    : s_(scene), p_(params), stream_(stream) {
+  W_ = scene.width;
+  H_ = scene.height;
+  size_t P = (size_t)W_ * H_;
+  dPixel_ = alloc<float>(P * 10);
+  dRadii_ = alloc<float>(P);
+  dCounters_ = alloc<uint32_t>(8);
+  dCellSize_ = alloc<float>(4);
+  dCand_ = alloc<float>(16);
+  dState_ = alloc<float>(4);
+  dBox_ = alloc<int>(8);
+  traceBlocks_ = 2048;
+  dOverflow_ = alloc<uint2>((size_t)traceBlocks_ * kB * kStackOvf);
+  partialBlocks_ = 512;
+  dPartial_ = alloc<float>((size_t)partialBlocks_ * kCandidates);
+  // per-thread photon slots: batch x cap x 48 bytes, at most 3 GiB
+  cap_ = (uint32_t)std::max(1, s_.maxDepth - 1);
+  size_t maxBatch = ((size_t)3 << 30) / ((size_t)cap_ * 48);
+  batchMax_ = (uint32_t)std::min<size_t>((size_t)1 << 22, maxBatch);
+  batchMax_ = std::max<uint32_t>(batchMax_, 4096);
+  dSlX_ = alloc<float4>((size_t)batchMax_ * cap_);
+  dSlWi_ = alloc<float4>((size_t)batchMax_ * cap_);
+  dSlF_ = alloc<float4>((size_t)batchMax_ * cap_);
+  dSlCount_ = alloc<uint32_t>(batchMax_);
+  dSlOffset_ = alloc<uint32_t>(batchMax_);
+  HC(hipcub::DeviceScan::ExclusiveSum(nullptr, scanTmpBytes_, dSlCount_, dSlOffset_, (int)batchMax_, stream_));
+  dScanTmp_ = alloc<char>(scanTmpBytes_);
+  // photons: at most storeLimit + cap per pass
+  emitLimit_ = (uint32_t)(int)((float)p_.emitFraction * H_ * W_);
+  storeLimit_ = (uint32_t)(int)((float)p_.storeFraction * H_ * W_);
+  photonCap_ = (size_t)storeLimit_ + cap_ + 1;
+  dPhX_ = alloc<float4>(photonCap_);
+  dPhWi_ = alloc<float4>(photonCap_);
+  dPhF_ = alloc<float4>(photonCap_);
+  dSX_ = alloc<float4>(photonCap_);
+  dSWi_ = alloc<float4>(photonCap_);
+  dSF_ = alloc<float4>(photonCap_);
+  dKeys_ = alloc<unsigned long long>(photonCap_);
+  dKeysSorted_ = alloc<unsigned long long>(photonCap_);
+  dIdx_ = alloc<uint32_t>(photonCap_);
+  dIdxSorted_ = alloc<uint32_t>(photonCap_);
+  HC(hipcub::DeviceRadixSort::SortPairs(nullptr, sortTmpBytes_, dKeys_, dKeysSorted_, dIdx_, dIdxSorted_,
+                                        (int)photonCap_, 0, 3 * kKeyBits, stream_));
+  dSortTmp_ = alloc<char>(sortTmpBytes_);
+  uint32_t ts = 1024;
+  while (ts < 2 * photonCap_) ts <<= 1;
+  tableSize_ = ts;
+  dTableKeys_ = alloc<unsigned long long>(tableSize_);
+  dTableVals_ = alloc<uint2>(tableSize_);
+  hipLaunchKernelGGL(kPixelInit, dim3(1024), dim3(kB), 0, stream_,
+                     SppmDev{dPixel_, dPixel_ + P, dPixel_ + 2 * P, dPixel_ + 3 * P, dPixel_ + 6 * P,
+                             (int*)(dPixel_ + 9 * P), dRadii_},
+                     (uint32_t)P);
+  HC(hipGetLastError());
+}
+
+SppmRenderer::~SppmRenderer() {
+  for (void* p : allocs_) (void)hipFree(p);
+}
+
+void SppmRenderer::initRadii() {
+  size_t P = (size_t)W_ * H_;
+  if (!s_.hasSurfaceLight) {
+    hipLaunchKernelGGL(kFill, dim3(1024), dim3(kB), 0, stream_, dRadii_, (uint32_t)P, 1.0f);
+    HC(hipGetLastError());
+    return;
+  }
+  hipLaunchKernelGGL(kInitRadii, dim3(traceBlocks_), dim3(kB), 0, stream_, s_, p_.footprintScale, dRadii_,
+                     dOverflow_);
+  HC(hipGetLastError());
+  // getCloseNonNegative (PhotonMapping.cpp:653-678), in raster order and in
+  // place as the reference does.
+  std::vector<float> r(P);
+  HC(hipMemcpyAsync(r.data(), dRadii_, P * sizeof(float), hipMemcpyDeviceToHost, stream_));
+  HC(hipStreamSynchronize(stream_));
+  int W = (int)W_, H = (int)H_;
+  auto inside = [&](int x, int y) { return x >= 0 && y >= 0 && x < W && y < H; };
+  auto at = [&](int x, int y) { return r[(size_t)y * W + x]; };
+  bool any = false;
+  for (int j = 0; j < H; ++j)
+    for (int i = 0; i < W; ++i) {
+      if (!(r[(size_t)j * W + i] < 0.0f)) continue;
+      any = true;
+      float v = 1e-3f;
+      for (int k = 1; k < std::max(H, W); ++k) {
+        bool found = false;
+        for (int m = 0; m <= k * 2 && !found; ++m) {
+          int i2 = i - k, j2 = j - k + m;
+          if (inside(i2, j2) && at(i2, j2) > 0.0f) {
+            v = at(i2, j2);
+            found = true;
+            break;
+          }
+          i2 = i - k + m;
+          j2 = j - k;
+          if (inside(i2, j2) && at(i2, j2) > 0.0f) {
+            v = at(i2, j2);
+            found = true;
+            break;
+          }
+          i2 = i + k;
+          j2 = j - k + m;
+          if (inside(i2, j2) && at(i2, j2) > 0.0f) {
+            v = at(i2, j2);
+            found = true;
+            break;
+          }
+        }
+        if (found) break;
+      }
+      r[(size_t)j * W + i] = v;
+    }
+  if (any) {
+    HC(hipMemcpyAsync(dRadii_, r.data(), P * sizeof(float), hipMemcpyHostToDevice, stream_));
+    HC(hipStreamSynchronize(stream_));
+  }
+}
+
+void SppmRenderer::pass(uint32_t passIndex, float offX, float offY, float* dColor, float* dNormal, float* dAlbedo,
+                        KernelTimer* timer) {
+  size_t P = (size_t)W_ * H_;
+  if (!initialised_) {
+    if (timer) timer->begin(kKSppmRadii, stream_);
+    initRadii();
+    if (timer) timer->end(kKSppmRadii, stream_);
+    initialised_ = true;
+  }
+  SppmDev g{};
+  g.radius = dPixel_;
+  g.N = dPixel_ + P;
+  g.Ne = dPixel_ + 2 * P;
+  g.flux = dPixel_ + 3 * P;
+  g.LSum = dPixel_ + 6 * P;
+  g.Nd = (int*)(dPixel_ + 9 * P);
+  g.radii = dRadii_;
+  g.phX = dPhX_, g.phWi = dPhWi_, g.phF = dPhF_;
+  g.sX = dSX_, g.sWi = dSWi_, g.sF = dSF_;
+  g.keys = dKeys_;
+  g.idx = dIdx_;
+  g.tKeys = dTableKeys_;
+  g.tVals = dTableVals_;
+  g.tMask = tableSize_ - 1;
+  g.bbox = dBox_;
+  g.cellSize = dCellSize_;
+  g.counters = dCounters_;
+  g.ovf = dOverflow_;
+
+  // 1. cell size
+  if (timer) timer->begin(kKSppmCellSize, stream_);
+  hipLaunchKernelGGL(kCellPartial, dim3(partialBlocks_), dim3(kB), 0, stream_, (const float*)dRadii_, (uint32_t)P,
+                     (const float*)dCand_, 0, dPartial_);
+  hipLaunchKernelGGL(kCellFold, dim3(1), dim3(kB), 0, stream_, (const float*)dPartial_, (int)partialBlocks_, 0,
+                     (uint32_t)P, dCand_, dState_, dCellSize_);
+  hipLaunchKernelGGL(kCellPartial, dim3(partialBlocks_), dim3(kB), 0, stream_, (const float*)dRadii_, (uint32_t)P,
+                     (const float*)dCand_, 11, dPartial_);
+  hipLaunchKernelGGL(kCellFold, dim3(1), dim3(kB), 0, stream_, (const float*)dPartial_, (int)partialBlocks_, 1,
+                     (uint32_t)P, dCand_, dState_, dCellSize_);
+  hipLaunchKernelGGL(kCellPartial, dim3(partialBlocks_), dim3(kB), 0, stream_, (const float*)dRadii_, (uint32_t)P,
+                     (const float*)dCand_, 10, dPartial_);
+  hipLaunchKernelGGL(kCellFold, dim3(1), dim3(kB), 0, stream_, (const float*)dPartial_, (int)partialBlocks_, 2,
+                     (uint32_t)P, dCand_, dState_, dCellSize_);
+  HC(hipGetLastError());
+  if (timer) timer->end(kKSppmCellSize, stream_);
+
+  // 2. photons
+  hipLaunchKernelGGL(kZeroU32, dim3(1), dim3(64), 0, stream_, dCounters_, 8u);
+  uint32_t stored = 0, emitted = 0, next = 0;
+  if (s_.hasSurfaceLight) {
+    uint32_t batch = std::min<uint32_t>(batchMax_, std::max<uint32_t>(65536, storeLimit_));
+    while (stored < storeLimit_ && next < emitLimit_) {
+      batch = std::min(batch, emitLimit_ - next);
+      if (timer) timer->begin(kKSppmPhotons, stream_);
+      hipLaunchKernelGGL(kTracePhotons, dim3(traceBlocks_), dim3(kB), 0, stream_, s_, g, next, batch, emitLimit_,
+                         passIndex, cap_, dSlX_, dSlWi_, dSlF_, dSlCount_);
+      HC(hipGetLastError());
+      if (timer) timer->end(kKSppmPhotons, stream_);
+      if (timer) timer->begin(kKSppmBuild, stream_);
+      size_t tmp = scanTmpBytes_;
+      HC(hipcub::DeviceScan::ExclusiveSum(dScanTmp_, tmp, dSlCount_, dSlOffset_, (int)batch, stream_));
+      uint32_t blocks = (batch + kB - 1) / kB;
+      hipLaunchKernelGGL(kEmitCut, dim3(blocks), dim3(kB), 0, stream_, g, (const uint32_t*)dSlOffset_, next, batch,
+                         emitLimit_, storeLimit_);
+      hipLaunchKernelGGL(kEmitCompact, dim3(std::min<uint32_t>(blocks, 4096)), dim3(kB), 0, stream_, g,
+                         (const uint32_t*)dSlOffset_, (const uint32_t*)dSlCount_, batch, (const float4*)dSlX_,
+                         (const float4*)dSlWi_, (const float4*)dSlF_);
+      hipLaunchKernelGGL(kEmitAdvance, dim3(1), dim3(1), 0, stream_, g, (const uint32_t*)dSlOffset_,
+                         (const uint32_t*)dSlCount_);
+      HC(hipGetLastError());
+      if (timer) timer->end(kKSppmBuild, stream_);
+      uint32_t c[2];
+      HC(hipMemcpyAsync(c, dCounters_, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, stream_));
+      HC(hipStreamSynchronize(stream_));
+      uint32_t newly = c[1] - emitted;
+      stored = c[0];
+      emitted = c[1];
+      next += batch;
+      if (newly < batch) break;  // the emission loop stopped inside this batch
+      // size the next batch from the observed store rate
+      double rate = emitted > 0 ? (double)stored / (double)emitted : 0.0;
+      double want = rate > 0.0 ? 1.25 * (double)(storeLimit_ - std::min(stored, storeLimit_)) / rate : (double)batch * 2;
+      batch = (uint32_t)std::min<double>((double)batchMax_, std::max(65536.0, want));
+    }
+  }
+  info_.emitted = emitted;
+  info_.stored = stored;
+
+  // 3. cells
+  if (timer) timer->begin(kKSppmBuild, stream_);
+  hipLaunchKernelGGL(kCellInit, dim3(1024), dim3(kB), 0, stream_, g, 1);
+  if (stored > 0) {
+    uint32_t blocks = std::min<uint32_t>((stored + kB - 1) / kB, 4096);
+    hipLaunchKernelGGL(kCellKeys, dim3(blocks), dim3(kB), 0, stream_, g, stored);
+    size_t tmp = sortTmpBytes_;
+    HC(hipcub::DeviceRadixSort::SortPairs(dSortTmp_, tmp, dKeys_, dKeysSorted_, dIdx_, dIdxSorted_, (int)stored, 0,
+                                          3 * kKeyBits, stream_));
+    hipLaunchKernelGGL(kCellBuild, dim3(blocks), dim3(kB), 0, stream_, g, (const unsigned long long*)dKeysSorted_,
+                       (const uint32_t*)dIdxSorted_, stored);
+    hipLaunchKernelGGL(kCellCount, dim3(blocks), dim3(kB), 0, stream_, g, (const unsigned long long*)dKeysSorted_,
+                       stored);
+  }
+  HC(hipGetLastError());
+  if (timer) timer->end(kKSppmBuild, stream_);
+
+  // 4. camera paths, gathers, statistics
+  if (timer) timer->begin(kKSppmCamera, stream_);
+  hipLaunchKernelGGL(kSppmCamera, dim3(traceBlocks_), dim3(kB), 0, stream_, s_, g, passIndex, offX, offY, p_.alpha,
+                     (int)emitted, dColor, dNormal, dAlbedo);
+  HC(hipGetLastError());
+  if (timer) timer->end(kKSppmCamera, stream_);
+  uint32_t c[4];
+  HC(hipMemcpyAsync(c, dCounters_, sizeof(c), hipMemcpyDeviceToHost, stream_));
+  float cs;
+  HC(hipMemcpyAsync(&cs, dCellSize_, sizeof(float), hipMemcpyDeviceToHost, stream_));
+  HC(hipStreamSynchronize(stream_));
+  info_.cellSize = cs;
+  if (c[kCtrKeyOverflow] != 0)
+    throw std::runtime_error("photon cell index outside the 21-bit key range (scene extent / cell size too large)");
+  passes_ += 1;
+}
+
+void SppmRenderer::pixelStats(float* host10) {
+  size_t P = (size_t)W_ * H_;
+  HC(hipMemcpyAsync(host10, dPixel_, P * 10 * sizeof(float), hipMemcpyDeviceToHost, stream_));
+  HC(hipStreamSynchronize(stream_));
+}
+void SppmRenderer::radii(float* host) {
+  size_t P = (size_t)W_ * H_;
+  HC(hipMemcpyAsync(host, dRadii_, P * sizeof(float), hipMemcpyDeviceToHost, stream_));
+  HC(hipStreamSynchronize(stream_));
+}
+void SppmRenderer::photons(float* host, size_t maxPhotons) {
+  size_t n = std::min<size_t>(info_.stored, maxPhotons);
+  std::vector<float4> a(n), b(n), c(n);
+  HC(hipMemcpyAsync(a.data(), dPhX_, n * sizeof(float4), hipMemcpyDeviceToHost, stream_));
+  HC(hipMemcpyAsync(b.data(), dPhWi_, n * sizeof(float4), hipMemcpyDeviceToHost, stream_));
+  HC(hipMemcpyAsync(c.data(), dPhF_, n * sizeof(float4), hipMemcpyDeviceToHost, stream_));
+  HC(hipStreamSynchronize(stream_));
+  for (size_t k = 0; k < n; ++k) {
+    float* o = host + k * 10;
+    o[0] = a[k].x, o[1] = a[k].y, o[2] = a[k].z, o[3] = a[k].w;
+    o[4] = b[k].x, o[5] = b[k].y, o[6] = b[k].z;
+    o[7] = c[k].x, o[8] = c[k].y, o[9] = c[k].z;
+  }
+}
+
+}  // namespace hasty

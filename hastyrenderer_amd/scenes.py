@@ -209,6 +209,85 @@ def write_cornell(folder, width=256, height=256, spp=64, max_depth=8, sky=False)
     return os.path.join(folder, "renderJob.json")
 
 
+def write_mirror_caustic(folder, width=32, height=32, passes=64, max_depth=2, rho=0.5, half=0.1,
+                         emission=10.0 * math.pi, mirror=True, extent=20.0):
+    """Caustic with a closed-form answer: a small quad light at y = 1 facing
+    down (-y) over a floor at y = 0 (a perfect mirror: metallic 1, albedo 1,
+    roughness 0; or, with mirror=False, diffuse) and a diffuse ceiling of
+    albedo `rho` at y = 2 facing down.  The camera at (3, 0.5, 0) looks at
+    (1.5, 2, 0): it sees only ceiling points whose only light is the mirror
+    image of the light (at y = -1, facing up), so their radiance is
+    rho / pi * E(x) with E the irradiance of that image.  No walls, black
+    background, RR from depth 5 (none for max_depth 2)."""
+    os.makedirs(folder, exist_ok=True)
+    obj = _Obj()
+    mtl = [
+        _mtl("ceiling", kd=(rho, rho, rho)),
+        _mtl("mirror", kd=(1.0, 1.0, 1.0), ka=(1.0, 1.0, 1.0), ns=1000.0) if mirror
+        else _mtl("mirror", kd=(0.5, 0.5, 0.5)),
+        _mtl("light", ke=(emission, emission, emission)),
+    ]
+    quad = [(0, 1, 2), (0, 2, 3)]
+    e = extent
+    obj.shape("floor")
+    obj.smooth(False)
+    obj.material("mirror")
+    base = obj.vertices([[-e, 0, -e], [-e, 0, e], [e, 0, e], [e, 0, -e]])  # normal +y
+    obj.faces(quad, base)
+    obj.shape("ceiling")
+    obj.smooth(False)
+    obj.material("ceiling")
+    base = obj.vertices([[-e, 2, -e], [e, 2, -e], [e, 2, e], [-e, 2, e]])  # normal -y
+    obj.faces(quad, base)
+    obj.shape("light")
+    obj.smooth(False)
+    obj.material("light")
+    base = obj.vertices([[-half, 1, -half], [half, 1, -half], [half, 1, half], [-half, 1, half]])  # normal -y
+    obj.faces(quad, base)
+    obj.write(os.path.join(folder, "scene.obj"), "scene.mtl")
+    with open(os.path.join(folder, "scene.mtl"), "w") as f:
+        f.write("\n".join(mtl))
+    scene = {
+        "camera": {"position": [3.0, 0.5, 0.0], "look_at": [1.5, 2.0, 0.0], "fov": 40.0},
+        "world": {"background_color": [0.0, 0.0, 0.0]},
+        "mesh": "scene.obj",
+    }
+    with open(os.path.join(folder, "scene.json"), "w") as f:
+        json.dump(scene, f, indent=2)
+    _write_job(
+        folder,
+        {
+            "width": width,
+            "height": height,
+            "num_samples": passes,
+            "max_depth": max_depth,
+            "roulette_start_depth": 5,
+            "roulette_q": 0.3,
+            "use_mis": True,
+            "num_threads": 1,
+        },
+    )
+    return os.path.join(folder, "renderJob.json")
+
+
+def mirror_caustic_radiance(points, rho=0.5, half=0.1, emission=10.0 * math.pi, n=64):
+    """rho / pi * E(x) for ceiling points x (N, 3) of write_mirror_caustic:
+    E = integral over the mirrored light (y = -1, facing up, radiance
+    emission / pi) of L cos cos' / d^2, by an n x n midpoint rule (float64)."""
+    pts = np.asarray(points, np.float64).reshape(-1, 3)
+    g = (np.arange(n) + 0.5) / n * 2.0 * half - half
+    lx, lz = np.meshgrid(g, g, indexing="ij")
+    lx, lz = lx.ravel(), lz.ravel()
+    dA = (2.0 * half / n) ** 2
+    Le = emission / math.pi
+    out = np.zeros(len(pts))
+    for k, (x, y, z) in enumerate(pts):
+        dy = y + 1.0
+        d2 = (x - lx) ** 2 + dy * dy + (z - lz) ** 2
+        out[k] = np.sum(Le * dy * dy / (d2 * d2)) * dA
+    return rho / math.pi * out
+
+
 def icosphere(subdiv):
     """Unit icosphere: returns (vertices float64 (N,3), triangles int (M,3))."""
     t = (1.0 + math.sqrt(5.0)) / 2.0

@@ -112,6 +112,50 @@ int hasty_counters(HastyScene* scene, uint64_t* out5);
  * together) and grid sizes.  0 keeps the current value. */
 int hasty_set_launch(HastyScene* scene, uint32_t samples_per_batch, uint32_t trace_blocks, uint32_t shade_blocks);
 
+/* ---- progressive photon mapping (caustic photon map + MIS path tracing) ----
+ * Replaces renderPhotonThread (core/src/PhotonMapping.cpp:814-933, declared
+ * core/include/Hasty/PhotonMapping.h:124) for one PMRenderJob
+ * (PhotonMapping.h:33-121).  A job keeps the per-pixel statistics and the
+ * radius image on the device between passes; pass k uses the k-th sample
+ * offset of the job (RenderJob.h:53-60) and random streams keyed by
+ * (seed, photon or pixel index, k).  The films receive the CURRENT estimate
+ * of every pixel (the reference sets numSamples = 1 after each pass,
+ * PhotonMapping.cpp:918-923), not sums. */
+typedef struct HastySppm HastySppm;
+
+typedef struct HastySppmParams {
+  float alpha;           /* radius reduction, PMRenderJob::alpha (PhotonMapping.h:65), default 0.4 */
+  float footprint_scale; /* initial radius / pixel footprint (PhotonMapping.cpp:840), default 10 */
+  double store_fraction; /* stop emitting at store_fraction*W*H stored photons (PhotonMapping.cpp:834), 0.5 */
+  double emit_fraction;  /* ... or emit_fraction*W*H emitted photons (PhotonMapping.cpp:835), 10 */
+} HastySppmParams;
+
+typedef struct HastySppmInfo {
+  uint64_t emitted;     /* photons emitted in the last pass (PhotonMap::emittedCount) */
+  uint64_t stored;      /* photons stored in the last pass (PhotonMap::photonCount) */
+  float cell_size;      /* computeCellSize of the last pass (PhotonMapping.cpp:522-576) */
+  uint32_t passes_done;
+  uint32_t emit_limit;
+  uint32_t store_limit;
+} HastySppmInfo;
+
+/* params may be NULL (reference defaults).  The scene must outlive the job. */
+int hasty_sppm_create(HastyScene* scene, uint32_t seed, const HastySppmParams* params, HastySppm** out);
+void hasty_sppm_destroy(HastySppm* job);
+/* One pass into device films (3*W*H floats each). */
+int hasty_sppm_pass_device(HastySppm* job, uint32_t pass, float* d_color, float* d_normal, float* d_albedo);
+/* One pass into host films. */
+int hasty_sppm_pass(HastySppm* job, uint32_t pass, float* color, float* normal, float* albedo);
+int hasty_sppm_info(const HastySppm* job, HastySppmInfo* info);
+/* PMRenderJobPixel planes (PhotonMapping.h:12-31), W*H floats each: radius,
+ * N, Ne, flux r/g/b, LSum r/g/b, Nd (int32 bits). */
+int hasty_sppm_pixels(HastySppm* job, float* out_10_planes);
+/* The radius image (W*H floats, row-major). */
+int hasty_sppm_radii(HastySppm* job, float* out);
+/* Photons of the last pass in emission order, 10 floats each: position,
+ * dot(n_g, w_i), w_i, flux (Photon, PhotonMapping.cpp:11-23). */
+int hasty_sppm_photons(HastySppm* job, float* out, uint64_t max_photons);
+
 const char* hasty_last_error(void);
 
 #ifdef __cplusplus

@@ -390,6 +390,29 @@ int oracle_sppm_initial_radii(void* h, float* out) {
   memcpy(out, r.data(), sizeof(float) * r.size());
   return 0;
 }
+// the radius image the next pass starts from
+void oracle_sppm_radii(void* h, float* out) {
+  SppmJob* job = (SppmJob*)h;
+  if (!job->radii.empty()) memcpy(out, job->radii.data(), sizeof(float) * job->radii.size());
+}
+float oracle_get_close(const float* img, int W, int H, int i, int j) {
+  std::vector<float> v(img, img + (size_t)W * H);
+  return getCloseNonNegative(v, W, H, i, j);
+}
+// radiusNeighbors over `n` points (3 floats each) with the given cell size:
+// writes the indices in query order, returns how many (at most maxOut).
+int64_t oracle_radius_neighbors(const float* pts, uint32_t n, float cellSize, const float* c, float r, uint32_t* out,
+                                int64_t maxOut) {
+  PhotonMap map;
+  map.clear(cellSize);
+  for (uint32_t k = 0; k < n; ++k) map.points.push_back(Vec3f(pts[3 * k], pts[3 * k + 1], pts[3 * k + 2]));
+  map.build();
+  std::vector<uint32_t> found;
+  map.radiusNeighbors(Vec3f(c[0], c[1], c[2]), r, found);
+  int64_t m = std::min<int64_t>((int64_t)found.size(), maxOut);
+  for (int64_t k = 0; k < m; ++k) out[k] = found[k];
+  return (int64_t)found.size();
+}
 float oracle_cell_size(const float* radii, uint32_t W, uint32_t H) {
   std::vector<float> r(radii, radii + (size_t)W * H);
   return computeCellSize(r, W, H);

@@ -65,6 +65,13 @@ def lib():
         L.oracle_sppm_info.argtypes = [P, np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")]
         L.oracle_sppm_pixels.argtypes = [P, fp]
         L.oracle_sppm_initial_radii.argtypes = [P, fp]
+        L.oracle_sppm_radii.argtypes = [P, fp]
+        L.oracle_get_close.argtypes = [fp, C.c_int, C.c_int, C.c_int, C.c_int]
+        L.oracle_get_close.restype = C.c_float
+        L.oracle_radius_neighbors.argtypes = [fp, C.c_uint32, C.c_float, fp, C.c_float,
+                                              np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS"),
+                                              C.c_int64]
+        L.oracle_radius_neighbors.restype = C.c_int64
         L.oracle_cell_size.argtypes = [fp, C.c_uint32, C.c_uint32]
         L.oracle_cell_size.restype = C.c_float
         L.oracle_trace_photons.argtypes = [P, C.c_uint32, C.c_uint32, C.c_uint32, fp, C.c_int64]
@@ -261,10 +268,28 @@ class OracleSppm:
         lib().oracle_sppm_initial_radii(self.h, out)
         return out
 
+    def radii(self, w, h):
+        out = np.zeros((h, w), np.float32)
+        lib().oracle_sppm_radii(self.h, out)
+        return out
+
     def trace_photons(self, k, first, count, max_out=1 << 22):
         out = np.zeros((max_out, 11), np.float32)
         n = lib().oracle_trace_photons(self.h, k, first, count, out, max_out)
         return out[:n]
+
+
+def get_close_non_negative(img, i, j):
+    img = f32(img)
+    h, w = img.shape
+    return float(lib().oracle_get_close(img, w, h, i, j))
+
+
+def radius_neighbors(points, cell_size, center, r):
+    pts = f32(points).reshape(-1, 3)
+    out = np.zeros(max(1, len(pts)), np.uint32)
+    n = lib().oracle_radius_neighbors(pts, len(pts), cell_size, f32(center), r, out, len(out))
+    return out[:n]
 
 
 def cell_size(radii):
