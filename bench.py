@@ -20,7 +20,15 @@ records each ray moves, times the rays one launch processes; divided by the
 launch's average duration from HIP events on the library's stream.
 
 cpu_baseline: the oracle (the project's C++ restatement of the reference path
-tracer) timed on the host on 8 passes (8 spp) of the same 1024x1024 workload.
+tracer) timed on the host on 48 passes (48 spp) of the same 1024x1024 workload.
+
+roofline.traffic: HBM bytes per pt_extend launch measured by rocprofv3
+FETCH_SIZE and WRITE_SIZE runs, scaled by the factor the same counters read
+for a plain copy of known size (--copy-calibration); read from the committed
+summary in profiles/ when present.
+
+Other workloads (not the headline line): --workload sppm times photon-mapping
+passes on the caustic box (prism + 2M-triangle torus, 1920x1080).
 """
 from __future__ import annotations
 
@@ -36,6 +44,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "pt_extend_ggx_grid_1024_traffic.md")
 STATE_BYTES_PER_RAY = 52  # queue index 4 + ray origin/direction 32 + hit record 16
 
 
@@ -95,6 +104,107 @@ def cpu_baseline(job_path, seed, threads, spp):
     }, cnt
 
 
+def measured_traffic():
+    """HBM bytes per pt_extend launch from the committed rocprofv3 summary
+    (line 'traffic_bytes_per_launch: N'), or None."""
+    try:
+        with open(TRAFFIC_SUMMARY) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return float(line.split(":", 1)[1])
+    except OSError:
+        return None
+    return None
+
+
+def copy_calibration(mib):
+    """A plain device copy of `mib` MiB, five times (the known-size kernel the
+    FETCH_SIZE / WRITE_SIZE counters are scaled by)."""
+    import torch
+
+    n = mib * (1 << 20) // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    for _ in range(5):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    log("copy calibration: 5 copies of %d bytes" % (n * 4))
+
+
+def bench_sppm(args):
+    """Photon-mapping passes on the caustic box (dispersive prism, GGX gold
+    torus of n_major x n_minor x 2 triangles, 1% ceiling light) at
+    1920x1080; reference parameters unless overridden."""
+    import hastyrenderer_amd as ha
+    from hastyrenderer_amd import scenes
+
+    scene_dir = args.scene_dir or os.path.join(tempfile.gettempdir(), "hasty_bench_caustic")
+    job_path = os.path.join(scene_dir, "renderJob.json")
+    if not os.path.exists(job_path):
+        log("generating caustic scene in %s" % scene_dir)
+        scenes.write_caustic_scene(scene_dir)
+    t0 = time.perf_counter()
+    scene = ha.Scene(job_path)
+    load_s = time.perf_counter() - t0
+    info = scene.info()
+    W, H = info["width"], info["height"]
+    log("scene loaded in %.2f s: %d triangles" % (load_s, info["num_triangles"]))
+    sp = ha.Sppm(scene, 3, alpha=args.sppm_alpha, store_fraction=args.sppm_store_fraction,
+                 emit_fraction=args.sppm_emit_fraction)
+    hip = Hip()
+    nbytes = W * H * 3 * 4
+    films = [hip.malloc(nbytes) for _ in range(3)]
+    k = 0
+    for _ in range(args.warmup):
+        sp.pass_device(k, *films)
+        k += 1
+    scene.reset_stats()
+    scene.set_profiling(events=True, counters=False)
+    hip.sync()
+    t_start = time.perf_counter()
+    emitted = stored = 0
+    for _ in range(args.steps):
+        t = time.perf_counter()
+        sp.pass_device(k, *films)
+        i = sp.info()
+        emitted += i["emitted"]
+        stored += i["stored"]
+        log("pass %d: %.3f s, emitted %d stored %d cell %.4g" % (k, time.perf_counter() - t, i["emitted"],
+                                                              i["stored"], i["cell_size"]))
+        k += 1
+    hip.sync()
+    elapsed = time.perf_counter() - t_start
+    kstats = scene.kernel_stats()
+    scene.set_profiling(events=False, counters=False)
+    line = {
+        "metric": "photon-mapping passes/s at 1920x1080 (caustic box)",
+        "value": args.steps / elapsed,
+        "unit": "passes/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "sppm_caustic_box", "width": W, "height": H, "triangles": info["num_triangles"],
+                   "alpha": args.sppm_alpha, "store_fraction": args.sppm_store_fraction,
+                   "emit_fraction": args.sppm_emit_fraction, "parallelism": "single-gpu"},
+        "photons_emitted_per_s": emitted / elapsed,
+        "photons_stored_per_pass": stored / max(1, args.steps),
+        "load_s": load_s,
+        "kernel_ms_per_pass": {k2: round(v[1] / args.steps, 3) for k2, v in kstats.items()},
+    }
+    for f in films:
+        hip.free(f)
+    sp.close()
+    scene.close()
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -104,7 +214,13 @@ def main():
     ap.add_argument("--scene-dir", default=None)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
-    ap.add_argument("--cpu-spp", type=int, default=8)
+    ap.add_argument("--cpu-spp", type=int, default=48)
+    ap.add_argument("--workload", choices=("ggx", "sppm"), default="ggx")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="also run a plain copy of MIB MiB (counter calibration)")
+    ap.add_argument("--sppm-alpha", type=float, default=0.4)
+    ap.add_argument("--sppm-store-fraction", type=float, default=0.5)
+    ap.add_argument("--sppm-emit-fraction", type=float, default=10.0)
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -112,6 +228,12 @@ def main():
               "(multi-GPU sharding is a design note, not a measured configuration); run with --gpus 1.",
               file=sys.stderr)
         sys.exit(2)
+
+    if args.workload == "sppm":
+        bench_sppm(args)
+        if args.copy_calibration:
+            copy_calibration(args.copy_calibration)
+        return
 
     import hastyrenderer_amd as ha
     from hastyrenderer_amd import scenes
@@ -190,7 +312,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": measured_traffic(),
             "kernel": "pt_extend",
             "bytes_per_ray_canonical": round(b_ray, 1),
             "rays_per_launch": round(rays_per_launch, 1),
@@ -231,6 +353,8 @@ def main():
     for f in films:
         hip.free(f)
     scene.close()
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
     print(json.dumps(line), flush=True)
 
 

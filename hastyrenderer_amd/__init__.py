@@ -206,8 +206,9 @@ class Scene:
             setattr(s, k, int(v) if k != "roulette_q" else float(v))
         _err(lib().hasty_scene_set_settings(self._h, C.byref(s)))
 
-    def set_launch(self, samples_per_batch=0, trace_blocks=0, shade_blocks=0):
-        _err(lib().hasty_set_launch(self._h, samples_per_batch, trace_blocks, shade_blocks))
+    def set_launch(self, path_slots=0, trace_blocks=0, shade_blocks=0):
+        """Path slots in flight and grid sizes (0 keeps the current value)."""
+        _err(lib().hasty_set_launch(self._h, path_slots, trace_blocks, shade_blocks))
 
     def render_pt(self, seed: int, first_sample: int, num_samples: int, films=None):
         """Adds samples [first, first+n) into host films; returns (color, normal, albedo) sums."""

@@ -133,7 +133,9 @@ struct HastyScene {
   hasty::PathBuffers pb{};
   std::vector<void*> pathAllocs;
   uint32_t pathCapacity = 0;
-  uint32_t samplesPerBatch = 0;  // 0 = automatic
+  uint32_t slotsWanted = 0;  // 0 = automatic
+  uint32_t* dTileOrder = nullptr;
+  uint32_t tileW = 0, tileH = 0;
   hasty::LaunchConfig launch;
   float2* dOffsets = nullptr;
   uint32_t offsetsCapacity = 0;
@@ -178,6 +180,7 @@ struct HastyScene {
     freePath();
     for (void* p : sceneAllocs) (void)hipFree(p);
     if (dOffsets) (void)hipFree(dOffsets);
+    if (dTileOrder) (void)hipFree(dTileOrder);
     if (dFilm) (void)hipFree(dFilm);
     if (traceOverflow) (void)hipFree(traceOverflow);
     if (stream) (void)hipStreamDestroy(stream);
@@ -193,14 +196,25 @@ struct HastyScene {
     dev.height = st.height;
   }
 
-  void ensurePathBuffers(uint32_t K) {
-    uint32_t Pp = hasty::paddedPixels(dev.width, dev.height);
-    uint64_t need = (uint64_t)K * Pp;
-    if (need > 0xFFFFFFFFull / 2) throw std::runtime_error("too many paths in flight");
+  // Path slots in flight (regenerated as paths end) and the results ring.
+  uint32_t slotCount() const {
+    uint64_t P = (uint64_t)dev.width * dev.height;
+    uint64_t want = slotsWanted ? slotsWanted : (1ull << 22);
+    return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(want, std::max<uint64_t>(P, 1ull << 16)));
+  }
+  uint32_t ringPathsWanted() const {
+    uint64_t P = (uint64_t)dev.width * dev.height;
+    uint64_t r = std::max<uint64_t>(P, (uint64_t)(1u << 26) / P * P);  // about 64M paths, whole samples
+    return (uint32_t)std::min<uint64_t>(r, 0x7FFFFFFFull / P * P);
+  }
+
+  void ensurePathBuffers() {
+    uint32_t need = slotCount();
+    uint32_t ring = ringPathsWanted();
     uint32_t traceThreads = launch.traceBlocks * hasty::kBlock;
-    if (need <= pathCapacity && pb.traceThreads == traceThreads) return;
+    if (need == pathCapacity && ring == pb.ringPaths && pb.traceThreads == traceThreads) return;
     freePath();
-    size_t n = (size_t)need;
+    size_t n = need;
     pb.rayO = allocPath<float4>(n);
     pb.rayD = allocPath<float4>(n);
     pb.hit = allocPath<float4>(n);
@@ -220,34 +234,44 @@ struct HastyScene {
     pb.qA = allocPath<uint32_t>(n);
     pb.qB = allocPath<uint32_t>(n);
     pb.qS = allocPath<uint32_t>(n);
+    pb.qR = allocPath<uint32_t>(n);
+    pb.qI = allocPath<uint32_t>(n);
     pb.counters = allocPath<uint32_t>(hasty::kNumCounters);
     pb.stats = allocPath<unsigned long long>(hasty::kNumStats);
     pb.stackOverflow = allocPath<uint2>((size_t)traceThreads * hasty::kStackOvf);
+    pb.ringL = allocPath<float4>(ring);
+    pb.ringN = allocPath<float4>(ring);
+    pb.ringA = allocPath<float4>(ring);
+    pb.ringPaths = ring;
     pb.traceThreads = traceThreads;
-    pb.capacity = (uint32_t)need;
-    pathCapacity = (uint32_t)need;
+    pb.capacity = need;
+    pathCapacity = need;
     HIPCHECK(hipMemset(pb.stats, 0, sizeof(unsigned long long) * hasty::kNumStats));
   }
 
-  uint32_t chooseK(uint32_t nSamples) const {
-    uint32_t Pp = hasty::paddedPixels(dev.width, dev.height);
-    uint32_t K = samplesPerBatch;
-    if (K == 0) {
-      const uint32_t target = 1u << 21;  // paths in flight per batch
-      K = std::max<uint32_t>(1, target / std::max<uint32_t>(1, Pp));
-    }
-    return std::max<uint32_t>(1, std::min(K, nSamples));
+  void ensureTileOrder() {
+    if (tileW == dev.width && tileH == dev.height && dTileOrder) return;
+    if (dTileOrder) (void)hipFree(dTileOrder);
+    std::vector<uint32_t> t((size_t)dev.width * dev.height);
+    hasty::tileOrder(dev.width, dev.height, t.data());
+    HIPCHECK(hipMalloc(&dTileOrder, sizeof(uint32_t) * t.size()));
+    HIPCHECK(hipMemcpy(dTileOrder, t.data(), sizeof(uint32_t) * t.size(), hipMemcpyHostToDevice));
+    tileW = dev.width;
+    tileH = dev.height;
   }
 
   void renderDevice(uint32_t seed, uint32_t first, uint32_t nSamples, float* dC, float* dN, float* dA) {
     if (dev.width == 0 || dev.height == 0) throw std::runtime_error("empty film");
     dev.seed = seed;
-    uint32_t K = chooseK(nSamples);
-    ensurePathBuffers(K);
-    if (offsetsCapacity < K) {
+    ensurePathBuffers();
+    ensureTileOrder();
+    uint64_t P = (uint64_t)dev.width * dev.height;
+    // one call covers at most 2^31 paths
+    uint32_t perCall = (uint32_t)std::max<uint64_t>(1, 0x7FFFFFFFull / P);
+    if (offsetsCapacity < std::min(nSamples, perCall)) {
       if (dOffsets) (void)hipFree(dOffsets);
-      HIPCHECK(hipMalloc(&dOffsets, sizeof(float2) * K));
-      offsetsCapacity = K;
+      offsetsCapacity = std::min(nSamples, perCall);
+      HIPCHECK(hipMalloc(&dOffsets, sizeof(float2) * offsetsCapacity));
     }
     // Halton offsets of samples first .. first+n-1 (point index k+1)
     Halton h2(2), h3(3);
@@ -258,11 +282,11 @@ struct HastyScene {
     }
     launch.countStats = counting;
     if (profiling && !timer) timer = std::make_unique<EventTimer>();
-    for (uint32_t done = 0; done < nSamples; done += K) {
-      uint32_t k = std::min(K, nSamples - done);
+    for (uint32_t done = 0; done < nSamples; done += perCall) {
+      uint32_t k = std::min(perCall, nSamples - done);
       HIPCHECK(hipMemcpyAsync(dOffsets, offs.data() + done, sizeof(float2) * k, hipMemcpyHostToDevice, stream));
-      hasty::launchPTBatch(dev, pb, first + done, k, dOffsets, dC, dN, dA, launch, stream,
-                           profiling ? timer.get() : nullptr);
+      hasty::renderPT(dev, pb, first + done, k, dOffsets, dTileOrder, dC, dN, dA, launch, stream,
+                      profiling ? timer.get() : nullptr);
       HIPCHECK(hipGetLastError());
     }
     HIPCHECK(hipStreamSynchronize(stream));
@@ -576,9 +600,9 @@ int hasty_counters(HastyScene* s, uint64_t* out) {
   return 0;
 }
 
-int hasty_set_launch(HastyScene* s, uint32_t spb, uint32_t traceBlocks, uint32_t shadeBlocks) {
+int hasty_set_launch(HastyScene* s, uint32_t slots, uint32_t traceBlocks, uint32_t shadeBlocks) {
   if (!s) return 1;
-  if (spb) s->samplesPerBatch = spb;
+  if (slots) s->slotsWanted = slots;
   if (traceBlocks) s->launch.traceBlocks = traceBlocks;
   if (shadeBlocks) s->launch.shadeBlocks = shadeBlocks;
   return 0;

@@ -7,15 +7,18 @@
 
 namespace hasty {
 
-// Structure-of-arrays path state for `capacity` paths (K samples x padded
-// pixels).  Each array is indexed by path slot; queues hold slot indices.
+// Structure-of-arrays path state for `capacity` path slots.  Slots are
+// recycled: when a path ends its radiance goes to the results ring and the
+// slot starts the next path of the render (regeneration), so every stage
+// keeps a full queue until the last paths of a render drain.  Queues hold
+// slot indices.
 struct PathBuffers {
   float4* rayO;   // origin; w unused
   float4* rayD;   // direction; w = pdf of the BSDF sample that produced it
   float4* hit;    // t, u, v, triangle (bits)
   float4* beta;   // path throughput
   float4* L;      // accumulated radiance
-  uint4* state;   // x rng draws, y flags, z previous triangle, w unused
+  uint4* state;   // x rng draws, y flags, z previous triangle, w path id
   uint4* media;   // 2 x uint4 per slot (8 entries)
   float4* auxN;   // depth-0 shading normal
   float4* auxA;   // depth-0 albedo
@@ -26,27 +29,55 @@ struct PathBuffers {
   float4* shThr;  // local throughput (Beer / (1 - q))
   float4* shBeta; // path throughput at the vertex
   float4* shT;    // transmittance of the light-ray media copy
-  uint32_t* qA;
-  uint32_t* qB;
-  uint32_t* qS;
-  uint32_t* counters;  // [0] qA size [1] qB size [2] shadow size
+  uint32_t* qA;   // extension queue of this bounce
+  uint32_t* qB;   // extension queue of the next bounce
+  uint32_t* qS;   // shadow queue
+  uint32_t* qR;   // paths that ended at this bounce
+  uint32_t* qI;   // idle slots (stack)
+  uint32_t* counters;         // see CounterId
   unsigned long long* stats;  // see StatId
   uint2* stackOverflow;
+  float4* ringL;  // finished paths: radiance, normal, albedo by
+  float4* ringN;  // (sample % ringSamples) * pixels + pixel
+  float4* ringA;
+  uint32_t ringPaths;
   uint32_t capacity;
   uint32_t traceThreads;  // threads of the traversal grids (overflow slices)
+};
+
+enum CounterId {
+  kCntA = 0,      // live paths in qA
+  kCntB = 1,      // paths queued for the next bounce
+  kCntS = 2,      // shadow rays
+  kCntR = 3,      // paths that ended at this bounce
+  kCntNext = 4,   // next path id to start
+  kCntLimit = 5,  // path ids below this may start (results ring space)
+  kCntIdle = 6,   // idle slots in qI
+  kNumCounters = 8
+};
+
+// One render call: samples first .. first + N / pixels - 1 of every pixel.
+// Path id = (sample - first) * pixels + rank, pixel = tileOrder[rank].
+struct PassInfo {
+  uint32_t first;
+  uint32_t pixels;
+  uint32_t paths;        // N
+  uint32_t ringSamples;  // ringPaths / pixels
+  const uint32_t* tileOrder;
+  const float2* offsets;  // per sample of the call
 };
 
 constexpr int kStackLds = 16;
 constexpr int kStackOvf = 64;
 constexpr int kBlock = 256;
 
-constexpr int kNumCounters = 4;
 // 64-bit device statistics (filled when LaunchConfig::countStats is set).
 enum StatId { kStatExtRays = 0, kStatShadowRays, kStatNodes, kStatTris, kStatMediaOverflow, kNumStats = 8 };
 
 struct LaunchConfig {
   uint32_t traceBlocks = 2048;  // persistent-style grid of the traversal kernels
   uint32_t shadeBlocks = 4096;
+  uint32_t chunk = 8;           // bounces between host checks of the live count
   bool countStats = false;
 };
 
@@ -57,6 +88,7 @@ enum KernelId {
   kKConnect,
   kKAccumulate,
   kKMisc,
+  kKRetire,
   kKSppmRadii,
   kKSppmCellSize,
   kKSppmPhotons,
@@ -73,19 +105,20 @@ struct KernelTimer {
   virtual ~KernelTimer() {}
 };
 
-// Renders samples [firstSample, firstSample + K) of every pixel and adds them,
-// in sample order, into the device films (row-major RGB float32).
-void launchPTBatch(const hdev::DevScene& scene, const PathBuffers& pb, uint32_t firstSample, uint32_t K,
-                   const float2* dOffsets, float* dColor, float* dNormal, float* dAlbedo, const LaunchConfig& cfg,
-                   hipStream_t stream, KernelTimer* timer);
+// Renders samples [first, first + n) of every pixel and adds them, in sample
+// order, into the device films (row-major RGB float32).  dOffsets holds the n
+// pixel offsets, dTileOrder the pixels in 8x8-tile order.  Synchronises with
+// the stream once per cfg.chunk bounces.
+void renderPT(const hdev::DevScene& scene, const PathBuffers& pb, uint32_t first, uint32_t n, const float2* dOffsets,
+              const uint32_t* dTileOrder, float* dColor, float* dNormal, float* dAlbedo, const LaunchConfig& cfg,
+              hipStream_t stream, KernelTimer* timer);
+
+// Pixels of a width x height film in 8x8-tile order (row-major inside a tile).
+void tileOrder(uint32_t width, uint32_t height, uint32_t* out);
 
 // Closest hit for n rays (6 floats each): tri, t, u, v.
 void launchTraceClosest(const hdev::DevScene& scene, const float* dRays, uint32_t n, int32_t* dTri, float* dTuv,
                         uint2* dOverflow, uint32_t overflowThreads, hipStream_t stream);
 
-// Slots per sample: pixels padded to whole 8x8 tiles.
-__host__ __device__ inline uint32_t paddedPixels(uint32_t width, uint32_t height) {
-  return ((width + 7) / 8) * ((height + 7) / 8) * 64;
-}
 
 }  // namespace hasty

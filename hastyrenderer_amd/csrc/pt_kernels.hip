@@ -1,5 +1,12 @@
-// Wavefront path tracer for gfx950: generate -> (extend -> shade -> connect)
-// x max_depth -> accumulate, over K samples of every pixel at once.
+// Wavefront path tracer for gfx950 with path regeneration:
+//   fill -> repeat { extend -> shade -> connect -> retire } -> accumulate.
+// A render call covers n samples of every pixel (n * pixels paths, numbered
+// sample-major, pixels in 8x8-tile order).  A fixed pool of path slots runs
+// them: `retire` writes a finished path's radiance into a results ring and
+// starts the next path id in the same slot, so the queues stay full until
+// the last paths drain.  Every path's random stream is keyed by (pixel,
+// sample), so the image does not depend on which slot ran which path, and
+// `accumulate` adds each pixel's samples in sample order.
 //
 // The recursion of estimateRadiance (core/src/PathTracing.cpp:21-95) becomes
 // one extension ray (BSDF strategy) plus one shadow ray (light strategy) per
@@ -15,6 +22,11 @@
 // in LDS, and the block reserves its range with one atomic.
 #include "pt_kernels.h"
 
+#include <algorithm>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
 namespace hasty {
 using namespace hdev;
 
@@ -25,6 +37,7 @@ const char* kernelName(int id) {
     case kKShade: return "pt_shade";
     case kKConnect: return "pt_connect";
     case kKAccumulate: return "pt_accumulate";
+    case kKRetire: return "pt_retire";
     case kKSppmRadii: return "sppm_init_radii";
     case kKSppmCellSize: return "sppm_cell_size";
     case kKSppmPhotons: return "sppm_photons";
@@ -35,19 +48,6 @@ const char* kernelName(int id) {
 }
 
 namespace {
-
-// 8x8 pixel tiles: 64 consecutive slots (one wavefront) cover one tile.
-__device__ inline bool slotPixel(uint32_t t, uint32_t width, uint32_t height, uint32_t* i, uint32_t* j) {
-  uint32_t tilesX = (width + 7) / 8;
-  uint32_t tile = t >> 6, w = t & 63u;
-  *i = (tile % tilesX) * 8 + (w & 7u);
-  *j = (tile / tilesX) * 8 + (w >> 3);
-  return *i < width && *j < height;
-}
-__device__ inline uint32_t pixelSlot(uint32_t i, uint32_t j, uint32_t width) {
-  uint32_t tilesX = (width + 7) / 8;
-  return (((j >> 3) * tilesX + (i >> 3)) << 6) | ((j & 7u) << 3) | (i & 7u);
-}
 
 // Block-wide append: returns this lane's index in the output queue (valid
 // only where `pred`).  Must be reached by every thread of the block.
@@ -105,46 +105,80 @@ __device__ inline uint32_t packFlags(int depth, int wavelength, const Media& md)
          (md.overflow ? (1u << 16) : 0u);
 }
 
-// ------------------------------------------------------------- generate ----
-__global__ void __launch_bounds__(kBlock) kGenerate(DevScene s, PathBuffers pb, uint32_t firstSample, uint32_t K,
-                                                    const float2* offsets) {
-  __shared__ uint32_t ldsWave[kBlock / 64];
-  __shared__ uint32_t ldsBase;
-  uint32_t Pp = paddedPixels(s.width, s.height);
-  uint32_t total = K * Pp;
+// ------------------------------------------------------------ path start ----
+// Camera ray of path `pid` (Camera.cpp:41-81 with the sample's offset,
+// RenderJob.h:53-60) into `slot`.
+__device__ inline void startPath(const DevScene& s, const PathBuffers& pb, const PassInfo& pi, uint32_t slot,
+                                 uint32_t pid) {
+  uint32_t k = pid / pi.pixels;
+  uint32_t pixel = pi.tileOrder[pid - k * pi.pixels];
+  uint32_t i = pixel % s.width, j = pixel / s.width;
+  hrng::Stream rng = hrng::make_stream(s.seed, pixel, pi.first + k, hrng::kDomainPath);
+  float2 off = pi.offsets[k];
+  V3 o, d;
+  cameraRay(s, rng, (float)i + off.x, (float)j + off.y, &o, &d);
+  pb.rayO[slot] = f4(o, 0.0f);
+  pb.rayD[slot] = f4(d, 0.0f);
+  pb.beta[slot] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
+  pb.L[slot] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  pb.auxN[slot] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  pb.auxA[slot] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  pb.state[slot] = make_uint4(rng.counter, 0u, 0xFFFFFFFFu, pid);
+}
+
+// Every slot idle, no path started.
+__global__ void __launch_bounds__(kBlock) kInitRender(PathBuffers pb, uint32_t limit) {
   uint32_t stride = gridDim.x * blockDim.x;
-  for (uint32_t base = blockIdx.x * blockDim.x; base < total; base += stride) {
-    uint32_t slot = base + threadIdx.x;
-    bool live = false;
-    if (slot < total) {
-      uint32_t k = slot / Pp, t = slot % Pp;
-      uint32_t i, j;
-      if (slotPixel(t, s.width, s.height, &i, &j)) {
-        live = true;
-        uint32_t sample = firstSample + k;
-        hrng::Stream rng = hrng::make_stream(s.seed, j * s.width + i, sample, hrng::kDomainPath);
-        float2 off = offsets[k];
-        V3 o, d;
-        cameraRay(s, rng, (float)i + off.x, (float)j + off.y, &o, &d);
-        pb.rayO[slot] = f4(o, 0.0f);
-        pb.rayD[slot] = f4(d, 0.0f);
-        pb.beta[slot] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
-        pb.L[slot] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        pb.auxN[slot] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        pb.auxA[slot] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        pb.state[slot] = make_uint4(rng.counter, 0u, 0xFFFFFFFFu, sample);
-      }
-    }
-    uint32_t idx = blockAppend(&pb.counters[0], live, ldsWave, &ldsBase);
-    if (live) pb.qA[idx] = slot;
+  for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < pb.capacity; k += stride) pb.qI[k] = k;
+  if (blockIdx.x == 0 && threadIdx.x < kNumCounters) {
+    uint32_t v = 0u;
+    if (threadIdx.x == kCntLimit) v = limit;
+    if (threadIdx.x == kCntIdle) v = pb.capacity;
+    pb.counters[threadIdx.x] = v;
   }
 }
 
+// Idle slots take the next path ids (up to the limit) and join qA; the
+// counters are advanced by kFillCommit.
+__device__ inline uint32_t fillCount(const uint32_t* c) {
+  uint32_t idle = c[kCntIdle], next = c[kCntNext], lim = c[kCntLimit];
+  uint32_t avail = lim > next ? lim - next : 0u;
+  return idle < avail ? idle : avail;
+}
+__global__ void __launch_bounds__(kBlock) kFill(DevScene s, PathBuffers pb, PassInfo pi) {
+  __shared__ uint32_t ldsWave[kBlock / 64];
+  __shared__ uint32_t ldsBase;
+  uint32_t m = fillCount(pb.counters);
+  uint32_t idle = pb.counters[kCntIdle], next = pb.counters[kCntNext];
+  uint32_t stride = gridDim.x * blockDim.x;
+  for (uint32_t base = blockIdx.x * blockDim.x; base < m; base += stride) {
+    uint32_t q = base + threadIdx.x;
+    bool live = q < m;
+    uint32_t slot = 0;
+    if (live) {
+      slot = pb.qI[idle - 1 - q];
+      startPath(s, pb, pi, slot, next + q);
+    }
+    uint32_t idx = blockAppend(&pb.counters[kCntA], live, ldsWave, &ldsBase);
+    if (live) pb.qA[idx] = slot;
+  }
+}
+__global__ void kFillCommit(uint32_t* c) {
+  if (threadIdx.x == 0) {
+    uint32_t m = fillCount(c);
+    c[kCntIdle] -= m;
+    c[kCntNext] += m;
+  }
+}
+__global__ void kSetLimit(uint32_t* c, uint32_t limit) {
+  if (threadIdx.x == 0) c[kCntLimit] = limit;
+}
+
 // --------------------------------------------------------------- extend ----
-__global__ void __launch_bounds__(kBlock) kExtend(DevScene s, PathBuffers pb, const uint32_t* queue,
-                                                  const uint32_t* count, int countStats) {
+__global__ void __launch_bounds__(kBlock) kExtend(DevScene s, PathBuffers pb, int countStats) {
   __shared__ uint2 stack[kStackLds * kBlock];
-  uint32_t n = *count;
+  uint32_t n = pb.counters[kCntA];
+  const uint32_t* queue = pb.qA;
   uint32_t gtid = xcdBlock() * blockDim.x + threadIdx.x;
   uint32_t stride = gridDim.x * blockDim.x;
   uint2* ovf = pb.stackOverflow + (size_t)gtid * kStackOvf;
@@ -165,20 +199,19 @@ __global__ void __launch_bounds__(kBlock) kExtend(DevScene s, PathBuffers pb, co
 }
 
 // ---------------------------------------------------------------- shade ----
-__global__ void __launch_bounds__(kBlock) kShade(DevScene s, PathBuffers pb, const uint32_t* qIn,
-                                                 const uint32_t* countIn, uint32_t* qOut, uint32_t* countOut,
-                                                 uint32_t* countShadow) {
+__global__ void __launch_bounds__(kBlock) kShade(DevScene s, PathBuffers pb, PassInfo pi) {
   __shared__ uint32_t ldsWave[kBlock / 64];
   __shared__ uint32_t ldsBase;
-  uint32_t n = *countIn;
+  uint32_t n = pb.counters[kCntA];
   uint32_t stride = gridDim.x * blockDim.x;
   const bool twoStrategies = s.hasLight && s.useMIS;
   for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += stride) {
     uint32_t q = base + threadIdx.x;
-    bool pushNext = false, pushShadow = false;
+    bool pushNext = false, pushShadow = false, live = false;
     uint32_t slot = 0;
     if (q < n) {
-      slot = qIn[q];
+      live = true;
+      slot = pb.qA[q];
       uint4 st = pb.state[slot];
       int depth = (int)(st.y & 0xFFu);
       int wavelength = (int)((st.y >> 8) & 0x3u) - 1;
@@ -194,10 +227,9 @@ __global__ void __launch_bounds__(kBlock) kShade(DevScene s, PathBuffers pb, con
       V3 L = f4xyz(L4);
       hrng::Stream rng;
       {
-        uint32_t Pp = paddedPixels(s.width, s.height);
-        uint32_t i, j;
-        slotPixel(slot % Pp, s.width, s.height, &i, &j);
-        rng = hrng::make_stream(s.seed, j * s.width + i, st.w, hrng::kDomainPath);
+        uint32_t k = st.w / pi.pixels;
+        uint32_t pixel = pi.tileOrder[st.w - k * pi.pixels];
+        rng = hrng::make_stream(s.seed, pixel, pi.first + k, hrng::kDomainPath);
         rng.counter = st.x;
       }
       bool alive = true;
@@ -335,17 +367,20 @@ __global__ void __launch_bounds__(kBlock) kShade(DevScene s, PathBuffers pb, con
       }
       pb.L[slot] = f4(L, 0.0f);
     }
-    uint32_t idxN = blockAppend(countOut, pushNext, ldsWave, &ldsBase);
-    if (pushNext) qOut[idxN] = slot;
-    uint32_t idxS = blockAppend(countShadow, pushShadow, ldsWave, &ldsBase);
+    uint32_t idxN = blockAppend(&pb.counters[kCntB], pushNext, ldsWave, &ldsBase);
+    if (pushNext) pb.qB[idxN] = slot;
+    uint32_t idxS = blockAppend(&pb.counters[kCntS], pushShadow, ldsWave, &ldsBase);
     if (pushShadow) pb.qS[idxS] = slot;
+    bool retire = live && !pushNext;  // the path ends here (after its shadow ray, if any)
+    uint32_t idxR = blockAppend(&pb.counters[kCntR], retire, ldsWave, &ldsBase);
+    if (retire) pb.qR[idxR] = slot;
   }
 }
 
 // -------------------------------------------------------------- connect ----
-__global__ void __launch_bounds__(kBlock) kConnect(DevScene s, PathBuffers pb, const uint32_t* count, int countStats) {
+__global__ void __launch_bounds__(kBlock) kConnect(DevScene s, PathBuffers pb, int countStats) {
   __shared__ uint2 stack[kStackLds * kBlock];
-  uint32_t n = *count;
+  uint32_t n = pb.counters[kCntS];
   uint32_t gtid = xcdBlock() * blockDim.x + threadIdx.x;
   uint32_t stride = gridDim.x * blockDim.x;
   uint2* ovf = pb.stackOverflow + (size_t)gtid * kStackOvf;
@@ -425,23 +460,49 @@ __global__ void __launch_bounds__(kBlock) kConnect(DevScene s, PathBuffers pb, c
   }
 }
 
+// --------------------------------------------------------------- retire ----
+// Finished paths: results into the ring; the slot becomes idle and the next
+// kFill gives it a new path id.
+__global__ void __launch_bounds__(kBlock) kRetire(PathBuffers pb, PassInfo pi) {
+  __shared__ uint32_t ldsWave[kBlock / 64];
+  __shared__ uint32_t ldsBase;
+  uint32_t n = pb.counters[kCntR];
+  uint32_t stride = gridDim.x * blockDim.x;
+  for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += stride) {
+    uint32_t q = base + threadIdx.x;
+    bool live = q < n;
+    uint32_t slot = 0;
+    if (live) {
+      slot = pb.qR[q];
+      uint32_t pid = pb.state[slot].w;
+      uint32_t k = pid / pi.pixels;
+      uint32_t pixel = pi.tileOrder[pid - k * pi.pixels];
+      size_t ri = (size_t)(k % pi.ringSamples) * pi.pixels + pixel;
+      pb.ringL[ri] = pb.L[slot];
+      pb.ringN[ri] = pb.auxN[slot];
+      pb.ringA[ri] = pb.auxA[slot];
+    }
+    uint32_t idx = blockAppend(&pb.counters[kCntIdle], live, ldsWave, &ldsBase);
+    if (live) pb.qI[idx] = slot;
+  }
+}
+
 // ----------------------------------------------------------- accumulate ----
-__global__ void __launch_bounds__(kBlock) kAccumulate(DevScene s, PathBuffers pb, uint32_t K, float* color,
-                                                      float* normal, float* albedo) {
-  uint32_t P = s.width * s.height;
-  uint32_t Pp = paddedPixels(s.width, s.height);
+// Samples [k0, k1) of the call, from the ring, added pixel by pixel in
+// sample order; NaN samples are skipped (PathTracing.cpp:132-135).
+__global__ void __launch_bounds__(kBlock) kAccumulate(PathBuffers pb, PassInfo pi, uint32_t k0, uint32_t k1,
+                                                      float* color, float* normal, float* albedo) {
+  uint32_t P = pi.pixels;
   uint32_t stride = gridDim.x * blockDim.x;
   for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < P; p += stride) {
-    uint32_t i = p % s.width, j = p / s.width;
-    uint32_t t = pixelSlot(i, j, s.width);
     float c0 = color[p * 3], c1 = color[p * 3 + 1], c2 = color[p * 3 + 2];
     float n0 = normal[p * 3], n1 = normal[p * 3 + 1], n2 = normal[p * 3 + 2];
     float a0 = albedo[p * 3], a1 = albedo[p * 3 + 1], a2 = albedo[p * 3 + 2];
-    for (uint32_t k = 0; k < K; ++k) {
-      uint32_t slot = k * Pp + t;
-      float4 L = pb.L[slot];
-      if (L.x != L.x || L.y != L.y || L.z != L.z) continue;  // PathTracing.cpp:132-135
-      float4 an = pb.auxN[slot], aa = pb.auxA[slot];
+    for (uint32_t k = k0; k < k1; ++k) {
+      size_t ri = (size_t)(k % pi.ringSamples) * P + p;
+      float4 L = pb.ringL[ri];
+      if (L.x != L.x || L.y != L.y || L.z != L.z) continue;
+      float4 an = pb.ringN[ri], aa = pb.ringA[ri];
       c0 += L.x, c1 += L.y, c2 += L.z;
       n0 += an.x, n1 += an.y, n2 += an.z;
       a0 += aa.x, a1 += aa.y, a2 += aa.z;
@@ -452,18 +513,16 @@ __global__ void __launch_bounds__(kBlock) kAccumulate(DevScene s, PathBuffers pb
   }
 }
 
-__global__ void kResetCounters(uint32_t* c) {
-  if (threadIdx.x < kNumCounters) c[threadIdx.x] = 0u;
-}
-// Next bounce: qB becomes the input queue.
+// Next bounce: qB becomes the input queue (the host swaps the pointers).
 __global__ void kAdvance(uint32_t* c) {
   if (threadIdx.x == 0) {
-    c[0] = c[1];
-    c[1] = 0u;
-    c[2] = 0u;
+    c[kCntA] = c[kCntB];
+    c[kCntB] = 0u;
+    c[kCntS] = 0u;
+    c[kCntR] = 0u;
   }
 }
-// --------------------------------------------------------- trace only ----
+
 __global__ void __launch_bounds__(kBlock) kTraceRays(DevScene s, const float* rays, uint32_t n, int32_t* tri,
                                                      float* tuv, uint2* overflow) {
   __shared__ uint2 stack[kStackLds * kBlock];
@@ -483,50 +542,98 @@ __global__ void __launch_bounds__(kBlock) kTraceRays(DevScene s, const float* ra
 
 }  // namespace
 
-void launchPTBatch(const DevScene& s, const PathBuffers& pbIn, uint32_t firstSample, uint32_t K, const float2* dOffsets,
-                   float* dColor, float* dNormal, float* dAlbedo, const LaunchConfig& cfg, hipStream_t stream,
-                   KernelTimer* timer) {
+void tileOrder(uint32_t width, uint32_t height, uint32_t* out) {
+  uint32_t n = 0;
+  for (uint32_t ty = 0; ty < height; ty += 8)
+    for (uint32_t tx = 0; tx < width; tx += 8)
+      for (uint32_t y = ty; y < ty + 8 && y < height; ++y)
+        for (uint32_t x = tx; x < tx + 8 && x < width; ++x) out[n++] = y * width + x;
+}
+
+void renderPT(const DevScene& s, const PathBuffers& pbIn, uint32_t first, uint32_t n, const float2* dOffsets,
+              const uint32_t* dTileOrder, float* dColor, float* dNormal, float* dAlbedo, const LaunchConfig& cfg,
+              hipStream_t stream, KernelTimer* timer) {
   PathBuffers pb = pbIn;  // queue pointers swap per bounce
-  uint32_t Pp = paddedPixels(s.width, s.height);
-  uint32_t total = K * Pp;
   auto t0 = [&](int k) {
     if (timer) timer->begin(k, stream);
   };
   auto t1 = [&](int k) {
     if (timer) timer->end(k, stream);
   };
-  uint32_t genBlocks = (total + kBlock - 1) / kBlock;
-  if (genBlocks > cfg.shadeBlocks) genBlocks = cfg.shadeBlocks;
+  auto check = [](hipError_t e) {
+    if (e != hipSuccess) throw std::runtime_error(std::string("renderPT: ") + hipGetErrorString(e));
+  };
+  PassInfo pi;
+  pi.first = first;
+  pi.pixels = s.width * s.height;
+  pi.paths = n * pi.pixels;  // the caller keeps this below 2^31
+  pi.ringSamples = pb.ringPaths / pi.pixels;
+  if (pi.ringSamples > n) pi.ringSamples = n;
+  pi.tileOrder = dTileOrder;
+  pi.offsets = dOffsets;
+  if (pi.ringSamples == 0) throw std::runtime_error("renderPT: results ring smaller than one sample");
+  const uint32_t shadeBlocks = cfg.shadeBlocks;
+  const int countStats = cfg.countStats ? 1 : 0;
+  uint32_t limit = (uint32_t)std::min<uint64_t>(pi.paths, (uint64_t)pi.ringSamples * pi.pixels);
   t0(kKMisc);
-  kResetCounters<<<1, 64, 0, stream>>>(pb.counters);
+  kInitRender<<<256, kBlock, 0, stream>>>(pb, limit);
   t1(kKMisc);
-  t0(kKGenerate);
-  kGenerate<<<genBlocks, kBlock, 0, stream>>>(s, pb, firstSample, K, dOffsets);
-  t1(kKGenerate);
-  int countStats = cfg.countStats ? 1 : 0;
-  for (int d = 0; d < s.maxDepth; ++d) {
-    t0(kKExtend);
-    kExtend<<<cfg.traceBlocks, kBlock, 0, stream>>>(s, pb, pb.qA, &pb.counters[0], countStats);
-    t1(kKExtend);
-    t0(kKShade);
-    kShade<<<genBlocks, kBlock, 0, stream>>>(s, pb, pb.qA, &pb.counters[0], pb.qB, &pb.counters[1], &pb.counters[2]);
-    t1(kKShade);
-    t0(kKConnect);
-    kConnect<<<cfg.traceBlocks, kBlock, 0, stream>>>(s, pb, &pb.counters[2], countStats);
-    t1(kKConnect);
-    t0(kKMisc);
-    kAdvance<<<1, 64, 0, stream>>>(pb.counters);
-    t1(kKMisc);
-    // swap queues: the next extension reads what shade wrote
-    uint32_t* tmp = pb.qA;
-    pb.qA = pb.qB;
-    pb.qB = tmp;
+  // Paths started by the end of chunk c are finished `lag` chunks later: a
+  // path has at most maxDepth vertices, one per bounce, and starts at the
+  // beginning of a bounce.
+  const uint32_t chunk = cfg.chunk ? cfg.chunk : 8;
+  const uint32_t lag = ((uint32_t)s.maxDepth + chunk - 1) / chunk;
+  std::vector<uint32_t> startedBy;
+  uint32_t accumulated = 0;  // samples already added to the films
+  uint32_t accBlocks = (pi.pixels + kBlock - 1) / kBlock;
+  for (uint32_t c = 0;; ++c) {
+    for (uint32_t it = 0; it < chunk; ++it) {
+      t0(kKGenerate);
+      kFill<<<shadeBlocks, kBlock, 0, stream>>>(s, pb, pi);
+      t1(kKGenerate);
+      t0(kKMisc);
+      kFillCommit<<<1, 64, 0, stream>>>(pb.counters);
+      t1(kKMisc);
+      t0(kKExtend);
+      kExtend<<<cfg.traceBlocks, kBlock, 0, stream>>>(s, pb, countStats);
+      t1(kKExtend);
+      t0(kKShade);
+      kShade<<<shadeBlocks, kBlock, 0, stream>>>(s, pb, pi);
+      t1(kKShade);
+      t0(kKConnect);
+      kConnect<<<cfg.traceBlocks, kBlock, 0, stream>>>(s, pb, countStats);
+      t1(kKConnect);
+      t0(kKRetire);
+      kRetire<<<shadeBlocks, kBlock, 0, stream>>>(pb, pi);
+      t1(kKRetire);
+      t0(kKMisc);
+      kAdvance<<<1, 64, 0, stream>>>(pb.counters);
+      t1(kKMisc);
+      uint32_t* tmp = pb.qA;
+      pb.qA = pb.qB;
+      pb.qB = tmp;
+    }
+    uint32_t cnt[kNumCounters];
+    check(hipMemcpyAsync(cnt, pb.counters, sizeof(cnt), hipMemcpyDeviceToHost, stream));
+    check(hipStreamSynchronize(stream));
+    uint32_t live = cnt[kCntA], next = cnt[kCntNext];
+    startedBy.push_back(next);
+    uint32_t done = live == 0 ? next : (c >= lag ? startedBy[c - lag] : 0u);
+    uint32_t complete = done / pi.pixels;  // samples whose paths have all finished
+    if (complete > accumulated) {
+      t0(kKAccumulate);
+      kAccumulate<<<accBlocks, kBlock, 0, stream>>>(pb, pi, accumulated, complete, dColor, dNormal, dAlbedo);
+      t1(kKAccumulate);
+      accumulated = complete;
+      uint32_t newLimit =
+          (uint32_t)std::min<uint64_t>(pi.paths, (uint64_t)(accumulated + pi.ringSamples) * pi.pixels);
+      t0(kKMisc);
+      kSetLimit<<<1, 64, 0, stream>>>(pb.counters, newLimit);
+      t1(kKMisc);
+    }
+    if (live == 0 && next >= pi.paths) break;
   }
-  uint32_t P = s.width * s.height;
-  uint32_t accBlocks = (P + kBlock - 1) / kBlock;
-  t0(kKAccumulate);
-  kAccumulate<<<accBlocks, kBlock, 0, stream>>>(s, pb, K, dColor, dNormal, dAlbedo);
-  t1(kKAccumulate);
+  if (accumulated != n) throw std::runtime_error("renderPT: not every sample was accumulated");
 }
 
 void launchTraceClosest(const DevScene& s, const float* dRays, uint32_t n, int32_t* dTri, float* dTuv, uint2* dOverflow,

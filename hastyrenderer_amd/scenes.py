@@ -288,6 +288,107 @@ def mirror_caustic_radiance(points, rho=0.5, half=0.1, emission=10.0 * math.pi, 
     return rho / math.pi * out
 
 
+_CUBE_CORNERS = np.array([[x, y, z] for x in (-0.5, 0.5) for y in (-0.5, 0.5) for z in (-0.5, 0.5)])
+_CUBE_TRIS = np.array([(q[0], q[1], q[2]) for q in ((0, 1, 3, 2), (4, 6, 7, 5), (0, 4, 5, 1), (2, 3, 7, 6),
+                                                   (0, 2, 6, 4), (1, 5, 7, 3))] +
+                      [(q[0], q[2], q[3]) for q in ((0, 1, 3, 2), (4, 6, 7, 5), (0, 4, 5, 1), (2, 3, 7, 6),
+                                                   (0, 2, 6, 4), (1, 5, 7, 3))])
+
+
+def _random_rotations(rng, n):
+    """Uniform random rotations (normalised Gaussian quaternions) as (n, 3, 3)."""
+    q = rng.normal(size=(n, 4))
+    q /= np.linalg.norm(q, axis=1, keepdims=True)
+    w, x, y, z = q.T
+    return np.stack([
+        np.stack([1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w)], -1),
+        np.stack([2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w)], -1),
+        np.stack([2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)], -1),
+    ], 1)
+
+
+def write_cuboid_scene(folder, n_cuboids, width, height, spp, seed, max_depth=8):
+    """One batching job: `n_cuboids` random cuboids (positions uniform in the
+    unit cube, edge lengths uniform in [0.002, 0.05], uniform random
+    rotations) split evenly at random over three material classes drawn once
+    per job -- Lambertian, GGX conductor with alpha in U[0.05, 0.8], smooth
+    glass with IOR in U[1.3, 1.8] -- merged into one shape per class (12
+    triangles per cuboid), one 0.3 x 0.3 area light above the cube, dim grey
+    background, pinhole camera looking at the cube."""
+    os.makedirs(folder, exist_ok=True)
+    rng = np.random.default_rng(seed)
+    alpha = rng.uniform(0.05, 0.8)
+    ior = rng.uniform(1.3, 1.8)
+    pos = rng.uniform(0.0, 1.0, (n_cuboids, 3))
+    size = rng.uniform(0.002, 0.05, (n_cuboids, 3))
+    rot = _random_rotations(rng, n_cuboids)
+    cls = rng.integers(0, 3, n_cuboids)
+    # corners (n, 8, 3) = rot @ (corner * size) + pos
+    local = _CUBE_CORNERS[None, :, :] * size[:, None, :]
+    verts = np.einsum("nij,nkj->nki", rot, local) + pos[:, None, :]
+    albedo = rng.uniform(0.3, 0.9, 3)
+    f0 = rng.uniform(0.5, 1.0, 3)
+    ns = 1000.0 * (1.0 - math.sqrt(alpha)) ** 2  # roughness = 1 - sqrt(Ns / 1000), alpha = roughness^2
+    mtl = [
+        _mtl("lambert", kd=tuple(albedo)),
+        _mtl("ggx", kd=tuple(f0), ka=(1.0, 1.0, 1.0), ns=ns),
+        _mtl("glass", kd=(1.0, 1.0, 1.0), ns=1000.0, ni=ior, d=0.0),
+        _mtl("light", ke=(20.0, 20.0, 20.0)),
+    ]
+    lines = ["mtllib scene.mtl"]
+    vbase = 0
+    for c, name in enumerate(("lambert", "ggx", "glass")):
+        sel = np.nonzero(cls == c)[0]
+        if len(sel) == 0:
+            continue
+        lines.append("o %s" % name)
+        lines.append("s off")
+        lines.append("usemtl %s" % name)
+        v = verts[sel].reshape(-1, 3).astype(np.float32)
+        lines.append("\n".join("v %.9g %.9g %.9g" % tuple(r) for r in v.tolist()))
+        f = (_CUBE_TRIS[None, :, :] + 8 * np.arange(len(sel))[:, None, None]).reshape(-1, 3) + vbase + 1
+        lines.append("\n".join("f %d %d %d" % tuple(r) for r in f.tolist()))
+        vbase += len(v)
+    lines.append("o light")
+    lines.append("s off")
+    lines.append("usemtl light")
+    lines.append("v 0.35 1.6 0.35\nv 0.35 1.6 0.65\nv 0.65 1.6 0.65\nv 0.65 1.6 0.35")
+    lines.append("f %d %d %d\nf %d %d %d" % (vbase + 1, vbase + 2, vbase + 3, vbase + 1, vbase + 3, vbase + 4))
+    with open(os.path.join(folder, "scene.obj"), "w") as fh:
+        fh.write("\n".join(lines) + "\n")
+    with open(os.path.join(folder, "scene.mtl"), "w") as fh:
+        fh.write("\n".join(mtl))
+    scene = {
+        "camera": {"position": [0.5, 0.9, 2.6], "look_at": [0.5, 0.5, 0.5], "fov": 40.0},
+        "world": {"background_color": [0.05, 0.05, 0.05]},
+        "mesh": "scene.obj",
+    }
+    with open(os.path.join(folder, "scene.json"), "w") as fh:
+        json.dump(scene, fh, indent=2)
+    _write_job(folder, {"width": int(width), "height": int(height), "num_samples": int(spp),
+                        "max_depth": max_depth, "roulette_start_depth": 3, "roulette_q": 0.3, "use_mis": True,
+                        "num_threads": 1})
+    return os.path.join(folder, "renderJob.json")
+
+
+def batch_jobs(folder, n_jobs=32, seed=2, max_pixels=1024 * 1024, min_pixels=128 * 128, spp_range=(16, 256),
+               cuboid_range=(1000, 160000)):
+    """The batching workload: `n_jobs` cuboid scenes with resolution (pixel
+    count) log-uniform in [min_pixels, max_pixels] (square films), spp
+    log-uniform in spp_range and cuboid count log-uniform in cuboid_range.
+    Returns the job paths."""
+    rng = np.random.default_rng(seed)
+    paths = []
+    for j in range(n_jobs):
+        px = math.exp(rng.uniform(math.log(min_pixels), math.log(max_pixels)))
+        side = max(8, int(round(math.sqrt(px))))
+        spp = int(round(math.exp(rng.uniform(math.log(spp_range[0]), math.log(spp_range[1])))))
+        n = int(round(math.exp(rng.uniform(math.log(cuboid_range[0]), math.log(cuboid_range[1])))))
+        paths.append(write_cuboid_scene(os.path.join(folder, "job%02d" % j), n, side, side, spp,
+                                        seed=int(rng.integers(1 << 31))))
+    return paths
+
+
 def icosphere(subdiv):
     """Unit icosphere: returns (vertices float64 (N,3), triangles int (M,3))."""
     t = (1.0 + math.sqrt(5.0)) / 2.0

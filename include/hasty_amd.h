@@ -108,9 +108,11 @@ int hasty_reset_stats(HastyScene* scene);
  * out[3] triangle tests, out[4] paths whose media stack overflowed. */
 int hasty_counters(HastyScene* scene, uint64_t* out5);
 
-/* Launch-shape knobs: paths in flight per batch (samples per pixel rendered
- * together) and grid sizes.  0 keeps the current value. */
-int hasty_set_launch(HastyScene* scene, uint32_t samples_per_batch, uint32_t trace_blocks, uint32_t shade_blocks);
+/* Launch-shape knobs: path slots in flight (paths are regenerated into
+ * freed slots; default 2^22, at least 2^16 unless the film is smaller) and
+ * the grid sizes of the traversal and shading kernels.  0 keeps the current
+ * value.  Images do not depend on these. */
+int hasty_set_launch(HastyScene* scene, uint32_t path_slots, uint32_t trace_blocks, uint32_t shade_blocks);
 
 /* ---- progressive photon mapping (caustic photon map + MIS path tracing) ----
  * Replaces renderPhotonThread (core/src/PhotonMapping.cpp:814-933, declared

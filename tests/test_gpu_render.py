@@ -130,12 +130,15 @@ def test_caustic_scene_parity(caustic_small_job):
 
 
 def test_deterministic_across_launch_shapes(cornell_job):
+    """Bit-identical films whatever the slot pool (heavy regeneration with
+    1000 slots) and grid sizes: each path's stream is keyed by (pixel,
+    sample), not by the slot that runs it."""
     scene = ha.Scene(cornell_job)
     scene.set_settings(width=128, height=96)
     ref = scene.render_pt(seed=3, first_sample=5, num_samples=6)
-    scene.set_launch(samples_per_batch=1, trace_blocks=64, shade_blocks=32)
+    scene.set_launch(path_slots=1000, trace_blocks=64, shade_blocks=32)
     alt = scene.render_pt(seed=3, first_sample=5, num_samples=6)
-    scene.set_launch(samples_per_batch=4, trace_blocks=1024, shade_blocks=4096)
+    scene.set_launch(path_slots=1 << 24, trace_blocks=1024, shade_blocks=4096)
     alt2 = scene.render_pt(seed=3, first_sample=5, num_samples=6)
     for x, y, z in zip(ref, alt, alt2):
         assert (x.view(np.uint32) == y.view(np.uint32)).all()

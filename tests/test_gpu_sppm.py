@@ -121,3 +121,19 @@ def test_render_photon_thread_mirror(caustic_small_job):
     assert rel_ok(bufs[0].getReadBuffer().data, rc).mean() >= 0.999
     est = ha.computeEstimate([bufs[0]])
     assert np.allclose(est, bufs[0].getReadBuffer().data, rtol=1e-6)
+
+
+def test_sppm_bitexact_run_to_run(caustic_small_job):
+    """Two jobs with the same seed give bit-identical films and statistics:
+    every pixel sums its photons in a fixed order (cells x -> y -> z, photons
+    in emission order), so no atomics or fixed point are needed."""
+    outs = []
+    for _ in range(2):
+        sc, g = _gpu_sppm(caustic_small_job, seed=6)
+        for k in range(2):
+            c, n, a = g.pass_(k)
+        outs.append((c.copy(), g.pixels()["flux"].copy(), g.radii().copy()))
+        g.close()
+        sc.close()
+    for x, y in zip(outs[0], outs[1]):
+        np.testing.assert_array_equal(x, y)
