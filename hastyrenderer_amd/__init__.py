@@ -41,6 +41,11 @@ EXPORTS = (
     "hasty_reset_stats",
     "hasty_counters",
     "hasty_set_launch",
+    "hasty_get_launch",
+    "hasty_set_launch_ex",
+    "hasty_render_batch_device",
+    "hasty_autotune_batch",
+    "hasty_autotune",
     "hasty_sppm_create",
     "hasty_sppm_destroy",
     "hasty_sppm_pass_device",
@@ -81,6 +86,28 @@ class SceneInfoC(C.Structure):
         ("bvh_nodes", C.c_uint32),
         ("bvh_depth", C.c_int32),
         ("device_bytes", C.c_uint64),
+    ]
+
+
+class LaunchC(C.Structure):
+    _fields_ = [
+        ("path_slots", C.c_uint32),
+        ("trace_blocks", C.c_uint32),
+        ("shade_blocks", C.c_uint32),
+        ("chunk", C.c_uint32),
+        ("use_graph", C.c_int32),
+    ]
+
+
+class BatchJobC(C.Structure):
+    _fields_ = [
+        ("scene", C.c_void_p),
+        ("seed", C.c_uint32),
+        ("first_sample", C.c_uint32),
+        ("num_samples", C.c_uint32),
+        ("d_color", C.c_void_p),
+        ("d_normal", C.c_void_p),
+        ("d_albedo", C.c_void_p),
     ]
 
 
@@ -131,6 +158,11 @@ def lib():
     L.hasty_reset_stats.argtypes = [P]
     L.hasty_counters.argtypes = [P, C.POINTER(C.c_uint64)]
     L.hasty_set_launch.argtypes = [P, C.c_uint32, C.c_uint32, C.c_uint32]
+    L.hasty_get_launch.argtypes = [P, C.POINTER(LaunchC)]
+    L.hasty_set_launch_ex.argtypes = [P, C.POINTER(LaunchC)]
+    L.hasty_render_batch_device.argtypes = [C.POINTER(BatchJobC), C.c_uint32]
+    L.hasty_autotune_batch.argtypes = [C.POINTER(BatchJobC), C.c_uint32, C.c_uint32, C.POINTER(LaunchC)]
+    L.hasty_autotune.argtypes = [P, C.c_uint32, C.c_uint32, C.POINTER(LaunchC)]
     L.hasty_sppm_create.argtypes = [P, C.c_uint32, C.POINTER(SppmParamsC), C.POINTER(P)]
     L.hasty_sppm_destroy.argtypes = [P]
     L.hasty_sppm_destroy.restype = None
@@ -226,6 +258,24 @@ class Scene:
         _err(lib().hasty_render_pt_device(self._h, seed, first_sample, num_samples, C.c_void_p(d_color),
                                           C.c_void_p(d_normal), C.c_void_p(d_albedo)))
 
+    def get_launch(self) -> dict:
+        lc = LaunchC()
+        _err(lib().hasty_get_launch(self._h, C.byref(lc)))
+        return {k: int(getattr(lc, k)) for k, _ in LaunchC._fields_}
+
+    def set_launch_ex(self, **kw):
+        """path_slots, trace_blocks, shade_blocks, chunk, use_graph."""
+        cur = self.get_launch()
+        cur.update(kw)
+        lc = LaunchC(*(int(cur[k]) for k, _ in LaunchC._fields_))
+        _err(lib().hasty_set_launch_ex(self._h, C.byref(lc)))
+
+    def autotune(self, seed: int = 0, trial_samples: int = 1) -> dict:
+        """Times short renders over the launch knobs and keeps the fastest."""
+        lc = LaunchC()
+        _err(lib().hasty_autotune(self._h, seed, trial_samples, C.byref(lc)))
+        return {k: int(getattr(lc, k)) for k, _ in LaunchC._fields_}
+
     def trace_closest(self, rays: np.ndarray):
         rays = np.ascontiguousarray(rays, dtype=np.float32).reshape(-1, 6)
         n = rays.shape[0]
@@ -254,6 +304,77 @@ class Scene:
         _err(lib().hasty_counters(self._h, out))
         keys = ("extension_rays", "shadow_rays", "node_visits", "triangle_tests", "media_overflow")
         return dict(zip(keys, (int(x) for x in out)))
+
+
+class DeviceFilms:
+    """Color, normal and albedo films in device memory (hipMalloc), zeroed."""
+
+    _hip = None
+
+    def __init__(self, width: int, height: int):
+        if DeviceFilms._hip is None:
+            h = C.CDLL("libamdhip64.so")
+            h.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+            h.hipMemset.argtypes = [C.c_void_p, C.c_int, C.c_size_t]
+            h.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+            h.hipFree.argtypes = [C.c_void_p]
+            DeviceFilms._hip = h
+        self.width, self.height = width, height
+        self.nbytes = width * height * 3 * 4
+        self.ptrs = []
+        for _ in range(3):
+            p = C.c_void_p()
+            if DeviceFilms._hip.hipMalloc(C.byref(p), self.nbytes) != 0:
+                raise HastyError("hipMalloc failed")
+            self.ptrs.append(p.value)
+        self.zero()
+
+    def zero(self):
+        for p in self.ptrs:
+            if DeviceFilms._hip.hipMemset(C.c_void_p(p), 0, self.nbytes) != 0:
+                raise HastyError("hipMemset failed")
+
+    def download(self):
+        out = []
+        for p in self.ptrs:
+            a = np.zeros((self.height, self.width, 3), np.float32)
+            if DeviceFilms._hip.hipMemcpy(a.ctypes.data, C.c_void_p(p), self.nbytes, 2) != 0:  # device to host
+                raise HastyError("hipMemcpy failed")
+            out.append(a)
+        return tuple(out)
+
+    def close(self):
+        for p in self.ptrs:
+            DeviceFilms._hip.hipFree(C.c_void_p(p))
+        self.ptrs = []
+
+    def __del__(self):
+        if self.ptrs and DeviceFilms._hip is not None:
+            self.close()
+
+
+def _batch_array(jobs):
+    arr = (BatchJobC * len(jobs))()
+    for i, (scene, seed, first, n, films) in enumerate(jobs):
+        d = films.ptrs if films is not None else (None, None, None)
+        arr[i] = BatchJobC(scene.handle, seed, first, n, *d)
+    return arr
+
+
+def render_batch(jobs):
+    """jobs: list of (Scene, seed, first_sample, num_samples, DeviceFilms).
+    Renders all of them through the first scene's path pool in one batch."""
+    arr = _batch_array(jobs)
+    _err(lib().hasty_render_batch_device(arr, len(jobs)))
+
+
+def autotune_batch(jobs, trial_samples: int = 1) -> dict:
+    """jobs: list of (Scene, seed, first_sample, num_samples, films-or-None).
+    Applies the fastest launch configuration to jobs[0]'s scene."""
+    arr = _batch_array(jobs)
+    lc = LaunchC()
+    _err(lib().hasty_autotune_batch(arr, len(jobs), trial_samples, C.byref(lc)))
+    return {k: int(getattr(lc, k)) for k, _ in LaunchC._fields_}
 
 
 class Sppm:

@@ -7,6 +7,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <chrono>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -120,6 +121,16 @@ struct EventTimer : hasty::KernelTimer {
 
 }  // namespace
 
+struct HastyScene;
+struct BatchItem {
+  HastyScene* scene;
+  uint32_t seed, first, n;
+  float* color;
+  float* normal;
+  float* albedo;
+};
+static void renderBatch(HastyScene* owner, const BatchItem* items, uint32_t nItems);
+
 struct HastyScene {
   hasty::Job job;
   hasty::Bvh4 bvh;
@@ -137,8 +148,6 @@ struct HastyScene {
   uint32_t* dTileOrder = nullptr;
   uint32_t tileW = 0, tileH = 0;
   hasty::LaunchConfig launch;
-  float2* dOffsets = nullptr;
-  uint32_t offsetsCapacity = 0;
 
   // films for the host-pointer entry point
   float* dFilm = nullptr;
@@ -179,7 +188,6 @@ struct HastyScene {
     if (timer) timer->collect();
     freePath();
     for (void* p : sceneAllocs) (void)hipFree(p);
-    if (dOffsets) (void)hipFree(dOffsets);
     if (dTileOrder) (void)hipFree(dTileOrder);
     if (dFilm) (void)hipFree(dFilm);
     if (traceOverflow) (void)hipFree(traceOverflow);
@@ -252,53 +260,101 @@ struct HastyScene {
   void ensureTileOrder() {
     if (tileW == dev.width && tileH == dev.height && dTileOrder) return;
     if (dTileOrder) (void)hipFree(dTileOrder);
-    std::vector<uint32_t> t((size_t)dev.width * dev.height);
-    hasty::tileOrder(dev.width, dev.height, t.data());
+    size_t P = (size_t)dev.width * dev.height;
+    std::vector<uint32_t> t(2 * P);
+    hasty::tileOrder(dev.width, dev.height, t.data(), t.data() + P);
     HIPCHECK(hipMalloc(&dTileOrder, sizeof(uint32_t) * t.size()));
     HIPCHECK(hipMemcpy(dTileOrder, t.data(), sizeof(uint32_t) * t.size(), hipMemcpyHostToDevice));
     tileW = dev.width;
     tileH = dev.height;
   }
+  const uint32_t* rankOf() const { return dTileOrder + (size_t)dev.width * dev.height; }
 
   void renderDevice(uint32_t seed, uint32_t first, uint32_t nSamples, float* dC, float* dN, float* dA) {
-    if (dev.width == 0 || dev.height == 0) throw std::runtime_error("empty film");
-    dev.seed = seed;
-    ensurePathBuffers();
-    ensureTileOrder();
-    uint64_t P = (uint64_t)dev.width * dev.height;
-    // one call covers at most 2^31 paths
-    uint32_t perCall = (uint32_t)std::max<uint64_t>(1, 0x7FFFFFFFull / P);
-    if (offsetsCapacity < std::min(nSamples, perCall)) {
-      if (dOffsets) (void)hipFree(dOffsets);
-      offsetsCapacity = std::min(nSamples, perCall);
-      HIPCHECK(hipMalloc(&dOffsets, sizeof(float2) * offsetsCapacity));
-    }
-    // Halton offsets of samples first .. first+n-1 (point index k+1)
-    Halton h2(2), h3(3);
-    std::vector<float2> offs(nSamples);
-    for (uint32_t k = 0; k < first + nSamples; ++k) {
-      float a = h2.next(), b = h3.next();
-      if (k >= first) offs[k - first] = make_float2(a, b);
-    }
-    launch.countStats = counting;
-    if (profiling && !timer) timer = std::make_unique<EventTimer>();
-    for (uint32_t done = 0; done < nSamples; done += perCall) {
-      uint32_t k = std::min(perCall, nSamples - done);
-      HIPCHECK(hipMemcpyAsync(dOffsets, offs.data() + done, sizeof(float2) * k, hipMemcpyHostToDevice, stream));
-      hasty::renderPT(dev, pb, first + done, k, dOffsets, dTileOrder, dC, dN, dA, launch, stream,
-                      profiling ? timer.get() : nullptr);
-      HIPCHECK(hipGetLastError());
-    }
-    HIPCHECK(hipStreamSynchronize(stream));
-    if (counting) {
-      unsigned long long s[hasty::kNumStats];
-      HIPCHECK(hipMemcpy(s, pb.stats, sizeof(s), hipMemcpyDeviceToHost));
-      HIPCHECK(hipMemset(pb.stats, 0, sizeof(s)));
-      for (int i = 0; i < hasty::kNumStats; ++i) stats[i] += s[i];
-    }
-    if (profiling) timer->collect();
+    BatchItem it{this, seed, first, nSamples, dC, dN, dA};
+    renderBatch(this, &it, 1);
   }
 };
+
+// Halton offsets of samples first .. first+n-1 (point index k+1).
+static std::vector<float2> haltonOffsets(uint32_t first, uint32_t n) {
+  Halton h2(2), h3(3);
+  std::vector<float2> offs(n);
+  for (uint32_t k = 0; k < first + n; ++k) {
+    float a = h2.next(), b = h3.next();
+    if (k >= first) offs[k - first] = make_float2(a, b);
+  }
+  return offs;
+}
+
+// Renders the items through the path pool and stream of `owner`.  Items are
+// grouped so that each renderJobs call stays below 2^31 paths; a single item
+// larger than that is split by samples.
+static void renderBatch(HastyScene* owner, const BatchItem* items, uint32_t nItems) {
+  owner->ensurePathBuffers();
+  std::vector<void*> tmp;
+  struct FreeAll {
+    std::vector<void*>& v;
+    ~FreeAll() {
+      for (void* p : v) (void)hipFree(p);
+    }
+  } freeAll{tmp};
+  std::vector<hasty::RenderJobDesc> group;
+  uint64_t groupPaths = 0;
+  const uint64_t maxPaths = 0x7FFFFFFFull;
+  owner->launch.countStats = owner->counting;
+  if (owner->profiling && !owner->timer) owner->timer = std::make_unique<EventTimer>();
+  hasty::KernelTimer* timer = owner->profiling ? owner->timer.get() : nullptr;
+  auto flush = [&]() {
+    if (group.empty()) return;
+    hasty::renderJobs(group, owner->pb, owner->launch, owner->stream, timer);
+    HIPCHECK(hipGetLastError());
+    group.clear();
+    groupPaths = 0;
+    for (void* p : tmp) (void)hipFree(p);
+    tmp.clear();
+  };
+  for (uint32_t i = 0; i < nItems; ++i) {
+    const BatchItem& it = items[i];
+    HastyScene* sc = it.scene;
+    if (sc->device != owner->device) throw std::runtime_error("batch jobs must live on one device");
+    if (sc->dev.width == 0 || sc->dev.height == 0) throw std::runtime_error("empty film");
+    sc->ensureTileOrder();
+    uint64_t P = (uint64_t)sc->dev.width * sc->dev.height;
+    uint32_t perCall = (uint32_t)std::max<uint64_t>(1, maxPaths / P);
+    for (uint32_t done = 0; done < it.n; done += perCall) {
+      uint32_t k = std::min(perCall, it.n - done);
+      if (groupPaths + (uint64_t)k * P > maxPaths) flush();
+      std::vector<float2> offs = haltonOffsets(it.first + done, k);
+      void* d = nullptr;
+      HIPCHECK(hipMalloc(&d, sizeof(float2) * std::max<uint32_t>(1, k)));
+      tmp.push_back(d);
+      HIPCHECK(hipMemcpy(d, offs.data(), sizeof(float2) * k, hipMemcpyHostToDevice));
+      hasty::RenderJobDesc jd;
+      jd.scene = sc->dev;
+      jd.scene.seed = it.seed;
+      jd.first = it.first + done;
+      jd.n = k;
+      jd.dOffsets = (const float2*)d;
+      jd.dTileOrder = sc->dTileOrder;
+      jd.dRankOf = sc->rankOf();
+      jd.dColor = it.color;
+      jd.dNormal = it.normal;
+      jd.dAlbedo = it.albedo;
+      group.push_back(jd);
+      groupPaths += (uint64_t)k * P;
+    }
+  }
+  flush();
+  HIPCHECK(hipStreamSynchronize(owner->stream));
+  if (owner->counting) {
+    unsigned long long s[hasty::kNumStats];
+    HIPCHECK(hipMemcpy(s, owner->pb.stats, sizeof(s), hipMemcpyDeviceToHost));
+    HIPCHECK(hipMemset(owner->pb.stats, 0, sizeof(s)));
+    for (int i = 0; i < hasty::kNumStats; ++i) owner->stats[i] += s[i];
+  }
+  if (owner->profiling) owner->timer->collect();
+}
 
 struct HastySppm {
   HastyScene* scene = nullptr;
@@ -606,6 +662,151 @@ int hasty_set_launch(HastyScene* s, uint32_t slots, uint32_t traceBlocks, uint32
   if (traceBlocks) s->launch.traceBlocks = traceBlocks;
   if (shadeBlocks) s->launch.shadeBlocks = shadeBlocks;
   return 0;
+}
+
+int hasty_get_launch(const HastyScene* s, HastyLaunch* out) {
+  if (!s || !out) {
+    g_lastError = "null argument";
+    return 1;
+  }
+  out->path_slots = s->slotCount();
+  out->trace_blocks = s->launch.traceBlocks;
+  out->shade_blocks = s->launch.shadeBlocks;
+  out->chunk = s->launch.chunk;
+  out->use_graph = s->launch.useGraph ? 1 : 0;
+  return 0;
+}
+
+int hasty_set_launch_ex(HastyScene* s, const HastyLaunch* l) {
+  if (!s || !l) {
+    g_lastError = "null argument";
+    return 1;
+  }
+  if (l->path_slots) s->slotsWanted = l->path_slots;
+  if (l->trace_blocks) s->launch.traceBlocks = l->trace_blocks;
+  if (l->shade_blocks) s->launch.shadeBlocks = l->shade_blocks;
+  if (l->chunk) s->launch.chunk = l->chunk;
+  s->launch.useGraph = l->use_graph != 0;
+  return 0;
+}
+
+static int batchItems(const HastyBatchJob* jobs, uint32_t n, std::vector<BatchItem>& items) {
+  if (!jobs || n == 0) {
+    g_lastError = "null argument";
+    return 1;
+  }
+  items.resize(n);
+  for (uint32_t i = 0; i < n; ++i) {
+    const HastyBatchJob& j = jobs[i];
+    if (!j.scene || !j.d_color || !j.d_normal || !j.d_albedo) {
+      g_lastError = "null argument";
+      return 1;
+    }
+    items[i] = BatchItem{j.scene, j.seed, j.first_sample, j.num_samples, j.d_color, j.d_normal, j.d_albedo};
+  }
+  return 0;
+}
+
+int hasty_render_batch_device(const HastyBatchJob* jobs, uint32_t n) {
+  std::vector<BatchItem> items;
+  if (int rc = batchItems(jobs, n, items)) return rc;
+  try {
+    HIPCHECK(hipSetDevice(jobs[0].scene->device));
+    renderBatch(jobs[0].scene, items.data(), n);
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e, 3);
+  }
+}
+
+// Autotuner: one pass of coordinate descent over the launch knobs, each
+// candidate timed on a short render of the real jobs (trial_samples samples
+// of each, into scratch films).  Images never depend on the knobs.
+static int autotuneItems(std::vector<BatchItem> items, uint32_t trialSamples, HastyLaunch* chosen) {
+  HastyScene* owner = items[0].scene;
+  std::vector<void*> films;
+  try {
+    HIPCHECK(hipSetDevice(owner->device));
+    for (BatchItem& it : items) {
+      size_t P = (size_t)it.scene->dev.width * it.scene->dev.height * 3;
+      void* d = nullptr;
+      HIPCHECK(hipMalloc(&d, 3 * P * sizeof(float)));
+      films.push_back(d);
+      it.color = (float*)d;
+      it.normal = it.color + P;
+      it.albedo = it.normal + P;
+      it.first = 0;
+      it.n = std::max<uint32_t>(1, trialSamples);
+    }
+    bool prof = owner->profiling, cnt = owner->counting;
+    owner->profiling = owner->counting = false;
+    auto timeIt = [&]() {
+      auto t = std::chrono::steady_clock::now();
+      renderBatch(owner, items.data(), (uint32_t)items.size());
+      return std::chrono::duration<double>(std::chrono::steady_clock::now() - t).count();
+    };
+    HastyLaunch best;
+    hasty_get_launch(owner, &best);
+    hasty_set_launch_ex(owner, &best);
+    timeIt();  // warm-up (allocations)
+    double bestT = timeIt();
+    const std::vector<uint32_t> slots = {1u << 20, 1u << 21, 1u << 22, 1u << 23};
+    const std::vector<uint32_t> trace = {1024, 2048, 4096};
+    const std::vector<uint32_t> shade = {2048, 4096, 8192};
+    const std::vector<uint32_t> chunks = {4, 8, 16};
+    auto tryKnob = [&](auto setter, const std::vector<uint32_t>& values) {
+      HastyLaunch base = best;
+      for (uint32_t v : values) {
+        HastyLaunch c = base;
+        setter(c, v);
+        hasty_set_launch_ex(owner, &c);
+        double t = timeIt();
+        if (t < bestT) {
+          bestT = t;
+          best = c;
+        }
+      }
+      hasty_set_launch_ex(owner, &best);
+    };
+    tryKnob([](HastyLaunch& c, uint32_t v) { c.path_slots = v; }, slots);
+    tryKnob([](HastyLaunch& c, uint32_t v) { c.trace_blocks = v; }, trace);
+    tryKnob([](HastyLaunch& c, uint32_t v) { c.shade_blocks = v; }, shade);
+    tryKnob([](HastyLaunch& c, uint32_t v) { c.chunk = v; }, chunks);
+    tryKnob([](HastyLaunch& c, uint32_t v) { c.use_graph = (int32_t)v; }, {0u, 1u});
+    owner->profiling = prof;
+    owner->counting = cnt;
+    hasty_get_launch(owner, &best);
+    if (chosen) *chosen = best;
+    for (void* d : films) (void)hipFree(d);
+    return 0;
+  } catch (const std::exception& e) {
+    for (void* d : films) (void)hipFree(d);
+    return fail(e, 3);
+  }
+}
+
+int hasty_autotune_batch(const HastyBatchJob* jobs, uint32_t n, uint32_t trialSamples, HastyLaunch* chosen) {
+  if (!jobs || n == 0) {
+    g_lastError = "null argument";
+    return 1;
+  }
+  std::vector<BatchItem> items(n);
+  for (uint32_t i = 0; i < n; ++i) {
+    if (!jobs[i].scene) {
+      g_lastError = "null argument";
+      return 1;
+    }
+    items[i] = BatchItem{jobs[i].scene, jobs[i].seed, 0, trialSamples, nullptr, nullptr, nullptr};
+  }
+  return autotuneItems(items, trialSamples, chosen);
+}
+
+int hasty_autotune(HastyScene* s, uint32_t seed, uint32_t trialSamples, HastyLaunch* chosen) {
+  if (!s) {
+    g_lastError = "null argument";
+    return 1;
+  }
+  return autotuneItems({BatchItem{s, seed, 0, trialSamples, nullptr, nullptr, nullptr}}, trialSamples, chosen);
 }
 
 int hasty_sppm_create(HastyScene* s, uint32_t seed, const HastySppmParams* params, HastySppm** out) {

@@ -5,6 +5,8 @@
 
 #include "pt_device.h"
 
+#include <vector>
+
 namespace hasty {
 
 // Structure-of-arrays path state for `capacity` path slots.  Slots are
@@ -37,8 +39,8 @@ struct PathBuffers {
   uint32_t* counters;         // see CounterId
   unsigned long long* stats;  // see StatId
   uint2* stackOverflow;
-  float4* ringL;  // finished paths: radiance, normal, albedo by
-  float4* ringN;  // (sample % ringSamples) * pixels + pixel
+  float4* ringL;  // finished paths: radiance, normal, albedo at
+  float4* ringN;  // ring slot (global path id % ringPaths)
   float4* ringA;
   uint32_t ringPaths;
   uint32_t capacity;
@@ -56,15 +58,30 @@ enum CounterId {
   kNumCounters = 8
 };
 
-// One render call: samples first .. first + N / pixels - 1 of every pixel.
-// Path id = (sample - first) * pixels + rank, pixel = tileOrder[rank].
+// One job of a render call: samples first .. first + paths / pixels - 1 of
+// every pixel of its scene.  Path ids are global over the jobs of a call:
+// job j owns [pidBase, pidBase + paths); inside a job, local id =
+// (sample - first) * pixels + rank with pixel = tileOrder[rank] (8x8 tiles).
+// Finished paths go to ring slot (global id % ringPaths).
 struct PassInfo {
   uint32_t first;
   uint32_t pixels;
-  uint32_t paths;        // N
-  uint32_t ringSamples;  // ringPaths / pixels
-  const uint32_t* tileOrder;
-  const float2* offsets;  // per sample of the call
+  uint32_t paths;
+  uint32_t pidBase;
+  const uint32_t* tileOrder;  // rank -> pixel
+  const uint32_t* rankOf;     // pixel -> rank
+  const float2* offsets;      // per sample of the call
+};
+
+// The jobs of a render call.  With one job its scene and pass travel in the
+// kernel arguments (`s`, `pi`); with several, kernels look them up by path id.
+struct JobSet {
+  hdev::DevScene s;
+  PassInfo pi;
+  const hdev::DevScene* scenes;
+  const PassInfo* passes;
+  uint32_t nJobs;
+  uint32_t paths;  // over all jobs
 };
 
 constexpr int kStackLds = 16;
@@ -78,6 +95,7 @@ struct LaunchConfig {
   uint32_t traceBlocks = 2048;  // persistent-style grid of the traversal kernels
   uint32_t shadeBlocks = 4096;
   uint32_t chunk = 8;           // bounces between host checks of the live count
+  bool useGraph = false;        // replay each chunk of bounces as one captured hipGraph
   bool countStats = false;
 };
 
@@ -89,6 +107,7 @@ enum KernelId {
   kKAccumulate,
   kKMisc,
   kKRetire,
+  kKGraph,
   kKSppmRadii,
   kKSppmCellSize,
   kKSppmPhotons,
@@ -105,16 +124,29 @@ struct KernelTimer {
   virtual ~KernelTimer() {}
 };
 
-// Renders samples [first, first + n) of every pixel and adds them, in sample
-// order, into the device films (row-major RGB float32).  dOffsets holds the n
-// pixel offsets, dTileOrder the pixels in 8x8-tile order.  Synchronises with
-// the stream once per cfg.chunk bounces.
-void renderPT(const hdev::DevScene& scene, const PathBuffers& pb, uint32_t first, uint32_t n, const float2* dOffsets,
-              const uint32_t* dTileOrder, float* dColor, float* dNormal, float* dAlbedo, const LaunchConfig& cfg,
-              hipStream_t stream, KernelTimer* timer);
+// One job of renderJobs: scene (seed set), samples [first, first + n), the n
+// pixel offsets, the tile order and its inverse, device films (sums, added to
+// in sample order).
+struct RenderJobDesc {
+  hdev::DevScene scene;
+  uint32_t first, n;
+  const float2* dOffsets;
+  const uint32_t* dTileOrder;
+  const uint32_t* dRankOf;
+  float* dColor;
+  float* dNormal;
+  float* dAlbedo;
+};
 
-// Pixels of a width x height film in 8x8-tile order (row-major inside a tile).
-void tileOrder(uint32_t width, uint32_t height, uint32_t* out);
+// Renders every job through one shared pool of path slots (paths of all jobs
+// in flight together; at most 2^31 paths per call).  Synchronises with the
+// stream once per cfg.chunk bounces.
+void renderJobs(const std::vector<RenderJobDesc>& jobs, const PathBuffers& pb, const LaunchConfig& cfg,
+                hipStream_t stream, KernelTimer* timer);
+
+// Pixels of a width x height film in 8x8-tile order (row-major inside a
+// tile), and the inverse map.
+void tileOrder(uint32_t width, uint32_t height, uint32_t* order, uint32_t* rankOf);
 
 // Closest hit for n rays (6 floats each): tri, t, u, v.
 void launchTraceClosest(const hdev::DevScene& scene, const float* dRays, uint32_t n, int32_t* dTri, float* dTuv,

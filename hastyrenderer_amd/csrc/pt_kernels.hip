@@ -38,6 +38,7 @@ const char* kernelName(int id) {
     case kKConnect: return "pt_connect";
     case kKAccumulate: return "pt_accumulate";
     case kKRetire: return "pt_retire";
+    case kKGraph: return "pt_graph_chunk";
     case kKSppmRadii: return "sppm_init_radii";
     case kKSppmCellSize: return "sppm_cell_size";
     case kKSppmPhotons: return "sppm_photons";
@@ -105,13 +106,42 @@ __device__ inline uint32_t packFlags(int depth, int wavelength, const Media& md)
          (md.overflow ? (1u << 16) : 0u);
 }
 
+// ------------------------------------------------------------------ jobs ----
+// Job of a global path id: the last job whose pidBase <= pid.
+template <bool MULTI>
+__device__ inline uint32_t findJob(const JobSet& js, uint32_t pid) {
+  if (!MULTI) return 0u;
+  uint32_t lo = 0, hi = js.nJobs;
+  while (hi - lo > 1u) {
+    uint32_t mid = (lo + hi) >> 1;
+    if (js.passes[mid].pidBase <= pid) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+template <bool MULTI>
+__device__ inline const DevScene& sceneOf(const JobSet& js, uint32_t j) {
+  return MULTI ? js.scenes[j] : js.s;
+}
+template <bool MULTI>
+__device__ inline const PassInfo& passOf(const JobSet& js, uint32_t j) {
+  return MULTI ? js.passes[j] : js.pi;
+}
+// Pixel and sample of a path (global id) of job pi.
+__device__ inline void pathPixel(const PassInfo& pi, uint32_t pid, uint32_t* pixel, uint32_t* k) {
+  uint32_t local = pid - pi.pidBase;
+  uint32_t kk = local / pi.pixels;
+  *pixel = pi.tileOrder[local - kk * pi.pixels];
+  *k = kk;
+}
+
 // ------------------------------------------------------------ path start ----
 // Camera ray of path `pid` (Camera.cpp:41-81 with the sample's offset,
 // RenderJob.h:53-60) into `slot`.
 __device__ inline void startPath(const DevScene& s, const PathBuffers& pb, const PassInfo& pi, uint32_t slot,
                                  uint32_t pid) {
-  uint32_t k = pid / pi.pixels;
-  uint32_t pixel = pi.tileOrder[pid - k * pi.pixels];
+  uint32_t k, pixel;
+  pathPixel(pi, pid, &pixel, &k);
   uint32_t i = pixel % s.width, j = pixel / s.width;
   hrng::Stream rng = hrng::make_stream(s.seed, pixel, pi.first + k, hrng::kDomainPath);
   float2 off = pi.offsets[k];
@@ -145,7 +175,8 @@ __device__ inline uint32_t fillCount(const uint32_t* c) {
   uint32_t avail = lim > next ? lim - next : 0u;
   return idle < avail ? idle : avail;
 }
-__global__ void __launch_bounds__(kBlock) kFill(DevScene s, PathBuffers pb, PassInfo pi) {
+template <bool MULTI>
+__global__ void __launch_bounds__(kBlock) kFill(JobSet js, PathBuffers pb) {
   __shared__ uint32_t ldsWave[kBlock / 64];
   __shared__ uint32_t ldsBase;
   uint32_t m = fillCount(pb.counters);
@@ -157,7 +188,9 @@ __global__ void __launch_bounds__(kBlock) kFill(DevScene s, PathBuffers pb, Pass
     uint32_t slot = 0;
     if (live) {
       slot = pb.qI[idle - 1 - q];
-      startPath(s, pb, pi, slot, next + q);
+      uint32_t pid = next + q;
+      uint32_t j = findJob<MULTI>(js, pid);
+      startPath(sceneOf<MULTI>(js, j), pb, passOf<MULTI>(js, j), slot, pid);
     }
     uint32_t idx = blockAppend(&pb.counters[kCntA], live, ldsWave, &ldsBase);
     if (live) pb.qA[idx] = slot;
@@ -175,7 +208,8 @@ __global__ void kSetLimit(uint32_t* c, uint32_t limit) {
 }
 
 // --------------------------------------------------------------- extend ----
-__global__ void __launch_bounds__(kBlock) kExtend(DevScene s, PathBuffers pb, int countStats) {
+template <bool MULTI>
+__global__ void __launch_bounds__(kBlock) kExtend(JobSet js, PathBuffers pb, int countStats) {
   __shared__ uint2 stack[kStackLds * kBlock];
   uint32_t n = pb.counters[kCntA];
   const uint32_t* queue = pb.qA;
@@ -186,6 +220,7 @@ __global__ void __launch_bounds__(kBlock) kExtend(DevScene s, PathBuffers pb, in
   for (uint32_t q = gtid; q < n; q += stride) {
     uint32_t slot = queue[q];
     float4 o = pb.rayO[slot], d = pb.rayD[slot];
+    const DevScene& s = sceneOf<MULTI>(js, MULTI ? findJob<MULTI>(js, pb.state[slot].w) : 0u);
     Hit h = traceClosest<kStackLds, kStackOvf>(s, f4xyz(o), f4xyz(d), stack + threadIdx.x, kBlock, ovf,
                                     countStats ? &nodes : nullptr, countStats ? &tris : nullptr);
     pb.hit[slot] = make_float4(h.t, h.u, h.v, __int_as_float(h.tri));
@@ -199,12 +234,12 @@ __global__ void __launch_bounds__(kBlock) kExtend(DevScene s, PathBuffers pb, in
 }
 
 // ---------------------------------------------------------------- shade ----
-__global__ void __launch_bounds__(kBlock) kShade(DevScene s, PathBuffers pb, PassInfo pi) {
+template <bool MULTI>
+__global__ void __launch_bounds__(kBlock) kShade(JobSet js, PathBuffers pb) {
   __shared__ uint32_t ldsWave[kBlock / 64];
   __shared__ uint32_t ldsBase;
   uint32_t n = pb.counters[kCntA];
   uint32_t stride = gridDim.x * blockDim.x;
-  const bool twoStrategies = s.hasLight && s.useMIS;
   for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += stride) {
     uint32_t q = base + threadIdx.x;
     bool pushNext = false, pushShadow = false, live = false;
@@ -213,6 +248,10 @@ __global__ void __launch_bounds__(kBlock) kShade(DevScene s, PathBuffers pb, Pas
       live = true;
       slot = pb.qA[q];
       uint4 st = pb.state[slot];
+      const uint32_t job = findJob<MULTI>(js, st.w);
+      const DevScene& s = sceneOf<MULTI>(js, job);
+      const PassInfo& pi = passOf<MULTI>(js, job);
+      const bool twoStrategies = s.hasLight && s.useMIS;
       int depth = (int)(st.y & 0xFFu);
       int wavelength = (int)((st.y >> 8) & 0x3u) - 1;
       Media md;
@@ -227,8 +266,8 @@ __global__ void __launch_bounds__(kBlock) kShade(DevScene s, PathBuffers pb, Pas
       V3 L = f4xyz(L4);
       hrng::Stream rng;
       {
-        uint32_t k = st.w / pi.pixels;
-        uint32_t pixel = pi.tileOrder[st.w - k * pi.pixels];
+        uint32_t k, pixel;
+        pathPixel(pi, st.w, &pixel, &k);
         rng = hrng::make_stream(s.seed, pixel, pi.first + k, hrng::kDomainPath);
         rng.counter = st.x;
       }
@@ -378,7 +417,8 @@ __global__ void __launch_bounds__(kBlock) kShade(DevScene s, PathBuffers pb, Pas
 }
 
 // -------------------------------------------------------------- connect ----
-__global__ void __launch_bounds__(kBlock) kConnect(DevScene s, PathBuffers pb, int countStats) {
+template <bool MULTI>
+__global__ void __launch_bounds__(kBlock) kConnect(JobSet js, PathBuffers pb, int countStats) {
   __shared__ uint2 stack[kStackLds * kBlock];
   uint32_t n = pb.counters[kCntS];
   uint32_t gtid = xcdBlock() * blockDim.x + threadIdx.x;
@@ -387,6 +427,7 @@ __global__ void __launch_bounds__(kBlock) kConnect(DevScene s, PathBuffers pb, i
   uint32_t nodes = 0, tris = 0, rays = 0;
   for (uint32_t q = gtid; q < n; q += stride) {
     uint32_t slot = pb.qS[q];
+    const DevScene& s = sceneOf<MULTI>(js, MULTI ? findJob<MULTI>(js, pb.state[slot].w) : 0u);
     float4 o4 = pb.shO[slot], d4 = pb.shD[slot];
     V3 so = f4xyz(o4), sd = f4xyz(d4);
     int32_t target = __float_as_int(d4.w);
@@ -463,7 +504,7 @@ __global__ void __launch_bounds__(kBlock) kConnect(DevScene s, PathBuffers pb, i
 // --------------------------------------------------------------- retire ----
 // Finished paths: results into the ring; the slot becomes idle and the next
 // kFill gives it a new path id.
-__global__ void __launch_bounds__(kBlock) kRetire(PathBuffers pb, PassInfo pi) {
+__global__ void __launch_bounds__(kBlock) kRetire(PathBuffers pb) {
   __shared__ uint32_t ldsWave[kBlock / 64];
   __shared__ uint32_t ldsBase;
   uint32_t n = pb.counters[kCntR];
@@ -474,10 +515,7 @@ __global__ void __launch_bounds__(kBlock) kRetire(PathBuffers pb, PassInfo pi) {
     uint32_t slot = 0;
     if (live) {
       slot = pb.qR[q];
-      uint32_t pid = pb.state[slot].w;
-      uint32_t k = pid / pi.pixels;
-      uint32_t pixel = pi.tileOrder[pid - k * pi.pixels];
-      size_t ri = (size_t)(k % pi.ringSamples) * pi.pixels + pixel;
+      uint32_t ri = pb.state[slot].w % pb.ringPaths;
       pb.ringL[ri] = pb.L[slot];
       pb.ringN[ri] = pb.auxN[slot];
       pb.ringA[ri] = pb.auxA[slot];
@@ -488,8 +526,8 @@ __global__ void __launch_bounds__(kBlock) kRetire(PathBuffers pb, PassInfo pi) {
 }
 
 // ----------------------------------------------------------- accumulate ----
-// Samples [k0, k1) of the call, from the ring, added pixel by pixel in
-// sample order; NaN samples are skipped (PathTracing.cpp:132-135).
+// Samples [k0, k1) of job pi, from the ring, added pixel by pixel in sample
+// order; NaN samples are skipped (PathTracing.cpp:132-135).
 __global__ void __launch_bounds__(kBlock) kAccumulate(PathBuffers pb, PassInfo pi, uint32_t k0, uint32_t k1,
                                                       float* color, float* normal, float* albedo) {
   uint32_t P = pi.pixels;
@@ -498,8 +536,9 @@ __global__ void __launch_bounds__(kBlock) kAccumulate(PathBuffers pb, PassInfo p
     float c0 = color[p * 3], c1 = color[p * 3 + 1], c2 = color[p * 3 + 2];
     float n0 = normal[p * 3], n1 = normal[p * 3 + 1], n2 = normal[p * 3 + 2];
     float a0 = albedo[p * 3], a1 = albedo[p * 3 + 1], a2 = albedo[p * 3 + 2];
+    uint32_t pid0 = pi.pidBase + pi.rankOf[p];
     for (uint32_t k = k0; k < k1; ++k) {
-      size_t ri = (size_t)(k % pi.ringSamples) * P + p;
+      uint32_t ri = (pid0 + k * P) % pb.ringPaths;
       float4 L = pb.ringL[ri];
       if (L.x != L.x || L.y != L.y || L.z != L.z) continue;
       float4 an = pb.ringN[ri], aa = pb.ringA[ri];
@@ -542,98 +581,194 @@ __global__ void __launch_bounds__(kBlock) kTraceRays(DevScene s, const float* ra
 
 }  // namespace
 
-void tileOrder(uint32_t width, uint32_t height, uint32_t* out) {
+void tileOrder(uint32_t width, uint32_t height, uint32_t* order, uint32_t* rankOf) {
   uint32_t n = 0;
   for (uint32_t ty = 0; ty < height; ty += 8)
     for (uint32_t tx = 0; tx < width; tx += 8)
       for (uint32_t y = ty; y < ty + 8 && y < height; ++y)
-        for (uint32_t x = tx; x < tx + 8 && x < width; ++x) out[n++] = y * width + x;
+        for (uint32_t x = tx; x < tx + 8 && x < width; ++x) {
+          order[n] = y * width + x;
+          rankOf[y * width + x] = n;
+          ++n;
+        }
 }
 
-void renderPT(const DevScene& s, const PathBuffers& pbIn, uint32_t first, uint32_t n, const float2* dOffsets,
-              const uint32_t* dTileOrder, float* dColor, float* dNormal, float* dAlbedo, const LaunchConfig& cfg,
-              hipStream_t stream, KernelTimer* timer) {
-  PathBuffers pb = pbIn;  // queue pointers swap per bounce
+namespace {
+
+void hipOk(hipError_t e, const char* what) {
+  if (e != hipSuccess) throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+// The bounces of one chunk: fill, extend, shade, connect, retire, advance.
+// `chunk` is even, so the queue pointers are back in place afterwards and
+// every chunk issues identical launches (what a captured graph replays).
+template <bool MULTI>
+void launchChunk(const JobSet& js, const PathBuffers& pbIn, const LaunchConfig& cfg, uint32_t chunk,
+                 hipStream_t stream, KernelTimer* timer) {
+  PathBuffers pb = pbIn;
   auto t0 = [&](int k) {
     if (timer) timer->begin(k, stream);
   };
   auto t1 = [&](int k) {
     if (timer) timer->end(k, stream);
   };
-  auto check = [](hipError_t e) {
-    if (e != hipSuccess) throw std::runtime_error(std::string("renderPT: ") + hipGetErrorString(e));
-  };
-  PassInfo pi;
-  pi.first = first;
-  pi.pixels = s.width * s.height;
-  pi.paths = n * pi.pixels;  // the caller keeps this below 2^31
-  pi.ringSamples = pb.ringPaths / pi.pixels;
-  if (pi.ringSamples > n) pi.ringSamples = n;
-  pi.tileOrder = dTileOrder;
-  pi.offsets = dOffsets;
-  if (pi.ringSamples == 0) throw std::runtime_error("renderPT: results ring smaller than one sample");
-  const uint32_t shadeBlocks = cfg.shadeBlocks;
   const int countStats = cfg.countStats ? 1 : 0;
-  uint32_t limit = (uint32_t)std::min<uint64_t>(pi.paths, (uint64_t)pi.ringSamples * pi.pixels);
+  for (uint32_t it = 0; it < chunk; ++it) {
+    t0(kKGenerate);
+    kFill<MULTI><<<cfg.shadeBlocks, kBlock, 0, stream>>>(js, pb);
+    t1(kKGenerate);
+    t0(kKMisc);
+    kFillCommit<<<1, 64, 0, stream>>>(pb.counters);
+    t1(kKMisc);
+    t0(kKExtend);
+    kExtend<MULTI><<<cfg.traceBlocks, kBlock, 0, stream>>>(js, pb, countStats);
+    t1(kKExtend);
+    t0(kKShade);
+    kShade<MULTI><<<cfg.shadeBlocks, kBlock, 0, stream>>>(js, pb);
+    t1(kKShade);
+    t0(kKConnect);
+    kConnect<MULTI><<<cfg.traceBlocks, kBlock, 0, stream>>>(js, pb, countStats);
+    t1(kKConnect);
+    t0(kKRetire);
+    kRetire<<<cfg.shadeBlocks, kBlock, 0, stream>>>(pb);
+    t1(kKRetire);
+    t0(kKMisc);
+    kAdvance<<<1, 64, 0, stream>>>(pb.counters);
+    t1(kKMisc);
+    uint32_t* tmp = pb.qA;
+    pb.qA = pb.qB;
+    pb.qB = tmp;
+  }
+}
+
+}  // namespace
+
+void renderJobs(const std::vector<RenderJobDesc>& jobs, const PathBuffers& pb, const LaunchConfig& cfg,
+                hipStream_t stream, KernelTimer* timer) {
+  const uint32_t nJobs = (uint32_t)jobs.size();
+  if (nJobs == 0) return;
+  std::vector<DevScene> scenes(nJobs);
+  std::vector<PassInfo> passes(nJobs);
+  uint64_t total = 0;
+  int maxDepth = 0;
+  for (uint32_t j = 0; j < nJobs; ++j) {
+    const RenderJobDesc& d = jobs[j];
+    scenes[j] = d.scene;
+    PassInfo& pi = passes[j];
+    pi.first = d.first;
+    pi.pixels = d.scene.width * d.scene.height;
+    pi.paths = d.n * pi.pixels;
+    pi.pidBase = (uint32_t)total;
+    pi.tileOrder = d.dTileOrder;
+    pi.rankOf = d.dRankOf;
+    pi.offsets = d.dOffsets;
+    if (pi.pixels > pb.ringPaths) throw std::runtime_error("renderJobs: results ring smaller than one sample");
+    total += (uint64_t)d.n * pi.pixels;
+    maxDepth = std::max(maxDepth, d.scene.maxDepth);
+  }
+  if (total >= (1ull << 31)) throw std::runtime_error("renderJobs: more than 2^31 paths in one call");
+  JobSet js{};
+  js.s = scenes[0];
+  js.pi = passes[0];
+  js.nJobs = nJobs;
+  js.paths = (uint32_t)total;
+  void* dScenes = nullptr;
+  void* dPasses = nullptr;
+  struct Free {
+    void*& a;
+    void*& b;
+    ~Free() {
+      if (a) (void)hipFree(a);
+      if (b) (void)hipFree(b);
+    }
+  } freeOnExit{dScenes, dPasses};
+  if (nJobs > 1) {
+    hipOk(hipMalloc(&dScenes, sizeof(DevScene) * nJobs), "hipMalloc");
+    hipOk(hipMalloc(&dPasses, sizeof(PassInfo) * nJobs), "hipMalloc");
+    hipOk(hipMemcpyAsync(dScenes, scenes.data(), sizeof(DevScene) * nJobs, hipMemcpyHostToDevice, stream), "copy");
+    hipOk(hipMemcpyAsync(dPasses, passes.data(), sizeof(PassInfo) * nJobs, hipMemcpyHostToDevice, stream), "copy");
+    js.scenes = (const DevScene*)dScenes;
+    js.passes = (const PassInfo*)dPasses;
+  }
+  if (total == 0) return;
+  auto t0 = [&](int k) {
+    if (timer) timer->begin(k, stream);
+  };
+  auto t1 = [&](int k) {
+    if (timer) timer->end(k, stream);
+  };
+  const uint32_t chunk = std::max<uint32_t>(2, (cfg.chunk + 1) & ~1u);
+  // Paths started during chunk c have finished `lag` chunks later: a path
+  // has at most maxDepth vertices, one per bounce.
+  const uint32_t lag = ((uint32_t)maxDepth + chunk - 1) / chunk;
+  std::vector<uint32_t> acc(nJobs, 0);  // samples of each job added to its films
+  uint32_t front = 0;                    // first job not completely added
+  auto accPid = [&]() -> uint64_t {
+    return front < nJobs ? (uint64_t)passes[front].pidBase + (uint64_t)acc[front] * passes[front].pixels : total;
+  };
+  uint32_t limit = (uint32_t)std::min<uint64_t>(total, accPid() + pb.ringPaths);
   t0(kKMisc);
   kInitRender<<<256, kBlock, 0, stream>>>(pb, limit);
   t1(kKMisc);
-  // Paths started by the end of chunk c are finished `lag` chunks later: a
-  // path has at most maxDepth vertices, one per bounce, and starts at the
-  // beginning of a bounce.
-  const uint32_t chunk = cfg.chunk ? cfg.chunk : 8;
-  const uint32_t lag = ((uint32_t)s.maxDepth + chunk - 1) / chunk;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  struct GraphFree {
+    hipGraph_t& g;
+    hipGraphExec_t& e;
+    ~GraphFree() {
+      if (e) (void)hipGraphExecDestroy(e);
+      if (g) (void)hipGraphDestroy(g);
+    }
+  } graphFree{graph, exec};
+  if (cfg.useGraph) {
+    hipOk(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+    if (nJobs > 1) launchChunk<true>(js, pb, cfg, chunk, stream, nullptr);
+    else launchChunk<false>(js, pb, cfg, chunk, stream, nullptr);
+    hipOk(hipStreamEndCapture(stream, &graph), "hipStreamEndCapture");
+    hipOk(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0), "hipGraphInstantiate");
+  }
   std::vector<uint32_t> startedBy;
-  uint32_t accumulated = 0;  // samples already added to the films
-  uint32_t accBlocks = (pi.pixels + kBlock - 1) / kBlock;
   for (uint32_t c = 0;; ++c) {
-    for (uint32_t it = 0; it < chunk; ++it) {
-      t0(kKGenerate);
-      kFill<<<shadeBlocks, kBlock, 0, stream>>>(s, pb, pi);
-      t1(kKGenerate);
-      t0(kKMisc);
-      kFillCommit<<<1, 64, 0, stream>>>(pb.counters);
-      t1(kKMisc);
-      t0(kKExtend);
-      kExtend<<<cfg.traceBlocks, kBlock, 0, stream>>>(s, pb, countStats);
-      t1(kKExtend);
-      t0(kKShade);
-      kShade<<<shadeBlocks, kBlock, 0, stream>>>(s, pb, pi);
-      t1(kKShade);
-      t0(kKConnect);
-      kConnect<<<cfg.traceBlocks, kBlock, 0, stream>>>(s, pb, countStats);
-      t1(kKConnect);
-      t0(kKRetire);
-      kRetire<<<shadeBlocks, kBlock, 0, stream>>>(pb, pi);
-      t1(kKRetire);
-      t0(kKMisc);
-      kAdvance<<<1, 64, 0, stream>>>(pb.counters);
-      t1(kKMisc);
-      uint32_t* tmp = pb.qA;
-      pb.qA = pb.qB;
-      pb.qB = tmp;
+    if (exec) {
+      t0(kKGraph);
+      hipOk(hipGraphLaunch(exec, stream), "hipGraphLaunch");
+      t1(kKGraph);
+    } else if (nJobs > 1) {
+      launchChunk<true>(js, pb, cfg, chunk, stream, timer);
+    } else {
+      launchChunk<false>(js, pb, cfg, chunk, stream, timer);
     }
     uint32_t cnt[kNumCounters];
-    check(hipMemcpyAsync(cnt, pb.counters, sizeof(cnt), hipMemcpyDeviceToHost, stream));
-    check(hipStreamSynchronize(stream));
+    hipOk(hipMemcpyAsync(cnt, pb.counters, sizeof(cnt), hipMemcpyDeviceToHost, stream), "counter readback");
+    hipOk(hipStreamSynchronize(stream), "hipStreamSynchronize");
     uint32_t live = cnt[kCntA], next = cnt[kCntNext];
     startedBy.push_back(next);
-    uint32_t done = live == 0 ? next : (c >= lag ? startedBy[c - lag] : 0u);
-    uint32_t complete = done / pi.pixels;  // samples whose paths have all finished
-    if (complete > accumulated) {
-      t0(kKAccumulate);
-      kAccumulate<<<accBlocks, kBlock, 0, stream>>>(pb, pi, accumulated, complete, dColor, dNormal, dAlbedo);
-      t1(kKAccumulate);
-      accumulated = complete;
-      uint32_t newLimit =
-          (uint32_t)std::min<uint64_t>(pi.paths, (uint64_t)(accumulated + pi.ringSamples) * pi.pixels);
+    uint64_t done = live == 0 ? next : (c >= lag ? startedBy[c - lag] : 0u);
+    for (uint32_t j = front; j < nJobs && passes[j].pidBase < done; ++j) {
+      const PassInfo& pi = passes[j];
+      uint32_t n = pi.paths / pi.pixels;
+      uint32_t complete = (uint32_t)std::min<uint64_t>(n, (done - pi.pidBase) / pi.pixels);
+      if (complete > acc[j]) {
+        uint32_t blocks = (pi.pixels + kBlock - 1) / kBlock;
+        t0(kKAccumulate);
+        kAccumulate<<<blocks, kBlock, 0, stream>>>(pb, pi, acc[j], complete, jobs[j].dColor, jobs[j].dNormal,
+                                                   jobs[j].dAlbedo);
+        t1(kKAccumulate);
+        acc[j] = complete;
+      }
+    }
+    while (front < nJobs && acc[front] * (uint64_t)passes[front].pixels == passes[front].paths) ++front;
+    uint32_t newLimit = (uint32_t)std::min<uint64_t>(total, accPid() + pb.ringPaths);
+    if (newLimit != limit) {
       t0(kKMisc);
       kSetLimit<<<1, 64, 0, stream>>>(pb.counters, newLimit);
       t1(kKMisc);
+      limit = newLimit;
     }
-    if (live == 0 && next >= pi.paths) break;
+    if (live == 0 && next >= total) break;
   }
-  if (accumulated != n) throw std::runtime_error("renderPT: not every sample was accumulated");
+  hipOk(hipStreamSynchronize(stream), "hipStreamSynchronize");
+  if (front != nJobs) throw std::runtime_error("renderJobs: not every sample was accumulated");
 }
 
 void launchTraceClosest(const DevScene& s, const float* dRays, uint32_t n, int32_t* dTri, float* dTuv, uint2* dOverflow,

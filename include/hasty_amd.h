@@ -114,6 +114,45 @@ int hasty_counters(HastyScene* scene, uint64_t* out5);
  * value.  Images do not depend on these. */
 int hasty_set_launch(HastyScene* scene, uint32_t path_slots, uint32_t trace_blocks, uint32_t shade_blocks);
 
+/* Launch configuration of a scene's path pool (also used for batches whose
+ * first job is this scene).  chunk = bounces between host checks (even);
+ * use_graph = replay each chunk as one captured hipGraph.  Fields that are 0
+ * keep their value (except use_graph). */
+typedef struct HastyLaunch {
+  uint32_t path_slots;
+  uint32_t trace_blocks;
+  uint32_t shade_blocks;
+  uint32_t chunk;
+  int32_t use_graph;
+} HastyLaunch;
+int hasty_get_launch(const HastyScene* scene, HastyLaunch* out);
+int hasty_set_launch_ex(HastyScene* scene, const HastyLaunch* launch);
+
+/* ---- batching: many render jobs through one pool of path slots ----
+ * No reference counterpart (the reference renders one job per process,
+ * apps/PathTracingViewer.cpp).  Each job's films receive exactly what
+ * hasty_render_pt_device(scene, seed, first, n, ...) adds for it alone; the
+ * paths of all jobs are in flight together, so every wavefront stage is one
+ * launch for the whole batch.  Uses the path pool, stream and launch settings
+ * of jobs[0].scene; all scenes must be on one device. */
+typedef struct HastyBatchJob {
+  HastyScene* scene;
+  uint32_t seed;
+  uint32_t first_sample;
+  uint32_t num_samples;
+  float* d_color;
+  float* d_normal;
+  float* d_albedo;
+} HastyBatchJob;
+int hasty_render_batch_device(const HastyBatchJob* jobs, uint32_t n_jobs);
+
+/* Autotuner (no reference counterpart): times short renders (trial_samples
+ * samples of every job into scratch films) over path slots, grid sizes,
+ * chunk length and graph replay by coordinate descent, applies the fastest
+ * configuration to jobs[0].scene and returns it in *chosen (may be NULL). */
+int hasty_autotune_batch(const HastyBatchJob* jobs, uint32_t n_jobs, uint32_t trial_samples, HastyLaunch* chosen);
+int hasty_autotune(HastyScene* scene, uint32_t seed, uint32_t trial_samples, HastyLaunch* chosen);
+
 /* ---- progressive photon mapping (caustic photon map + MIS path tracing) ----
  * Replaces renderPhotonThread (core/src/PhotonMapping.cpp:814-933, declared
  * core/include/Hasty/PhotonMapping.h:124) for one PMRenderJob
