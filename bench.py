@@ -28,7 +28,8 @@ for a plain copy of known size (--copy-calibration); read from the committed
 summary in profiles/ when present.
 
 Other workloads (not the headline line): --workload sppm times photon-mapping
-passes on the caustic box (prism + 2M-triangle torus, 1920x1080).
+passes on the caustic box (prism + 2M-triangle torus, 1920x1080);
+--workload batch renders 32 random-cuboid jobs in one batch (autotuned).
 """
 from __future__ import annotations
 
@@ -205,6 +206,81 @@ def bench_sppm(args):
     print(json.dumps(line), flush=True)
 
 
+def bench_batch(args):
+    """32 render jobs (random-cuboid scenes, resolutions log-uniform in
+    128^2..1024^2, spp in 16..256, 1k..160k cuboids) rendered in one batch
+    through one path pool, after the autotuner picked the launch shape; the
+    same jobs rendered one after another are reported beside it."""
+    import hastyrenderer_amd as ha
+    from hastyrenderer_amd import scenes
+
+    scene_dir = args.scene_dir or os.path.join(tempfile.gettempdir(), "hasty_bench_batch")
+    marker = os.path.join(scene_dir, "jobs.json")
+    if os.path.exists(marker):
+        with open(marker) as f:
+            paths = json.load(f)
+    else:
+        log("generating 32 cuboid jobs in %s" % scene_dir)
+        paths = scenes.batch_jobs(scene_dir, n_jobs=32, seed=2)
+        with open(marker, "w") as f:
+            json.dump(paths, f)
+    t0 = time.perf_counter()
+    scs = []
+    for i, pth in enumerate(paths):
+        scs.append(ha.Scene(pth))
+        log("loaded job %d: %d triangles" % (i, scs[-1].info()["num_triangles"]))
+    load_s = time.perf_counter() - t0
+    infos = [sc.info() for sc in scs]
+    films = [ha.DeviceFilms(i["width"], i["height"]) for i in infos]
+    jobs = [(sc, j, 0, i["num_samples"], f) for j, (sc, i, f) in enumerate(zip(scs, infos, films))]
+    paths_total = sum(i["width"] * i["height"] * i["num_samples"] for i in infos)
+    tuned = ha.autotune_batch([(sc, j, 0, 1, None) for sc, j, _, _, _ in jobs], trial_samples=1)
+    log("autotuned launch: %s" % json.dumps(tuned))
+    hip = Hip()
+    for _ in range(args.warmup):
+        ha.render_batch(jobs)
+    hip.sync()
+    t_start = time.perf_counter()
+    for i in range(args.steps):
+        t = time.perf_counter()
+        for f in films:
+            f.zero()
+        ha.render_batch(jobs)
+        log("batch step %d: %.3f s" % (i, time.perf_counter() - t))
+    hip.sync()
+    elapsed = time.perf_counter() - t_start
+    t = time.perf_counter()
+    for sc, seed, first, n, f in jobs:
+        f.zero()
+        sc.set_launch_ex(**{k: v for k, v in tuned.items()})
+        sc.render_pt_device(seed, first, n, *f.ptrs)
+    hip.sync()
+    seq = time.perf_counter() - t
+    line = {
+        "metric": "paths/s over a batch of 32 render jobs (one path pool)",
+        "value": paths_total * args.steps / elapsed,
+        "unit": "paths/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "batch_32_cuboid_jobs", "jobs": len(jobs), "paths": paths_total,
+                   "triangles": sum(i["num_triangles"] for i in infos),
+                   "pixels": sum(i["width"] * i["height"] for i in infos), "launch": tuned,
+                   "parallelism": "single-gpu"},
+        "sequential_paths_per_s": paths_total / seq,
+        "load_s": load_s,
+    }
+    for f in films:
+        f.close()
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -215,7 +291,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--cpu-spp", type=int, default=48)
-    ap.add_argument("--workload", choices=("ggx", "sppm"), default="ggx")
+    ap.add_argument("--workload", choices=("ggx", "sppm", "batch"), default="ggx")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="also run a plain copy of MIB MiB (counter calibration)")
     ap.add_argument("--sppm-alpha", type=float, default=0.4)
@@ -229,6 +305,9 @@ def main():
               file=sys.stderr)
         sys.exit(2)
 
+    if args.workload == "batch":
+        bench_batch(args)
+        return
     if args.workload == "sppm":
         bench_sppm(args)
         if args.copy_calibration:

@@ -46,6 +46,17 @@ EXPORTS = (
     "hasty_render_batch_device",
     "hasty_autotune_batch",
     "hasty_autotune",
+    "hasty_guide_num_params",
+    "hasty_guide_create",
+    "hasty_guide_destroy",
+    "hasty_guide_set_box",
+    "hasty_guide_set_params",
+    "hasty_guide_get_params",
+    "hasty_guide_infer",
+    "hasty_guide_gradient",
+    "hasty_guide_train_step",
+    "hasty_scene_set_guide",
+    "hasty_scene_guide_stats",
     "hasty_sppm_create",
     "hasty_sppm_destroy",
     "hasty_sppm_pass_device",
@@ -163,6 +174,18 @@ def lib():
     L.hasty_render_batch_device.argtypes = [C.POINTER(BatchJobC), C.c_uint32]
     L.hasty_autotune_batch.argtypes = [C.POINTER(BatchJobC), C.c_uint32, C.c_uint32, C.POINTER(LaunchC)]
     L.hasty_autotune.argtypes = [P, C.c_uint32, C.c_uint32, C.POINTER(LaunchC)]
+    L.hasty_guide_num_params.restype = C.c_uint32
+    L.hasty_guide_create.argtypes = [P, C.c_uint32, C.POINTER(P)]
+    L.hasty_guide_destroy.argtypes = [P]
+    L.hasty_guide_destroy.restype = None
+    L.hasty_guide_set_box.argtypes = [P, fp, fp]
+    L.hasty_guide_set_params.argtypes = [P, fp]
+    L.hasty_guide_get_params.argtypes = [P, fp]
+    L.hasty_guide_infer.argtypes = [P, fp, C.c_uint32, fp]
+    L.hasty_guide_gradient.argtypes = [P, fp, C.c_uint32, fp, fp]
+    L.hasty_guide_train_step.argtypes = [P, fp, C.c_uint32, C.c_float, fp]
+    L.hasty_scene_set_guide.argtypes = [P, P, C.c_float, C.c_float]
+    L.hasty_scene_guide_stats.argtypes = [P, fp, C.POINTER(C.c_uint32)]
     L.hasty_sppm_create.argtypes = [P, C.c_uint32, C.POINTER(SppmParamsC), C.POINTER(P)]
     L.hasty_sppm_destroy.argtypes = [P]
     L.hasty_sppm_destroy.restype = None
@@ -257,6 +280,19 @@ class Scene:
         """Films are device pointers (ints), e.g. torch tensors' data_ptr()."""
         _err(lib().hasty_render_pt_device(self._h, seed, first_sample, num_samples, C.c_void_p(d_color),
                                           C.c_void_p(d_normal), C.c_void_p(d_albedo)))
+
+    def set_guide(self, guide, fraction=0.5, train_lr=0.0):
+        """Guided rendering with `guide` (None: plain); online Adam training
+        after every sample index when train_lr > 0."""
+        self._guide = guide
+        _err(lib().hasty_scene_set_guide(self._h, guide.handle if guide is not None else None, fraction,
+                                         train_lr))
+
+    def guide_stats(self):
+        loss = np.zeros(1, np.float32)
+        n = C.c_uint32()
+        _err(lib().hasty_scene_guide_stats(self._h, _fptr(loss), C.byref(n)))
+        return float(loss[0]), int(n.value)
 
     def get_launch(self) -> dict:
         lc = LaunchC()
@@ -375,6 +411,62 @@ def autotune_batch(jobs, trial_samples: int = 1) -> dict:
     lc = LaunchC()
     _err(lib().hasty_autotune_batch(arr, len(jobs), trial_samples, C.byref(lc)))
     return {k: int(getattr(lc, k)) for k, _ in LaunchC._fields_}
+
+
+class Guide:
+    """The learned path guide of a scene (HastyGuide*): MLP -> vMF mixture."""
+
+    def __init__(self, scene: Scene, seed: int = 0):
+        self.scene = scene
+        h = C.c_void_p()
+        _err(lib().hasty_guide_create(scene.handle, seed, C.byref(h)))
+        self._h = h
+        self.n_params = int(lib().hasty_guide_num_params())
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and _lib is not None:
+            _lib.hasty_guide_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    @property
+    def handle(self):
+        return self._h
+
+    def set_box(self, lo, hi):
+        _err(lib().hasty_guide_set_box(self._h, _fptr(np.ascontiguousarray(lo, np.float32)),
+                                       _fptr(np.ascontiguousarray(hi, np.float32))))
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, np.float32)
+        assert p.size == self.n_params
+        _err(lib().hasty_guide_set_params(self._h, _fptr(p)))
+
+    def get_params(self):
+        p = np.zeros(self.n_params, np.float32)
+        _err(lib().hasty_guide_get_params(self._h, _fptr(p)))
+        return p
+
+    def infer(self, queries):
+        q = np.ascontiguousarray(queries, np.float32).reshape(-1, 9)
+        raw = np.zeros((q.shape[0], 48), np.float32)
+        _err(lib().hasty_guide_infer(self._h, _fptr(q), q.shape[0], _fptr(raw)))
+        return raw
+
+    def gradient(self, samples):
+        sm = np.ascontiguousarray(samples, np.float32).reshape(-1, 13)
+        g = np.zeros(self.n_params, np.float32)
+        loss = np.zeros(1, np.float32)
+        _err(lib().hasty_guide_gradient(self._h, _fptr(sm), sm.shape[0], _fptr(g), _fptr(loss)))
+        return g, float(loss[0])
+
+    def train_step(self, samples, lr=1e-3):
+        sm = np.ascontiguousarray(samples, np.float32).reshape(-1, 13)
+        loss = np.zeros(1, np.float32)
+        _err(lib().hasty_guide_train_step(self._h, _fptr(sm), sm.shape[0], lr, _fptr(loss)))
+        return float(loss[0])
 
 
 class Sppm:

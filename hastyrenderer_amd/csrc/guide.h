@@ -1,0 +1,249 @@
+// Learned path guide: a small MLP (BF16 weights on MFMA, FP32 accumulation)
+// maps (position, normal, outgoing direction) to a mixture of 8 von
+// Mises-Fisher lobes; trained online on the tracer's own radiance samples.
+// No counterpart in the reference renderer.
+//
+// Network: 80 inputs (12-frequency sin/cos encoding of the position
+// normalised to the scene box = 72, normal 3, outgoing direction 3, 2 zero
+// pads) -> 64 -> 64 -> 64 -> 64 (ReLU) -> 48 raw outputs (40 used: per lobe
+// mean direction 3, log concentration 1, logit 1).  Weights are kept as FP32
+// master copies, as BF16 W^T [out][in] for the forward pass and BF16 W
+// [in][out] for the backward pass.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string.h>
+
+#include "pt_device.h"
+
+namespace guide {
+
+constexpr int kFreq = 12;
+constexpr int kInPad = 80;
+constexpr int kHidden = 64;
+constexpr int kLobes = 8;
+constexpr int kOut = 40;
+constexpr int kOutPad = 48;
+constexpr int kLayers = 5;
+constexpr float kCLo = -4.0f, kCHi = 9.0f;  // log-concentration clamp
+
+__host__ __device__ constexpr int layerIn(int l) { return l == 0 ? kInPad : kHidden; }
+__host__ __device__ constexpr int layerOut(int l) { return l == kLayers - 1 ? kOutPad : kHidden; }
+// flat parameter layout: per layer W^T (out x in) then b (out)
+__host__ __device__ constexpr int wOffset(int l) {
+  return l == 0 ? 0 : wOffset(l - 1) + layerIn(l - 1) * layerOut(l - 1) + layerOut(l - 1);
+}
+__host__ __device__ constexpr int bOffset(int l) { return wOffset(l) + layerIn(l) * layerOut(l); }
+constexpr int kParams = bOffset(kLayers - 1) + layerOut(kLayers - 1);
+// offsets of the BF16 weight matrices (biases excluded)
+__host__ __device__ constexpr int wqOffset(int l) {
+  return l == 0 ? 0 : wqOffset(l - 1) + layerIn(l - 1) * layerOut(l - 1);
+}
+constexpr int kWeights = wqOffset(kLayers - 1) + layerIn(kLayers - 1) * layerOut(kLayers - 1);
+// rows of the training activations: layer inputs (H) and output gradients (G)
+__host__ __device__ constexpr int hOffset(int l) { return l == 0 ? 0 : hOffset(l - 1) + layerIn(l - 1); }
+__host__ __device__ constexpr int gOffset(int l) { return l == 0 ? 0 : gOffset(l - 1) + layerOut(l - 1); }
+
+// Device view of the network (read by inference and training kernels).
+struct Net {
+  const short* wT;     // BF16 W^T, kWeights
+  const short* w;      // BF16 W (transposed copies), kWeights
+  const float* param;  // FP32 master parameters (biases read from here)
+  float boxLo[3], boxInv[3];  // position normalisation: (x - lo) * inv
+};
+
+// ------------------------------------------------------------ BF16 ------
+__host__ __device__ inline short toBf16(float f) {
+  uint32_t u;
+  memcpy(&u, &f, 4);
+  if ((u & 0x7F800000u) == 0x7F800000u && (u & 0x7FFFFFu)) return (short)((u >> 16) | 0x40u);  // NaN stays NaN
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return (short)(u >> 16);
+}
+__host__ __device__ inline float fromBf16(short h) {
+  uint32_t u = ((uint32_t)(uint16_t)h) << 16;
+  float f;
+  memcpy(&f, &u, 4);
+  return f;
+}
+
+// ----------------------------------------------------- input features ----
+// Feature k of a query (x normalised to [0,1]^3): 24a + 2f -> sin(2^f pi x_a),
+// 24a + 2f + 1 -> cos(2^f pi x_a); 72..74 normal; 75..77 wo; 78, 79 zero.
+__device__ inline float feature(int k, hdev::V3 x, hdev::V3 n, hdev::V3 wo) {
+  if (k < 72) {
+    int a = k / 24, r = k - 24 * a, f = r >> 1;
+    float xa = a == 0 ? x.x : (a == 1 ? x.y : x.z);
+    float ang = (float)(1u << f) * 3.14159265358979323846f * xa;
+    return (r & 1) ? cosf(ang) : sinf(ang);
+  }
+  if (k < 75) return k == 72 ? n.x : (k == 73 ? n.y : n.z);
+  if (k < 78) return k == 75 ? wo.x : (k == 76 ? wo.y : wo.z);
+  return 0.0f;
+}
+
+// --------------------------------------------------------- vMF mixture ----
+struct Mixture {
+  hdev::V3 mu[kLobes];
+  float kappa[kLobes];
+  float w[kLobes];
+};
+
+__device__ inline Mixture mixtureFromRaw(const float* raw) {
+  Mixture m;
+  float amax = -3.0e38f;
+#pragma unroll
+  for (int k = 0; k < kLobes; ++k) amax = fmaxf(amax, raw[5 * k + 4]);
+  float sum = 0.0f;
+#pragma unroll
+  for (int k = 0; k < kLobes; ++k) {
+    hdev::V3 r = hdev::v3(raw[5 * k], raw[5 * k + 1], raw[5 * k + 2]);
+    float nr = sqrtf(hdev::dot(r, r));
+    m.mu[k] = nr > 1e-12f ? r / nr : hdev::v3(0.0f, 0.0f, 1.0f);
+    m.kappa[k] = __expf(fminf(fmaxf(raw[5 * k + 3], kCLo), kCHi));
+    m.w[k] = __expf(raw[5 * k + 4] - amax);
+    sum += m.w[k];
+  }
+#pragma unroll
+  for (int k = 0; k < kLobes; ++k) m.w[k] /= sum;
+  return m;
+}
+
+__device__ inline float vmfPdf(hdev::V3 mu, float kappa, hdev::V3 d) {
+  float norm = kappa / (2.0f * 3.14159265358979323846f * -expm1f(-2.0f * kappa));
+  return norm * __expf(kappa * (hdev::dot(mu, d) - 1.0f));
+}
+
+__device__ inline float mixturePdf(const Mixture& m, hdev::V3 d) {
+  float p = 0.0f;
+#pragma unroll
+  for (int k = 0; k < kLobes; ++k) p += m.w[k] * vmfPdf(m.mu[k], m.kappa[k], d);
+  return p;
+}
+
+// Lobe by its weight (xi0), then Jakob's stable inversion about mu.
+__device__ inline hdev::V3 mixtureSample(const Mixture& m, float xi0, float xi1, float xi2) {
+  int k = kLobes - 1;
+  float c = 0.0f;
+#pragma unroll
+  for (int j = 0; j < kLobes; ++j) {
+    c += m.w[j];
+    if (xi0 < c) {
+      k = j;
+      break;
+    }
+  }
+  float kappa = m.kappa[k];
+  float wz = 1.0f + __logf(xi1 + (1.0f - xi1) * __expf(-2.0f * kappa)) / kappa;
+  wz = fminf(1.0f, fmaxf(-1.0f, wz));
+  float s = sqrtf(fmaxf(0.0f, 1.0f - wz * wz));
+  float phi = 2.0f * 3.14159265358979323846f * xi2;
+  hdev::V3 mu = m.mu[k];
+  hdev::V3 t = fabsf(mu.x) > 0.9f ? hdev::v3(0.0f, 1.0f, 0.0f) : hdev::v3(1.0f, 0.0f, 0.0f);
+  hdev::V3 b1 = hdev::normalize(hdev::cross(mu, t));
+  hdev::V3 b2 = hdev::cross(mu, b1);
+  return hdev::normalize(b1 * (s * __cosf(phi)) + b2 * (s * __sinf(phi)) + mu * wz);
+}
+
+// ------------------------------------------------------------- MFMA MLP ----
+// Every layer is computed transposed, H_out^T (out x 16 queries) =
+// W^T (out x in) . H_in^T (in x 16), by 16x16x16 tiles of
+// v_mfma_f32_16x16x16_bf16.  Operand lanes:
+//   A (16 x 16, row i, column k): lane l holds i = l % 16, k = 4 (l / 16) + v
+//   B (16 x 16, row k, column j): lane l holds j = l % 16, k = 4 (l / 16) + v
+//   D (16 x 16, row i, column j): lane l holds j = l % 16, i = 4 (l / 16) + v
+// so an output tile in D layout is, after bias/ReLU and rounding, already the
+// B operand of the next layer for the matching k-step.
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+__device__ inline f32x4 mfma(bf16x4 a, bf16x4 b, f32x4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+}
+__device__ inline bf16x4 pack4(float a, float b, float c, float d) {
+  bf16x4 r;
+  r.x = toBf16(a);
+  r.y = toBf16(b);
+  r.z = toBf16(c);
+  r.w = toBf16(d);
+  return r;
+}
+__device__ inline bf16x4 load4(const short* p) { return *reinterpret_cast<const bf16x4*>(p); }
+
+// One layer, transposed: z[mt] (D layout) = W^T tile rows . x[ks] + b.
+template <int KS, int MT>
+__device__ inline void layerFwd(const short* wT, const float* bias, int din, const bf16x4* x, f32x4* z) {
+  const int lane = threadIdx.x & 63, r = lane & 15, q = lane >> 4;
+#pragma unroll
+  for (int mt = 0; mt < MT; ++mt) {
+    f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+    const short* row = wT + (size_t)(16 * mt + r) * din + 4 * q;
+#pragma unroll
+    for (int ks = 0; ks < KS; ++ks) acc = mfma(load4(row + 16 * ks), x[ks], acc);
+    const float* b = bias + 16 * mt + 4 * q;
+    acc.x += b[0];
+    acc.y += b[1];
+    acc.z += b[2];
+    acc.w += b[3];
+    z[mt] = acc;
+  }
+}
+__device__ inline bf16x4 reluPack(f32x4 z) {
+  return pack4(fmaxf(z.x, 0.0f), fmaxf(z.y, 0.0f), fmaxf(z.z, 0.0f), fmaxf(z.w, 0.0f));
+}
+
+// Input features of query (x, n, wo) for the 5 k-steps of layer 0 (B layout).
+__device__ inline void inputOperands(hdev::V3 x, hdev::V3 n, hdev::V3 wo, bf16x4* in) {
+  const int q = (threadIdx.x & 63) >> 4;
+#pragma unroll
+  for (int ks = 0; ks < 5; ++ks) {
+    int k = 16 * ks + 4 * q;
+    in[ks] = pack4(feature(k, x, n, wo), feature(k + 1, x, n, wo), feature(k + 2, x, n, wo), feature(k + 3, x, n, wo));
+  }
+}
+
+// Forward through all layers: raw[3] (D layout of the 48 outputs); hidden
+// operands and pre-activations kept when the caller wants them.
+struct Acts {
+  bf16x4 h[kLayers][5];  // layer inputs (B layout); h[0] has 5 k-steps, others 4
+  f32x4 z[kLayers - 1][4];
+};
+__device__ inline void forwardAll(const Net& net, Acts& a, f32x4* raw) {
+  layerFwd<5, 4>(net.wT + wqOffset(0), net.param + bOffset(0), kInPad, a.h[0], a.z[0]);
+#pragma unroll
+  for (int mt = 0; mt < 4; ++mt) a.h[1][mt] = reluPack(a.z[0][mt]);
+  layerFwd<4, 4>(net.wT + wqOffset(1), net.param + bOffset(1), kHidden, a.h[1], a.z[1]);
+#pragma unroll
+  for (int mt = 0; mt < 4; ++mt) a.h[2][mt] = reluPack(a.z[1][mt]);
+  layerFwd<4, 4>(net.wT + wqOffset(2), net.param + bOffset(2), kHidden, a.h[2], a.z[2]);
+#pragma unroll
+  for (int mt = 0; mt < 4; ++mt) a.h[3][mt] = reluPack(a.z[2][mt]);
+  layerFwd<4, 4>(net.wT + wqOffset(3), net.param + bOffset(3), kHidden, a.h[3], a.z[3]);
+#pragma unroll
+  for (int mt = 0; mt < 4; ++mt) a.h[4][mt] = reluPack(a.z[3][mt]);
+  layerFwd<4, 3>(net.wT + wqOffset(4), net.param + bOffset(4), kHidden, a.h[4], raw);
+}
+
+__device__ inline hdev::V3 normPos(const Net& net, hdev::V3 p) {
+  return hdev::v3((p.x - net.boxLo[0]) * net.boxInv[0], (p.y - net.boxLo[1]) * net.boxInv[1],
+            (p.z - net.boxLo[2]) * net.boxInv[2]);
+}
+
+// Raw outputs of one query per lane group: every lane of the wave calls this
+// (MFMA is wave-wide); lane l supplies query l % 16 and receives outputs
+// 16 mt + 4 (l / 16) + v (mt < 3, v < 4) of that query in raw[4 mt + v].
+__device__ inline void inferOne(const Net& net, hdev::V3 x, hdev::V3 n, hdev::V3 wo, float* raw) {
+  Acts a;
+  inputOperands(normPos(net, x), n, wo, a.h[0]);
+  f32x4 o[3];
+  forwardAll(net, a, o);
+#pragma unroll
+  for (int mt = 0; mt < 3; ++mt) {
+    raw[4 * mt] = o[mt].x;
+    raw[4 * mt + 1] = o[mt].y;
+    raw[4 * mt + 2] = o[mt].z;
+    raw[4 * mt + 3] = o[mt].w;
+  }
+}
+
+}  // namespace guide

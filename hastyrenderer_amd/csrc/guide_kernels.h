@@ -1,0 +1,61 @@
+// Host side of the MLP path guide: parameters, Adam state, training buffers.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <vector>
+
+#include "guide.h"
+
+namespace guide {
+
+class Guide {
+ public:
+  explicit Guide(hipStream_t stream);
+  ~Guide();
+  Guide(const Guide&) = delete;
+  Guide& operator=(const Guide&) = delete;
+
+  // He-uniform hidden layers, a small output layer, zero biases and padding.
+  static void initParams(uint32_t seed, float* host);
+  void setParams(const float* host);  // also resets Adam
+  void getParams(float* host);
+  void setBox(const float lo[3], const float hi[3]);
+  Net net() const;
+
+  // n queries (pos, normal, wo as 3 float4 each) -> 48 raw outputs each.
+  void inferDevice(const float4* dIn, uint32_t n, float4* dRaw);
+  // Adds d(mean loss)/d(params) of n samples (pos + F, normal, wo, wi) to
+  // the gradient buffer; returns the mean loss.
+  float gradientDevice(const float4* dSamples, uint32_t n);
+  void adamStep(float lr);  // consumes and clears the gradient
+  // One Adam step on n samples whose weights are first normalised to mean 1
+  // and capped at weightCap (in place); returns the mean loss.
+  float trainOnDevice(float4* dSamples, uint32_t n, float lr, float weightCap);
+  void getGrad(float* host);
+  void zeroGrad();
+  uint32_t steps() const { return step_; }
+  hipStream_t stream() const { return stream_; }
+
+ private:
+  void ensureTrain(uint32_t batch);
+  hipStream_t stream_;
+  float* dParam_ = nullptr;
+  float* dM_ = nullptr;
+  float* dV_ = nullptr;
+  float* dGrad_ = nullptr;
+  short* dWT_ = nullptr;
+  short* dW_ = nullptr;
+  float* dLoss_ = nullptr;
+  void* dTileJobs_ = nullptr;
+  uint32_t nTileJobs_ = 0;
+  short* dH_ = nullptr;
+  short* dG_ = nullptr;
+  uint32_t trainCap_ = 0;
+  float4* dIn_ = nullptr;
+  float4* dRaw_ = nullptr;
+  float boxLo_[3], boxInv_[3];
+  uint32_t step_ = 0;
+};
+
+}  // namespace guide

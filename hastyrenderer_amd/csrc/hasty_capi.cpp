@@ -18,6 +18,7 @@
 #include "rng.h"
 #include "scene_io.h"
 #include "sppm.h"
+#include "guide_kernels.h"
 
 namespace {
 
@@ -157,6 +158,19 @@ struct HastyScene {
   uint2* traceOverflow = nullptr;
   uint32_t traceOverflowThreads = 0;
 
+  // guided rendering
+  HastyGuide* guide = nullptr;
+  float guideFraction = 0.5f;
+  float guideLr = 0.0f;
+  float4* guideRaw = nullptr;
+  float4* guideRec = nullptr;
+  float4* guideSamples = nullptr;
+  uint32_t* guideCount = nullptr;
+  uint32_t guideSlots = 0;
+  static constexpr uint32_t kGuideSampleCap = 1u << 18;
+  float lastGuideLoss = 0.0f;
+  uint32_t lastGuideSamples = 0;
+
   bool profiling = false;  // HIP-event timing of every launch
   bool counting = false;   // device ray / node / triangle counters
   std::unique_ptr<EventTimer> timer;
@@ -189,6 +203,8 @@ struct HastyScene {
     freePath();
     for (void* p : sceneAllocs) (void)hipFree(p);
     if (dTileOrder) (void)hipFree(dTileOrder);
+    for (void* p : {(void*)guideRaw, (void*)guideRec, (void*)guideSamples, (void*)guideCount})
+      if (p) (void)hipFree(p);
     if (dFilm) (void)hipFree(dFilm);
     if (traceOverflow) (void)hipFree(traceOverflow);
     if (stream) (void)hipStreamDestroy(stream);
@@ -290,7 +306,14 @@ static std::vector<float2> haltonOffsets(uint32_t first, uint32_t n) {
 // Renders the items through the path pool and stream of `owner`.  Items are
 // grouped so that each renderJobs call stays below 2^31 paths; a single item
 // larger than that is split by samples.
+static void renderGuided(HastyScene* s, const BatchItem& it);
+
 static void renderBatch(HastyScene* owner, const BatchItem* items, uint32_t nItems) {
+  if (owner->guide) {
+    if (nItems != 1 || items[0].scene != owner) throw std::runtime_error("guided rendering takes one job of its scene");
+    renderGuided(owner, items[0]);
+    return;
+  }
   owner->ensurePathBuffers();
   std::vector<void*> tmp;
   struct FreeAll {
@@ -356,6 +379,26 @@ static void renderBatch(HastyScene* owner, const BatchItem* items, uint32_t nIte
   if (owner->profiling) owner->timer->collect();
 }
 
+struct HastyGuide {
+  HastyScene* scene = nullptr;
+  std::unique_ptr<guide::Guide> g;
+  float4* dBuf = nullptr;
+  size_t bufFloats = 0;
+  float4* buffer(size_t floats) {
+    if (floats > bufFloats) {
+      if (dBuf) (void)hipFree(dBuf);
+      dBuf = nullptr;
+      if (hipMalloc(&dBuf, floats * sizeof(float)) != hipSuccess) throw std::runtime_error("hipMalloc failed");
+      bufFloats = floats;
+    }
+    return dBuf;
+  }
+  ~HastyGuide() {
+    g.reset();
+    if (dBuf) (void)hipFree(dBuf);
+  }
+};
+
 struct HastySppm {
   HastyScene* scene = nullptr;
   std::unique_ptr<hasty::SppmRenderer> r;
@@ -365,6 +408,76 @@ struct HastySppm {
     if (dFilm) (void)hipFree(dFilm);
   }
 };
+
+// Guided rendering, one sample index at a time: render it with the current
+// guide, then (when training) one Adam step on the radiance samples its paths
+// produced.
+static void renderGuided(HastyScene* s, const BatchItem& it) {
+  s->ensurePathBuffers();
+  s->ensureTileOrder();
+  uint32_t slots = s->pb.capacity;
+  if (s->guideSlots != slots) {
+    for (void* p : {(void*)s->guideRaw, (void*)s->guideRec})
+      if (p) (void)hipFree(p);
+    HIPCHECK(hipMalloc(&s->guideRaw, sizeof(float4) * 10 * (size_t)slots));
+    HIPCHECK(hipMalloc(&s->guideRec, sizeof(float4) * 12 * (size_t)slots));
+    s->guideSlots = slots;
+  }
+  if (!s->guideSamples) {
+    HIPCHECK(hipMalloc(&s->guideSamples, sizeof(float4) * 4 * (size_t)HastyScene::kGuideSampleCap));
+    HIPCHECK(hipMalloc(&s->guideCount, sizeof(uint32_t)));
+  }
+  hasty::GuideRun gr;
+  gr.net = s->guide->g->net();
+  gr.raw = s->guideRaw;
+  gr.rec = s->guideRec;
+  gr.samples = s->guideSamples;
+  gr.sampleCount = s->guideCount;
+  gr.sampleCap = HastyScene::kGuideSampleCap;
+  gr.fraction = s->guideFraction;
+  s->launch.countStats = s->counting;
+  if (s->profiling && !s->timer) s->timer = std::make_unique<EventTimer>();
+  hasty::KernelTimer* timer = s->profiling ? s->timer.get() : nullptr;
+  void* dOff = nullptr;
+  HIPCHECK(hipMalloc(&dOff, sizeof(float2)));
+  struct FreeOff {
+    void* p;
+    ~FreeOff() { (void)hipFree(p); }
+  } freeOff{dOff};
+  for (uint32_t k = 0; k < it.n; ++k) {
+    std::vector<float2> off = haltonOffsets(it.first + k, 1);
+    HIPCHECK(hipMemcpy(dOff, off.data(), sizeof(float2), hipMemcpyHostToDevice));
+    HIPCHECK(hipMemsetAsync(s->guideCount, 0, sizeof(uint32_t), s->stream));
+    hasty::RenderJobDesc jd;
+    jd.scene = s->dev;
+    jd.scene.seed = it.seed;
+    jd.first = it.first + k;
+    jd.n = 1;
+    jd.dOffsets = (const float2*)dOff;
+    jd.dTileOrder = s->dTileOrder;
+    jd.dRankOf = s->rankOf();
+    jd.dColor = it.color;
+    jd.dNormal = it.normal;
+    jd.dAlbedo = it.albedo;
+    hasty::renderJobs({jd}, s->pb, s->launch, s->stream, timer, &gr);
+    if (s->guideLr > 0.0f) {
+      uint32_t n = 0;
+      HIPCHECK(hipMemcpy(&n, s->guideCount, sizeof(uint32_t), hipMemcpyDeviceToHost));
+      n = std::min(n, HastyScene::kGuideSampleCap);
+      s->lastGuideSamples = n;
+      if (n > 0) s->lastGuideLoss = s->guide->g->trainOnDevice(s->guideSamples, n, s->guideLr, 20.0f);
+      gr.net = s->guide->g->net();
+    }
+  }
+  HIPCHECK(hipStreamSynchronize(s->stream));
+  if (s->counting) {
+    unsigned long long st[hasty::kNumStats];
+    HIPCHECK(hipMemcpy(st, s->pb.stats, sizeof(st), hipMemcpyDeviceToHost));
+    HIPCHECK(hipMemset(s->pb.stats, 0, sizeof(st)));
+    for (int i = 0; i < hasty::kNumStats; ++i) s->stats[i] += st[i];
+  }
+  if (s->profiling) s->timer->collect();
+}
 
 static void buildScene(HastyScene& S, const char* path) {
   hasty::loadJob(path, S.job);
@@ -930,6 +1043,188 @@ int hasty_sppm_photons(HastySppm* h, float* out, uint64_t maxPhotons) {
   try {
     HIPCHECK(hipSetDevice(h->scene->device));
     h->r->photons(out, (size_t)maxPhotons);
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e, 3);
+  }
+}
+
+}  // extern "C"
+// ------------------------------------------------------------ path guide ----
+extern "C" {
+
+uint32_t hasty_guide_num_params(void) { return (uint32_t)guide::kParams; }
+
+int hasty_guide_create(HastyScene* s, uint32_t seed, HastyGuide** out) {
+  if (!s || !out) {
+    g_lastError = "null argument";
+    return 1;
+  }
+  try {
+    HIPCHECK(hipSetDevice(s->device));
+    auto h = std::make_unique<HastyGuide>();
+    h->scene = s;
+    h->g = std::make_unique<guide::Guide>(s->stream);
+    std::vector<float> p(guide::kParams);
+    guide::Guide::initParams(seed, p.data());
+    h->g->setParams(p.data());
+    // scene box from the triangle positions
+    const std::vector<float>& tp = s->job.scene.triPos;
+    float lo[3] = {3.0e38f, 3.0e38f, 3.0e38f}, hi[3] = {-3.0e38f, -3.0e38f, -3.0e38f};
+    for (size_t i = 0; i + 2 < tp.size(); i += 3)
+      for (int c = 0; c < 3; ++c) {
+        lo[c] = std::min(lo[c], tp[i + c]);
+        hi[c] = std::max(hi[c], tp[i + c]);
+      }
+    if (tp.empty()) lo[0] = lo[1] = lo[2] = 0.0f, hi[0] = hi[1] = hi[2] = 1.0f;
+    h->g->setBox(lo, hi);
+    *out = h.release();
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e, 2);
+  }
+}
+
+void hasty_guide_destroy(HastyGuide* g) { delete g; }
+
+int hasty_guide_set_box(HastyGuide* g, const float* lo, const float* hi) {
+  if (!g || !lo || !hi) {
+    g_lastError = "null argument";
+    return 1;
+  }
+  g->g->setBox(lo, hi);
+  return 0;
+}
+
+int hasty_guide_set_params(HastyGuide* g, const float* p) {
+  if (!g || !p) {
+    g_lastError = "null argument";
+    return 1;
+  }
+  try {
+    g->g->setParams(p);
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e, 3);
+  }
+}
+
+int hasty_guide_get_params(HastyGuide* g, float* p) {
+  if (!g || !p) {
+    g_lastError = "null argument";
+    return 1;
+  }
+  try {
+    g->g->getParams(p);
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e, 3);
+  }
+}
+
+// queries: n x 9 floats (position, normal, wo); raw: n x 48 floats.
+int hasty_guide_infer(HastyGuide* g, const float* queries, uint32_t n, float* raw) {
+  if (!g || (n && (!queries || !raw))) {
+    g_lastError = "null argument";
+    return 1;
+  }
+  if (n == 0) return 0;
+  try {
+    HIPCHECK(hipSetDevice(g->scene->device));
+    std::vector<float4> in((size_t)n * 3);
+    for (uint32_t i = 0; i < n; ++i)
+      for (int k = 0; k < 3; ++k)
+        in[3 * i + k] = make_float4(queries[9 * i + 3 * k], queries[9 * i + 3 * k + 1], queries[9 * i + 3 * k + 2], 0.0f);
+    float4* d = g->buffer((size_t)n * 12 + (size_t)n * 48);
+    float4* dRaw = d + (size_t)n * 3;
+    hipStream_t st = g->g->stream();
+    HIPCHECK(hipMemcpyAsync(d, in.data(), sizeof(float4) * in.size(), hipMemcpyHostToDevice, st));
+    g->g->inferDevice(d, n, dRaw);
+    HIPCHECK(hipMemcpyAsync(raw, dRaw, sizeof(float) * 48 * (size_t)n, hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipStreamSynchronize(st));
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e, 3);
+  }
+}
+
+// samples: n x 13 floats (position, normal, wo, wi, weight F).  Adds the
+// gradient of the mean loss -F log q(wi) to the guide's gradient buffer (the
+// buffer is cleared first when `clear`), optionally copies it to grad_out,
+// and returns the mean loss in *loss.
+static void guideGradient(HastyGuide* g, const float* smp, uint32_t n, bool clear, float* gradOut, float* loss) {
+  std::vector<float4> in((size_t)n * 4);
+  for (uint32_t i = 0; i < n; ++i) {
+    const float* s = smp + 13 * (size_t)i;
+    in[4 * i] = make_float4(s[0], s[1], s[2], s[12]);
+    in[4 * i + 1] = make_float4(s[3], s[4], s[5], 0.0f);
+    in[4 * i + 2] = make_float4(s[6], s[7], s[8], 0.0f);
+    in[4 * i + 3] = make_float4(s[9], s[10], s[11], 0.0f);
+  }
+  float4* d = g->buffer((size_t)n * 16);
+  hipStream_t st = g->g->stream();
+  HIPCHECK(hipMemcpyAsync(d, in.data(), sizeof(float4) * in.size(), hipMemcpyHostToDevice, st));
+  if (clear) g->g->zeroGrad();
+  float l = g->g->gradientDevice(d, n);
+  if (loss) *loss = l;
+  if (gradOut) g->g->getGrad(gradOut);
+}
+
+int hasty_guide_gradient(HastyGuide* g, const float* samples, uint32_t n, float* grad, float* loss) {
+  if (!g || !samples || n == 0) {
+    g_lastError = "null argument";
+    return 1;
+  }
+  try {
+    HIPCHECK(hipSetDevice(g->scene->device));
+    guideGradient(g, samples, n, true, grad, loss);
+    g->g->zeroGrad();
+    HIPCHECK(hipStreamSynchronize(g->g->stream()));
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e, 3);
+  }
+}
+
+int hasty_scene_set_guide(HastyScene* s, HastyGuide* g, float fraction, float trainLr) {
+  if (!s) {
+    g_lastError = "null argument";
+    return 1;
+  }
+  if (g && g->scene != s) {
+    g_lastError = "the guide belongs to another scene";
+    return 1;
+  }
+  if (g && !(fraction > 0.0f && fraction < 1.0f)) {
+    g_lastError = "guide fraction must be in (0, 1)";
+    return 1;
+  }
+  s->guide = g;
+  s->guideFraction = fraction;
+  s->guideLr = trainLr;
+  return 0;
+}
+
+int hasty_scene_guide_stats(const HastyScene* s, float* lastLoss, uint32_t* lastSamples) {
+  if (!s) {
+    g_lastError = "null argument";
+    return 1;
+  }
+  if (lastLoss) *lastLoss = s->lastGuideLoss;
+  if (lastSamples) *lastSamples = s->lastGuideSamples;
+  return 0;
+}
+
+int hasty_guide_train_step(HastyGuide* g, const float* samples, uint32_t n, float lr, float* loss) {
+  if (!g || !samples || n == 0) {
+    g_lastError = "null argument";
+    return 1;
+  }
+  try {
+    HIPCHECK(hipSetDevice(g->scene->device));
+    guideGradient(g, samples, n, true, nullptr, loss);
+    g->g->adamStep(lr);
+    HIPCHECK(hipStreamSynchronize(g->g->stream()));
     return 0;
   } catch (const std::exception& e) {
     return fail(e, 3);

@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "guide.h"
 #include "pt_device.h"
 
 #include <vector>
@@ -73,6 +74,20 @@ struct PassInfo {
   const float2* offsets;      // per sample of the call
 };
 
+// Guided rendering (single job): per-slot raw guide outputs, per-slot
+// training records (2 x {position, normal, wo, wi, throughput before the
+// sample, radiance before the continuation}) and the training samples
+// harvested when paths end (4 float4 each: position + weight, normal, wo, wi).
+struct GuideRun {
+  guide::Net net;
+  float4* raw;      // 10 float4 per slot
+  float4* rec;      // 12 float4 per slot
+  float4* samples;  // sampleCap x 4 float4
+  uint32_t* sampleCount;
+  uint32_t sampleCap;
+  float fraction;   // probability of drawing the direction from the guide
+};
+
 // The jobs of a render call.  With one job its scene and pass travel in the
 // kernel arguments (`s`, `pi`); with several, kernels look them up by path id.
 struct JobSet {
@@ -82,6 +97,7 @@ struct JobSet {
   const PassInfo* passes;
   uint32_t nJobs;
   uint32_t paths;  // over all jobs
+  GuideRun g;      // read by the guided kernels only
 };
 
 constexpr int kStackLds = 16;
@@ -108,6 +124,7 @@ enum KernelId {
   kKMisc,
   kKRetire,
   kKGraph,
+  kKGuideInfer,
   kKSppmRadii,
   kKSppmCellSize,
   kKSppmPhotons,
@@ -142,7 +159,7 @@ struct RenderJobDesc {
 // in flight together; at most 2^31 paths per call).  Synchronises with the
 // stream once per cfg.chunk bounces.
 void renderJobs(const std::vector<RenderJobDesc>& jobs, const PathBuffers& pb, const LaunchConfig& cfg,
-                hipStream_t stream, KernelTimer* timer);
+                hipStream_t stream, KernelTimer* timer, const GuideRun* guide = nullptr);
 
 // Pixels of a width x height film in 8x8-tile order (row-major inside a
 // tile), and the inverse map.

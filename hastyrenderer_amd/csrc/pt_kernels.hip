@@ -39,6 +39,7 @@ const char* kernelName(int id) {
     case kKAccumulate: return "pt_accumulate";
     case kKRetire: return "pt_retire";
     case kKGraph: return "pt_graph_chunk";
+    case kKGuideInfer: return "guide_infer";
     case kKSppmRadii: return "sppm_init_radii";
     case kKSppmCellSize: return "sppm_cell_size";
     case kKSppmPhotons: return "sppm_photons";
@@ -234,7 +235,10 @@ __global__ void __launch_bounds__(kBlock) kExtend(JobSet js, PathBuffers pb, int
 }
 
 // ---------------------------------------------------------------- shade ----
-template <bool MULTI>
+// Bits 20-21 of the flags: training records of the path (guided rendering).
+constexpr uint32_t kRecShift = 20;
+
+template <bool MULTI, bool GUIDED>
 __global__ void __launch_bounds__(kBlock) kShade(JobSet js, PathBuffers pb) {
   __shared__ uint32_t ldsWave[kBlock / 64];
   __shared__ uint32_t ldsBase;
@@ -264,6 +268,11 @@ __global__ void __launch_bounds__(kBlock) kShade(JobSet js, PathBuffers pb) {
       V3 beta = f4xyz(b4);
       float4 L4 = pb.L[slot];
       V3 L = f4xyz(L4);
+      uint32_t nrec = GUIDED ? (st.y >> kRecShift) & 3u : 0u;
+      if (GUIDED && depth > 0 && nrec > 0 && ((st.y >> (kRecShift + 2)) & 1u)) {
+        // radiance before the continuation of the last recorded vertex
+        js.g.rec[(size_t)slot * 12 + 6 * (nrec - 1) + 5] = L4;
+      }
       hrng::Stream rng;
       {
         uint32_t k, pixel;
@@ -329,6 +338,20 @@ __global__ void __launch_bounds__(kBlock) kShade(JobSet js, PathBuffers pb) {
               if (depth + 1 >= s.maxDepth) {
                 alive = false;  // every deeper estimate is zero
               } else {
+                // guided vertex: glossy or diffuse reflection (no delta lobe)
+                const bool guided = GUIDED && m.alpha > 0.0f && m.transmission == 0.0f;
+                guide::Mixture gm;
+                if (GUIDED && guided) {
+                  float raw[guide::kOut];
+                  const float4* gr = js.g.raw + (size_t)slot * 10;
+#pragma unroll
+                  for (int i = 0; i < 10; ++i) {
+                    float4 v = gr[i];
+                    raw[4 * i] = v.x, raw[4 * i + 1] = v.y, raw[4 * i + 2] = v.z, raw[4 * i + 3] = v.w;
+                  }
+                  gm = guide::mixtureFromRaw(raw);
+                }
+                const float gf = GUIDED ? js.g.fraction : 0.0f;
                 if (twoStrategies) {
                   // ---- light strategy (MIS.cpp:32-54, Scene.cpp:538-586)
                   float pSurf = !s.hasEnv ? 1.0f : (s.hasSurfaceLight ? 0.5f : 0.0f);
@@ -357,6 +380,7 @@ __global__ void __launch_bounds__(kBlock) kShade(JobSet js, PathBuffers pb) {
                   BsdfEval ev = bsdfEvaluate(m, it, wo, sd);
                   float outsideIOR = mediaOutsideIOR(s, md, it, wo, wavelength);
                   float pB = bsdfPdf(m, it, wo, sd, outsideIOR);
+                  if (GUIDED && guided) pB = gf * guide::mixturePdf(gm, sd) + (1.0f - gf) * pB;
                   Media mc = md;
                   mediaUpdate(mc, it, wo, sd);
                   V3 Tc = mediaTransmittance(s, mc);
@@ -380,7 +404,33 @@ __global__ void __launch_bounds__(kBlock) kShade(JobSet js, PathBuffers pb) {
                   sampleProbability = p0;
                 }
                 float outsideIOR = mediaOutsideIOR(s, md, it, wo, wavelength);
-                BsdfSample bs = bsdfSample(m, rng, it, wo, wavelength, outsideIOR, false, 3);
+                BsdfSample bs;
+                if (GUIDED && guided) {
+                  // one-sample MIS of guide and BSDF: p = f pG + (1 - f) pB
+                  if (rng.uniform01f() < gf) {
+                    float x0 = rng.uniform01f(), x1 = rng.uniform01f(), x2 = rng.uniform01f();
+                    bs.dir = guide::mixtureSample(gm, x0, x1, x2);
+                    bs.outside = dot(it.ng, wo) >= 0.0f;
+                    BsdfEval ev = bsdfEvaluate(m, it, wo, bs.dir);
+                    float pB = bsdfPdf(m, it, wo, bs.dir, outsideIOR);
+                    bs.pdf = gf * guide::mixturePdf(gm, bs.dir) + (1.0f - gf) * pB;
+                    float mm = bs.pdf > 0.0f ? __builtin_fabsf(dot(bs.dir, it.ng)) / bs.pdf : 0.0f;
+                    bs.tD = ev.fD * mm;
+                    bs.tC = ev.fC * mm;
+                    if (!(bs.pdf > 0.0f)) bs.pdf = 0.0f;
+                  } else {
+                    bs = bsdfSample(m, rng, it, wo, wavelength, outsideIOR, false, 3);
+                    if (bs.pdf != 0.0f) {
+                      float pMix = gf * guide::mixturePdf(gm, bs.dir) + (1.0f - gf) * bs.pdf;
+                      float sc = bs.pdf / pMix;
+                      bs.tD = bs.tD * sc;
+                      bs.tC = bs.tC * sc;
+                      bs.pdf = pMix;
+                    }
+                  }
+                } else {
+                  bs = bsdfSample(m, rng, it, wo, wavelength, outsideIOR, false, 3);
+                }
                 bs.tD = bs.tD / sampleProbability;
                 bs.tC = bs.tC / sampleProbability;
                 if (bs.pdf != 0.0f) {
@@ -396,7 +446,24 @@ __global__ void __launch_bounds__(kBlock) kShade(JobSet js, PathBuffers pb) {
                 }
                 alive = false;
                 if (pushNext) {
-                  pb.state[slot] = make_uint4(rng.counter, packFlags(depth + 1, wavelength, md), (uint32_t)tri, st.w);
+                  uint32_t recBits = 0u;
+                  if (GUIDED) {
+                    if (guided && nrec < 2u) {
+                      float4* rr = js.g.rec + (size_t)slot * 12 + 6 * nrec;
+                      V3 nf = dot(it.ng, wo) >= 0.0f ? it.ng : -it.ng;
+                      rr[0] = f4(it.x, 0.0f);
+                      rr[1] = f4(nf, 0.0f);
+                      rr[2] = f4(wo, 0.0f);
+                      rr[3] = f4(bs.dir, 0.0f);
+                      rr[4] = f4(beta, 0.0f);
+                      nrec += 1u;
+                      recBits = (nrec << kRecShift) | (1u << (kRecShift + 2));
+                    } else {
+                      recBits = nrec << kRecShift;
+                    }
+                  }
+                  pb.state[slot] =
+                      make_uint4(rng.counter, packFlags(depth + 1, wavelength, md) | recBits, (uint32_t)tri, st.w);
                   storeMedia(pb, slot, md);
                 }
               }
@@ -504,7 +571,8 @@ __global__ void __launch_bounds__(kBlock) kConnect(JobSet js, PathBuffers pb, in
 // --------------------------------------------------------------- retire ----
 // Finished paths: results into the ring; the slot becomes idle and the next
 // kFill gives it a new path id.
-__global__ void __launch_bounds__(kBlock) kRetire(PathBuffers pb) {
+template <bool GUIDED>
+__global__ void __launch_bounds__(kBlock) kRetire(JobSet js, PathBuffers pb) {
   __shared__ uint32_t ldsWave[kBlock / 64];
   __shared__ uint32_t ldsBase;
   uint32_t n = pb.counters[kCntR];
@@ -515,8 +583,33 @@ __global__ void __launch_bounds__(kBlock) kRetire(PathBuffers pb) {
     uint32_t slot = 0;
     if (live) {
       slot = pb.qR[q];
-      uint32_t ri = pb.state[slot].w % pb.ringPaths;
-      pb.ringL[ri] = pb.L[slot];
+      uint4 st = pb.state[slot];
+      uint32_t ri = st.w % pb.ringPaths;
+      float4 Lf = pb.L[slot];
+      pb.ringL[ri] = Lf;
+      if (GUIDED) {
+        // training samples: weight = luminance of the radiance the
+        // continuation brought, over the throughput before the sample
+        uint32_t nrec = (st.y >> kRecShift) & 3u;
+        for (uint32_t r = 0; r < nrec; ++r) {
+          const float4* rr = js.g.rec + (size_t)slot * 12 + 6 * r;
+          float4 b = rr[4], ls = rr[5];
+          float cx = b.x > 0.0f ? (Lf.x - ls.x) / b.x : 0.0f;
+          float cy = b.y > 0.0f ? (Lf.y - ls.y) / b.y : 0.0f;
+          float cz = b.z > 0.0f ? (Lf.z - ls.z) / b.z : 0.0f;
+          float F = 0.2126f * cx + 0.7152f * cy + 0.0722f * cz;
+          if (F > 0.0f && F < 3.0e38f) {
+            uint32_t k = atomicAdd(js.g.sampleCount, 1u);
+            if (k < js.g.sampleCap) {
+              float4 x = rr[0];
+              js.g.samples[4 * (size_t)k] = make_float4(x.x, x.y, x.z, F);
+              js.g.samples[4 * (size_t)k + 1] = rr[1];
+              js.g.samples[4 * (size_t)k + 2] = rr[2];
+              js.g.samples[4 * (size_t)k + 3] = rr[3];
+            }
+          }
+        }
+      }
       pb.ringN[ri] = pb.auxN[slot];
       pb.ringA[ri] = pb.auxA[slot];
     }
@@ -549,6 +642,36 @@ __global__ void __launch_bounds__(kBlock) kAccumulate(PathBuffers pb, PassInfo p
     color[p * 3] = c0, color[p * 3 + 1] = c1, color[p * 3 + 2] = c2;
     normal[p * 3] = n0, normal[p * 3 + 1] = n1, normal[p * 3 + 2] = n2;
     albedo[p * 3] = a0, albedo[p * 3 + 1] = a1, albedo[p * 3 + 2] = a2;
+  }
+}
+
+// ------------------------------------------------------------ guide infer ----
+// Raw guide outputs for every path in qA that hit a surface: inputs are the
+// hit point, the geometric normal facing wo, and wo; 16 paths per wave.
+__global__ void __launch_bounds__(kBlock) kGuideInfer(JobSet js, PathBuffers pb) {
+  const DevScene& s = js.s;
+  const uint32_t n = pb.counters[kCntA];
+  const int lane = threadIdx.x & 63, r = lane & 15, q = lane >> 4;
+  uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  uint32_t waves = (gridDim.x * blockDim.x) >> 6;
+  for (uint32_t t = wave; t * 16 < n; t += waves) {
+    uint32_t e = t * 16 + r;
+    bool valid = e < n;
+    uint32_t slot = pb.qA[valid ? e : n - 1];
+    float4 h4 = pb.hit[slot];
+    int32_t tri = __float_as_int(h4.w);
+    V3 x = v3(0.0f, 0.0f, 0.0f), ng = v3(0.0f, 0.0f, 1.0f);
+    if (tri >= 0) hitPointNormal(s, tri, h4.y, h4.z, &x, &ng);
+    V3 wo = -f4xyz(pb.rayD[slot]);
+    if (dot(ng, wo) < 0.0f) ng = -ng;
+    float raw[12];
+    guide::inferOne(js.g.net, x, ng, wo, raw);
+    if (valid && tri >= 0) {
+      float4* out = js.g.raw + (size_t)slot * 10;
+#pragma unroll
+      for (int mt = 0; mt < 3; ++mt)
+        if (4 * mt + q < 10) out[4 * mt + q] = make_float4(raw[4 * mt], raw[4 * mt + 1], raw[4 * mt + 2], raw[4 * mt + 3]);
+    }
   }
 }
 
@@ -602,7 +725,7 @@ void hipOk(hipError_t e, const char* what) {
 // The bounces of one chunk: fill, extend, shade, connect, retire, advance.
 // `chunk` is even, so the queue pointers are back in place afterwards and
 // every chunk issues identical launches (what a captured graph replays).
-template <bool MULTI>
+template <bool MULTI, bool GUIDED>
 void launchChunk(const JobSet& js, const PathBuffers& pbIn, const LaunchConfig& cfg, uint32_t chunk,
                  hipStream_t stream, KernelTimer* timer) {
   PathBuffers pb = pbIn;
@@ -623,14 +746,19 @@ void launchChunk(const JobSet& js, const PathBuffers& pbIn, const LaunchConfig& 
     t0(kKExtend);
     kExtend<MULTI><<<cfg.traceBlocks, kBlock, 0, stream>>>(js, pb, countStats);
     t1(kKExtend);
+    if (GUIDED) {
+      t0(kKGuideInfer);
+      kGuideInfer<<<2048, kBlock, 0, stream>>>(js, pb);
+      t1(kKGuideInfer);
+    }
     t0(kKShade);
-    kShade<MULTI><<<cfg.shadeBlocks, kBlock, 0, stream>>>(js, pb);
+    kShade<MULTI, GUIDED><<<cfg.shadeBlocks, kBlock, 0, stream>>>(js, pb);
     t1(kKShade);
     t0(kKConnect);
     kConnect<MULTI><<<cfg.traceBlocks, kBlock, 0, stream>>>(js, pb, countStats);
     t1(kKConnect);
     t0(kKRetire);
-    kRetire<<<cfg.shadeBlocks, kBlock, 0, stream>>>(pb);
+    kRetire<GUIDED><<<cfg.shadeBlocks, kBlock, 0, stream>>>(js, pb);
     t1(kKRetire);
     t0(kKMisc);
     kAdvance<<<1, 64, 0, stream>>>(pb.counters);
@@ -644,7 +772,7 @@ void launchChunk(const JobSet& js, const PathBuffers& pbIn, const LaunchConfig& 
 }  // namespace
 
 void renderJobs(const std::vector<RenderJobDesc>& jobs, const PathBuffers& pb, const LaunchConfig& cfg,
-                hipStream_t stream, KernelTimer* timer) {
+                hipStream_t stream, KernelTimer* timer, const GuideRun* guide) {
   const uint32_t nJobs = (uint32_t)jobs.size();
   if (nJobs == 0) return;
   std::vector<DevScene> scenes(nJobs);
@@ -672,6 +800,10 @@ void renderJobs(const std::vector<RenderJobDesc>& jobs, const PathBuffers& pb, c
   js.pi = passes[0];
   js.nJobs = nJobs;
   js.paths = (uint32_t)total;
+  if (guide) {
+    if (nJobs != 1) throw std::runtime_error("renderJobs: guided rendering takes one job");
+    js.g = *guide;
+  }
   void* dScenes = nullptr;
   void* dPasses = nullptr;
   struct Free {
@@ -722,8 +854,9 @@ void renderJobs(const std::vector<RenderJobDesc>& jobs, const PathBuffers& pb, c
   } graphFree{graph, exec};
   if (cfg.useGraph) {
     hipOk(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
-    if (nJobs > 1) launchChunk<true>(js, pb, cfg, chunk, stream, nullptr);
-    else launchChunk<false>(js, pb, cfg, chunk, stream, nullptr);
+    if (nJobs > 1) launchChunk<true, false>(js, pb, cfg, chunk, stream, nullptr);
+    else if (guide) launchChunk<false, true>(js, pb, cfg, chunk, stream, nullptr);
+    else launchChunk<false, false>(js, pb, cfg, chunk, stream, nullptr);
     hipOk(hipStreamEndCapture(stream, &graph), "hipStreamEndCapture");
     hipOk(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0), "hipGraphInstantiate");
   }
@@ -734,9 +867,11 @@ void renderJobs(const std::vector<RenderJobDesc>& jobs, const PathBuffers& pb, c
       hipOk(hipGraphLaunch(exec, stream), "hipGraphLaunch");
       t1(kKGraph);
     } else if (nJobs > 1) {
-      launchChunk<true>(js, pb, cfg, chunk, stream, timer);
+      launchChunk<true, false>(js, pb, cfg, chunk, stream, timer);
+    } else if (guide) {
+      launchChunk<false, true>(js, pb, cfg, chunk, stream, timer);
     } else {
-      launchChunk<false>(js, pb, cfg, chunk, stream, timer);
+      launchChunk<false, false>(js, pb, cfg, chunk, stream, timer);
     }
     uint32_t cnt[kNumCounters];
     hipOk(hipMemcpyAsync(cnt, pb.counters, sizeof(cnt), hipMemcpyDeviceToHost, stream), "counter readback");

@@ -138,7 +138,7 @@ def _box(center, size, angle_deg):
     return v, tris
 
 
-def write_cornell(folder, width=256, height=256, spp=64, max_depth=8, sky=False):
+def write_cornell(folder, width=256, height=256, spp=64, max_depth=8, sky=False, light_up=False):
     """Procedural Cornell box: 5 walls (10 triangles), two boxes rotated
     +-17 degrees (12 triangles each) and a 0.25 x 0.25 quad light of radiance
     (17, 12, 4) just under the ceiling; 36 triangles, black background,
@@ -176,8 +176,12 @@ def write_cornell(folder, width=256, height=256, spp=64, max_depth=8, sky=False)
     obj.smooth(False)
     obj.material("light")
     s = 0.125
-    base = obj.vertices([[-s, 1.98, -s], [s, 1.98, -s], [s, 1.98, s], [-s, 1.98, s]])
-    obj.faces(quad, base)  # normal -y: emits downwards
+    if light_up:  # variant: the light faces the ceiling, the room is lit indirectly
+        base = obj.vertices([[-s, 1.9, -s], [-s, 1.9, s], [s, 1.9, s], [s, 1.9, -s]])
+        obj.faces(quad, base)  # normal +y
+    else:
+        base = obj.vertices([[-s, 1.98, -s], [s, 1.98, -s], [s, 1.98, s], [-s, 1.98, s]])
+        obj.faces(quad, base)  # normal -y: emits downwards
     obj.write(os.path.join(folder, "scene.obj"), "scene.mtl")
     with open(os.path.join(folder, "scene.mtl"), "w") as f:
         f.write("\n".join(mtl))

@@ -153,6 +153,41 @@ int hasty_render_batch_device(const HastyBatchJob* jobs, uint32_t n_jobs);
 int hasty_autotune_batch(const HastyBatchJob* jobs, uint32_t n_jobs, uint32_t trial_samples, HastyLaunch* chosen);
 int hasty_autotune(HastyScene* scene, uint32_t seed, uint32_t trial_samples, HastyLaunch* chosen);
 
+/* ---- learned path guide (no counterpart in the reference) ----
+ * A small MLP (BF16 weights on MFMA, FP32 accumulation) maps (position
+ * normalised to the scene box, normal, outgoing direction) to a mixture of 8
+ * von Mises-Fisher lobes (raw outputs per lobe: mean direction 3, log
+ * concentration 1, logit 1; 48 floats with padding).  Trained with Adam on
+ * the loss -F log q(w_i).  The guide lives on its scene's device and stream. */
+typedef struct HastyGuide HastyGuide;
+uint32_t hasty_guide_num_params(void);
+int hasty_guide_create(HastyScene* scene, uint32_t seed, HastyGuide** out);
+void hasty_guide_destroy(HastyGuide* guide);
+int hasty_guide_set_box(HastyGuide* guide, const float* lo3, const float* hi3);
+int hasty_guide_set_params(HastyGuide* guide, const float* params);  /* resets Adam */
+int hasty_guide_get_params(HastyGuide* guide, float* params);
+/* queries: n x 9 floats (position, normal, wo) -> raw: n x 48 floats */
+int hasty_guide_infer(HastyGuide* guide, const float* queries, uint32_t n, float* raw);
+/* samples: n x 13 floats (position, normal, wo, wi, weight F).  gradient of
+ * the mean loss (params-sized) without changing the guide */
+int hasty_guide_gradient(HastyGuide* guide, const float* samples, uint32_t n, float* grad, float* loss);
+/* one Adam step (beta 0.9 / 0.999, eps 1e-8) on the mean loss of the samples */
+int hasty_guide_train_step(HastyGuide* guide, const float* samples, uint32_t n, float lr, float* loss);
+
+/* Guided rendering of a scene: hasty_render_pt[_device] then draws the
+ * continuation direction at glossy and diffuse vertices from the guide with
+ * probability `fraction` and from the BSDF otherwise, weighting by the
+ * mixture pdf (one-sample MIS; NEE's MIS uses the same mixture), so the
+ * estimate stays unbiased.  Samples are rendered one index at a time; when
+ * train_lr > 0 each index is followed by one Adam step on the radiance
+ * samples its paths produced (weight = luminance of the radiance the
+ * continuation brought over the throughput before the sample, normalised to
+ * mean 1 and capped at 20).  guide NULL returns to plain rendering.  The
+ * guide must belong to the scene; the scene must outlive its use. */
+int hasty_scene_set_guide(HastyScene* scene, HastyGuide* guide, float fraction, float train_lr);
+/* loss and sample count of the last online training step */
+int hasty_scene_guide_stats(const HastyScene* scene, float* last_loss, uint32_t* last_samples);
+
 /* ---- progressive photon mapping (caustic photon map + MIS path tracing) ----
  * Replaces renderPhotonThread (core/src/PhotonMapping.cpp:814-933, declared
  * core/include/Hasty/PhotonMapping.h:124) for one PMRenderJob

@@ -82,6 +82,7 @@ def load_testkernels():
         L.tk_vndf.argtypes = [fp, C.c_uint32, fp]
         L.tk_sampler.argtypes = [C.c_int, C.c_uint32, C.c_uint32, fp, fp, C.c_float, fp]
         L.tk_bsdf_eval.argtypes = [fp, fp, fp, C.c_uint32, fp]
+        L.tk_mfma.argtypes = [fp, fp, fp]
         L.tk_bsdf_sample.argtypes = [fp, fp, fp, C.c_uint32, C.c_uint32, fp]
         _TK = L
     return _TK

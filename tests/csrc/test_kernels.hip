@@ -160,7 +160,48 @@ int finish(float* dOut, float* hOut, size_t n) {
 
 }  // namespace
 
+// One v_mfma_f32_16x16x16_bf16 of A (16x16) and B (16x16), both given
+// row-major as floats and rounded to BF16, with the operand lane layout the
+// guide kernels assume (A: i = l%16, k = 4(l/16)+v; B: j = l%16,
+// k = 4(l/16)+v; D: j = l%16, i = 4(l/16)+v).  One wave, in bounds.
+typedef short tkbf16x4 __attribute__((ext_vector_type(4)));
+typedef float tkf32x4 __attribute__((ext_vector_type(4)));
+__device__ inline short tkBf16(float f) {
+  uint32_t u = __float_as_uint(f);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return (short)(u >> 16);
+}
+__global__ void kMfma(const float* A, const float* B, float* D) {
+  int l = threadIdx.x, r = l & 15, q = l >> 4;
+  tkbf16x4 a, b;
+  a.x = tkBf16(A[r * 16 + 4 * q + 0]);
+  a.y = tkBf16(A[r * 16 + 4 * q + 1]);
+  a.z = tkBf16(A[r * 16 + 4 * q + 2]);
+  a.w = tkBf16(A[r * 16 + 4 * q + 3]);
+  b.x = tkBf16(B[(4 * q + 0) * 16 + r]);
+  b.y = tkBf16(B[(4 * q + 1) * 16 + r]);
+  b.z = tkBf16(B[(4 * q + 2) * 16 + r]);
+  b.w = tkBf16(B[(4 * q + 3) * 16 + r]);
+  tkf32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+  acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
+  D[(4 * q + 0) * 16 + r] = acc.x;
+  D[(4 * q + 1) * 16 + r] = acc.y;
+  D[(4 * q + 2) * 16 + r] = acc.z;
+  D[(4 * q + 3) * 16 + r] = acc.w;
+}
+
 extern "C" {
+
+int tk_mfma(const float* A, const float* B, float* D) {
+  float* dA = toDevice(A, 256);
+  float* dB = toDevice(B, 256);
+  float* dD = devAlloc<float>(256);
+  if (!dA || !dB || !dD) return 1;
+  kMfma<<<1, 64>>>(dA, dB, dD);
+  int rc = finish(dD, D, 256);
+  (void)hipFree(dA), (void)hipFree(dB), (void)hipFree(dD);
+  return rc;
+}
 
 int tk_math(int fn, const float* a, const float* b, uint32_t n, float* out) {
   float* da = toDevice(a, n);

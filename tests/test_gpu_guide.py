@@ -1,0 +1,125 @@
+"""The learned guide's GPU kernels against the CPU model (oracle/guide_ref.py,
+itself pinned by finite differences): the BF16 MFMA lane layout, inference,
+the fused forward/loss/backward + weight-gradient GEMM, Adam, and that
+training lowers the loss on a fixed batch."""
+import math
+
+import numpy as np
+import pytest
+
+import hastyrenderer_amd as ha
+from oracle import guide_ref as gr
+
+from conftest import load_testkernels
+
+pytestmark = pytest.mark.gpu
+
+
+def _guide(cornell_job, seed=3, scale_out=10.0):
+    sc = ha.Scene(cornell_job)
+    g = ha.Guide(sc, seed=seed)
+    g.set_box([0.0, 0.0, 0.0], [1.0, 1.0, 1.0])
+    p = gr.init_params(seed)
+    w4, b4, din, dout = gr.LAYOUT[4]
+    p[w4:w4 + din * dout] *= scale_out
+    g.set_params(p)
+    return sc, g, p
+
+
+def _queries(rng, B):
+    x = rng.uniform(0, 1, (B, 3)).astype(np.float32)
+    n = rng.normal(size=(B, 3))
+    n /= np.linalg.norm(n, axis=1, keepdims=True)
+    wo = rng.normal(size=(B, 3))
+    wo /= np.linalg.norm(wo, axis=1, keepdims=True)
+    return x, n.astype(np.float32), wo.astype(np.float32)
+
+
+def test_mfma_lane_layout():
+    L = load_testkernels()
+    rng = np.random.default_rng(0)
+    A = rng.integers(-8, 9, (16, 16)).astype(np.float32) / 4  # exact in BF16
+    B = rng.integers(-8, 9, (16, 16)).astype(np.float32) / 4
+    D = np.zeros((16, 16), np.float32)
+    assert L.tk_mfma(A, B, D) == 0
+    np.testing.assert_array_equal(D, A @ B)
+
+
+def test_infer_matches_reference(cornell_job):
+    sc, g, p = _guide(cornell_job)
+    rng = np.random.default_rng(1)
+    x, n, wo = _queries(rng, 1000)
+    raw = g.infer(np.concatenate([x, n, wo], 1))
+    ref, _ = gr.forward(p, gr.encode(x, n, wo), bf16_ops=True)
+    err = np.abs(raw - ref)
+    scale = np.sqrt(np.mean(ref[:, :40] ** 2))
+    print("infer: median |err| / rms %.2e, max %.2e" % (np.median(err) / scale, err.max() / scale), flush=True)
+    assert np.median(err) <= 1e-5 * scale
+    assert np.mean(err[:, :40] <= 1e-3 * scale) >= 0.99
+    assert np.all(raw[:, 40:] == 0.0)
+
+
+def _samples(rng, B, target=None):
+    x, n, wo = _queries(rng, B)
+    if target is None:
+        wi = rng.normal(size=(B, 3))
+        wi /= np.linalg.norm(wi, axis=1, keepdims=True)
+    else:
+        wi = target + 0.15 * rng.normal(size=(B, 3))
+        wi /= np.linalg.norm(wi, axis=1, keepdims=True)
+    F = rng.uniform(0.2, 2.0, B).astype(np.float32)
+    return x, n, wo, wi.astype(np.float32), F
+
+
+def test_gradient_matches_reference(cornell_job):
+    sc, g, p = _guide(cornell_job)
+    rng = np.random.default_rng(2)
+    x, n, wo, wi, F = _samples(rng, 1000)
+    grad, loss = g.gradient(np.concatenate([x, n, wo, wi, F[:, None]], 1))
+    raw, cache = gr.forward(p, gr.encode(x, n, wo), bf16_ops=True)
+    lref, g_raw = gr.loss_and_grad_raw(raw, wi.astype(np.float64), F.astype(np.float64))
+    gref = gr.backward(p, cache, g_raw, bf16_ops=True)
+    assert loss == pytest.approx(lref, rel=1e-3)
+    rel = np.linalg.norm(grad - gref) / np.linalg.norm(gref)
+    print("gradient: loss %.6f (ref %.6f), relative L2 error %.2e" % (loss, lref, rel), flush=True)
+    assert rel < 1e-2
+    for (w, b, din, dout) in gr.LAYOUT:
+        a, r = grad[w:w + din * dout], gref[w:w + din * dout]
+        cos = float(a @ r / (np.linalg.norm(a) * np.linalg.norm(r) + 1e-30))
+        assert cos > 0.999
+    # padding stays untouched
+    w0, b0, din0, dout0 = gr.LAYOUT[0]
+    assert np.all(grad[w0:w0 + din0 * dout0].reshape(dout0, din0)[:, 78:] == 0.0)
+
+
+def test_adam_step_matches_reference(cornell_job):
+    sc, g, p = _guide(cornell_job)
+    rng = np.random.default_rng(4)
+    x, n, wo, wi, F = _samples(rng, 512)
+    sm = np.concatenate([x, n, wo, wi, F[:, None]], 1)
+    grad, _ = g.gradient(sm)
+    g.train_step(sm, lr=1e-3)
+    p1 = g.get_params()
+    pref, _, _ = gr.adam_step(p.copy(), np.zeros_like(p), np.zeros_like(p), grad, t=1, lr=1e-3)
+    close = np.abs(p1 - pref) <= 1e-6 + 1e-6 * np.abs(pref)
+    assert close.mean() >= 0.999
+    assert np.all(np.abs(p1 - p) <= 1.01e-3)
+
+
+def test_training_lowers_loss(cornell_job):
+    """Directions clustered about a fixed target: Adam on one fixed batch
+    raises the learned density at the target well above uniform (1 / 4 pi)."""
+    sc, g, p = _guide(cornell_job, scale_out=1.0)
+    rng = np.random.default_rng(5)
+    target = np.array([0.3, 0.8, -0.52])
+    target /= np.linalg.norm(target)
+    x, n, wo, wi, F = _samples(rng, 4096, target)
+    sm = np.concatenate([x, n, wo, wi, F[:, None]], 1)
+    first = g.train_step(sm, lr=3e-3)
+    for _ in range(150):
+        last = g.train_step(sm, lr=3e-3)
+    print("training: loss %.4f -> %.4f" % (first, last), flush=True)
+    assert last < first - 1.0
+    raw = g.infer(np.concatenate([x[:64], n[:64], wo[:64]], 1)).astype(np.float64)
+    dens = gr.pdf(raw, np.tile(target, (64, 1)))
+    assert np.median(dens) > 5.0 / (4 * math.pi)

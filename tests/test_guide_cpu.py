@@ -1,0 +1,117 @@
+"""Pins the CPU model of the learned guide (oracle/guide_ref.py) without a GPU:
+central finite differences of its loss against its analytic backward pass,
+closed forms of the von Mises-Fisher lobe (normalisation, mean direction
+A(kappa) = coth(kappa) - 1/kappa), and Adam's first step."""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import guide_ref as gr
+
+
+def _batch(rng, B):
+    x = rng.uniform(0, 1, (B, 3))
+    n = rng.normal(size=(B, 3))
+    n /= np.linalg.norm(n, axis=1, keepdims=True)
+    wo = rng.normal(size=(B, 3))
+    wo /= np.linalg.norm(wo, axis=1, keepdims=True)
+    wi = rng.normal(size=(B, 3))
+    wi /= np.linalg.norm(wi, axis=1, keepdims=True)
+    F = rng.uniform(0.1, 2.0, B)
+    return x, n, wo, wi, F
+
+
+def _loss(params, feats, wi, F):
+    raw, _ = gr.forward(params, feats)
+    loss, _ = gr.loss_and_grad_raw(raw, wi, F)
+    return loss
+
+
+def test_backward_matches_finite_differences():
+    rng = np.random.default_rng(3)
+    params = gr.init_params(1).astype(np.float64)
+    # make the output layer non-trivial so every output gradient matters
+    w4, b4, din, dout = gr.LAYOUT[4]
+    params[w4:w4 + din * dout] *= 10.0
+    x, n, wo, wi, F = _batch(rng, 6)
+    feats = gr.encode(x, n, wo)
+    raw, cache = gr.forward(params, feats)
+    _, g_raw = gr.loss_and_grad_raw(raw, wi, F)
+    grad = gr.backward(params, cache, g_raw)
+    idx = []
+    for (w, b, din, dout), nin, nout in zip(gr.LAYOUT, (78, 64, 64, 64, 64), (64, 64, 64, 64, 40)):
+        for _ in range(6):
+            idx.append(w + rng.integers(nout) * din + rng.integers(nin))
+        idx.append(b + rng.integers(nout))
+    h = 1e-6
+    for i in idx:
+        p = params.copy()
+        p[i] += h
+        lp = _loss(p, feats, wi, F)
+        p[i] -= 2 * h
+        lm = _loss(p, feats, wi, F)
+        fd = (lp - lm) / (2 * h)
+        assert abs(fd - grad[i]) <= 1e-5 * max(1.0, abs(fd)), (i, fd, grad[i])
+
+
+def test_raw_gradient_matches_finite_differences():
+    """Covers the vMF/softmax/normalisation derivatives directly (all 40 raw
+    outputs, lobes with small and large concentration)."""
+    rng = np.random.default_rng(5)
+    raw = rng.normal(size=(4, 48))
+    raw[:, 3:40:5] = rng.uniform(-3, 8, (4, 8))
+    _, n, wo, wi, F = _batch(rng, 4)
+    _, g = gr.loss_and_grad_raw(raw, wi, F)
+    h = 1e-6
+    for b in range(4):
+        for i in range(40):
+            r = raw.copy()
+            r[b, i] += h
+            lp, _ = gr.loss_and_grad_raw(r, wi, F)
+            r[b, i] -= 2 * h
+            lm, _ = gr.loss_and_grad_raw(r, wi, F)
+            fd = (lp - lm) / (2 * h)
+            assert abs(fd - g[b, i]) <= 1e-5 * max(1.0, abs(fd)), (b, i)
+
+
+@pytest.mark.parametrize("kappa", [0.05, 1.0, 20.0, 300.0])
+def test_vmf_normalised_and_mean(kappa):
+    # integrate in (cos theta, phi) about mu = z: the density depends on cos only
+    t = np.linspace(-1.0, 1.0, 400001)
+    mu = np.array([[[0.0, 0.0, 1.0]]])
+    d = np.stack([np.sqrt(1 - t * t), np.zeros_like(t), t], -1)
+    p = gr.vmf_pdf(np.repeat(mu, len(t), 0), np.full((len(t), 1), kappa), d)[:, 0]
+    total = 2 * math.pi * np.trapezoid(p, t)
+    assert total == pytest.approx(1.0, rel=1e-6)
+    mean_cos = 2 * math.pi * np.trapezoid(p * t, t)
+    assert mean_cos == pytest.approx(1.0 / math.tanh(kappa) - 1.0 / kappa, rel=1e-5, abs=1e-9)
+
+
+@pytest.mark.parametrize("kappa", [0.5, 5.0, 80.0])
+def test_vmf_sampling_mean_direction(kappa):
+    rng = np.random.default_rng(7)
+    mu = rng.normal(size=3)
+    mu /= np.linalg.norm(mu)
+    N = 200000
+    d = gr.sample_vmf(np.tile(mu, (N, 1)), np.full(N, kappa), rng.uniform(size=N), rng.uniform(size=N))
+    assert np.allclose(np.linalg.norm(d, axis=1), 1.0, atol=1e-9)
+    m = d.mean(axis=0)
+    A = 1.0 / math.tanh(kappa) - 1.0 / kappa
+    assert np.allclose(m, A * mu, atol=5.0 / math.sqrt(N))
+
+
+def test_adam_first_step_closed_form():
+    p = np.array([1.0, -2.0, 0.5], np.float32)
+    g = np.array([0.3, -4.0, 1e-3], np.float32)
+    m = np.zeros(3, np.float32)
+    v = np.zeros(3, np.float32)
+    p1, m1, v1 = gr.adam_step(p, m, v, g, t=1, lr=1e-3)
+    # bias-corrected first step: m_hat = g, v_hat = g^2 -> lr * g / (|g| + eps)
+    assert np.allclose(p1, p - 1e-3 * g / (np.abs(g) + 1e-8), rtol=1e-6)
+
+
+def test_bf16_rounding_known_values():
+    v = np.array([1.0, 1.00390625, 1.005859375, 3.0e38, -0.0], np.float32)
+    # 1 + 2^-8 is a tie between 1 and 1 + 2^-7: nearest even keeps 1
+    np.testing.assert_array_equal(gr.bf16(v)[:3], [1.0, 1.0, 1.0078125])
