@@ -29,7 +29,8 @@ summary in profiles/ when present.
 
 Other workloads (not the headline line): --workload sppm times photon-mapping
 passes on the caustic box (prism + 2M-triangle torus, 1920x1080);
---workload batch renders 32 random-cuboid jobs in one batch (autotuned).
+--workload batch renders 32 random-cuboid jobs in one batch (autotuned);
+--workload guide measures the learned guide's equal-time relMSE gain.
 """
 from __future__ import annotations
 
@@ -281,6 +282,94 @@ def bench_batch(args):
     print(json.dumps(line), flush=True)
 
 
+def bench_guide(args):
+    """Equal-time relMSE, guided vs plain GPU tracer, on the metal-ring box
+    (dispersive prism, GGX gold torus, 1% ceiling light) at 1024^2, max depth
+    32.  The reference is a plain render of --guide-ref-spp samples; both
+    methods render --guide-spp samples; the guided one trains online after
+    each of its first --guide-train-spp sample indices, then renders the rest
+    with the trained guide (training time included).  Since relMSE falls as
+    1 / spp, time to reach the plain render's error with the guide =
+    T_guided * relMSE_guided / relMSE_plain; value = T_plain / that time."""
+    import numpy as np
+
+    import hastyrenderer_amd as ha
+    from hastyrenderer_amd import scenes
+
+    scene_dir = args.scene_dir or os.path.join(tempfile.gettempdir(), "hasty_bench_ring")
+    job_path = os.path.join(scene_dir, "renderJob.json")
+    if not os.path.exists(job_path):
+        log("generating metal-ring scene in %s" % scene_dir)
+        scenes.write_caustic_scene(scene_dir, width=1024, height=1024, passes=64, max_depth=32)
+    hip = Hip()
+
+    def render(sc, seed, n):
+        info = sc.info()
+        f = ha.DeviceFilms(info["width"], info["height"])
+        hip.sync()
+        t = time.perf_counter()
+        sc.render_pt_device(seed, 0, n, *f.ptrs)
+        hip.sync()
+        dt = time.perf_counter() - t
+        c = f.download()[0] / n
+        f.close()
+        return c, dt
+
+    def relmse(img, ref):
+        return float(np.mean((img - ref) ** 2 / (ref ** 2 + 1e-2)))
+
+    plain = ha.Scene(job_path)
+    info = plain.info()
+    ref, tref = render(plain, 101, args.guide_ref_spp)
+    log("reference: %d spp in %.1f s" % (args.guide_ref_spp, tref))
+    n = args.guide_spp
+    pc, tp = render(plain, 5, n)
+    ep = relmse(pc, ref)
+    log("plain: %d spp in %.3f s, relMSE %.5f" % (n, tp, ep))
+    gsc = ha.Scene(job_path)
+    g = ha.Guide(gsc, seed=0)
+    ntrain = min(args.guide_train_spp, n)
+    f = ha.DeviceFilms(info["width"], info["height"])
+    hip.sync()
+    t = time.perf_counter()
+    gsc.set_guide(g, fraction=0.5, train_lr=1e-3)  # online training, one sample index per step
+    gsc.render_pt_device(5, 0, ntrain, *f.ptrs)
+    gsc.set_guide(g, fraction=0.5, train_lr=0.0)  # trained guide, remaining samples in one call
+    if n > ntrain:
+        gsc.render_pt_device(5, ntrain, n - ntrain, *f.ptrs)
+    hip.sync()
+    tg = time.perf_counter() - t
+    gc = f.download()[0] / n
+    f.close()
+    eg = relmse(gc, ref)
+    loss, ns = gsc.guide_stats()
+    log("guided (%d spp online training + %d spp): %.3f s, relMSE %.5f, last loss %.4f on %d samples" % (
+        ntrain, n - ntrain, tg, eg, loss, ns))
+    t_equal = tg * eg / ep
+    line = {
+        "metric": "equal-time relMSE gain of the learned guide over the plain GPU tracer",
+        "value": tp / t_equal,
+        "unit": "x (plain time / guided time to equal relMSE)",
+        "n_gpus": 1,
+        "steps": 1,
+        "warmup": 0,
+        "ms_per_step": tg * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 path tracing, bf16 MLP weights",
+        "data": "synthetic",
+        "config": {"workload": "metal_ring_box_guided", "width": info["width"], "height": info["height"],
+                   "max_depth": info["max_depth"], "triangles": info["num_triangles"], "spp": n,
+                   "reference_spp": args.guide_ref_spp, "train_spp": ntrain, "guide_fraction": 0.5,
+                   "adam_lr": 1e-3,
+                   "parallelism": "single-gpu"},
+        "plain": {"seconds": tp, "relmse": ep},
+        "guided": {"seconds": tg, "relmse": eg, "equal_error_seconds": t_equal},
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -291,7 +380,10 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--cpu-spp", type=int, default=48)
-    ap.add_argument("--workload", choices=("ggx", "sppm", "batch"), default="ggx")
+    ap.add_argument("--workload", choices=("ggx", "sppm", "batch", "guide"), default="ggx")
+    ap.add_argument("--guide-spp", type=int, default=64)
+    ap.add_argument("--guide-train-spp", type=int, default=16)
+    ap.add_argument("--guide-ref-spp", type=int, default=2048)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="also run a plain copy of MIB MiB (counter calibration)")
     ap.add_argument("--sppm-alpha", type=float, default=0.4)
@@ -307,6 +399,9 @@ def main():
 
     if args.workload == "batch":
         bench_batch(args)
+        return
+    if args.workload == "guide":
+        bench_guide(args)
         return
     if args.workload == "sppm":
         bench_sppm(args)

@@ -221,10 +221,12 @@ struct HastyScene {
   }
 
   // Path slots in flight (regenerated as paths end) and the results ring.
-  uint32_t slotCount() const {
-    uint64_t P = (uint64_t)dev.width * dev.height;
+  // Slots for a render of `paths` paths: the wanted pool (default 2^22), but
+  // no more than the render needs (at least 2^16).
+  uint32_t slotCount(uint64_t paths = 0) const {
+    if (paths == 0) paths = (uint64_t)dev.width * dev.height;
     uint64_t want = slotsWanted ? slotsWanted : (1ull << 22);
-    return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(want, std::max<uint64_t>(P, 1ull << 16)));
+    return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(want, std::max<uint64_t>(paths, 1ull << 16)));
   }
   uint32_t ringPathsWanted() const {
     uint64_t P = (uint64_t)dev.width * dev.height;
@@ -232,8 +234,8 @@ struct HastyScene {
     return (uint32_t)std::min<uint64_t>(r, 0x7FFFFFFFull / P * P);
   }
 
-  void ensurePathBuffers() {
-    uint32_t need = slotCount();
+  void ensurePathBuffers(uint64_t paths = 0) {
+    uint32_t need = slotCount(paths);
     uint32_t ring = ringPathsWanted();
     uint32_t traceThreads = launch.traceBlocks * hasty::kBlock;
     if (need == pathCapacity && ring == pb.ringPaths && pb.traceThreads == traceThreads) return;
@@ -314,7 +316,10 @@ static void renderBatch(HastyScene* owner, const BatchItem* items, uint32_t nIte
     renderGuided(owner, items[0]);
     return;
   }
-  owner->ensurePathBuffers();
+  uint64_t allPaths = 0;
+  for (uint32_t i = 0; i < nItems; ++i)
+    allPaths += (uint64_t)items[i].n * items[i].scene->dev.width * items[i].scene->dev.height;
+  owner->ensurePathBuffers(allPaths);
   std::vector<void*> tmp;
   struct FreeAll {
     std::vector<void*>& v;
@@ -413,7 +418,7 @@ struct HastySppm {
 // guide, then (when training) one Adam step on the radiance samples its paths
 // produced.
 static void renderGuided(HastyScene* s, const BatchItem& it) {
-  s->ensurePathBuffers();
+  s->ensurePathBuffers((uint64_t)it.n * s->dev.width * s->dev.height);
   s->ensureTileOrder();
   uint32_t slots = s->pb.capacity;
   if (s->guideSlots != slots) {
@@ -444,15 +449,27 @@ static void renderGuided(HastyScene* s, const BatchItem& it) {
     void* p;
     ~FreeOff() { (void)hipFree(p); }
   } freeOff{dOff};
-  for (uint32_t k = 0; k < it.n; ++k) {
-    std::vector<float2> off = haltonOffsets(it.first + k, 1);
-    HIPCHECK(hipMemcpy(dOff, off.data(), sizeof(float2), hipMemcpyHostToDevice));
+  // Without training the guide is fixed: all samples in one call.
+  const bool training = s->guideLr > 0.0f;
+  const uint64_t P = (uint64_t)s->dev.width * s->dev.height;
+  const uint32_t perCall =
+      training ? 1u : (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(it.n, 0x7FFFFFFFull / P));
+  if (!training && it.n > 1) {
+    (void)hipFree(dOff);
+    freeOff.p = nullptr;
+    HIPCHECK(hipMalloc(&dOff, sizeof(float2) * perCall));
+    freeOff.p = dOff;
+  }
+  for (uint32_t k = 0; k < it.n; k += perCall) {
+    const uint32_t cnt = std::min(perCall, it.n - k);
+    std::vector<float2> off = haltonOffsets(it.first + k, cnt);
+    HIPCHECK(hipMemcpy(dOff, off.data(), sizeof(float2) * cnt, hipMemcpyHostToDevice));
     HIPCHECK(hipMemsetAsync(s->guideCount, 0, sizeof(uint32_t), s->stream));
     hasty::RenderJobDesc jd;
     jd.scene = s->dev;
     jd.scene.seed = it.seed;
     jd.first = it.first + k;
-    jd.n = 1;
+    jd.n = cnt;
     jd.dOffsets = (const float2*)dOff;
     jd.dTileOrder = s->dTileOrder;
     jd.dRankOf = s->rankOf();
@@ -460,7 +477,7 @@ static void renderGuided(HastyScene* s, const BatchItem& it) {
     jd.dNormal = it.normal;
     jd.dAlbedo = it.albedo;
     hasty::renderJobs({jd}, s->pb, s->launch, s->stream, timer, &gr);
-    if (s->guideLr > 0.0f) {
+    if (training) {
       uint32_t n = 0;
       HIPCHECK(hipMemcpy(&n, s->guideCount, sizeof(uint32_t), hipMemcpyDeviceToHost));
       n = std::min(n, HastyScene::kGuideSampleCap);

@@ -100,7 +100,12 @@ struct JobSet {
   GuideRun g;      // read by the guided kernels only
 };
 
-constexpr int kStackLds = 16;
+// Traversal stack entries kept in LDS per lane (the rest spill to a global
+// slice); a build knob because it trades LDS (occupancy) for spills.
+#ifndef HASTY_STACK_LDS
+#define HASTY_STACK_LDS 16
+#endif
+constexpr int kStackLds = HASTY_STACK_LDS;
 constexpr int kStackOvf = 64;
 constexpr int kBlock = 256;
 

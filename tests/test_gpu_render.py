@@ -7,10 +7,13 @@ relative on at least 99.9% of pixels.  Both sides use the same counter-based
 RNG keys, so paths take the same decisions; only float reassociation of the
 accumulated radiance differs.
 """
+import os
+
 import numpy as np
 import pytest
 
 import hastyrenderer_amd as ha
+from hastyrenderer_amd import scenes
 from conftest import rel_ok
 from oracle import pyoracle as po
 
@@ -186,3 +189,34 @@ def test_stop_flag_stops_between_batches(cornell_job):
     td = ha.RenderThreadData(seed=0)
     ha.renderThread(*bufs, job, td, batch=4)
     assert td.stoppedFlag and bufs[0].getWriteBuffer().numSamples == 0
+
+
+def test_headless_cli_matches_library(cornell_job, caustic_small_job, tmp_path):
+    """hastyrenderer_amd/hasty_render (C++ host layer with the reference's
+    renderThread / renderPhotonThread signatures) writes what the library
+    renders: PT estimate = sums / (1e-7 + samples), SPPM = last estimate."""
+    import subprocess
+
+    cli = os.path.join(os.path.dirname(ha.__file__), "hasty_render")
+    out = tmp_path / "pt"
+    out.mkdir()
+    r = subprocess.run([cli, cornell_job, str(out)], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    img = scenes.read_pfm(str(out / "output.pfm"))
+    info = ha.Scene(cornell_job).info()
+    c, n, a = ha.Scene(cornell_job).render_pt(seed=0, first_sample=0, num_samples=info["num_samples"])
+    np.testing.assert_allclose(img, c / np.float32(1e-7 + info["num_samples"]), rtol=1e-6, atol=1e-9)
+    out2 = tmp_path / "pm"
+    out2.mkdir()
+    r = subprocess.run([cli, "--sppm", caustic_small_job, str(out2)], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    pm = scenes.read_pfm(str(out2 / "output.pfm"))
+    job = ha.PMRenderJob()
+    job.loadJSON(caustic_small_job)
+    st = job.renderSettings
+    bufs = [ha.Image3fAccDoubleBuffer(st.width, st.height) for _ in range(3)]
+    ha.renderPhotonThread(*bufs, job, ha.RenderThreadData(seed=0))
+    np.testing.assert_allclose(pm, bufs[0].getReadBuffer().data / np.float32(1e-7 + 1), rtol=1e-6, atol=1e-9)
+    # reference error convention: load failure -> exit code 2
+    r = subprocess.run([cli, str(tmp_path / "missing.json"), str(out)], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 2
