@@ -284,7 +284,8 @@ def bench_batch(args):
 
 def bench_guide(args):
     """Equal-time relMSE, guided vs plain GPU tracer, on the metal-ring box
-    (dispersive prism, GGX gold torus, 1% ceiling light) at 1024^2, max depth
+    (dispersive prism, GGX gold torus, 1% ceiling light) at --guide-res^2
+    (512^2 by default so that the reference can take 16384 spp), max depth
     32.  The reference is a plain render of --guide-ref-spp samples; both
     methods render --guide-spp samples; the guided one trains online after
     each of its first --guide-train-spp sample indices, then renders the rest
@@ -296,11 +297,11 @@ def bench_guide(args):
     import hastyrenderer_amd as ha
     from hastyrenderer_amd import scenes
 
-    scene_dir = args.scene_dir or os.path.join(tempfile.gettempdir(), "hasty_bench_ring")
+    scene_dir = args.scene_dir or os.path.join(tempfile.gettempdir(), "hasty_bench_ring%d" % args.guide_res)
     job_path = os.path.join(scene_dir, "renderJob.json")
     if not os.path.exists(job_path):
         log("generating metal-ring scene in %s" % scene_dir)
-        scenes.write_caustic_scene(scene_dir, width=1024, height=1024, passes=64, max_depth=32)
+        scenes.write_caustic_scene(scene_dir, width=args.guide_res, height=args.guide_res, passes=64, max_depth=32)
     hip = Hip()
 
     def render(sc, seed, n):
@@ -381,9 +382,11 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--cpu-spp", type=int, default=48)
     ap.add_argument("--workload", choices=("ggx", "sppm", "batch", "guide"), default="ggx")
+    ap.add_argument("--tune-spp", type=int, default=4, help="samples per autotuner trial (0: no tuning)")
     ap.add_argument("--guide-spp", type=int, default=64)
+    ap.add_argument("--guide-res", type=int, default=512)
     ap.add_argument("--guide-train-spp", type=int, default=16)
-    ap.add_argument("--guide-ref-spp", type=int, default=2048)
+    ap.add_argument("--guide-ref-spp", type=int, default=16384)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="also run a plain copy of MIB MiB (counter calibration)")
     ap.add_argument("--sppm-alpha", type=float, default=0.4)
@@ -428,6 +431,9 @@ def main():
     log("scene loaded in %.2f s: %d triangles, BVH %d nodes depth %d, %.1f MB in HBM" % (
         load_s, info["num_triangles"], info["bvh_nodes"], info["bvh_depth"], info["device_bytes"] / 1e6))
 
+    # launch shape: the autotuner times short renders of this scene (untimed)
+    tuned = scene.autotune(seed=seed, trial_samples=args.tune_spp) if args.tune_spp > 0 else scene.get_launch()
+    log("launch: %s" % json.dumps(tuned))
     hip = Hip()
     nbytes = W * H * 3 * 4
     films = [hip.malloc(nbytes) for _ in range(3)]
@@ -515,6 +521,7 @@ def main():
             "triangles": info["num_triangles"],
             "environment": "2048x1024 lat-long, importance sampled",
             "camera": "thin lens f/2.8, 8 blades",
+            "launch": tuned,
             "parallelism": "single-gpu",
         },
         "rays_per_s": value * rays_per_path,

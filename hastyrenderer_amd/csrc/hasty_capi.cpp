@@ -799,7 +799,7 @@ int hasty_get_launch(const HastyScene* s, HastyLaunch* out) {
     g_lastError = "null argument";
     return 1;
   }
-  out->path_slots = s->slotCount();
+  out->path_slots = s->slotsWanted ? s->slotsWanted : (1u << 22);
   out->trace_blocks = s->launch.traceBlocks;
   out->shade_blocks = s->launch.shadeBlocks;
   out->chunk = s->launch.chunk;
