@@ -556,9 +556,10 @@ static void buildScene(HastyScene& S, const char* path) {
     d.envRGB = S.upload(sc.env.rgb.data(), sc.env.rgb.size());
     d.envPdf = S.upload(sc.envPdf.data(), sc.envPdf.size());
     d.envCdf = S.upload(sc.envCdf.data(), sc.envCdf.size());
-    std::vector<uint32_t> guide(1025);
+    std::vector<uint32_t> guide(hdev::kEnvGuide + 1);
     uint32_t n = (uint32_t)sc.envCdf.size();
-    for (uint32_t g = 0; g <= 1024; ++g) guide[g] = hrng::upper_bound(sc.envCdf.data(), n, g / 1024.0);
+    for (uint32_t g = 0; g <= hdev::kEnvGuide; ++g)
+      guide[g] = hrng::upper_bound(sc.envCdf.data(), n, g / (double)hdev::kEnvGuide);
     d.envCdfGuide = S.upload(guide.data(), guide.size());
     d.envW = sc.env.width;
     d.envH = sc.env.height;

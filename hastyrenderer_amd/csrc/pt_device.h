@@ -133,7 +133,7 @@ struct DevScene {
   const float* envRGB;
   const float* envPdf;
   const double* envCdf;
-  const uint32_t* envCdfGuide;  // 1024 + 1 coarse search brackets
+  const uint32_t* envCdfGuide;  // kEnvGuide + 1 search brackets
   uint32_t envW, envH;
   float envValueSum;
   int hasEnv;
@@ -817,11 +817,15 @@ __device__ inline float envPdf(const DevScene& s, V3 dir) {
   float d = W * H * s.envPdf[(size_t)y * s.envW + x] / s.envValueSum;
   return d / fmaxf(1e-7f, 2.0f * (float)(kPi * kPi) * sinT);
 }
-// upper_bound over the environment CDF, narrowed by a coarse guide table.
+// upper_bound over the environment CDF, narrowed by a guide table:
+// guide[g] = upper_bound(cdf, g / kEnvGuide), so for x in [g, g + 1) / kEnvGuide
+// the answer lies in [guide[g], guide[g + 1]] (upper_bound is monotone) and
+// the result is exactly the full search's.
+constexpr uint32_t kEnvGuide = 65536;
 __device__ inline uint32_t envCdfSearch(const DevScene& s, double x) {
   uint32_t n = s.envW * s.envH;
-  uint32_t g = (uint32_t)(x * 1024.0);
-  if (g > 1023) g = 1023;
+  uint32_t g = (uint32_t)(x * (double)kEnvGuide);
+  if (g > kEnvGuide - 1) g = kEnvGuide - 1;
   uint32_t lo = s.envCdfGuide[g], hi = s.envCdfGuide[g + 1];
   uint32_t count = hi - lo;
   while (count > 0) {

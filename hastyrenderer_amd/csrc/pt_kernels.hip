@@ -91,12 +91,19 @@ __device__ inline void waveAddStat(unsigned long long* c, uint32_t v) {
   if ((threadIdx.x & 63u) == 0 && v) atomicAdd(c, (unsigned long long)v);
 }
 
+// Entries beyond md.n are never read, so an empty stack is neither loaded
+// nor stored.
 __device__ inline void loadMedia(const PathBuffers& pb, uint32_t slot, uint32_t flags, Media& md) {
+  md.n = (int)((flags >> 12) & 0xFu);
+  md.overflow = (flags >> 16) & 1u;
+  if (md.n == 0) {
+#pragma unroll
+    for (int i = 0; i < 8; ++i) md.e[i] = 0u;
+    return;
+  }
   uint4 a = pb.media[(size_t)slot * 2], b = pb.media[(size_t)slot * 2 + 1];
   md.e[0] = a.x, md.e[1] = a.y, md.e[2] = a.z, md.e[3] = a.w;
   md.e[4] = b.x, md.e[5] = b.y, md.e[6] = b.z, md.e[7] = b.w;
-  md.n = (int)((flags >> 12) & 0xFu);
-  md.overflow = (flags >> 16) & 1u;
 }
 __device__ inline void storeMedia(const PathBuffers& pb, uint32_t slot, const Media& md) {
   pb.media[(size_t)slot * 2] = make_uint4(md.e[0], md.e[1], md.e[2], md.e[3]);
@@ -464,7 +471,7 @@ __global__ void __launch_bounds__(kBlock) kShade(JobSet js, PathBuffers pb) {
                   }
                   pb.state[slot] =
                       make_uint4(rng.counter, packFlags(depth + 1, wavelength, md) | recBits, (uint32_t)tri, st.w);
-                  storeMedia(pb, slot, md);
+                  if (md.n > 0) storeMedia(pb, slot, md);
                 }
               }
             }
