@@ -153,7 +153,7 @@ def bench_sppm(args):
     W, H = info["width"], info["height"]
     log("scene loaded in %.2f s: %d triangles" % (load_s, info["num_triangles"]))
     sp = ha.Sppm(scene, 3, alpha=args.sppm_alpha, store_fraction=args.sppm_store_fraction,
-                 emit_fraction=args.sppm_emit_fraction)
+                 emit_fraction=args.sppm_emit_fraction, initial_radius=args.sppm_initial_radius)
     hip = Hip()
     nbytes = W * H * 3 * 4
     films = [hip.malloc(nbytes) for _ in range(3)]
@@ -194,7 +194,8 @@ def bench_sppm(args):
         "data": "synthetic",
         "config": {"workload": "sppm_caustic_box", "width": W, "height": H, "triangles": info["num_triangles"],
                    "alpha": args.sppm_alpha, "store_fraction": args.sppm_store_fraction,
-                   "emit_fraction": args.sppm_emit_fraction, "parallelism": "single-gpu"},
+                   "emit_fraction": args.sppm_emit_fraction, "initial_radius": args.sppm_initial_radius or "footprint",
+                   "parallelism": "single-gpu"},
         "photons_emitted_per_s": emitted / elapsed,
         "photons_stored_per_pass": stored / max(1, args.steps),
         "load_s": load_s,
@@ -392,6 +393,8 @@ def main():
     ap.add_argument("--sppm-alpha", type=float, default=0.4)
     ap.add_argument("--sppm-store-fraction", type=float, default=0.5)
     ap.add_argument("--sppm-emit-fraction", type=float, default=10.0)
+    ap.add_argument("--sppm-initial-radius", type=float, default=0.0,
+                    help="constant initial radius (0: 10x pixel footprint as the reference)")
     args = ap.parse_args()
 
     if args.gpus != 1:

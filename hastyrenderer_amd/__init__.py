@@ -128,6 +128,7 @@ class SppmParamsC(C.Structure):
         ("footprint_scale", C.c_float),
         ("store_fraction", C.c_double),
         ("emit_fraction", C.c_double),
+        ("initial_radius", C.c_float),
     ]
 
 
@@ -474,9 +475,9 @@ class Sppm:
     statistics and the radius image stay on the device between passes."""
 
     def __init__(self, scene: Scene, seed: int, alpha=0.4, footprint_scale=10.0, store_fraction=0.5,
-                 emit_fraction=10.0):
+                 emit_fraction=10.0, initial_radius=0.0):
         self.scene = scene  # keeps the scene alive
-        p = SppmParamsC(alpha, footprint_scale, store_fraction, emit_fraction)
+        p = SppmParamsC(alpha, footprint_scale, store_fraction, emit_fraction, initial_radius)
         h = C.c_void_p()
         _err(lib().hasty_sppm_create(scene.handle, seed, C.byref(p), C.byref(h)))
         self._h = h

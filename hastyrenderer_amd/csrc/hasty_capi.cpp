@@ -952,6 +952,7 @@ int hasty_sppm_create(HastyScene* s, uint32_t seed, const HastySppmParams* param
     if (params) {
       p.alpha = params->alpha;
       p.footprintScale = params->footprint_scale;
+      p.initialRadius = params->initial_radius;
       p.storeFraction = params->store_fraction;
       p.emitFraction = params->emit_fraction;
     }

@@ -16,6 +16,7 @@ struct SppmParams {
   float footprintScale = 10.0f;  // initial radius = 10 x pixel footprint (PhotonMapping.cpp:840)
   double storeFraction = 0.5;    // stop emitting once 0.5 W H photons are stored (PhotonMapping.cpp:834)
   double emitFraction = 10.0;    // ... or 10 W H photons were emitted (PhotonMapping.cpp:835)
+  float initialRadius = 0.0f;    // > 0: this constant initial radius instead of the footprint estimate
 };
 
 struct SppmPassInfo {

@@ -802,8 +802,9 @@ SppmRenderer::~SppmRenderer() {
 
 void SppmRenderer::initRadii() {
   size_t P = (size_t)W_ * H_;
-  if (!s_.hasSurfaceLight) {
-    hipLaunchKernelGGL(kFill, dim3(1024), dim3(kB), 0, stream_, dRadii_, (uint32_t)P, 1.0f);
+  if (!s_.hasSurfaceLight || p_.initialRadius > 0.0f) {
+    float r0 = p_.initialRadius > 0.0f ? p_.initialRadius : 1.0f;
+    hipLaunchKernelGGL(kFill, dim3(1024), dim3(kB), 0, stream_, dRadii_, (uint32_t)P, r0);
     HC(hipGetLastError());
     return;
   }

@@ -86,7 +86,7 @@ void renderPhotonThread(Image3fAccDoubleBuffer& colorBuffer, Image3fAccDoubleBuf
   {
     std::lock_guard<std::mutex> lock(job.sppmMutex);
     if (!job.sppm) {
-      HastySppmParams p{job.alpha, 10.0f, 0.5, 10.0};
+      HastySppmParams p{job.alpha, 10.0f, 0.5, 10.0, 0.0f};
       std::lock_guard<std::mutex> slock(job.scene->mutex());
       if (hasty_sppm_create(job.scene->handle(), threadData.seed, &p, &job.sppm) != 0) {
         fprintf(stderr, "error: %s\n", hasty_last_error());

@@ -204,6 +204,8 @@ typedef struct HastySppmParams {
   float footprint_scale; /* initial radius / pixel footprint (PhotonMapping.cpp:840), default 10 */
   double store_fraction; /* stop emitting at store_fraction*W*H stored photons (PhotonMapping.cpp:834), 0.5 */
   double emit_fraction;  /* ... or emit_fraction*W*H emitted photons (PhotonMapping.cpp:835), 10 */
+  float initial_radius;  /* > 0: constant initial radius (no counterpart; the reference uses the
+                            footprint estimate, PhotonMapping.cpp:838-841), default 0 */
 } HastySppmParams;
 
 typedef struct HastySppmInfo {
