@@ -289,7 +289,8 @@ def bench_guide(args):
     (512^2 by default so that the reference can take 16384 spp), max depth
     32.  The reference is a plain render of --guide-ref-spp samples; both
     methods render --guide-spp samples; the guided one trains online after
-    each of its first --guide-train-spp sample indices, then renders the rest
+    each of its first --guide-train-spp sample indices (one pass of Adam steps
+    over --guide-minibatch samples each), then renders the rest
     with the trained guide (training time included).  Since relMSE falls as
     1 / spp, time to reach the plain render's error with the guide =
     T_guided * relMSE_guided / relMSE_plain; value = T_plain / that time."""
@@ -298,11 +299,16 @@ def bench_guide(args):
     import hastyrenderer_amd as ha
     from hastyrenderer_amd import scenes
 
-    scene_dir = args.scene_dir or os.path.join(tempfile.gettempdir(), "hasty_bench_ring%d" % args.guide_res)
+    scene_dir = args.scene_dir or os.path.join(tempfile.gettempdir(),
+                                               "hasty_bench_%s%d" % (args.guide_scene, args.guide_res))
     job_path = os.path.join(scene_dir, "renderJob.json")
     if not os.path.exists(job_path):
-        log("generating metal-ring scene in %s" % scene_dir)
-        scenes.write_caustic_scene(scene_dir, width=args.guide_res, height=args.guide_res, passes=64, max_depth=32)
+        log("generating %s scene in %s" % (args.guide_scene, scene_dir))
+        if args.guide_scene == "ring":
+            scenes.write_caustic_scene(scene_dir, width=args.guide_res, height=args.guide_res, passes=64, max_depth=32)
+        else:  # Cornell room whose light faces the ceiling: lit indirectly
+            scenes.write_cornell(scene_dir, width=args.guide_res, height=args.guide_res, spp=64, light_up=True,
+                                 max_depth=32)
     hip = Hip()
 
     def render(sc, seed, n):
@@ -334,7 +340,8 @@ def bench_guide(args):
     f = ha.DeviceFilms(info["width"], info["height"])
     hip.sync()
     t = time.perf_counter()
-    gsc.set_guide(g, fraction=0.5, train_lr=1e-3)  # online training, one sample index per step
+    gsc.set_guide_training(minibatch=args.guide_minibatch)
+    gsc.set_guide(g, fraction=0.5, train_lr=args.guide_lr)  # online training after each sample index
     gsc.render_pt_device(5, 0, ntrain, *f.ptrs)
     gsc.set_guide(g, fraction=0.5, train_lr=0.0)  # trained guide, remaining samples in one call
     if n > ntrain:
@@ -361,10 +368,12 @@ def bench_guide(args):
         "vs_baseline": None,
         "dtype": "f32 path tracing, bf16 MLP weights",
         "data": "synthetic",
-        "config": {"workload": "metal_ring_box_guided", "width": info["width"], "height": info["height"],
+        "config": {"workload": "metal_ring_box_guided" if args.guide_scene == "ring" else "indirect_room_guided",
+                   "width": info["width"], "height": info["height"],
                    "max_depth": info["max_depth"], "triangles": info["num_triangles"], "spp": n,
                    "reference_spp": args.guide_ref_spp, "train_spp": ntrain, "guide_fraction": 0.5,
-                   "adam_lr": 1e-3,
+                   "minibatch": args.guide_minibatch,
+                   "adam_lr": args.guide_lr,
                    "parallelism": "single-gpu"},
         "plain": {"seconds": tp, "relmse": ep},
         "guided": {"seconds": tg, "relmse": eg, "equal_error_seconds": t_equal},
@@ -387,6 +396,10 @@ def main():
     ap.add_argument("--guide-spp", type=int, default=64)
     ap.add_argument("--guide-res", type=int, default=512)
     ap.add_argument("--guide-train-spp", type=int, default=16)
+    ap.add_argument("--guide-scene", choices=("ring", "room"), default="ring",
+                    help="ring: the metal-ring caustic box; room: a Cornell room lit from behind its light")
+    ap.add_argument("--guide-lr", type=float, default=1e-3, help="Adam learning rate of the online training")
+    ap.add_argument("--guide-minibatch", type=int, default=16384, help="training samples per Adam step")
     ap.add_argument("--guide-ref-spp", type=int, default=16384)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="also run a plain copy of MIB MiB (counter calibration)")

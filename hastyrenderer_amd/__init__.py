@@ -56,6 +56,7 @@ EXPORTS = (
     "hasty_guide_gradient",
     "hasty_guide_train_step",
     "hasty_scene_set_guide",
+    "hasty_scene_set_guide_training",
     "hasty_scene_guide_stats",
     "hasty_sppm_create",
     "hasty_sppm_destroy",
@@ -186,6 +187,7 @@ def lib():
     L.hasty_guide_gradient.argtypes = [P, fp, C.c_uint32, fp, fp]
     L.hasty_guide_train_step.argtypes = [P, fp, C.c_uint32, C.c_float, fp]
     L.hasty_scene_set_guide.argtypes = [P, P, C.c_float, C.c_float]
+    L.hasty_scene_set_guide_training.argtypes = [P, C.c_uint32, C.c_float]
     L.hasty_scene_guide_stats.argtypes = [P, fp, C.POINTER(C.c_uint32)]
     L.hasty_sppm_create.argtypes = [P, C.c_uint32, C.POINTER(SppmParamsC), C.POINTER(P)]
     L.hasty_sppm_destroy.argtypes = [P]
@@ -284,10 +286,14 @@ class Scene:
 
     def set_guide(self, guide, fraction=0.5, train_lr=0.0):
         """Guided rendering with `guide` (None: plain); online Adam training
-        after every sample index when train_lr > 0."""
+        (one pass of minibatch steps) after every sample index when
+        train_lr > 0."""
         self._guide = guide
         _err(lib().hasty_scene_set_guide(self._h, guide.handle if guide is not None else None, fraction,
                                          train_lr))
+
+    def set_guide_training(self, minibatch=16384, weight_cap=20.0):
+        _err(lib().hasty_scene_set_guide_training(self._h, minibatch, weight_cap))
 
     def guide_stats(self):
         loss = np.zeros(1, np.float32)

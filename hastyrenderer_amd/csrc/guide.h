@@ -74,8 +74,8 @@ __device__ inline float feature(int k, hdev::V3 x, hdev::V3 n, hdev::V3 wo) {
   if (k < 72) {
     int a = k / 24, r = k - 24 * a, f = r >> 1;
     float xa = a == 0 ? x.x : (a == 1 ? x.y : x.z);
-    float ang = (float)(1u << f) * 3.14159265358979323846f * xa;
-    return (r & 1) ? cosf(ang) : sinf(ang);
+    float t = (float)(1u << f) * xa;
+    return (r & 1) ? cospif(t) : sinpif(t);
   }
   if (k < 75) return k == 72 ? n.x : (k == 73 ? n.y : n.z);
   if (k < 78) return k == 75 ? wo.x : (k == 76 ? wo.y : wo.z);
@@ -122,23 +122,28 @@ __device__ inline float mixturePdf(const Mixture& m, hdev::V3 d) {
 }
 
 // Lobe by its weight (xi0), then Jakob's stable inversion about mu.
+// The first lobe whose cumulative weight exceeds xi0 (the last one if none
+// does) is picked by selects with constant indices, so the mixture stays in
+// registers.
 __device__ inline hdev::V3 mixtureSample(const Mixture& m, float xi0, float xi1, float xi2) {
-  int k = kLobes - 1;
+  float kappa = m.kappa[kLobes - 1];
+  hdev::V3 mu = m.mu[kLobes - 1];
   float c = 0.0f;
+  bool found = false;
 #pragma unroll
-  for (int j = 0; j < kLobes; ++j) {
+  for (int j = 0; j < kLobes - 1; ++j) {
     c += m.w[j];
-    if (xi0 < c) {
-      k = j;
-      break;
-    }
+    const bool take = !found && xi0 < c;
+    kappa = take ? m.kappa[j] : kappa;
+    mu.x = take ? m.mu[j].x : mu.x;
+    mu.y = take ? m.mu[j].y : mu.y;
+    mu.z = take ? m.mu[j].z : mu.z;
+    found = found || take;
   }
-  float kappa = m.kappa[k];
   float wz = 1.0f + __logf(xi1 + (1.0f - xi1) * __expf(-2.0f * kappa)) / kappa;
   wz = fminf(1.0f, fmaxf(-1.0f, wz));
   float s = sqrtf(fmaxf(0.0f, 1.0f - wz * wz));
   float phi = 2.0f * 3.14159265358979323846f * xi2;
-  hdev::V3 mu = m.mu[k];
   hdev::V3 t = fabsf(mu.x) > 0.9f ? hdev::v3(0.0f, 1.0f, 0.0f) : hdev::v3(1.0f, 0.0f, 0.0f);
   hdev::V3 b1 = hdev::normalize(hdev::cross(mu, t));
   hdev::V3 b2 = hdev::cross(mu, b1);
@@ -160,12 +165,13 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 __device__ inline f32x4 mfma(bf16x4 a, bf16x4 b, f32x4 c) {
   return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
 }
+// Round to nearest even, as toBf16 (gfx950 converts pairs in one instruction).
 __device__ inline bf16x4 pack4(float a, float b, float c, float d) {
   bf16x4 r;
-  r.x = toBf16(a);
-  r.y = toBf16(b);
-  r.z = toBf16(c);
-  r.w = toBf16(d);
+  r.x = __builtin_bit_cast(short, (__bf16)a);
+  r.y = __builtin_bit_cast(short, (__bf16)b);
+  r.z = __builtin_bit_cast(short, (__bf16)c);
+  r.w = __builtin_bit_cast(short, (__bf16)d);
   return r;
 }
 __device__ inline bf16x4 load4(const short* p) { return *reinterpret_cast<const bf16x4*>(p); }
@@ -193,12 +199,27 @@ __device__ inline bf16x4 reluPack(f32x4 z) {
 }
 
 // Input features of query (x, n, wo) for the 5 k-steps of layer 0 (B layout).
+// A lane's 4 consecutive features k..k+3 (k a multiple of 4): below 72 they
+// are the sine/cosine pairs of octaves f and f+1 of one axis, computed as
+// sinpi / cospi of 2^f x_a (the scaling is exact, so no angle is rounded
+// before the reduction).
 __device__ inline void inputOperands(hdev::V3 x, hdev::V3 n, hdev::V3 wo, bf16x4* in) {
   const int q = (threadIdx.x & 63) >> 4;
 #pragma unroll
   for (int ks = 0; ks < 5; ++ks) {
-    int k = 16 * ks + 4 * q;
-    in[ks] = pack4(feature(k, x, n, wo), feature(k + 1, x, n, wo), feature(k + 2, x, n, wo), feature(k + 3, x, n, wo));
+    const int k = 16 * ks + 4 * q;
+    if (k < 72) {
+      const int a = k / 24, f = (k - 24 * a) >> 1;
+      const float xa = a == 0 ? x.x : (a == 1 ? x.y : x.z);
+      const float t = xa * (float)(1u << f);
+      float s0, c0, s1, c1;
+      sincospif(t, &s0, &c0);
+      sincospif(t + t, &s1, &c1);
+      in[ks] = pack4(s0, c0, s1, c1);
+    } else {
+      in[ks] = pack4(feature(k, x, n, wo), feature(k + 1, x, n, wo), feature(k + 2, x, n, wo),
+                     feature(k + 3, x, n, wo));
+    }
   }
 }
 

@@ -32,6 +32,10 @@ class Guide {
   // One Adam step on n samples whose weights are first normalised to mean 1
   // and capped at weightCap (in place); returns the mean loss.
   float trainOnDevice(float4* dSamples, uint32_t n, float lr, float weightCap);
+  // One pass over n samples in Adam steps of `minibatch` samples, in a fixed
+  // pseudo-random order (the samples arrive grouped by image region); weights
+  // are normalised over all n first.  Returns the mean loss of the last step.
+  float trainEpoch(float4* dSamples, uint32_t n, float lr, float weightCap, uint32_t minibatch);
   void getGrad(float* host);
   void zeroGrad();
   uint32_t steps() const { return step_; }
@@ -39,6 +43,8 @@ class Guide {
 
  private:
   void ensureTrain(uint32_t batch);
+  void gradientAsync(const float4* dSamples, uint32_t n);
+  float normaliseWeights(float4* dSamples, uint32_t n, float weightCap);
   hipStream_t stream_;
   float* dParam_ = nullptr;
   float* dM_ = nullptr;
@@ -54,6 +60,8 @@ class Guide {
   uint32_t trainCap_ = 0;
   float4* dIn_ = nullptr;
   float4* dRaw_ = nullptr;
+  float4* dShuffled_ = nullptr;
+  uint32_t shuffledCap_ = 0;
   float boxLo_[3], boxInv_[3];
   uint32_t step_ = 0;
 };

@@ -277,6 +277,15 @@ __global__ void __launch_bounds__(256) kWeightScale(float4* smp, uint32_t n, flo
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
     smp[4 * i].w = fminf(smp[4 * i].w * inv, cap);
 }
+// dst[i] = src[(a i + b) mod n] with a coprime to n: a permutation of the
+// 4-float4 samples.
+__global__ void __launch_bounds__(256) kPermute(const float4* src, float4* dst, uint32_t n, uint32_t a, uint32_t b) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const uint32_t j = (uint32_t)(((uint64_t)a * i + b) % n);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) dst[(size_t)i * 4 + k] = src[(size_t)j * 4 + k];
+  }
+}
 __global__ void kZero(float* a, uint32_t n) {
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) a[i] = 0.0f;
 }
@@ -307,7 +316,7 @@ Guide::Guide(hipStream_t stream) : stream_(stream) {
 
 Guide::~Guide() {
   for (void* q : {(void*)dParam_, (void*)dM_, (void*)dV_, (void*)dGrad_, (void*)dWT_, (void*)dW_, (void*)dLoss_,
-                  (void*)dTileJobs_, (void*)dH_, (void*)dG_, (void*)dIn_, (void*)dRaw_})
+                  (void*)dTileJobs_, (void*)dH_, (void*)dG_, (void*)dIn_, (void*)dRaw_, (void*)dShuffled_})
     if (q) (void)hipFree(q);
 }
 
@@ -373,8 +382,7 @@ void Guide::ensureTrain(uint32_t batch) {
   trainCap_ = batch;
 }
 
-float Guide::gradientDevice(const float4* dSamples, uint32_t n) {
-  if (n == 0) return 0.0f;
+void Guide::gradientAsync(const float4* dSamples, uint32_t n) {
   const uint32_t chunk = 512;
   uint32_t batch = ((n + chunk - 1) / chunk) * chunk;  // multiple of the GEMM chunk (and of 16)
   ensureTrain(batch);
@@ -387,6 +395,11 @@ float Guide::gradientDevice(const float4* dSamples, uint32_t n) {
   kWeightGrad<<<(waves + 3) / 4, 256, 0, stream_>>>(dH_, dG_, batch, chunk, (const TileJob*)dTileJobs_, nTileJobs_,
                                                     dGrad_);
   hipOk(hipGetLastError(), "kWeightGrad");
+}
+
+float Guide::gradientDevice(const float4* dSamples, uint32_t n) {
+  if (n == 0) return 0.0f;
+  gradientAsync(dSamples, n);
   float loss = 0.0f;
   hipOk(hipMemcpyAsync(&loss, dLoss_, sizeof(float), hipMemcpyDeviceToHost, stream_), "copy");
   hipOk(hipStreamSynchronize(stream_), "sync");
@@ -402,8 +415,8 @@ void Guide::adamStep(float lr) {
   hipOk(hipGetLastError(), "kAdam");
 }
 
-float Guide::trainOnDevice(float4* dSamples, uint32_t n, float lr, float weightCap) {
-  if (n == 0) return 0.0f;
+// Scales the weights to mean 1 and caps them; false if they sum to zero.
+float Guide::normaliseWeights(float4* dSamples, uint32_t n, float weightCap) {
   kZero<<<1, 64, 0, stream_>>>(dLoss_, 1);
   kWeightSum<<<256, 256, 0, stream_>>>(dSamples, n, dLoss_);
   float sum = 0.0f;
@@ -411,10 +424,51 @@ float Guide::trainOnDevice(float4* dSamples, uint32_t n, float lr, float weightC
   hipOk(hipStreamSynchronize(stream_), "sync");
   if (!(sum > 0.0f)) return 0.0f;
   kWeightScale<<<256, 256, 0, stream_>>>(dSamples, n, (float)n / sum, weightCap);
+  return sum;
+}
+
+float Guide::trainOnDevice(float4* dSamples, uint32_t n, float lr, float weightCap) {
+  if (n == 0 || !(normaliseWeights(dSamples, n, weightCap) > 0.0f)) return 0.0f;
   zeroGrad();
   float loss = gradientDevice(dSamples, n);
   adamStep(lr);
   return loss;
+}
+
+float Guide::trainEpoch(float4* dSamples, uint32_t n, float lr, float weightCap, uint32_t minibatch) {
+  if (n == 0 || minibatch == 0 || !(normaliseWeights(dSamples, n, weightCap) > 0.0f)) return 0.0f;
+  if (shuffledCap_ < n) {
+    if (dShuffled_) (void)hipFree(dShuffled_);
+    dShuffled_ = nullptr;
+    hipOk(hipMalloc(&dShuffled_, sizeof(float4) * 4 * (size_t)n), "hipMalloc");
+    shuffledCap_ = n;
+  }
+  // multiplier near n * (golden ratio - 1), made coprime to n
+  uint32_t a = (uint32_t)((double)n * 0.6180339887) | 1u;
+  auto gcd = [](uint32_t x, uint32_t y) {
+    while (y) {
+      uint32_t t = x % y;
+      x = y;
+      y = t;
+    }
+    return x;
+  };
+  while (gcd(a, n) != 1u) a += 2u;
+  const uint32_t b = (uint32_t)(((uint64_t)step_ * 7919u) % n);
+  kPermute<<<512, 256, 0, stream_>>>(dSamples, dShuffled_, n, a % n == 0u ? 1u : a, b);
+  hipOk(hipGetLastError(), "kPermute");
+  for (uint32_t off = 0; off < n; off += minibatch) {
+    const uint32_t m = std::min(minibatch, n - off);
+    zeroGrad();
+    if (off + m >= n) {  // the last step reports its loss
+      float loss = gradientDevice(dShuffled_ + (size_t)off * 4, m);
+      adamStep(lr);
+      return loss;
+    }
+    gradientAsync(dShuffled_ + (size_t)off * 4, m);
+    adamStep(lr);
+  }
+  return 0.0f;
 }
 
 void Guide::getGrad(float* host) {

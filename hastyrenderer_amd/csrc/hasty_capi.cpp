@@ -162,12 +162,14 @@ struct HastyScene {
   HastyGuide* guide = nullptr;
   float guideFraction = 0.5f;
   float guideLr = 0.0f;
+  uint32_t guideMinibatch = 16384;
+  float guideWeightCap = 20.0f;
   float4* guideRaw = nullptr;
   float4* guideRec = nullptr;
   float4* guideSamples = nullptr;
   uint32_t* guideCount = nullptr;
   uint32_t guideSlots = 0;
-  static constexpr uint32_t kGuideSampleCap = 1u << 18;
+  static constexpr uint32_t kGuideSampleCap = 1u << 19;
   float lastGuideLoss = 0.0f;
   uint32_t lastGuideSamples = 0;
 
@@ -262,6 +264,7 @@ struct HastyScene {
     pb.qS = allocPath<uint32_t>(n);
     pb.qR = allocPath<uint32_t>(n);
     pb.qI = allocPath<uint32_t>(n);
+    pb.mark = allocPath<uint32_t>(n);
     pb.counters = allocPath<uint32_t>(hasty::kNumCounters);
     pb.stats = allocPath<unsigned long long>(hasty::kNumStats);
     pb.stackOverflow = allocPath<uint2>((size_t)traceThreads * hasty::kStackOvf);
@@ -438,7 +441,9 @@ static void renderGuided(HastyScene* s, const BatchItem& it) {
   gr.rec = s->guideRec;
   gr.samples = s->guideSamples;
   gr.sampleCount = s->guideCount;
-  gr.sampleCap = HastyScene::kGuideSampleCap;
+  // Without training no radiance samples are recorded.
+  const bool training = s->guideLr > 0.0f;
+  gr.sampleCap = training ? HastyScene::kGuideSampleCap : 0u;
   gr.fraction = s->guideFraction;
   s->launch.countStats = s->counting;
   if (s->profiling && !s->timer) s->timer = std::make_unique<EventTimer>();
@@ -450,7 +455,6 @@ static void renderGuided(HastyScene* s, const BatchItem& it) {
     ~FreeOff() { (void)hipFree(p); }
   } freeOff{dOff};
   // Without training the guide is fixed: all samples in one call.
-  const bool training = s->guideLr > 0.0f;
   const uint64_t P = (uint64_t)s->dev.width * s->dev.height;
   const uint32_t perCall =
       training ? 1u : (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(it.n, 0x7FFFFFFFull / P));
@@ -482,7 +486,9 @@ static void renderGuided(HastyScene* s, const BatchItem& it) {
       HIPCHECK(hipMemcpy(&n, s->guideCount, sizeof(uint32_t), hipMemcpyDeviceToHost));
       n = std::min(n, HastyScene::kGuideSampleCap);
       s->lastGuideSamples = n;
-      if (n > 0) s->lastGuideLoss = s->guide->g->trainOnDevice(s->guideSamples, n, s->guideLr, 20.0f);
+      if (n > 0)
+        s->lastGuideLoss =
+            s->guide->g->trainEpoch(s->guideSamples, n, s->guideLr, s->guideWeightCap, s->guideMinibatch);
       gr.net = s->guide->g->net();
     }
   }
@@ -1221,6 +1227,20 @@ int hasty_scene_set_guide(HastyScene* s, HastyGuide* g, float fraction, float tr
   s->guide = g;
   s->guideFraction = fraction;
   s->guideLr = trainLr;
+  return 0;
+}
+
+int hasty_scene_set_guide_training(HastyScene* s, uint32_t minibatch, float weightCap) {
+  if (!s) {
+    g_lastError = "null argument";
+    return 1;
+  }
+  if (minibatch < 16 || minibatch > HastyScene::kGuideSampleCap || !(weightCap > 0.0f)) {
+    g_lastError = "minibatch must be in [16, 2^19] and the weight cap positive";
+    return 1;
+  }
+  s->guideMinibatch = minibatch;
+  s->guideWeightCap = weightCap;
   return 0;
 }
 

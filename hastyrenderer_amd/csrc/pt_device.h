@@ -50,6 +50,32 @@ __device__ inline V3 mix3(V3 a, V3 b, float l) {
 }
 __device__ inline float comp(V3 a, int i) { return i == 0 ? a.x : (i == 1 ? a.y : a.z); }
 __device__ inline V3 load3(const float* p) { return V3{p[0], p[1], p[2]}; }
+// Loads from scene arrays (device allocations) through global-address-space
+// pointers: a pointer the compiler cannot trace to a kernel argument (a
+// batched job's scene) would otherwise be read with flat loads, which also
+// wait on the LDS counter.  Only for global memory, never for locals.
+typedef float HNf4 __attribute__((ext_vector_type(4)));
+template <class T>
+__device__ inline T ldg(const T* p) {
+  return *(const __attribute__((address_space(1))) T*)p;
+}
+__device__ inline float4 ldg4(const float4* p) {
+  const HNf4 v = *(const __attribute__((address_space(1))) HNf4*)p;
+  return make_float4(v.x, v.y, v.z, v.w);
+}
+__device__ inline V3 load3g(const float* p) { return V3{ldg(p), ldg(p + 1), ldg(p + 2)}; }
+// Copy of a plain struct in global memory, by words.
+template <class T>
+__device__ inline T ldgStruct(const T* p) {
+  static_assert(sizeof(T) % 4 == 0, "word-sized");
+  const uint32_t* src = (const uint32_t*)p;
+  uint32_t w[sizeof(T) / 4];
+#pragma unroll
+  for (int i = 0; i < (int)(sizeof(T) / 4); ++i) w[i] = ldg(src + i);
+  T out;
+  __builtin_memcpy(&out, w, sizeof(T));
+  return out;
+}
 __device__ inline V3 f4xyz(float4 a) { return V3{a.x, a.y, a.z}; }
 __device__ inline float4 f4(V3 a, float w) { return make_float4(a.x, a.y, a.z, w); }
 
@@ -241,8 +267,14 @@ __device__ inline void cswap(float& ta, uint32_t& ra, float& tb, uint32_t& rb) {
 // in the lane's global overflow slice.  Children are ordered near-to-far with
 // a 4-input sorting network held in registers.
 template <int LDS_DEPTH, int OVF_DEPTH, bool CLOSEST>
-__device__ inline Hit traverse(const DevScene& s, const RayPre& r, float tBound, int32_t idBound, uint2* lds,
+__device__ inline Hit traverse(const DevScene& s, const RayPre& r, float tBound, int32_t idBound, uint2* ldsGeneric,
                                int laneStride, uint2* ovf, uint32_t* nodeCount, uint32_t* triCount) {
+  // The stack's LDS part through an LDS-qualified pointer: the compiler
+  // then keeps LDS and overflow accesses apart (ds_* and global_*) instead
+  // of merging them into one flat access, which would wait on both counters.
+  // Entries are (ref, entry distance) packed as ref | bits(t) << 32.
+  typedef __attribute__((address_space(3))) unsigned long long LdsEntry;
+  LdsEntry* lds = (LdsEntry*)ldsGeneric;
   Hit best;
   best.t = tBound;
   best.u = best.v = 0.0f;
@@ -254,8 +286,10 @@ __device__ inline Hit traverse(const DevScene& s, const RayPre& r, float tBound,
   while (!done) {
     if ((ref & kLeafFlag) == 0) {
       const float4* n = s.nodes + (size_t)ref * 8;
-      float4 lox = n[0], hix = n[1], loy = n[2], hiy = n[3], loz = n[4], hiz = n[5];
-      int4 refs = *reinterpret_cast<const int4*>(n + 6);
+      float4 lox = ldg4(n), hix = ldg4(n + 1), loy = ldg4(n + 2), hiy = ldg4(n + 3), loz = ldg4(n + 4),
+             hiz = ldg4(n + 5);
+      const float4 rf = ldg4(n + 6);
+      int4 refs = make_int4(__float_as_int(rf.x), __float_as_int(rf.y), __float_as_int(rf.z), __float_as_int(rf.w));
       nNodes++;
       float tb = (CLOSEST && best.tri < 0) ? __builtin_inff() : best.t * kCullSlack;
 #define HDEV_SLAB(LX, HX, LY, HY, LZ, HZ, RF, TOUT)                                          \
@@ -289,7 +323,7 @@ __device__ inline Hit traverse(const DevScene& s, const RayPre& r, float tBound,
 #define HDEV_PUSH(RR, EE)                                       \
   if (sp < LDS_DEPTH + OVF_DEPTH) {                            \
     uint2 e_ = make_uint2(RR, __float_as_uint(EE));            \
-    if (sp < LDS_DEPTH) lds[sp * laneStride] = e_;             \
+    if (sp < LDS_DEPTH) lds[sp * laneStride] = ((unsigned long long)e_.y << 32) | e_.x; \
     else ovf[sp - LDS_DEPTH] = e_;                             \
     ++sp;                                                      \
   }
@@ -305,7 +339,7 @@ __device__ inline Hit traverse(const DevScene& s, const RayPre& r, float tBound,
       uint32_t first = ref & 0x00FFFFFFu;
       for (uint32_t k = 0; k < count; ++k) {
         const float4* tr = s.tris + (size_t)(first + k) * 3;
-        float4 p0 = tr[0], p1 = tr[1], p2 = tr[2];
+        float4 p0 = ldg4(tr), p1 = ldg4(tr + 1), p2 = ldg4(tr + 2);
         int32_t id = (int32_t)__float_as_uint(p0.w);
         nTris++;
         float t, u, v;
@@ -333,7 +367,14 @@ __device__ inline Hit traverse(const DevScene& s, const RayPre& r, float tBound,
     bool found = false;
     while (sp > 0) {
       --sp;
-      uint2 e = sp < LDS_DEPTH ? lds[sp * laneStride] : ovf[sp - LDS_DEPTH];
+      uint2 e;
+      if (sp < LDS_DEPTH) {
+        const unsigned long long v = lds[sp * laneStride];
+        e = make_uint2((uint32_t)v, (uint32_t)(v >> 32));
+      } else {  // rare; volatile so that it is not merged with the LDS load
+        const volatile uint2* o = ovf + (sp - LDS_DEPTH);
+        e = make_uint2(o->x, o->y);
+      }
       float lim = (CLOSEST && best.tri < 0) ? __builtin_inff() : best.t * kCullSlack;
       if (__uint_as_float(e.y) <= lim) {
         ref = e.x;
@@ -378,15 +419,15 @@ struct Interaction {
 __device__ inline Interaction makeInteractionB(const DevScene& s, int32_t tri, float b0, float b1, float b2, V3 ng) {
   Interaction it;
   const float* p = s.triPos + (size_t)tri * 9;
-  V3 p0 = load3(p), p1 = load3(p + 3), p2 = load3(p + 6);
+  V3 p0 = load3g(p), p1 = load3g(p + 3), p2 = load3g(p + 6);
   it.tri = tri;
-  it.geomID = s.triGeom[tri];
-  it.mat = s.triMat[tri];
+  it.geomID = ldg(s.triGeom + tri);
+  it.mat = ldg(s.triMat + tri);
   it.ng = ng;
   it.x = p0 * b0 + p1 * b1 + p2 * b2;
-  if (s.triSmooth[tri]) {
+  if (ldg(s.triSmooth + tri)) {
     const float* q = s.triNrm + (size_t)tri * 9;
-    V3 n0 = load3(q), n1 = load3(q + 3), n2 = load3(q + 6);
+    V3 n0 = load3g(q), n1 = load3g(q + 3), n2 = load3g(q + 6);
     it.ns = normalize(n0 * b0 + n1 * b1 + n2 * b2);
     float h = 0.0f;
     V3 xv[3] = {p0, p1, p2};
@@ -411,7 +452,7 @@ __device__ inline Interaction makeInteractionB(const DevScene& s, int32_t tri, f
 // geometric normal normalize((v1-v0) x (v2-v0)).
 __device__ inline Interaction makeInteraction(const DevScene& s, int32_t tri, float u, float v) {
   const float* p = s.triPos + (size_t)tri * 9;
-  V3 p0 = load3(p), p1 = load3(p + 3), p2 = load3(p + 6);
+  V3 p0 = load3g(p), p1 = load3g(p + 3), p2 = load3g(p + 6);
   V3 ng = normalize(cross(p1 - p0, p2 - p0));
   return makeInteractionB(s, tri, 1.0f - u - v, u, v, ng);
 }
@@ -795,7 +836,7 @@ __device__ inline V3 envLookup(const DevScene& s, float u, float v) {
   int col = (int)(uu * s.envW);
   int row = (int)(om * s.envH);
   if (col > (int)s.envW - 1) col = (int)s.envW - 1;
-  return load3(s.envRGB + ((size_t)row * s.envW + col) * 3);
+  return load3g(s.envRGB + ((size_t)row * s.envW + col) * 3);
 }
 __device__ inline V3 evalEnvironment(const DevScene& s, V3 dir) {  // Scene.cpp:482-485
   if (!s.hasEnv) return v3(s.bg[0], s.bg[1], s.bg[2]);
@@ -873,7 +914,7 @@ __device__ inline V3 envSample(const DevScene& s, hrng::Stream& rng, float* dens
 // --------------------------------------------------------------- lights ----
 // Scene.cpp:587-616: light-strategy solid-angle density of (ray, hit).
 __device__ inline float lightDensity(const DevScene& s, V3 rayO, V3 rayD, int32_t hitTri, V3 hitX, V3 hitNg) {
-  bool surfaceLight = hitTri >= 0 && s.mats[s.triMat[hitTri]].isLight;
+  bool surfaceLight = hitTri >= 0 && ldg(&s.mats[ldg(s.triMat + hitTri)].isLight);
   bool infinite = hitTri < 0;
   if (!surfaceLight && !infinite) return 0.0f;
   float pSurf = !s.hasEnv ? 1.0f : (s.hasSurfaceLight ? 0.5f : 0.0f);
@@ -893,7 +934,7 @@ __device__ inline float lightDensity(const DevScene& s, V3 rayO, V3 rayD, int32_
 // Hit point and normalized geometric normal only (for light densities).
 __device__ inline void hitPointNormal(const DevScene& s, int32_t tri, float u, float v, V3* x, V3* ng) {
   const float* p = s.triPos + (size_t)tri * 9;
-  V3 p0 = load3(p), p1 = load3(p + 3), p2 = load3(p + 6);
+  V3 p0 = load3g(p), p1 = load3g(p + 3), p2 = load3g(p + 6);
   float b0 = 1.0f - u - v;
   *ng = normalize(cross(p1 - p0, p2 - p0));
   *x = p0 * b0 + p1 * u + p2 * v;

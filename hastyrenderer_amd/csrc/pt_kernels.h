@@ -37,6 +37,7 @@ struct PathBuffers {
   uint32_t* qS;   // shadow queue
   uint32_t* qR;   // paths that ended at this bounce
   uint32_t* qI;   // idle slots (stack)
+  uint32_t* mark; // bounce (epoch) whose extension queue holds the slot
   uint32_t* counters;         // see CounterId
   unsigned long long* stats;  // see StatId
   uint2* stackOverflow;
@@ -56,6 +57,7 @@ enum CounterId {
   kCntNext = 4,   // next path id to start
   kCntLimit = 5,  // path ids below this may start (results ring space)
   kCntIdle = 6,   // idle slots in qI
+  kCntEpoch = 7,  // bounces advanced since the render started
   kNumCounters = 8
 };
 
@@ -103,10 +105,10 @@ struct JobSet {
 // Traversal stack entries kept in LDS per lane (the rest spill to a global
 // slice); a build knob because it trades LDS (occupancy) for spills.
 #ifndef HASTY_STACK_LDS
-#define HASTY_STACK_LDS 16
+#define HASTY_STACK_LDS 8
 #endif
 constexpr int kStackLds = HASTY_STACK_LDS;
-constexpr int kStackOvf = 64;
+constexpr int kStackOvf = 80 - kStackLds;  // 80 entries per lane in all
 constexpr int kBlock = 256;
 
 // 64-bit device statistics (filled when LaunchConfig::countStats is set).

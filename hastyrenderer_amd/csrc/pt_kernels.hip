@@ -127,13 +127,69 @@ __device__ inline uint32_t findJob(const JobSet& js, uint32_t pid) {
   }
   return lo;
 }
+// Batched rendering keeps the job index of a path in flag bits 23-31 (set
+// when the path starts) for up to 512 jobs; larger batches search.
+constexpr uint32_t kJobShift = 23;
+constexpr uint32_t kFlagJobs = 512;
 template <bool MULTI>
-__device__ inline const DevScene& sceneOf(const JobSet& js, uint32_t j) {
-  return MULTI ? js.scenes[j] : js.s;
+__device__ inline uint32_t jobOf(const JobSet& js, const uint4& st) {
+  if (!MULTI) return 0u;
+  return js.nJobs <= kFlagJobs ? st.y >> kJobShift : findJob<true>(js, st.w);
 }
-template <bool MULTI>
-__device__ inline const PassInfo& passOf(const JobSet& js, uint32_t j) {
-  return MULTI ? js.passes[j] : js.pi;
+__device__ inline uint32_t jobBits(const JobSet& js, uint32_t job) {
+  return js.nJobs <= kFlagJobs ? job << kJobShift : 0u;
+}
+// A copy of *p, which every active lane of the wave passes, read through
+// the scalar unit.
+template <class T>
+__device__ inline T loadUniform(const T* p) {
+  static_assert(sizeof(T) % 4 == 0, "word-sized");
+  typedef __attribute__((address_space(4))) const uint32_t CWord;
+  const CWord* w = (const CWord*)p;
+  uint32_t words[sizeof(T) / 4];
+#pragma unroll
+  for (int i = 0; i < (int)(sizeof(T) / 4); ++i) words[i] = w[i];
+  T out;
+  __builtin_memcpy(&out, words, sizeof(T));
+  return out;
+}
+// A pointer read from memory is generic to the compiler (flat loads, which
+// wait on the LDS counter too); these round trips through the global address
+// space give every scene and pass array global loads.
+template <class T>
+__device__ inline const T* globalPtr(const T* p) {
+  return (const T*)(const __attribute__((address_space(1))) T*)p;
+}
+__device__ inline void makeGlobal(DevScene& s) {
+  s.nodes = globalPtr(s.nodes), s.tris = globalPtr(s.tris), s.triPos = globalPtr(s.triPos);
+  s.triNrm = globalPtr(s.triNrm), s.triMat = globalPtr(s.triMat), s.triGeom = globalPtr(s.triGeom);
+  s.triSmooth = globalPtr(s.triSmooth), s.mats = globalPtr(s.mats), s.lightTris = globalPtr(s.lightTris);
+  s.lightCdf = globalPtr(s.lightCdf), s.envRGB = globalPtr(s.envRGB), s.envPdf = globalPtr(s.envPdf);
+  s.envCdf = globalPtr(s.envCdf), s.envCdfGuide = globalPtr(s.envCdfGuide);
+}
+__device__ inline void makeGlobal(PassInfo& pi) {
+  pi.tileOrder = globalPtr(pi.tileOrder), pi.rankOf = globalPtr(pi.rankOf), pi.offsets = globalPtr(pi.offsets);
+}
+// f(scene, pass) of the lane's job.  Batched, the wave's active lanes are
+// served one distinct job at a time (normally one: paths are handed out in
+// id order), each with its scene and pass in scalar registers.
+template <bool MULTI, class F>
+__device__ inline void forJob(const JobSet& js, uint32_t job, F&& f) {
+  if constexpr (!MULTI) {
+    f(js.s, js.pi);
+  } else {
+    for (;;) {
+      const uint32_t j0 = __builtin_amdgcn_readfirstlane(job);
+      if (job == j0) {
+        DevScene sc = loadUniform(js.scenes + j0);
+        PassInfo pi = loadUniform(js.passes + j0);
+        makeGlobal(sc);
+        makeGlobal(pi);
+        f(sc, pi);
+        break;
+      }
+    }
+  }
 }
 // Pixel and sample of a path (global id) of job pi.
 __device__ inline void pathPixel(const PassInfo& pi, uint32_t pid, uint32_t* pixel, uint32_t* k) {
@@ -147,7 +203,7 @@ __device__ inline void pathPixel(const PassInfo& pi, uint32_t pid, uint32_t* pix
 // Camera ray of path `pid` (Camera.cpp:41-81 with the sample's offset,
 // RenderJob.h:53-60) into `slot`.
 __device__ inline void startPath(const DevScene& s, const PathBuffers& pb, const PassInfo& pi, uint32_t slot,
-                                 uint32_t pid) {
+                                 uint32_t pid, uint32_t flags) {
   uint32_t k, pixel;
   pathPixel(pi, pid, &pixel, &k);
   uint32_t i = pixel % s.width, j = pixel / s.width;
@@ -161,13 +217,72 @@ __device__ inline void startPath(const DevScene& s, const PathBuffers& pb, const
   pb.L[slot] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   pb.auxN[slot] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   pb.auxA[slot] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-  pb.state[slot] = make_uint4(rng.counter, 0u, 0xFFFFFFFFu, pid);
+  pb.state[slot] = make_uint4(rng.counter, flags, 0xFFFFFFFFu, pid);
 }
 
 // Every slot idle, no path started.
+// Occupancy targets (waves per SIMD) of the three ray-bound stages: the
+// register budget each may use is 512 / waves VGPRs.  0 leaves it to the
+// compiler.  Extend at 8 waves (64 VGPRs, no spills) is ~7% faster than at
+// the compiler's 7; shade and connect lose when squeezed.
+#ifndef HASTY_EXTEND_WAVES
+#define HASTY_EXTEND_WAVES 8
+#endif
+#ifndef HASTY_SHADE_WAVES
+#define HASTY_SHADE_WAVES 0
+#endif
+#ifndef HASTY_CONNECT_WAVES
+#define HASTY_CONNECT_WAVES 0
+#endif
+#define HASTY_WAVES_ATTR_(n) __attribute__((amdgpu_waves_per_eu(n)))
+#if HASTY_EXTEND_WAVES > 0
+#define HASTY_EXTEND_ATTR HASTY_WAVES_ATTR_(HASTY_EXTEND_WAVES)
+#else
+#define HASTY_EXTEND_ATTR
+#endif
+#if HASTY_SHADE_WAVES > 0
+#define HASTY_SHADE_ATTR HASTY_WAVES_ATTR_(HASTY_SHADE_WAVES)
+#else
+#define HASTY_SHADE_ATTR
+#endif
+#if HASTY_CONNECT_WAVES > 0
+#define HASTY_CONNECT_ATTR HASTY_WAVES_ATTR_(HASTY_CONNECT_WAVES)
+#else
+#define HASTY_CONNECT_ATTR
+#endif
+
+// The paths to extend and shade this bounce: while they fill more than half
+// the pool they are visited in slot order (entry q is slot q, taken if its
+// mark is this bounce), so that neighbouring lanes read neighbouring path
+// records; otherwise through the queue qA.  Either way every path is visited
+// once, and the image does not depend on the order.
+struct ExtSet {
+  bool dense;
+  uint32_t total, epoch;
+  __device__ bool member(const PathBuffers& pb, uint32_t q, uint32_t* slot) const {
+    if (!dense) {
+      *slot = pb.qA[q];
+      return true;
+    }
+    *slot = q;
+    return pb.mark[q] == epoch;
+  }
+};
+__device__ inline ExtSet extensionSet(const PathBuffers& pb) {
+  ExtSet e;
+  const uint32_t n = pb.counters[kCntA];
+  e.dense = 2ull * n > (unsigned long long)pb.capacity;
+  e.total = e.dense ? pb.capacity : n;
+  e.epoch = pb.counters[kCntEpoch];
+  return e;
+}
+
 __global__ void __launch_bounds__(kBlock) kInitRender(PathBuffers pb, uint32_t limit) {
   uint32_t stride = gridDim.x * blockDim.x;
-  for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < pb.capacity; k += stride) pb.qI[k] = k;
+  for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < pb.capacity; k += stride) {
+    pb.qI[k] = k;
+    pb.mark[k] = 0xFFFFFFFFu;
+  }
   if (blockIdx.x == 0 && threadIdx.x < kNumCounters) {
     uint32_t v = 0u;
     if (threadIdx.x == kCntLimit) v = limit;
@@ -197,8 +312,10 @@ __global__ void __launch_bounds__(kBlock) kFill(JobSet js, PathBuffers pb) {
     if (live) {
       slot = pb.qI[idle - 1 - q];
       uint32_t pid = next + q;
-      uint32_t j = findJob<MULTI>(js, pid);
-      startPath(sceneOf<MULTI>(js, j), pb, passOf<MULTI>(js, j), slot, pid);
+      const uint32_t j = findJob<MULTI>(js, pid);
+      const uint32_t flags = MULTI ? jobBits(js, j) : 0u;
+      pb.mark[slot] = pb.counters[kCntEpoch];
+      forJob<MULTI>(js, j, [&](const DevScene& sc, const PassInfo& pi) { startPath(sc, pb, pi, slot, pid, flags); });
     }
     uint32_t idx = blockAppend(&pb.counters[kCntA], live, ldsWave, &ldsBase);
     if (live) pb.qA[idx] = slot;
@@ -217,20 +334,22 @@ __global__ void kSetLimit(uint32_t* c, uint32_t limit) {
 
 // --------------------------------------------------------------- extend ----
 template <bool MULTI>
-__global__ void __launch_bounds__(kBlock) kExtend(JobSet js, PathBuffers pb, int countStats) {
+__global__ void __launch_bounds__(kBlock) HASTY_EXTEND_ATTR kExtend(JobSet js, PathBuffers pb, int countStats) {
   __shared__ uint2 stack[kStackLds * kBlock];
-  uint32_t n = pb.counters[kCntA];
-  const uint32_t* queue = pb.qA;
+  const ExtSet es = extensionSet(pb);
   uint32_t gtid = xcdBlock() * blockDim.x + threadIdx.x;
   uint32_t stride = gridDim.x * blockDim.x;
   uint2* ovf = pb.stackOverflow + (size_t)gtid * kStackOvf;
   uint32_t nodes = 0, tris = 0, rays = 0;
-  for (uint32_t q = gtid; q < n; q += stride) {
-    uint32_t slot = queue[q];
+  for (uint32_t q = gtid; q < es.total; q += stride) {
+    uint32_t slot;
+    if (!es.member(pb, q, &slot)) continue;
     float4 o = pb.rayO[slot], d = pb.rayD[slot];
-    const DevScene& s = sceneOf<MULTI>(js, MULTI ? findJob<MULTI>(js, pb.state[slot].w) : 0u);
-    Hit h = traceClosest<kStackLds, kStackOvf>(s, f4xyz(o), f4xyz(d), stack + threadIdx.x, kBlock, ovf,
-                                    countStats ? &nodes : nullptr, countStats ? &tris : nullptr);
+    const uint32_t job = MULTI ? jobOf<MULTI>(js, pb.state[slot]) : 0u;
+    Hit h;
+    forJob<MULTI>(js, job, [&](const DevScene& sc, const PassInfo&) {
+      h = traceClosest<kStackLds, kStackOvf>(sc, f4xyz(o), f4xyz(d), stack + threadIdx.x, kBlock, ovf, &nodes, &tris);
+    });
     pb.hit[slot] = make_float4(h.t, h.u, h.v, __int_as_float(h.tri));
     rays++;
   }
@@ -244,24 +363,23 @@ __global__ void __launch_bounds__(kBlock) kExtend(JobSet js, PathBuffers pb, int
 // ---------------------------------------------------------------- shade ----
 // Bits 20-21 of the flags: training records of the path (guided rendering).
 constexpr uint32_t kRecShift = 20;
+// Materials guided by the learned proposal: GGX alpha at least this.
+constexpr float kGuideMinAlpha = 0.25f;
 
 template <bool MULTI, bool GUIDED>
-__global__ void __launch_bounds__(kBlock) kShade(JobSet js, PathBuffers pb) {
+__global__ void __launch_bounds__(kBlock) HASTY_SHADE_ATTR kShade(JobSet js, PathBuffers pb) {
   __shared__ uint32_t ldsWave[kBlock / 64];
   __shared__ uint32_t ldsBase;
-  uint32_t n = pb.counters[kCntA];
+  const ExtSet es = extensionSet(pb);
   uint32_t stride = gridDim.x * blockDim.x;
-  for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += stride) {
+  for (uint32_t base = blockIdx.x * blockDim.x; base < es.total; base += stride) {
     uint32_t q = base + threadIdx.x;
     bool pushNext = false, pushShadow = false, live = false;
     uint32_t slot = 0;
-    if (q < n) {
+    if (q < es.total && es.member(pb, q, &slot)) {
       live = true;
-      slot = pb.qA[q];
-      uint4 st = pb.state[slot];
-      const uint32_t job = findJob<MULTI>(js, st.w);
-      const DevScene& s = sceneOf<MULTI>(js, job);
-      const PassInfo& pi = passOf<MULTI>(js, job);
+      const uint4 st = pb.state[slot];
+      forJob<MULTI>(js, jobOf<MULTI>(js, st), [&](const DevScene& s, const PassInfo& pi) {
       const bool twoStrategies = s.hasLight && s.useMIS;
       int depth = (int)(st.y & 0xFFu);
       int wavelength = (int)((st.y >> 8) & 0x3u) - 1;
@@ -324,7 +442,7 @@ __global__ void __launch_bounds__(kBlock) kShade(JobSet js, PathBuffers pb) {
           alive = false;
         } else {
           Interaction it = makeInteraction(s, tri, h4.y, h4.z);
-          const DevMaterial& m = s.mats[it.mat];
+          const DevMaterial m = ldgStruct(s.mats + it.mat);
           if (depth == 0) {
             pb.auxN[slot] = f4(it.ns, 0.0f);
             pb.auxA[slot] = make_float4(m.albedo[0], m.albedo[1], m.albedo[2], 0.0f);
@@ -345,8 +463,9 @@ __global__ void __launch_bounds__(kBlock) kShade(JobSet js, PathBuffers pb) {
               if (depth + 1 >= s.maxDepth) {
                 alive = false;  // every deeper estimate is zero
               } else {
-                // guided vertex: glossy or diffuse reflection (no delta lobe)
-                const bool guided = GUIDED && m.alpha > 0.0f && m.transmission == 0.0f;
+                // guided vertex: rough reflection (no delta lobe, and not so
+                // glossy that broad vMF lobes could only waste samples)
+                const bool guided = GUIDED && m.alpha >= kGuideMinAlpha && m.transmission == 0.0f;
                 guide::Mixture gm;
                 if (GUIDED && guided) {
                   float raw[guide::kOut];
@@ -367,9 +486,9 @@ __global__ void __launch_bounds__(kBlock) kShade(JobSet js, PathBuffers pb) {
                   int32_t target = -1;
                   if (xi < pSurf) {
                     uint32_t li = lightCdfSearch(s, rng.canonical_double());
-                    target = (int32_t)s.lightTris[li];
+                    target = (int32_t)ldg(s.lightTris + li);
                     const float* p = s.triPos + (size_t)target * 9;
-                    V3 p0 = load3(p), p1 = load3(p + 3), p2 = load3(p + 6);
+                    V3 p0 = load3g(p), p1 = load3g(p + 3), p2 = load3g(p + 6);
                     float su = rng.uniform01f();
                     float tt = hm::fsqrt(rng.uniform01f());
                     float b1 = 1.0f - tt;
@@ -455,7 +574,7 @@ __global__ void __launch_bounds__(kBlock) kShade(JobSet js, PathBuffers pb) {
                 if (pushNext) {
                   uint32_t recBits = 0u;
                   if (GUIDED) {
-                    if (guided && nrec < 2u) {
+                    if (guided && nrec < 2u && js.g.sampleCap > 0u) {  // records only when training
                       float4* rr = js.g.rec + (size_t)slot * 12 + 6 * nrec;
                       V3 nf = dot(it.ng, wo) >= 0.0f ? it.ng : -it.ng;
                       rr[0] = f4(it.x, 0.0f);
@@ -469,8 +588,11 @@ __global__ void __launch_bounds__(kBlock) kShade(JobSet js, PathBuffers pb) {
                       recBits = nrec << kRecShift;
                     }
                   }
+                  pb.mark[slot] = es.epoch + 1u;
                   pb.state[slot] =
-                      make_uint4(rng.counter, packFlags(depth + 1, wavelength, md) | recBits, (uint32_t)tri, st.w);
+                      make_uint4(rng.counter,
+                                 packFlags(depth + 1, wavelength, md) | recBits | (st.y & ~((1u << kJobShift) - 1u)),
+                                 (uint32_t)tri, st.w);
                   if (md.n > 0) storeMedia(pb, slot, md);
                 }
               }
@@ -479,6 +601,7 @@ __global__ void __launch_bounds__(kBlock) kShade(JobSet js, PathBuffers pb) {
         }
       }
       pb.L[slot] = f4(L, 0.0f);
+      });
     }
     uint32_t idxN = blockAppend(&pb.counters[kCntB], pushNext, ldsWave, &ldsBase);
     if (pushNext) pb.qB[idxN] = slot;
@@ -492,7 +615,7 @@ __global__ void __launch_bounds__(kBlock) kShade(JobSet js, PathBuffers pb) {
 
 // -------------------------------------------------------------- connect ----
 template <bool MULTI>
-__global__ void __launch_bounds__(kBlock) kConnect(JobSet js, PathBuffers pb, int countStats) {
+__global__ void __launch_bounds__(kBlock) HASTY_CONNECT_ATTR kConnect(JobSet js, PathBuffers pb, int countStats) {
   __shared__ uint2 stack[kStackLds * kBlock];
   uint32_t n = pb.counters[kCntS];
   uint32_t gtid = xcdBlock() * blockDim.x + threadIdx.x;
@@ -501,7 +624,8 @@ __global__ void __launch_bounds__(kBlock) kConnect(JobSet js, PathBuffers pb, in
   uint32_t nodes = 0, tris = 0, rays = 0;
   for (uint32_t q = gtid; q < n; q += stride) {
     uint32_t slot = pb.qS[q];
-    const DevScene& s = sceneOf<MULTI>(js, MULTI ? findJob<MULTI>(js, pb.state[slot].w) : 0u);
+    const uint32_t job = MULTI ? jobOf<MULTI>(js, pb.state[slot]) : 0u;
+    forJob<MULTI>(js, job, [&](const DevScene& s, const PassInfo&) {
     float4 o4 = pb.shO[slot], d4 = pb.shD[slot];
     V3 so = f4xyz(o4), sd = f4xyz(d4);
     int32_t target = __float_as_int(d4.w);
@@ -515,38 +639,39 @@ __global__ void __launch_bounds__(kBlock) kConnect(JobSet js, PathBuffers pb, in
       // closest hit == target  <=>  the target is hit at (tT, target) and no
       // other triangle hits before it (or at tT with a smaller index)
       const float* p = s.triPos + (size_t)target * 9;
-      float4 p0 = make_float4(p[0], p[1], p[2], 0.0f), p1 = make_float4(p[3], p[4], p[5], 0.0f),
-             p2 = make_float4(p[6], p[7], p[8], 0.0f);
+      float4 p0 = make_float4(ldg(p), ldg(p + 1), ldg(p + 2), 0.0f),
+             p1 = make_float4(ldg(p + 3), ldg(p + 4), ldg(p + 5), 0.0f),
+             p2 = make_float4(ldg(p + 6), ldg(p + 7), ldg(p + 8), 0.0f);
       float tT, uT, vT;
       visible = intersectTri(rp, p0, p1, p2, 1e-6f, &tT, &uT, &vT);
       if (visible) {
         visible = findBlocker<kStackLds, kStackOvf>(s, rp, tT, target, stack + threadIdx.x, kBlock, ovf,
-                                                    countStats ? &nodes : nullptr, countStats ? &tris : nullptr) < 0;
+                                                    &nodes, &tris) < 0;
         h.t = tT, h.u = uT, h.v = vT, h.tri = target;
       }
     } else if (!s.hasSurfaceLight) {
       // no emissive triangle: visible <=> nothing is hit at all
       visible = findBlocker<kStackLds, kStackOvf>(s, rp, __builtin_inff(), 0x7FFFFFFF, stack + threadIdx.x, kBlock,
-                                                  ovf, countStats ? &nodes : nullptr,
-                                                  countStats ? &tris : nullptr) < 0;
+                                                  ovf, &nodes,
+                                                  &tris) < 0;
       h.t = __builtin_inff(), h.u = h.v = 0.0f, h.tri = -1;
     } else {
       h = traceClosest<kStackLds, kStackOvf>(s, so, sd, stack + threadIdx.x, kBlock, ovf,
-                                             countStats ? &nodes : nullptr, countStats ? &tris : nullptr);
-      bool hitLight = h.tri >= 0 && s.mats[s.triMat[h.tri]].isLight;
+                                             &nodes, &tris);
+      bool hitLight = h.tri >= 0 && ldg(&s.mats[ldg(s.triMat + h.tri)].isLight);
       visible = h.tri < 0 || hitLight;
     }
-    if (!visible) continue;
+    if (!visible) return;
     V3 hx = v3(0.0f, 0.0f, 0.0f), hn = v3(0.0f, 0.0f, 0.0f);
     if (h.tri >= 0) hitPointNormal(s, h.tri, h.u, h.v, &hx, &hn);
     float pL = lightDensity(s, so, sd, h.tri, hx, hn);
-    if (pL == 0.0f) continue;
+    if (pL == 0.0f) return;
     float4 fd4 = pb.shFD[slot], fc4 = pb.shFC[slot];
     float mm = fd4.w / pL;
     V3 tD = f4xyz(fd4) * mm, tC = f4xyz(fc4) * mm;
     V3 thr = f4xyz(pb.shThr[slot]);
     V3 t2 = mul(thr, tD + tC);
-    if (isZero(t2) || h.tri == __float_as_int(fc4.w)) continue;
+    if (isZero(t2) || h.tri == __float_as_int(fc4.w)) return;
     float pB = o4.w;
     float sum = 0.0f;
     sum += powci<2>(pL);
@@ -558,7 +683,7 @@ __global__ void __launch_bounds__(kBlock) kConnect(JobSet js, PathBuffers pb, in
     if (h.tri < 0) {
       Li = mul(beersLaw(Tc, 1e6f), evalEnvironment(s, sd));
     } else {
-      const DevMaterial& lm = s.mats[s.triMat[h.tri]];
+      const DevMaterial lm = ldgStruct(s.mats + ldg(s.triMat + h.tri));
       V3 Tb = beersLaw(Tc, norm(hx - so));
       V3 Le = dot(hn, -sd) <= 0.0f ? v3(0.0f, 0.0f, 0.0f)
                                     : v3(lm.emission[0], lm.emission[1], lm.emission[2]) * (float)kInvPi;
@@ -567,6 +692,7 @@ __global__ void __launch_bounds__(kBlock) kConnect(JobSet js, PathBuffers pb, in
     V3 beta = f4xyz(pb.shBeta[slot]);
     float4 L4 = pb.L[slot];
     pb.L[slot] = f4(f4xyz(L4) + mul(beta, mul(Li, t2)), 0.0f);
+    });
   }
   if (countStats) {
     waveAddStat(&pb.stats[kStatShadowRays], rays);
@@ -667,13 +793,21 @@ __global__ void __launch_bounds__(kBlock) kGuideInfer(JobSet js, PathBuffers pb)
     uint32_t slot = pb.qA[valid ? e : n - 1];
     float4 h4 = pb.hit[slot];
     int32_t tri = __float_as_int(h4.w);
+    // only hits on guided materials read the outputs; a wave whose 16 paths
+    // have none skips the network (the test is uniform over the wave)
+    bool want = valid && tri >= 0;
+    if (want) {
+      const DevMaterial& m = s.mats[s.triMat[tri]];
+      want = !m.isLight && m.alpha >= kGuideMinAlpha && m.transmission == 0.0f;
+    }
+    if (__ballot(want) == 0ull) continue;
     V3 x = v3(0.0f, 0.0f, 0.0f), ng = v3(0.0f, 0.0f, 1.0f);
     if (tri >= 0) hitPointNormal(s, tri, h4.y, h4.z, &x, &ng);
     V3 wo = -f4xyz(pb.rayD[slot]);
     if (dot(ng, wo) < 0.0f) ng = -ng;
     float raw[12];
     guide::inferOne(js.g.net, x, ng, wo, raw);
-    if (valid && tri >= 0) {
+    if (want) {
       float4* out = js.g.raw + (size_t)slot * 10;
 #pragma unroll
       for (int mt = 0; mt < 3; ++mt)
@@ -685,6 +819,7 @@ __global__ void __launch_bounds__(kBlock) kGuideInfer(JobSet js, PathBuffers pb)
 // Next bounce: qB becomes the input queue (the host swaps the pointers).
 __global__ void kAdvance(uint32_t* c) {
   if (threadIdx.x == 0) {
+    c[kCntEpoch] += 1u;
     c[kCntA] = c[kCntB];
     c[kCntB] = 0u;
     c[kCntS] = 0u;

@@ -178,13 +178,18 @@ int hasty_guide_train_step(HastyGuide* guide, const float* samples, uint32_t n, 
  * continuation direction at glossy and diffuse vertices from the guide with
  * probability `fraction` and from the BSDF otherwise, weighting by the
  * mixture pdf (one-sample MIS; NEE's MIS uses the same mixture), so the
- * estimate stays unbiased.  Samples are rendered one index at a time; when
- * train_lr > 0 each index is followed by one Adam step on the radiance
- * samples its paths produced (weight = luminance of the radiance the
- * continuation brought over the throughput before the sample, normalised to
- * mean 1 and capped at 20).  guide NULL returns to plain rendering.  The
- * guide must belong to the scene; the scene must outlive its use. */
+ * estimate stays unbiased.  When train_lr > 0 samples are rendered one
+ * index at a time, each followed by one pass of Adam steps over the radiance
+ * samples its paths produced (up to 2^19; weight = luminance of the radiance
+ * the continuation brought over the throughput before the sample, normalised
+ * to mean 1 and capped), in minibatches of a pseudo-random order; without
+ * training all indices go in one call.  guide NULL returns to plain
+ * rendering.  The guide must belong to the scene; the scene must outlive its
+ * use. */
 int hasty_scene_set_guide(HastyScene* scene, HastyGuide* guide, float fraction, float train_lr);
+/* Online-training minibatch (default 16384, in [16, 2^19]) and weight cap
+ * (default 20).  No reference counterpart. */
+int hasty_scene_set_guide_training(HastyScene* scene, uint32_t minibatch, float weight_cap);
 /* loss and sample count of the last online training step */
 int hasty_scene_guide_stats(const HastyScene* scene, float* last_loss, uint32_t* last_samples);
 

@@ -81,15 +81,17 @@ def init_params(seed=0):
 def encode(x, n, wo):
     """Input features (B, 80): per axis a and frequency f, sin(2^f pi x_a),
     cos(2^f pi x_a) at 24a + 2f, 24a + 2f + 1; then n, wo, two zeros.  x is
-    the position normalised to [0, 1]^3.  Computed in float32 like the GPU."""
+    the position normalised to [0, 1]^3, in float32 like the GPU; 2^f x_a is
+    exact in float32 and pi (2^f x_a) is formed in float64, as the GPU's
+    sinpi / cospi of 2^f x_a do."""
     x = np.asarray(x, np.float32)
     B = x.shape[0]
     out = np.zeros((B, IN_PAD), np.float64)
     for a in range(3):
         for f in range(N_FREQ):
-            ang = (np.float32(2.0 ** f * math.pi) * x[:, a]).astype(np.float32)
-            out[:, 24 * a + 2 * f] = np.sin(ang.astype(np.float64))
-            out[:, 24 * a + 2 * f + 1] = np.cos(ang.astype(np.float64))
+            t = (np.float32(2.0 ** f) * x[:, a]).astype(np.float64)
+            out[:, 24 * a + 2 * f] = np.sin(math.pi * t)
+            out[:, 24 * a + 2 * f + 1] = np.cos(math.pi * t)
     out[:, 72:75] = n
     out[:, 75:78] = wo
     return out

@@ -115,3 +115,24 @@ def test_bf16_rounding_known_values():
     v = np.array([1.0, 1.00390625, 1.005859375, 3.0e38, -0.0], np.float32)
     # 1 + 2^-8 is a tie between 1 and 1 + 2^-7: nearest even keeps 1
     np.testing.assert_array_equal(gr.bf16(v)[:3], [1.0, 1.0, 1.0078125])
+
+
+def test_encoding_octaves():
+    """The positional encoding is sin / cos of 2^f pi x: unit pairs, each
+    octave the angle double of the one below (sin 2t = 2 sin t cos t,
+    cos 2t = 1 - 2 sin^2 t), exact values at dyadic points, and the normal
+    and outgoing direction copied after the 72 encoded features."""
+    rng = np.random.default_rng(5)
+    x, n, wo, _, _ = _batch(rng, 64)
+    x[0] = (0.5, 0.25, 0.0)
+    f = gr.encode(x, n, wo)
+    s = f[:, :72:2].reshape(-1, 3, 12)
+    c = f[:, 1:72:2].reshape(-1, 3, 12)
+    assert np.allclose(s ** 2 + c ** 2, 1.0, atol=1e-12)
+    assert np.allclose(s[:, :, 1:], 2 * s[:, :, :-1] * c[:, :, :-1], atol=1e-9)
+    assert np.allclose(c[:, :, 1:], 1 - 2 * s[:, :, :-1] ** 2, atol=1e-9)
+    # sin(pi / 2) = 1, cos(pi / 2) = 0 (x = 0.5, f = 0); sin(pi / 4 * 2) at x = 0.25, f = 1
+    assert abs(s[0, 0, 0] - 1.0) < 1e-15 and abs(c[0, 0, 0]) < 1e-15
+    assert abs(s[0, 1, 1] - 1.0) < 1e-15 and abs(s[0, 2, :]).max() == 0.0
+    assert np.array_equal(f[:, 72:75], n) and np.array_equal(f[:, 75:78], wo)
+    assert not f[:, 78:].any()
