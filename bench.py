@@ -289,8 +289,8 @@ def bench_guide(args):
     (512^2 by default so that the reference can take 16384 spp), max depth
     32.  The reference is a plain render of --guide-ref-spp samples; both
     methods render --guide-spp samples; the guided one trains online after
-    each of its first --guide-train-spp sample indices (one pass of Adam steps
-    over --guide-minibatch samples each), then renders the rest
+    groups of its first --guide-train-spp sample indices (one pass of Adam
+    steps of --guide-minibatch samples per group of ~2^20 paths), then renders the rest
     with the trained guide (training time included).  Since relMSE falls as
     1 / spp, time to reach the plain render's error with the guide =
     T_guided * relMSE_guided / relMSE_plain; value = T_plain / that time."""
@@ -341,7 +341,7 @@ def bench_guide(args):
     hip.sync()
     t = time.perf_counter()
     gsc.set_guide_training(minibatch=args.guide_minibatch)
-    gsc.set_guide(g, fraction=0.5, train_lr=args.guide_lr)  # online training after each sample index
+    gsc.set_guide(g, fraction=0.5, train_lr=args.guide_lr)  # online training
     gsc.render_pt_device(5, 0, ntrain, *f.ptrs)
     gsc.set_guide(g, fraction=0.5, train_lr=0.0)  # trained guide, remaining samples in one call
     if n > ntrain:
@@ -392,7 +392,12 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--cpu-spp", type=int, default=48)
     ap.add_argument("--workload", choices=("ggx", "sppm", "batch", "guide"), default="ggx")
-    ap.add_argument("--tune-spp", type=int, default=4, help="samples per autotuner trial (0: no tuning)")
+    ap.add_argument("--tune-spp", type=int, default=0,
+                    help="samples per autotuner trial before timing (0: the library's default launch, which is "
+                         "what the autotuner picks on this scene, without the run-to-run noise of short trials)")
+    ap.add_argument("--launch", default="", metavar="SLOTS,TRACE,SHADE,CHUNK,GRAPH",
+                    help="explicit launch shape (path slots, trace blocks, shade blocks, bounces per check, "
+                         "hipGraph 0/1) instead of autotuning")
     ap.add_argument("--guide-spp", type=int, default=64)
     ap.add_argument("--guide-res", type=int, default=512)
     ap.add_argument("--guide-train-spp", type=int, default=16)
@@ -447,8 +452,16 @@ def main():
     log("scene loaded in %.2f s: %d triangles, BVH %d nodes depth %d, %.1f MB in HBM" % (
         load_s, info["num_triangles"], info["bvh_nodes"], info["bvh_depth"], info["device_bytes"] / 1e6))
 
-    # launch shape: the autotuner times short renders of this scene (untimed)
-    tuned = scene.autotune(seed=seed, trial_samples=args.tune_spp) if args.tune_spp > 0 else scene.get_launch()
+    # launch shape: given (--launch), else what the autotuner picks from short
+    # renders of this scene (untimed), else the default
+    if args.launch:
+        keys = ("path_slots", "trace_blocks", "shade_blocks", "chunk", "use_graph")
+        scene.set_launch_ex(**dict(zip(keys, (int(x) for x in args.launch.split(",")))))
+        tuned = scene.get_launch()
+    elif args.tune_spp > 0:
+        tuned = scene.autotune(seed=seed, trial_samples=args.tune_spp)
+    else:
+        tuned = scene.get_launch()
     log("launch: %s" % json.dumps(tuned))
     hip = Hip()
     nbytes = W * H * 3 * 4

@@ -285,9 +285,9 @@ class Scene:
                                           C.c_void_p(d_normal), C.c_void_p(d_albedo)))
 
     def set_guide(self, guide, fraction=0.5, train_lr=0.0):
-        """Guided rendering with `guide` (None: plain); online Adam training
-        (one pass of minibatch steps) after every sample index when
-        train_lr > 0."""
+        """Guided rendering with `guide` (None: plain); with train_lr > 0,
+        online Adam training (one pass of minibatch steps) after every group
+        of sample indices of about 2^20 paths."""
         self._guide = guide
         _err(lib().hasty_scene_set_guide(self._h, guide.handle if guide is not None else None, fraction,
                                          train_lr))

@@ -150,6 +150,53 @@ __device__ inline hdev::V3 mixtureSample(const Mixture& m, float xi0, float xi1,
   return hdev::normalize(b1 * (s * __cosf(phi)) + b2 * (s * __sinf(phi)) + mu * wz);
 }
 
+// Mixture table of a shading point (10 float4): lobe k as (v_k, b_k) with
+// v_k = kappa_k mu_k and b_k = log(w_k kappa_k / (2 pi (1 - e^{-2 kappa_k})))
+// - kappa_k, so that lobe k's density term is exp(v_k . d + b_k); then the
+// eight weights w_k.  Same mixture as mixtureFromRaw.
+__device__ inline float4 lobeEntry(const float* r5, float wk) {
+  hdev::V3 mu = hdev::v3(r5[0], r5[1], r5[2]);
+  const float nr = sqrtf(hdev::dot(mu, mu));
+  mu = nr > 1e-12f ? mu / nr : hdev::v3(0.0f, 0.0f, 1.0f);
+  const float kappa = __expf(fminf(fmaxf(r5[3], kCLo), kCHi));
+  const float norm = kappa / (2.0f * 3.14159265358979323846f * -expm1f(-2.0f * kappa));
+  return make_float4(mu.x * kappa, mu.y * kappa, mu.z * kappa, __logf(wk * norm) - kappa);
+}
+__device__ inline float mixturePdfTable(const float4* g, hdev::V3 d) {
+  float p = 0.0f;
+#pragma unroll
+  for (int k = 0; k < kLobes; ++k) {
+    const float4 a = hdev::ldg4(g + k);
+    p += __expf(a.x * d.x + a.y * d.y + a.z * d.z + a.w);
+  }
+  return p;
+}
+__device__ inline hdev::V3 mixtureSampleTable(const float4* g, float xi0, float xi1, float xi2) {
+  const float4 w0 = hdev::ldg4(g + kLobes), w1 = hdev::ldg4(g + kLobes + 1);
+  const float w[kLobes] = {w0.x, w0.y, w0.z, w0.w, w1.x, w1.y, w1.z, w1.w};
+  int k = kLobes - 1;
+  float c = 0.0f;
+  bool found = false;
+#pragma unroll
+  for (int j = 0; j < kLobes - 1; ++j) {
+    c += w[j];
+    const bool take = !found && xi0 < c;
+    k = take ? j : k;
+    found = found || take;
+  }
+  const float4 a = hdev::ldg4(g + k);
+  const float kappa = sqrtf(a.x * a.x + a.y * a.y + a.z * a.z);
+  const hdev::V3 mu = hdev::v3(a.x, a.y, a.z) / kappa;
+  float wz = 1.0f + __logf(xi1 + (1.0f - xi1) * __expf(-2.0f * kappa)) / kappa;
+  wz = fminf(1.0f, fmaxf(-1.0f, wz));
+  const float s = sqrtf(fmaxf(0.0f, 1.0f - wz * wz));
+  const float phi = 2.0f * 3.14159265358979323846f * xi2;
+  const hdev::V3 t = fabsf(mu.x) > 0.9f ? hdev::v3(0.0f, 1.0f, 0.0f) : hdev::v3(1.0f, 0.0f, 0.0f);
+  const hdev::V3 b1 = hdev::normalize(hdev::cross(mu, t));
+  const hdev::V3 b2 = hdev::cross(mu, b1);
+  return hdev::normalize(b1 * (s * __cosf(phi)) + b2 * (s * __sinf(phi)) + mu * wz);
+}
+
 // ------------------------------------------------------------- MFMA MLP ----
 // Every layer is computed transposed, H_out^T (out x 16 queries) =
 // W^T (out x in) . H_in^T (in x 16), by 16x16x16 tiles of
@@ -243,6 +290,82 @@ __device__ inline void forwardAll(const Net& net, Acts& a, f32x4* raw) {
 #pragma unroll
   for (int mt = 0; mt < 4; ++mt) a.h[4][mt] = reluPack(a.z[3][mt]);
   layerFwd<4, 3>(net.wT + wqOffset(4), net.param + bOffset(4), kHidden, a.h[4], raw);
+}
+
+// ---------------------------------------------- LDS-staged inference ----
+// Inference kernels copy the BF16 W^T and the biases into LDS once per block
+// (42,176 B) and loop over many 16-query tiles, so that each MFMA's weight
+// operand is an LDS read instead of a global load per tile.
+constexpr int kBiasLds = 4 * kHidden + kOutPad;
+__host__ __device__ constexpr int biasLds(int l) { return l * kHidden; }
+struct LdsNet {
+  short wT[kWeights];
+  float bias[kBiasLds];
+};
+// Output row of the last layer held at LDS row p when the rows are grouped by
+// lobe: lane group q of a 16-query tile holds rows 16 mt + 4 q + v (mt < 3,
+// v < 4) of the output; with this order its 12 values are raw outputs
+// 10 q .. 10 q + 9 (lobes 2q and 2q + 1) then two pads.
+__device__ inline int lobeMajorRow(int p) {
+  const int mt = p >> 4, q = (p >> 2) & 3, v = p & 3, sl = 4 * mt + v;
+  return sl < 10 ? 10 * q + sl : kOut + 2 * q + (sl - 10);
+}
+// lobeMajor: the last layer's rows in lobeMajorRow order (kGuideInfer);
+// otherwise the network's own order.
+__device__ inline void stageNet(const Net& net, LdsNet* L, bool lobeMajor = false) {
+  const uint4* src = reinterpret_cast<const uint4*>(net.wT);
+  uint4* dst = reinterpret_cast<uint4*>(L->wT);
+  constexpr int lastRowVecs = kHidden / 8;  // uint4 per row of the last layer
+  constexpr int lastBase = wqOffset(kLayers - 1) / 8;
+  for (int i = threadIdx.x; i < kWeights / 8; i += blockDim.x) {
+    int from = i;
+    if (lobeMajor && i >= lastBase) {
+      const int row = (i - lastBase) / lastRowVecs, col = (i - lastBase) % lastRowVecs;
+      from = lastBase + lobeMajorRow(row) * lastRowVecs + col;
+    }
+    dst[i] = src[from];
+  }
+  for (int i = threadIdx.x; i < kBiasLds; i += blockDim.x) {
+    const int l = i < 4 * kHidden ? i / kHidden : 4;
+    int o = i - biasLds(l);
+    if (lobeMajor && l == kLayers - 1) o = lobeMajorRow(o);
+    L->bias[i] = net.param[bOffset(l) + o];
+  }
+  __syncthreads();
+}
+template <int KS, int MT>
+__device__ inline void layerFwdLds(const LdsNet* L, int l, const bf16x4* x, f32x4* z) {
+  const int lane = threadIdx.x & 63, r = lane & 15, q = lane >> 4;
+  const int din = layerIn(l);
+#pragma unroll
+  for (int mt = 0; mt < MT; ++mt) {
+    f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+    const short* row = L->wT + wqOffset(l) + (16 * mt + r) * din + 4 * q;
+#pragma unroll
+    for (int ks = 0; ks < KS; ++ks) acc = mfma(load4(row + 16 * ks), x[ks], acc);
+    const float* b = L->bias + biasLds(l) + 16 * mt + 4 * q;
+    acc.x += b[0];
+    acc.y += b[1];
+    acc.z += b[2];
+    acc.w += b[3];
+    z[mt] = acc;
+  }
+}
+// Forward pass of one 16-query tile from the LDS copy (inputs in h0); only
+// the raw outputs are kept.
+__device__ inline void forwardLds(const LdsNet* L, const bf16x4* h0, f32x4* raw) {
+  f32x4 z[4];
+  bf16x4 h[4];
+  layerFwdLds<5, 4>(L, 0, h0, z);
+#pragma unroll
+  for (int l = 1; l < kLayers - 1; ++l) {
+#pragma unroll
+    for (int mt = 0; mt < 4; ++mt) h[mt] = reluPack(z[mt]);
+    layerFwdLds<4, 4>(L, l, h, z);
+  }
+#pragma unroll
+  for (int mt = 0; mt < 4; ++mt) h[mt] = reluPack(z[mt]);
+  layerFwdLds<4, 3>(L, kLayers - 1, h, raw);
 }
 
 __device__ inline hdev::V3 normPos(const Net& net, hdev::V3 p) {

@@ -32,10 +32,12 @@ class Guide {
   // One Adam step on n samples whose weights are first normalised to mean 1
   // and capped at weightCap (in place); returns the mean loss.
   float trainOnDevice(float4* dSamples, uint32_t n, float lr, float weightCap);
-  // One pass over n samples in Adam steps of `minibatch` samples, in a fixed
-  // pseudo-random order (the samples arrive grouped by image region); weights
-  // are normalised over all n first.  Returns the mean loss of the last step.
-  float trainEpoch(float4* dSamples, uint32_t n, float lr, float weightCap, uint32_t minibatch);
+  // Adam steps of `minibatch` samples over n samples in pseudo-random order
+  // (the samples arrive grouped by image region): one pass, or more passes
+  // (each in another order) until minSteps steps are taken.  Weights are
+  // normalised over all n first.  Returns the mean loss of the last step.
+  float trainEpoch(float4* dSamples, uint32_t n, float lr, float weightCap, uint32_t minibatch,
+                   uint32_t minSteps = 1);
   void getGrad(float* host);
   void zeroGrad();
   uint32_t steps() const { return step_; }

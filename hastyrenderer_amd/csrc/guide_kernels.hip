@@ -31,7 +31,10 @@ void hipOk(hipError_t e, const char* what) {
 // ---------------------------------------------------------------- infer ----
 // Raw outputs (48 floats) of n queries given as (pos, normal, wo) float4
 // triples; queries are taken 16 per wave.
+// (The compiler keeps the loop-invariant LDS weights in registers here.)
 __global__ void __launch_bounds__(256) kInfer(Net net, const float4* in, uint32_t n, float4* raw) {
+  __shared__ LdsNet ldsNet;
+  stageNet(net, &ldsNet);
   const int lane = threadIdx.x & 63, r = lane & 15, q = lane >> 4;
   uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   uint32_t waves = (gridDim.x * blockDim.x) >> 6;
@@ -39,10 +42,10 @@ __global__ void __launch_bounds__(256) kInfer(Net net, const float4* in, uint32_
     uint32_t j = t * 16 + r;
     uint32_t jj = j < n ? j : n - 1;
     float4 p4 = in[3 * jj], n4 = in[3 * jj + 1], w4 = in[3 * jj + 2];
-    Acts a;
-    inputOperands(normPos(net, f4xyz(p4)), f4xyz(n4), f4xyz(w4), a.h[0]);
+    bf16x4 h0[5];
+    inputOperands(normPos(net, f4xyz(p4)), f4xyz(n4), f4xyz(w4), h0);
     f32x4 o[3];
-    forwardAll(net, a, o);
+    forwardLds(&ldsNet, h0, o);
     if (j < n) {
 #pragma unroll
       for (int mt = 0; mt < 3; ++mt) raw[(size_t)j * 12 + 4 * mt + q] = make_float4(o[mt].x, o[mt].y, o[mt].z, o[mt].w);
@@ -240,8 +243,10 @@ __global__ void __launch_bounds__(256) kWeightGrad(const short* H, const short* 
 }
 
 // ------------------------------------------------------------------ Adam ----
+// One Adam step of every parameter (the gradient is consumed and cleared),
+// with the BF16 W^T and W copies of the weights refreshed in the same pass.
 __global__ void __launch_bounds__(256) kAdam(float* param, float* m, float* v, float* grad, float lr, float b1,
-                                             float b2, float eps, float c1, float c2) {
+                                             float b2, float eps, float c1, float c2, short* wT, short* w) {
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < (uint32_t)kParams; i += gridDim.x * blockDim.x) {
     float g = grad[i];
     float mi = b1 * m[i] + (1.0f - b1) * g;
@@ -251,6 +256,17 @@ __global__ void __launch_bounds__(256) kAdam(float* param, float* m, float* v, f
     float p = param[i] - lr * (mi / c1) / (sqrtf(vi / c2) + eps);
     param[i] = p;
     grad[i] = 0.0f;
+    // BF16 copies of a weight (not of a bias), as kQuantize writes them
+    int l = 0;
+    while (l + 1 < kLayers && (uint32_t)wOffset(l + 1) <= i) ++l;
+    const int din = layerIn(l), dout = layerOut(l);
+    const uint32_t local = i - (uint32_t)wOffset(l);
+    if (local < (uint32_t)(din * dout)) {
+      const int o = local / din, k = local - o * din;
+      const short b = toBf16(p);
+      wT[wqOffset(l) + local] = b;
+      w[wqOffset(l) + (size_t)k * dout + o] = b;
+    }
   }
 }
 // BF16 copies of the weights: W^T as stored, W transposed.
@@ -368,7 +384,7 @@ Net Guide::net() const {
 void Guide::inferDevice(const float4* dIn, uint32_t n, float4* dRaw) {
   if (n == 0) return;
   uint32_t tiles = (n + 15) / 16;
-  uint32_t blocks = std::min<uint32_t>(4096, (tiles + 3) / 4);
+  uint32_t blocks = std::min<uint32_t>(768, (tiles + 3) / 4);  // about one resident round
   kInfer<<<blocks, 256, 0, stream_>>>(net(), dIn, n, dRaw);
   hipOk(hipGetLastError(), "kInfer");
 }
@@ -410,8 +426,7 @@ void Guide::adamStep(float lr) {
   step_ += 1;
   const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
   float c1 = 1.0f - (float)std::pow(0.9, step_), c2 = 1.0f - (float)std::pow(0.999, step_);
-  kAdam<<<64, 256, 0, stream_>>>(dParam_, dM_, dV_, dGrad_, lr, b1, b2, eps, c1, c2);
-  kQuantize<<<64, 256, 0, stream_>>>(dParam_, dWT_, dW_);
+  kAdam<<<64, 256, 0, stream_>>>(dParam_, dM_, dV_, dGrad_, lr, b1, b2, eps, c1, c2, dWT_, dW_);
   hipOk(hipGetLastError(), "kAdam");
 }
 
@@ -435,7 +450,8 @@ float Guide::trainOnDevice(float4* dSamples, uint32_t n, float lr, float weightC
   return loss;
 }
 
-float Guide::trainEpoch(float4* dSamples, uint32_t n, float lr, float weightCap, uint32_t minibatch) {
+float Guide::trainEpoch(float4* dSamples, uint32_t n, float lr, float weightCap, uint32_t minibatch,
+                        uint32_t minSteps) {
   if (n == 0 || minibatch == 0 || !(normaliseWeights(dSamples, n, weightCap) > 0.0f)) return 0.0f;
   if (shuffledCap_ < n) {
     if (dShuffled_) (void)hipFree(dShuffled_);
@@ -443,8 +459,6 @@ float Guide::trainEpoch(float4* dSamples, uint32_t n, float lr, float weightCap,
     hipOk(hipMalloc(&dShuffled_, sizeof(float4) * 4 * (size_t)n), "hipMalloc");
     shuffledCap_ = n;
   }
-  // multiplier near n * (golden ratio - 1), made coprime to n
-  uint32_t a = (uint32_t)((double)n * 0.6180339887) | 1u;
   auto gcd = [](uint32_t x, uint32_t y) {
     while (y) {
       uint32_t t = x % y;
@@ -453,22 +467,29 @@ float Guide::trainEpoch(float4* dSamples, uint32_t n, float lr, float weightCap,
     }
     return x;
   };
-  while (gcd(a, n) != 1u) a += 2u;
-  const uint32_t b = (uint32_t)(((uint64_t)step_ * 7919u) % n);
-  kPermute<<<512, 256, 0, stream_>>>(dSamples, dShuffled_, n, a % n == 0u ? 1u : a, b);
-  hipOk(hipGetLastError(), "kPermute");
-  for (uint32_t off = 0; off < n; off += minibatch) {
-    const uint32_t m = std::min(minibatch, n - off);
-    zeroGrad();
-    if (off + m >= n) {  // the last step reports its loss
-      float loss = gradientDevice(dShuffled_ + (size_t)off * 4, m);
-      adamStep(lr);
-      return loss;
+  const uint32_t perPass = (n + minibatch - 1) / minibatch;
+  const uint32_t steps = std::max(perPass, minSteps);
+  float loss = 0.0f;
+  for (uint32_t st = 0; st < steps; ++st) {
+    const uint32_t k = st % perPass;
+    if (k == 0) {  // a new pass: permutation i -> (a i + b) mod n, a coprime to n
+      const uint32_t pass = st / perPass;
+      uint32_t a = (uint32_t)((double)n * (0.6180339887 + 0.0731 * (pass % 5))) | 1u;
+      while (gcd(a, n) != 1u) a += 2u;
+      const uint32_t b = (uint32_t)(((uint64_t)step_ * 7919u) % n);
+      kPermute<<<512, 256, 0, stream_>>>(dSamples, dShuffled_, n, a % n == 0u ? 1u : a % n, b);
+      hipOk(hipGetLastError(), "kPermute");
     }
-    gradientAsync(dShuffled_ + (size_t)off * 4, m);
+    const uint32_t off = k * minibatch, m = std::min(minibatch, n - off);
+    if (st == 0) zeroGrad();  // each Adam step leaves the gradient cleared
+    if (st + 1 == steps) {  // the last step reports its loss
+      loss = gradientDevice(dShuffled_ + (size_t)off * 4, m);
+    } else {
+      gradientAsync(dShuffled_ + (size_t)off * 4, m);
+    }
     adamStep(lr);
   }
-  return 0.0f;
+  return loss;
 }
 
 void Guide::getGrad(float* host) {

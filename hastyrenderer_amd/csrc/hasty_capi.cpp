@@ -169,7 +169,8 @@ struct HastyScene {
   float4* guideSamples = nullptr;
   uint32_t* guideCount = nullptr;
   uint32_t guideSlots = 0;
-  static constexpr uint32_t kGuideSampleCap = 1u << 19;
+  static constexpr uint32_t kGuideSampleCap = 1u << 21;
+  static constexpr uint64_t kGuideTrainPaths = 1u << 20;  // paths per online training step
   float lastGuideLoss = 0.0f;
   uint32_t lastGuideSamples = 0;
 
@@ -223,11 +224,11 @@ struct HastyScene {
   }
 
   // Path slots in flight (regenerated as paths end) and the results ring.
-  // Slots for a render of `paths` paths: the wanted pool (default 2^22), but
+  // Slots for a render of `paths` paths: the wanted pool (default 2^23), but
   // no more than the render needs (at least 2^16).
   uint32_t slotCount(uint64_t paths = 0) const {
     if (paths == 0) paths = (uint64_t)dev.width * dev.height;
-    uint64_t want = slotsWanted ? slotsWanted : (1ull << 22);
+    uint64_t want = slotsWanted ? slotsWanted : (1ull << 23);
     return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(want, std::max<uint64_t>(paths, 1ull << 16)));
   }
   uint32_t ringPathsWanted() const {
@@ -263,7 +264,6 @@ struct HastyScene {
     pb.qB = allocPath<uint32_t>(n);
     pb.qS = allocPath<uint32_t>(n);
     pb.qR = allocPath<uint32_t>(n);
-    pb.qI = allocPath<uint32_t>(n);
     pb.mark = allocPath<uint32_t>(n);
     pb.counters = allocPath<uint32_t>(hasty::kNumCounters);
     pb.stats = allocPath<unsigned long long>(hasty::kNumStats);
@@ -417,9 +417,11 @@ struct HastySppm {
   }
 };
 
-// Guided rendering, one sample index at a time: render it with the current
-// guide, then (when training) one Adam step on the radiance samples its paths
-// produced.
+// Guided rendering.  Training: whole sample indices in groups of about 2^20
+// paths, each group rendered with the current guide and followed by Adam
+// steps over the radiance samples its paths produced: one pass of
+// minibatches, and at least one step per sample index of the group.  Without
+// training: every index in one call.
 static void renderGuided(HastyScene* s, const BatchItem& it) {
   s->ensurePathBuffers((uint64_t)it.n * s->dev.width * s->dev.height);
   s->ensureTileOrder();
@@ -454,11 +456,10 @@ static void renderGuided(HastyScene* s, const BatchItem& it) {
     void* p;
     ~FreeOff() { (void)hipFree(p); }
   } freeOff{dOff};
-  // Without training the guide is fixed: all samples in one call.
   const uint64_t P = (uint64_t)s->dev.width * s->dev.height;
-  const uint32_t perCall =
-      training ? 1u : (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(it.n, 0x7FFFFFFFull / P));
-  if (!training && it.n > 1) {
+  const uint64_t groupPaths = training ? HastyScene::kGuideTrainPaths : 0x7FFFFFFFull;
+  const uint32_t perCall = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(it.n, groupPaths / P));
+  if (perCall > 1) {
     (void)hipFree(dOff);
     freeOff.p = nullptr;
     HIPCHECK(hipMalloc(&dOff, sizeof(float2) * perCall));
@@ -488,7 +489,7 @@ static void renderGuided(HastyScene* s, const BatchItem& it) {
       s->lastGuideSamples = n;
       if (n > 0)
         s->lastGuideLoss =
-            s->guide->g->trainEpoch(s->guideSamples, n, s->guideLr, s->guideWeightCap, s->guideMinibatch);
+            s->guide->g->trainEpoch(s->guideSamples, n, s->guideLr, s->guideWeightCap, s->guideMinibatch, cnt);
       gr.net = s->guide->g->net();
     }
   }
@@ -585,6 +586,9 @@ static void buildScene(HastyScene& S, const char* path) {
   d.focalDistance = sc.camera.focalDistance;
   d.bladeRotation = sc.camera.bladeRotation;
   d.numBlades = sc.camera.numBlades;
+  d.tanHalfFov = hdev::tanHalfFov(d.fovDegree);
+  d.ngonNx = d.numBlades >= 3 ? hm::cosf(hdev::ngonSector(d.numBlades) * 0.5f) : 0.0f;
+  d.ngonNy = d.numBlades >= 3 ? hm::sinf(hdev::ngonSector(d.numBlades) * 0.5f) : 0.0f;
   S.applySettings();
 }
 
@@ -806,7 +810,7 @@ int hasty_get_launch(const HastyScene* s, HastyLaunch* out) {
     g_lastError = "null argument";
     return 1;
   }
-  out->path_slots = s->slotsWanted ? s->slotsWanted : (1u << 22);
+  out->path_slots = s->slotsWanted ? s->slotsWanted : (1u << 23);
   out->trace_blocks = s->launch.traceBlocks;
   out->shade_blocks = s->launch.shadeBlocks;
   out->chunk = s->launch.chunk;
@@ -1236,7 +1240,7 @@ int hasty_scene_set_guide_training(HastyScene* s, uint32_t minibatch, float weig
     return 1;
   }
   if (minibatch < 16 || minibatch > HastyScene::kGuideSampleCap || !(weightCap > 0.0f)) {
-    g_lastError = "minibatch must be in [16, 2^19] and the weight cap positive";
+    g_lastError = "minibatch must be in [16, 2^21] and the weight cap positive";
     return 1;
   }
   s->guideMinibatch = minibatch;

@@ -168,6 +168,10 @@ struct DevScene {
   float camPos[3], camFwd[3], camRight[3], camUp[3];
   float fovDegree, aperture, focalDistance, bladeRotation;
   int numBlades;
+  // per-scene constants of the camera, computed on the host with the same
+  // hmath functions: tan(fov / 2) and the n-gon's edge normal (cos, sin of
+  // half a sector)
+  float tanHalfFov, ngonNx, ngonNy;
   int maxDepth, rrStart;
   float rrQ;
   int useMIS;
@@ -774,22 +778,36 @@ __device__ inline V3 mediaTransmittance(const DevScene& s, const Media& md) {
   if (md.n == 0) return v3(1.0f, 1.0f, 1.0f);
   return albedoOf(s.mats[md.e[0] & 0xFFFFu]);
 }
+// Index of the first entry of shape geomID, or -1.
+__device__ inline int mediaFind(const Media& md, uint32_t geomID) {
+  int idx = -1;
+#pragma unroll
+  for (int i = 0; i < kMaxMedia; ++i)
+    if (idx < 0 && i < md.n && (md.e[i] >> 16) == geomID) idx = i;
+  return idx;
+}
 // updateMedia; `wavelength` is the path's wavelength at the time of the call.
 __device__ inline void mediaUpdate(Media& md, const Interaction& it, V3 wFrom, V3 wTo) {
   float ndFrom = dot(it.ng, wFrom);
   float ndTo = dot(it.ng, wTo);
+  // (entries are addressed with constant indices only, so the stack stays
+  // in registers)
   if (ndFrom >= 0.0f && ndTo < 0.0f) {
-    if (md.n < kMaxMedia) md.e[md.n++] = (it.geomID << 16) | (uint32_t)it.mat;
-    else md.overflow = true;
+    if (md.n < kMaxMedia) {
+      const uint32_t v = (it.geomID << 16) | (uint32_t)it.mat;
+#pragma unroll
+      for (int i = 0; i < kMaxMedia; ++i)
+        if (i == md.n) md.e[i] = v;
+      md.n += 1;
+    } else {
+      md.overflow = true;
+    }
   } else if (ndFrom < 0.0f && ndTo >= 0.0f) {
-    int idx = -1;
-    for (int i = 0; i < md.n; ++i)
-      if ((md.e[i] >> 16) == it.geomID) {
-        idx = i;
-        break;
-      }
+    const int idx = mediaFind(md, it.geomID);
     if (idx != -1) {
-      for (int j = idx; j < md.n - 1; ++j) md.e[j] = md.e[j + 1];
+#pragma unroll
+      for (int j = 0; j < kMaxMedia - 1; ++j)
+        if (j >= idx && j < md.n - 1) md.e[j] = md.e[j + 1];
       md.n -= 1;
     }
   }
@@ -797,12 +815,7 @@ __device__ inline void mediaUpdate(Media& md, const Interaction& it, V3 wFrom, V
 __device__ inline float mediaOutsideIOR(const DevScene& s, const Media& md, const Interaction& it, V3 wFrom,
                                         int wavelength) {
   if (dot(wFrom, it.ng) < 0.0f) {
-    int idx = -1;
-    for (int i = 0; i < md.n; ++i)
-      if ((md.e[i] >> 16) == it.geomID) {
-        idx = i;
-        break;
-      }
+    const int idx = mediaFind(md, it.geomID);
     if (idx == -1) return 1.0f;
     if (md.n == 1) return 1.0f;
     uint32_t other = idx == 0 ? md.e[1] : md.e[0];
@@ -947,9 +960,14 @@ __device__ inline uint32_t lightCdfSearch(const DevScene& s, double x) {
 
 // ----------------------------------------------------------- camera ----
 // Sample.cpp:5-42
-__device__ inline void sampleNGon(hrng::Stream& rng, int numBlades, float bladeRotation, float* ox, float* oy) {
-  float anglePerSector = (float)(2.0 * kPi / numBlades);
-  float nx = hm::cosf(anglePerSector * 0.5f), ny = hm::sinf(anglePerSector * 0.5f);
+__host__ __device__ inline float ngonSector(int numBlades) { return (float)(2.0 * kPi / numBlades); }
+__host__ __device__ inline float tanHalfFov(float fovDegree) {
+  float fovRad = (float)kPi * fovDegree / 180.0f;
+  return hm::tanf(fovRad * 0.5f);
+}
+__device__ inline void sampleNGon(hrng::Stream& rng, int numBlades, float bladeRotation, float nx, float ny, float* ox,
+                                  float* oy) {
+  float anglePerSector = ngonSector(numBlades);
   float d = -nx;
   float sx, sy;
   bool outside;
@@ -973,14 +991,13 @@ __device__ inline void sampleNGon(hrng::Stream& rng, int numBlades, float bladeR
 __device__ inline void cameraRay(const DevScene& s, hrng::Stream& rng, float px, float py, V3* o, V3* dOut) {
   V3 pos = load3(s.camPos), fwd = load3(s.camFwd), right = load3(s.camRight), up = load3(s.camUp);
   float W = (float)s.width, H = (float)s.height;
-  float fovRad = (float)kPi * s.fovDegree / 180.0f;
-  float q = hm::tanf(fovRad * 0.5f);
+  float q = s.tanHalfFov;
   float lx = (px - 0.5f * W) / W * 2.0f;
   float ly = (py - 0.5f * H) / W * 2.0f;
   V3 dir = normalize(fwd + right * (q * lx) - up * (q * ly));
   if (s.aperture > 0.0f) {
     float ox, oy;
-    sampleNGon(rng, s.numBlades, s.bladeRotation, &ox, &oy);
+    sampleNGon(rng, s.numBlades, s.bladeRotation, s.ngonNx, s.ngonNy, &ox, &oy);
     float t = s.focalDistance * dot(dir, fwd);
     V3 focal = pos + dir * t;
     V3 org = pos + up * (s.aperture * ox) + right * (s.aperture * oy);
@@ -995,8 +1012,7 @@ __device__ inline void cameraRay(const DevScene& s, hrng::Stream& rng, float px,
 // Camera::computeRayAngle (Camera.cpp:83-98): angle subtended by one pixel.
 __device__ inline float cameraRayAngle(const DevScene& s, float px, float py) {
   float W = (float)s.width, H = (float)s.height;
-  float fovRad = (float)kPi * s.fovDegree / 180.0f;
-  float q = hm::tanf(fovRad * 0.5f);
+  float q = s.tanHalfFov;
   float lx1 = (px + 0.5f - 0.5f * W) / W * 2.0f;
   float ly1 = (py + 0.5f - 0.5f * H) / W * 2.0f;
   float lx2 = (px - 0.5f - 0.5f * W) / W * 2.0f;

@@ -36,7 +36,6 @@ struct PathBuffers {
   uint32_t* qB;   // extension queue of the next bounce
   uint32_t* qS;   // shadow queue
   uint32_t* qR;   // paths that ended at this bounce
-  uint32_t* qI;   // idle slots (stack)
   uint32_t* mark; // bounce (epoch) whose extension queue holds the slot
   uint32_t* counters;         // see CounterId
   unsigned long long* stats;  // see StatId
@@ -56,7 +55,7 @@ enum CounterId {
   kCntR = 3,      // paths that ended at this bounce
   kCntNext = 4,   // next path id to start
   kCntLimit = 5,  // path ids below this may start (results ring space)
-  kCntIdle = 6,   // idle slots in qI
+  kCntFill = 6,   // idle slots seen by kFill this bounce (its cursor)
   kCntEpoch = 7,  // bounces advanced since the render started
   kNumCounters = 8
 };
@@ -116,7 +115,7 @@ enum StatId { kStatExtRays = 0, kStatShadowRays, kStatNodes, kStatTris, kStatMed
 
 struct LaunchConfig {
   uint32_t traceBlocks = 2048;  // persistent-style grid of the traversal kernels
-  uint32_t shadeBlocks = 4096;
+  uint32_t shadeBlocks = 8192;
   uint32_t chunk = 8;           // bounces between host checks of the live count
   bool useGraph = false;        // replay each chunk of bounces as one captured hipGraph
   bool countStats = false;

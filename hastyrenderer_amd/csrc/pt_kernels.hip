@@ -223,13 +223,15 @@ __device__ inline void startPath(const DevScene& s, const PathBuffers& pb, const
 // Every slot idle, no path started.
 // Occupancy targets (waves per SIMD) of the three ray-bound stages: the
 // register budget each may use is 512 / waves VGPRs.  0 leaves it to the
-// compiler.  Extend at 8 waves (64 VGPRs, no spills) is ~7% faster than at
-// the compiler's 7; shade and connect lose when squeezed.
+// compiler.  Extend at 8 waves (64 VGPRs) is ~7% faster than at the
+// compiler's 7; shade at 3 keeps the guided variant (which would take 190
+// VGPRs) at the plain one's occupancy without spills; squeezing connect or
+// shade further loses.
 #ifndef HASTY_EXTEND_WAVES
 #define HASTY_EXTEND_WAVES 8
 #endif
 #ifndef HASTY_SHADE_WAVES
-#define HASTY_SHADE_WAVES 0
+#define HASTY_SHADE_WAVES 3
 #endif
 #ifndef HASTY_CONNECT_WAVES
 #define HASTY_CONNECT_WAVES 0
@@ -279,53 +281,54 @@ __device__ inline ExtSet extensionSet(const PathBuffers& pb) {
 
 __global__ void __launch_bounds__(kBlock) kInitRender(PathBuffers pb, uint32_t limit) {
   uint32_t stride = gridDim.x * blockDim.x;
-  for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < pb.capacity; k += stride) {
-    pb.qI[k] = k;
-    pb.mark[k] = 0xFFFFFFFFu;
-  }
+  for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < pb.capacity; k += stride) pb.mark[k] = 0xFFFFFFFFu;
   if (blockIdx.x == 0 && threadIdx.x < kNumCounters) {
     uint32_t v = 0u;
     if (threadIdx.x == kCntLimit) v = limit;
-    if (threadIdx.x == kCntIdle) v = pb.capacity;
     pb.counters[threadIdx.x] = v;
   }
 }
 
-// Idle slots take the next path ids (up to the limit) and join qA; the
-// counters are advanced by kFillCommit.
-__device__ inline uint32_t fillCount(const uint32_t* c) {
-  uint32_t idle = c[kCntIdle], next = c[kCntNext], lim = c[kCntLimit];
-  uint32_t avail = lim > next ? lim - next : 0u;
-  return idle < avail ? idle : avail;
+// A slot is idle when its mark is not this bounce's (its path ended, or it
+// never had one).  kFill visits the slots in order: idle slots draw offsets
+// from the fill cursor (one atomic per block), those below the ring-space
+// limit start the paths next + offset and join qA; kFillCommit advances
+// next by the number started and resets the cursor.  Paths are started in
+// slot order, so their records are written in whole cache lines where idle
+// slots are adjacent.
+__device__ inline uint32_t fillAvail(const uint32_t* c) {
+  const uint32_t next = c[kCntNext], lim = c[kCntLimit];
+  return lim > next ? lim - next : 0u;
 }
 template <bool MULTI>
 __global__ void __launch_bounds__(kBlock) kFill(JobSet js, PathBuffers pb) {
   __shared__ uint32_t ldsWave[kBlock / 64];
   __shared__ uint32_t ldsBase;
-  uint32_t m = fillCount(pb.counters);
-  uint32_t idle = pb.counters[kCntIdle], next = pb.counters[kCntNext];
-  uint32_t stride = gridDim.x * blockDim.x;
-  for (uint32_t base = blockIdx.x * blockDim.x; base < m; base += stride) {
-    uint32_t q = base + threadIdx.x;
-    bool live = q < m;
-    uint32_t slot = 0;
+  const uint32_t avail = fillAvail(pb.counters);
+  if (avail == 0u) return;  // the same for every thread
+  const uint32_t next = pb.counters[kCntNext], epoch = pb.counters[kCntEpoch];
+  const uint32_t stride = gridDim.x * blockDim.x;
+  for (uint32_t base = blockIdx.x * blockDim.x; base < pb.capacity; base += stride) {
+    const uint32_t slot = base + threadIdx.x;
+    const bool idle = slot < pb.capacity && pb.mark[slot] != epoch;
+    const uint32_t off = blockAppend(&pb.counters[kCntFill], idle, ldsWave, &ldsBase);
+    const bool live = idle && off < avail;
     if (live) {
-      slot = pb.qI[idle - 1 - q];
-      uint32_t pid = next + q;
+      const uint32_t pid = next + off;
       const uint32_t j = findJob<MULTI>(js, pid);
       const uint32_t flags = MULTI ? jobBits(js, j) : 0u;
-      pb.mark[slot] = pb.counters[kCntEpoch];
+      pb.mark[slot] = epoch;
       forJob<MULTI>(js, j, [&](const DevScene& sc, const PassInfo& pi) { startPath(sc, pb, pi, slot, pid, flags); });
     }
-    uint32_t idx = blockAppend(&pb.counters[kCntA], live, ldsWave, &ldsBase);
+    const uint32_t idx = blockAppend(&pb.counters[kCntA], live, ldsWave, &ldsBase);
     if (live) pb.qA[idx] = slot;
   }
 }
 __global__ void kFillCommit(uint32_t* c) {
   if (threadIdx.x == 0) {
-    uint32_t m = fillCount(c);
-    c[kCntIdle] -= m;
-    c[kCntNext] += m;
+    const uint32_t avail = fillAvail(c), idle = c[kCntFill];
+    c[kCntNext] += idle < avail ? idle : avail;
+    c[kCntFill] = 0u;
   }
 }
 __global__ void kSetLimit(uint32_t* c, uint32_t limit) {
@@ -466,17 +469,8 @@ __global__ void __launch_bounds__(kBlock) HASTY_SHADE_ATTR kShade(JobSet js, Pat
                 // guided vertex: rough reflection (no delta lobe, and not so
                 // glossy that broad vMF lobes could only waste samples)
                 const bool guided = GUIDED && m.alpha >= kGuideMinAlpha && m.transmission == 0.0f;
-                guide::Mixture gm;
-                if (GUIDED && guided) {
-                  float raw[guide::kOut];
-                  const float4* gr = js.g.raw + (size_t)slot * 10;
-#pragma unroll
-                  for (int i = 0; i < 10; ++i) {
-                    float4 v = gr[i];
-                    raw[4 * i] = v.x, raw[4 * i + 1] = v.y, raw[4 * i + 2] = v.z, raw[4 * i + 3] = v.w;
-                  }
-                  gm = guide::mixtureFromRaw(raw);
-                }
+                // the vertex's mixture table, written by kGuideInfer
+                const float4* gm = GUIDED ? js.g.raw + (size_t)slot * 10 : nullptr;
                 const float gf = GUIDED ? js.g.fraction : 0.0f;
                 if (twoStrategies) {
                   // ---- light strategy (MIS.cpp:32-54, Scene.cpp:538-586)
@@ -506,7 +500,7 @@ __global__ void __launch_bounds__(kBlock) HASTY_SHADE_ATTR kShade(JobSet js, Pat
                   BsdfEval ev = bsdfEvaluate(m, it, wo, sd);
                   float outsideIOR = mediaOutsideIOR(s, md, it, wo, wavelength);
                   float pB = bsdfPdf(m, it, wo, sd, outsideIOR);
-                  if (GUIDED && guided) pB = gf * guide::mixturePdf(gm, sd) + (1.0f - gf) * pB;
+                  if (GUIDED && guided) pB = gf * guide::mixturePdfTable(gm, sd) + (1.0f - gf) * pB;
                   Media mc = md;
                   mediaUpdate(mc, it, wo, sd);
                   V3 Tc = mediaTransmittance(s, mc);
@@ -535,11 +529,11 @@ __global__ void __launch_bounds__(kBlock) HASTY_SHADE_ATTR kShade(JobSet js, Pat
                   // one-sample MIS of guide and BSDF: p = f pG + (1 - f) pB
                   if (rng.uniform01f() < gf) {
                     float x0 = rng.uniform01f(), x1 = rng.uniform01f(), x2 = rng.uniform01f();
-                    bs.dir = guide::mixtureSample(gm, x0, x1, x2);
+                    bs.dir = guide::mixtureSampleTable(gm, x0, x1, x2);
                     bs.outside = dot(it.ng, wo) >= 0.0f;
                     BsdfEval ev = bsdfEvaluate(m, it, wo, bs.dir);
                     float pB = bsdfPdf(m, it, wo, bs.dir, outsideIOR);
-                    bs.pdf = gf * guide::mixturePdf(gm, bs.dir) + (1.0f - gf) * pB;
+                    bs.pdf = gf * guide::mixturePdfTable(gm, bs.dir) + (1.0f - gf) * pB;
                     float mm = bs.pdf > 0.0f ? __builtin_fabsf(dot(bs.dir, it.ng)) / bs.pdf : 0.0f;
                     bs.tD = ev.fD * mm;
                     bs.tC = ev.fC * mm;
@@ -547,7 +541,7 @@ __global__ void __launch_bounds__(kBlock) HASTY_SHADE_ATTR kShade(JobSet js, Pat
                   } else {
                     bs = bsdfSample(m, rng, it, wo, wavelength, outsideIOR, false, 3);
                     if (bs.pdf != 0.0f) {
-                      float pMix = gf * guide::mixturePdf(gm, bs.dir) + (1.0f - gf) * bs.pdf;
+                      float pMix = gf * guide::mixturePdfTable(gm, bs.dir) + (1.0f - gf) * bs.pdf;
                       float sc = bs.pdf / pMix;
                       bs.tD = bs.tD * sc;
                       bs.tC = bs.tC * sc;
@@ -702,52 +696,43 @@ __global__ void __launch_bounds__(kBlock) HASTY_CONNECT_ATTR kConnect(JobSet js,
 }
 
 // --------------------------------------------------------------- retire ----
-// Finished paths: results into the ring; the slot becomes idle and the next
-// kFill gives it a new path id.
+// Finished paths: results into the ring; the slot is idle from the next
+// bounce on (its mark is not renewed) and kFill gives it a new path id.
 template <bool GUIDED>
 __global__ void __launch_bounds__(kBlock) kRetire(JobSet js, PathBuffers pb) {
-  __shared__ uint32_t ldsWave[kBlock / 64];
-  __shared__ uint32_t ldsBase;
   uint32_t n = pb.counters[kCntR];
   uint32_t stride = gridDim.x * blockDim.x;
-  for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += stride) {
-    uint32_t q = base + threadIdx.x;
-    bool live = q < n;
-    uint32_t slot = 0;
-    if (live) {
-      slot = pb.qR[q];
-      uint4 st = pb.state[slot];
-      uint32_t ri = st.w % pb.ringPaths;
-      float4 Lf = pb.L[slot];
-      pb.ringL[ri] = Lf;
-      if (GUIDED) {
-        // training samples: weight = luminance of the radiance the
-        // continuation brought, over the throughput before the sample
-        uint32_t nrec = (st.y >> kRecShift) & 3u;
-        for (uint32_t r = 0; r < nrec; ++r) {
-          const float4* rr = js.g.rec + (size_t)slot * 12 + 6 * r;
-          float4 b = rr[4], ls = rr[5];
-          float cx = b.x > 0.0f ? (Lf.x - ls.x) / b.x : 0.0f;
-          float cy = b.y > 0.0f ? (Lf.y - ls.y) / b.y : 0.0f;
-          float cz = b.z > 0.0f ? (Lf.z - ls.z) / b.z : 0.0f;
-          float F = 0.2126f * cx + 0.7152f * cy + 0.0722f * cz;
-          if (F > 0.0f && F < 3.0e38f) {
-            uint32_t k = atomicAdd(js.g.sampleCount, 1u);
-            if (k < js.g.sampleCap) {
-              float4 x = rr[0];
-              js.g.samples[4 * (size_t)k] = make_float4(x.x, x.y, x.z, F);
-              js.g.samples[4 * (size_t)k + 1] = rr[1];
-              js.g.samples[4 * (size_t)k + 2] = rr[2];
-              js.g.samples[4 * (size_t)k + 3] = rr[3];
-            }
+  for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += stride) {
+    const uint32_t slot = pb.qR[q];
+    uint4 st = pb.state[slot];
+    uint32_t ri = st.w % pb.ringPaths;
+    float4 Lf = pb.L[slot];
+    pb.ringL[ri] = Lf;
+    if (GUIDED) {
+      // training samples: weight = luminance of the radiance the
+      // continuation brought, over the throughput before the sample
+      uint32_t nrec = (st.y >> kRecShift) & 3u;
+      for (uint32_t r = 0; r < nrec; ++r) {
+        const float4* rr = js.g.rec + (size_t)slot * 12 + 6 * r;
+        float4 b = rr[4], ls = rr[5];
+        float cx = b.x > 0.0f ? (Lf.x - ls.x) / b.x : 0.0f;
+        float cy = b.y > 0.0f ? (Lf.y - ls.y) / b.y : 0.0f;
+        float cz = b.z > 0.0f ? (Lf.z - ls.z) / b.z : 0.0f;
+        float F = 0.2126f * cx + 0.7152f * cy + 0.0722f * cz;
+        if (F > 0.0f && F < 3.0e38f) {
+          uint32_t k = atomicAdd(js.g.sampleCount, 1u);
+          if (k < js.g.sampleCap) {
+            float4 x = rr[0];
+            js.g.samples[4 * (size_t)k] = make_float4(x.x, x.y, x.z, F);
+            js.g.samples[4 * (size_t)k + 1] = rr[1];
+            js.g.samples[4 * (size_t)k + 2] = rr[2];
+            js.g.samples[4 * (size_t)k + 3] = rr[3];
           }
         }
       }
-      pb.ringN[ri] = pb.auxN[slot];
-      pb.ringA[ri] = pb.auxA[slot];
     }
-    uint32_t idx = blockAppend(&pb.counters[kCntIdle], live, ldsWave, &ldsBase);
-    if (live) pb.qI[idx] = slot;
+    pb.ringN[ri] = pb.auxN[slot];
+    pb.ringA[ri] = pb.auxA[slot];
   }
 }
 
@@ -781,7 +766,13 @@ __global__ void __launch_bounds__(kBlock) kAccumulate(PathBuffers pb, PassInfo p
 // ------------------------------------------------------------ guide infer ----
 // Raw guide outputs for every path in qA that hit a surface: inputs are the
 // hit point, the geometric normal facing wo, and wo; 16 paths per wave.
-__global__ void __launch_bounds__(kBlock) kGuideInfer(JobSet js, PathBuffers pb) {
+// One 1024-thread block per CU (16 waves, 4 per SIMD, sharing one LDS copy
+// of the network): each block stages the network once, then its waves loop
+// over 16-path tiles.
+constexpr uint32_t kGuideInferBlocks = 256, kGuideInferThreads = 1024;
+__global__ void __launch_bounds__(kGuideInferThreads) kGuideInfer(JobSet js, PathBuffers pb) {
+  __shared__ guide::LdsNet ldsNet;
+  guide::stageNet(js.g.net, &ldsNet, true);
   const DevScene& s = js.s;
   const uint32_t n = pb.counters[kCntA];
   const int lane = threadIdx.x & 63, r = lane & 15, q = lane >> 4;
@@ -797,7 +788,7 @@ __global__ void __launch_bounds__(kBlock) kGuideInfer(JobSet js, PathBuffers pb)
     // have none skips the network (the test is uniform over the wave)
     bool want = valid && tri >= 0;
     if (want) {
-      const DevMaterial& m = s.mats[s.triMat[tri]];
+      const DevMaterial m = ldgStruct(s.mats + ldg(s.triMat + tri));
       want = !m.isLight && m.alpha >= kGuideMinAlpha && m.transmission == 0.0f;
     }
     if (__ballot(want) == 0ull) continue;
@@ -805,13 +796,26 @@ __global__ void __launch_bounds__(kBlock) kGuideInfer(JobSet js, PathBuffers pb)
     if (tri >= 0) hitPointNormal(s, tri, h4.y, h4.z, &x, &ng);
     V3 wo = -f4xyz(pb.rayD[slot]);
     if (dot(ng, wo) < 0.0f) ng = -ng;
-    float raw[12];
-    guide::inferOne(js.g.net, x, ng, wo, raw);
+    guide::bf16x4 h0[5];
+    guide::inputOperands(guide::normPos(js.g.net, x), ng, wo, h0);
+    guide::f32x4 o[3];
+    guide::forwardLds(&ldsNet, h0, o);
+    // lane group q holds lobes 2q and 2q + 1 (raw outputs 10q .. 10q + 9);
+    // the softmax over the 8 logits runs across the query's four lanes
+    const float r10[10] = {o[0].x, o[0].y, o[0].z, o[0].w, o[1].x, o[1].y, o[1].z, o[1].w, o[2].x, o[2].y};
+    float amax = fmaxf(r10[4], r10[9]);
+    amax = fmaxf(amax, __shfl_xor(amax, 16));
+    amax = fmaxf(amax, __shfl_xor(amax, 32));
+    const float e0 = __expf(r10[4] - amax), e1 = __expf(r10[9] - amax);
+    float sum = e0 + e1;
+    sum += __shfl_xor(sum, 16);
+    sum += __shfl_xor(sum, 32);
+    const float w0 = e0 / sum, w1 = e1 / sum;
     if (want) {
       float4* out = js.g.raw + (size_t)slot * 10;
-#pragma unroll
-      for (int mt = 0; mt < 3; ++mt)
-        if (4 * mt + q < 10) out[4 * mt + q] = make_float4(raw[4 * mt], raw[4 * mt + 1], raw[4 * mt + 2], raw[4 * mt + 3]);
+      out[2 * q] = guide::lobeEntry(r10, w0);
+      out[2 * q + 1] = guide::lobeEntry(r10 + 5, w1);
+      reinterpret_cast<float2*>(out + guide::kLobes)[q] = make_float2(w0, w1);
     }
   }
 }
@@ -890,7 +894,7 @@ void launchChunk(const JobSet& js, const PathBuffers& pbIn, const LaunchConfig& 
     t1(kKExtend);
     if (GUIDED) {
       t0(kKGuideInfer);
-      kGuideInfer<<<2048, kBlock, 0, stream>>>(js, pb);
+      kGuideInfer<<<kGuideInferBlocks, kGuideInferThreads, 0, stream>>>(js, pb);
       t1(kKGuideInfer);
     }
     t0(kKShade);

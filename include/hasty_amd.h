@@ -109,7 +109,7 @@ int hasty_reset_stats(HastyScene* scene);
 int hasty_counters(HastyScene* scene, uint64_t* out5);
 
 /* Launch-shape knobs: path slots in flight (paths are regenerated into
- * freed slots; default 2^22, at least 2^16 unless the film is smaller) and
+ * freed slots; default 2^23, at least 2^16 unless the film is smaller) and
  * the grid sizes of the traversal and shading kernels.  0 keeps the current
  * value.  Images do not depend on these. */
 int hasty_set_launch(HastyScene* scene, uint32_t path_slots, uint32_t trace_blocks, uint32_t shade_blocks);
@@ -178,16 +178,17 @@ int hasty_guide_train_step(HastyGuide* guide, const float* samples, uint32_t n, 
  * continuation direction at glossy and diffuse vertices from the guide with
  * probability `fraction` and from the BSDF otherwise, weighting by the
  * mixture pdf (one-sample MIS; NEE's MIS uses the same mixture), so the
- * estimate stays unbiased.  When train_lr > 0 samples are rendered one
- * index at a time, each followed by one pass of Adam steps over the radiance
- * samples its paths produced (up to 2^19; weight = luminance of the radiance
- * the continuation brought over the throughput before the sample, normalised
- * to mean 1 and capped), in minibatches of a pseudo-random order; without
- * training all indices go in one call.  guide NULL returns to plain
+ * estimate stays unbiased.  When train_lr > 0 sample indices are rendered
+ * in groups of about 2^20 paths, each group followed by a pass of Adam
+ * steps (at least one per sample index of the group, more passes if needed)
+ * over the radiance samples its paths produced (up to 2^21; weight =
+ * luminance of the radiance the continuation brought over the throughput
+ * before the sample, normalised to mean 1 and capped), in minibatches of a
+ * pseudo-random order; without training all indices go in one call.  guide NULL returns to plain
  * rendering.  The guide must belong to the scene; the scene must outlive its
  * use. */
 int hasty_scene_set_guide(HastyScene* scene, HastyGuide* guide, float fraction, float train_lr);
-/* Online-training minibatch (default 16384, in [16, 2^19]) and weight cap
+/* Online-training minibatch (default 16384, in [16, 2^21]) and weight cap
  * (default 20).  No reference counterpart. */
 int hasty_scene_set_guide_training(HastyScene* scene, uint32_t minibatch, float weight_cap);
 /* loss and sample count of the last online training step */

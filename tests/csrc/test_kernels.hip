@@ -82,7 +82,8 @@ __global__ void kSampler(int kind, uint32_t seed, uint32_t count, V3 n, V3 wo, f
       o[0] = d.x, o[1] = d.y, o[2] = d.z, o[3] = pdf, o[4] = vndfGlobalPdf(n, param, param, t, b, wo, d);
     } else if (kind == 3) {
       float x, y;
-      sampleNGon(rng, (int)param, 0.0f, &x, &y);
+      const float half = ngonSector((int)param) * 0.5f;
+      sampleNGon(rng, (int)param, 0.0f, hm::cosf(half), hm::sinf(half), &x, &y);
       float anglePerSector = (float)(2.0 * kPi / (int)param);
       float areaSector = 0.5 * hm::sinf(anglePerSector);
       float areaNGon = (int)param * areaSector;
