@@ -211,12 +211,11 @@ __device__ inline void startPath(const DevScene& s, const PathBuffers& pb, const
   float2 off = pi.offsets[k];
   V3 o, d;
   cameraRay(s, rng, (float)i + off.x, (float)j + off.y, &o, &d);
+  // throughput, radiance and the depth-0 normal / albedo are not stored
+  // here: kShade starts every path from throughput 1 and radiance 0 and
+  // writes the aux values at its first vertex
   pb.rayO[slot] = f4(o, 0.0f);
   pb.rayD[slot] = f4(d, 0.0f);
-  pb.beta[slot] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
-  pb.L[slot] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-  pb.auxN[slot] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-  pb.auxA[slot] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   pb.state[slot] = make_uint4(rng.counter, flags, 0xFFFFFFFFu, pid);
 }
 
@@ -300,28 +299,76 @@ __device__ inline uint32_t fillAvail(const uint32_t* c) {
   const uint32_t next = c[kCntNext], lim = c[kCntLimit];
   return lim > next ? lim - next : 0u;
 }
+// Block-wide sum of v (every thread of the block calls it).
+__device__ inline uint32_t blockSum(uint32_t v, uint32_t* ldsWave) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+  if ((threadIdx.x & 63u) == 0) ldsWave[threadIdx.x >> 6] = v;
+  __syncthreads();
+  uint32_t sum = 0;
+  for (uint32_t w = 0; w < (blockDim.x >> 6); ++w) sum += ldsWave[w];
+  __syncthreads();
+  return sum;
+}
+// Exclusive prefix count of pred in thread order over the block, and the
+// block's count in *total (every thread of the block calls it).
+__device__ inline uint32_t blockPrefix(bool pred, uint32_t* ldsWave, uint32_t* total) {
+  const uint64_t mask = __ballot(pred);
+  const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+  const uint32_t prefix = (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
+  if (lane == 0) ldsWave[wave] = (uint32_t)__popcll(mask);
+  __syncthreads();
+  uint32_t before = 0, sum = 0;
+  for (uint32_t w = 0; w < (blockDim.x >> 6); ++w) {
+    const uint32_t c = ldsWave[w];
+    before += w < wave ? c : 0u;
+    sum += c;
+  }
+  __syncthreads();
+  *total = sum;
+  return before + prefix;
+}
+// Each block owns a contiguous range of slots: it counts the range's idle
+// slots, takes offsets for all of them with one atomic on the fill cursor
+// and a qA range for those it can start with one more, then starts them in
+// slot order.  (Two atomics per block, instead of two per 256 slots: same-
+// address atomics from every block were what bounded this kernel.)
+constexpr uint32_t kFillBlocks = 2048;
 template <bool MULTI>
 __global__ void __launch_bounds__(kBlock) kFill(JobSet js, PathBuffers pb) {
   __shared__ uint32_t ldsWave[kBlock / 64];
-  __shared__ uint32_t ldsBase;
+  __shared__ uint32_t ldsBase[3];
   const uint32_t avail = fillAvail(pb.counters);
   if (avail == 0u) return;  // the same for every thread
   const uint32_t next = pb.counters[kCntNext], epoch = pb.counters[kCntEpoch];
-  const uint32_t stride = gridDim.x * blockDim.x;
-  for (uint32_t base = blockIdx.x * blockDim.x; base < pb.capacity; base += stride) {
+  const uint32_t per = (pb.capacity + gridDim.x * kBlock - 1) / (gridDim.x * kBlock) * kBlock;
+  const uint32_t lo = min(blockIdx.x * per, pb.capacity), hi = min(lo + per, pb.capacity);
+  uint32_t mine = 0;
+  for (uint32_t slot = lo + threadIdx.x; slot < hi; slot += kBlock) mine += pb.mark[slot] != epoch ? 1u : 0u;
+  const uint32_t idleTotal = blockSum(mine, ldsWave);
+  if (threadIdx.x == 0) {
+    const uint32_t b = idleTotal ? atomicAdd(&pb.counters[kCntFill], idleTotal) : 0u;
+    const uint32_t live = b < avail ? min(idleTotal, avail - b) : 0u;
+    ldsBase[0] = b;
+    ldsBase[1] = live;
+    ldsBase[2] = live ? atomicAdd(&pb.counters[kCntA], live) : 0u;
+  }
+  __syncthreads();
+  const uint32_t b = ldsBase[0], live = ldsBase[1], qa = ldsBase[2];
+  uint32_t rankBase = 0;
+  for (uint32_t base = lo; base < hi && rankBase < live; base += kBlock) {  // uniform bounds
     const uint32_t slot = base + threadIdx.x;
-    const bool idle = slot < pb.capacity && pb.mark[slot] != epoch;
-    const uint32_t off = blockAppend(&pb.counters[kCntFill], idle, ldsWave, &ldsBase);
-    const bool live = idle && off < avail;
-    if (live) {
-      const uint32_t pid = next + off;
+    const bool idle = slot < hi && pb.mark[slot] != epoch;
+    uint32_t count;
+    const uint32_t r = rankBase + blockPrefix(idle, ldsWave, &count);
+    if (idle && r < live) {
+      const uint32_t pid = next + b + r;
       const uint32_t j = findJob<MULTI>(js, pid);
       const uint32_t flags = MULTI ? jobBits(js, j) : 0u;
       pb.mark[slot] = epoch;
       forJob<MULTI>(js, j, [&](const DevScene& sc, const PassInfo& pi) { startPath(sc, pb, pi, slot, pid, flags); });
+      pb.qA[qa + r] = slot;
     }
-    const uint32_t idx = blockAppend(&pb.counters[kCntA], live, ldsWave, &ldsBase);
-    if (live) pb.qA[idx] = slot;
+    rankBase += count;
   }
 }
 __global__ void kFillCommit(uint32_t* c) {
@@ -392,9 +439,11 @@ __global__ void __launch_bounds__(kBlock) HASTY_SHADE_ATTR kShade(JobSet js, Pat
       float4 ro4 = pb.rayO[slot], rd4 = pb.rayD[slot], h4 = pb.hit[slot];
       V3 ro = f4xyz(ro4), rd = f4xyz(rd4);
       int32_t tri = __float_as_int(h4.w);
-      float4 b4 = pb.beta[slot];
+      // a path's first vertex starts from throughput 1 and radiance 0 (not
+      // stored by kFill)
+      const float4 b4 = depth == 0 ? make_float4(1.0f, 1.0f, 1.0f, 0.0f) : pb.beta[slot];
       V3 beta = f4xyz(b4);
-      float4 L4 = pb.L[slot];
+      const float4 L4 = depth == 0 ? make_float4(0.0f, 0.0f, 0.0f, 0.0f) : pb.L[slot];
       V3 L = f4xyz(L4);
       uint32_t nrec = GUIDED ? (st.y >> kRecShift) & 3u : 0u;
       if (GUIDED && depth > 0 && nrec > 0 && ((st.y >> (kRecShift + 2)) & 1u)) {
@@ -884,7 +933,7 @@ void launchChunk(const JobSet& js, const PathBuffers& pbIn, const LaunchConfig& 
   const int countStats = cfg.countStats ? 1 : 0;
   for (uint32_t it = 0; it < chunk; ++it) {
     t0(kKGenerate);
-    kFill<MULTI><<<cfg.shadeBlocks, kBlock, 0, stream>>>(js, pb);
+    kFill<MULTI><<<kFillBlocks, kBlock, 0, stream>>>(js, pb);
     t1(kKGenerate);
     t0(kKMisc);
     kFillCommit<<<1, 64, 0, stream>>>(pb.counters);
