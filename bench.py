@@ -311,12 +311,19 @@ def bench_guide(args):
                                  max_depth=32)
     hip = Hip()
 
-    def render(sc, seed, n):
+    def render(sc, seed, n, chunk=0):
+        """n samples into fresh films (in calls of `chunk` samples with a
+        progress line each when chunk > 0); returns the mean image and the time."""
         info = sc.info()
         f = ha.DeviceFilms(info["width"], info["height"])
         hip.sync()
         t = time.perf_counter()
-        sc.render_pt_device(seed, 0, n, *f.ptrs)
+        step = chunk if chunk > 0 else n
+        for first in range(0, n, step):
+            sc.render_pt_device(seed, first, min(step, n - first), *f.ptrs)
+            if chunk > 0:
+                hip.sync()
+                log("  %d / %d samples, %.1f s" % (min(first + step, n), n, time.perf_counter() - t))
         hip.sync()
         dt = time.perf_counter() - t
         c = f.download()[0] / n
@@ -328,7 +335,7 @@ def bench_guide(args):
 
     plain = ha.Scene(job_path)
     info = plain.info()
-    ref, tref = render(plain, 101, args.guide_ref_spp)
+    ref, tref = render(plain, 101, args.guide_ref_spp, chunk=1024)
     log("reference: %d spp in %.1f s" % (args.guide_ref_spp, tref))
     n = args.guide_spp
     pc, tp = render(plain, 5, n)
@@ -398,7 +405,7 @@ def main():
     ap.add_argument("--launch", default="", metavar="SLOTS,TRACE,SHADE,CHUNK,GRAPH",
                     help="explicit launch shape (path slots, trace blocks, shade blocks, bounces per check, "
                          "hipGraph 0/1) instead of autotuning")
-    ap.add_argument("--guide-spp", type=int, default=64)
+    ap.add_argument("--guide-spp", type=int, default=256)
     ap.add_argument("--guide-res", type=int, default=512)
     ap.add_argument("--guide-train-spp", type=int, default=16)
     ap.add_argument("--guide-scene", choices=("ring", "room"), default="ring",
