@@ -542,6 +542,8 @@ static void buildScene(HastyScene& S, const char* path) {
   S.bvh = hasty::buildBvh4(sc.triPos.data(), sc.numTris, opt);
   if (3 * S.bvh.depth + 1 > hasty::kStackLds + hasty::kStackOvf)
     throw std::runtime_error("BVH too deep for the traversal stack (depth " + std::to_string(S.bvh.depth) + ")");
+  if ((uint64_t)S.bvh.nodes.size() * sizeof(float) >= (1ull << 32))  // traversal uses 32-bit node offsets
+    throw std::runtime_error("BVH larger than 4 GiB (" + std::to_string(S.bvh.numNodes) + " nodes)");
 
   HIPCHECK(hipGetDevice(&S.device));
   HIPCHECK(hipStreamCreate(&S.stream));

@@ -180,9 +180,18 @@ struct DevScene {
 
 // ----------------------------------------------------------- traversal ----
 struct RayPre {
-  V3 o, d, inv;
+  V3 o, d;
   int kx, ky, kz;
   float Sx, Sy, Sz;
+  // box tests: t of plane L along an axis = (L - o) * inv.  Closest-hit
+  // traversal takes min / max of each axis' two planes with inv = 1 / d.
+  // Occlusion traversal picks the near and far plane up front (fewer
+  // instructions, one more address per plane) with binv = 1 / d whose
+  // components are clamped to |d| >= 1e-20 so that no t is NaN; near* = byte
+  // offset in a node of the float4 holding each axis' near planes (lo when
+  // binv >= 0, where (lo - o) binv <= (hi - o) binv).
+  V3 inv, binv;
+  uint32_t nearX, nearY, nearZ;
 };
 __device__ inline float c3(V3 a, int i) { return i == 0 ? a.x : (i == 1 ? a.y : a.z); }
 __device__ inline RayPre prepareRay(V3 o, V3 d) {
@@ -190,6 +199,12 @@ __device__ inline RayPre prepareRay(V3 o, V3 d) {
   r.o = o;
   r.d = d;
   r.inv = v3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
+  r.binv = v3(1.0f / copysignf(fmaxf(__builtin_fabsf(d.x), 1e-20f), d.x),
+              1.0f / copysignf(fmaxf(__builtin_fabsf(d.y), 1e-20f), d.y),
+              1.0f / copysignf(fmaxf(__builtin_fabsf(d.z), 1e-20f), d.z));
+  r.nearX = r.binv.x >= 0.0f ? 0u : 16u;
+  r.nearY = r.binv.y >= 0.0f ? 32u : 48u;
+  r.nearZ = r.binv.z >= 0.0f ? 64u : 80u;
   float ax = __builtin_fabsf(d.x), ay = __builtin_fabsf(d.y), az = __builtin_fabsf(d.z);
   r.kz = ax > ay ? (ax > az ? 0 : 2) : (ay > az ? 1 : 2);
   r.kx = r.kz + 1 == 3 ? 0 : r.kz + 1;
@@ -289,13 +304,18 @@ __device__ inline Hit traverse(const DevScene& s, const RayPre& r, float tBound,
   bool done = false;
   while (!done) {
     if ((ref & kLeafFlag) == 0) {
-      const float4* n = s.nodes + (size_t)ref * 8;
-      float4 lox = ldg4(n), hix = ldg4(n + 1), loy = ldg4(n + 2), hiy = ldg4(n + 3), loz = ldg4(n + 4),
-             hiz = ldg4(n + 5);
-      const float4 rf = ldg4(n + 6);
-      int4 refs = make_int4(__float_as_int(rf.x), __float_as_int(rf.y), __float_as_int(rf.z), __float_as_int(rf.w));
+      float e0, e1, e2, e3;
+      int4 refs;
       nNodes++;
-      float tb = (CLOSEST && best.tri < 0) ? __builtin_inff() : best.t * kCullSlack;
+      const float tb = (CLOSEST && best.tri < 0) ? __builtin_inff() : best.t * kCullSlack;
+      // Entry and exit distances of a child box; the exit is scaled by 1 + 6u
+      // (Ize, "Robust BVH ray traversal") so that rounding never culls a box
+      // the exact ray enters.
+      if constexpr (CLOSEST) {
+        const float4* n = s.nodes + (size_t)ref * 8;
+        const float4 lox = ldg4(n), hix = ldg4(n + 1), loy = ldg4(n + 2), hiy = ldg4(n + 3), loz = ldg4(n + 4),
+                     hiz = ldg4(n + 5), rf = ldg4(n + 6);
+        refs = make_int4(__float_as_int(rf.x), __float_as_int(rf.y), __float_as_int(rf.z), __float_as_int(rf.w));
 #define HDEV_SLAB(LX, HX, LY, HY, LZ, HZ, RF, TOUT)                                          \
   {                                                                                          \
     float t0x = (LX - r.o.x) * r.inv.x, t1x = (HX - r.o.x) * r.inv.x;                        \
@@ -306,12 +326,33 @@ __device__ inline Hit traverse(const DevScene& s, const RayPre& r, float tBound,
     bool h = (uint32_t)(RF) != kLeafFlag && tmin <= tmax && tmax > 0.0f && tmin <= tb;      \
     TOUT = h ? tmin : __builtin_inff();                                                      \
   }
-      float e0, e1, e2, e3;
-      HDEV_SLAB(lox.x, hix.x, loy.x, hiy.x, loz.x, hiz.x, refs.x, e0)
-      HDEV_SLAB(lox.y, hix.y, loy.y, hiy.y, loz.y, hiz.y, refs.y, e1)
-      HDEV_SLAB(lox.z, hix.z, loy.z, hiy.z, loz.z, hiz.z, refs.z, e2)
-      HDEV_SLAB(lox.w, hix.w, loy.w, hiy.w, loz.w, hiz.w, refs.w, e3)
+        HDEV_SLAB(lox.x, hix.x, loy.x, hiy.x, loz.x, hiz.x, refs.x, e0)
+        HDEV_SLAB(lox.y, hix.y, loy.y, hiy.y, loz.y, hiz.y, refs.y, e1)
+        HDEV_SLAB(lox.z, hix.z, loy.z, hiy.z, loz.z, hiz.z, refs.z, e2)
+        HDEV_SLAB(lox.w, hix.w, loy.w, hiy.w, loz.w, hiz.w, refs.w, e3)
 #undef HDEV_SLAB
+      } else {
+        // 32-bit byte offsets from the node array (< 4 GiB of nodes)
+        const char* nb = reinterpret_cast<const char*>(s.nodes);
+        const uint32_t no = ref * 128u;
+        auto plane = [&](uint32_t off) { return ldg4(reinterpret_cast<const float4*>(nb + (no + off))); };
+        const float4 nx = plane(r.nearX), fx = plane(r.nearX ^ 16u), ny = plane(r.nearY), fy = plane(r.nearY ^ 16u),
+                     nz = plane(r.nearZ), fz = plane(r.nearZ ^ 16u), rf = plane(96u);
+        refs = make_int4(__float_as_int(rf.x), __float_as_int(rf.y), __float_as_int(rf.z), __float_as_int(rf.w));
+#define HDEV_SLAB(NX, FX, NY, FY, NZ, FZ, RF, TOUT)                                                \
+  {                                                                                                \
+    float tmin = fmaxf(fmaxf((NX - r.o.x) * r.binv.x, (NY - r.o.y) * r.binv.y), (NZ - r.o.z) * r.binv.z);  \
+    float tmax = fminf(fminf((FX - r.o.x) * r.binv.x, (FY - r.o.y) * r.binv.y), (FZ - r.o.z) * r.binv.z) * \
+                 kTmaxScale;                                                                       \
+    bool h = (uint32_t)(RF) != kLeafFlag && tmin <= tmax && tmax > 0.0f && tmin <= tb;            \
+    TOUT = h ? tmin : __builtin_inff();                                                            \
+  }
+        HDEV_SLAB(nx.x, fx.x, ny.x, fy.x, nz.x, fz.x, refs.x, e0)
+        HDEV_SLAB(nx.y, fx.y, ny.y, fy.y, nz.y, fz.y, refs.y, e1)
+        HDEV_SLAB(nx.z, fx.z, ny.z, fy.z, nz.z, fz.z, refs.z, e2)
+        HDEV_SLAB(nx.w, fx.w, ny.w, fy.w, nz.w, fz.w, refs.w, e3)
+#undef HDEV_SLAB
+      }
       uint32_t r0 = (uint32_t)refs.x, r1 = (uint32_t)refs.y, r2 = (uint32_t)refs.z, r3 = (uint32_t)refs.w;
       int cnt = (e0 != __builtin_inff()) + (e1 != __builtin_inff()) + (e2 != __builtin_inff()) +
                 (e3 != __builtin_inff());
