@@ -383,8 +383,10 @@ __global__ void kSetLimit(uint32_t* c, uint32_t limit) {
 }
 
 // --------------------------------------------------------------- extend ----
-template <bool MULTI>
-__global__ void __launch_bounds__(kBlock) HASTY_EXTEND_ATTR kExtend(JobSet js, PathBuffers pb, int countStats) {
+// COUNT: add rays, node visits and triangle tests to the statistics (the
+// counting is compiled out otherwise).
+template <bool MULTI, bool COUNT>
+__global__ void __launch_bounds__(kBlock) HASTY_EXTEND_ATTR kExtend(JobSet js, PathBuffers pb) {
   __shared__ uint2 stack[kStackLds * kBlock];
   const ExtSet es = extensionSet(pb);
   uint32_t gtid = xcdBlock() * blockDim.x + threadIdx.x;
@@ -403,7 +405,7 @@ __global__ void __launch_bounds__(kBlock) HASTY_EXTEND_ATTR kExtend(JobSet js, P
     pb.hit[slot] = make_float4(h.t, h.u, h.v, __int_as_float(h.tri));
     rays++;
   }
-  if (countStats) {
+  if (COUNT) {
     waveAddStat(&pb.stats[kStatExtRays], rays);
     waveAddStat(&pb.stats[kStatNodes], nodes);
     waveAddStat(&pb.stats[kStatTris], tris);
@@ -657,8 +659,8 @@ __global__ void __launch_bounds__(kBlock) HASTY_SHADE_ATTR kShade(JobSet js, Pat
 }
 
 // -------------------------------------------------------------- connect ----
-template <bool MULTI>
-__global__ void __launch_bounds__(kBlock) HASTY_CONNECT_ATTR kConnect(JobSet js, PathBuffers pb, int countStats) {
+template <bool MULTI, bool COUNT>
+__global__ void __launch_bounds__(kBlock) HASTY_CONNECT_ATTR kConnect(JobSet js, PathBuffers pb) {
   __shared__ uint2 stack[kStackLds * kBlock];
   uint32_t n = pb.counters[kCntS];
   uint32_t gtid = xcdBlock() * blockDim.x + threadIdx.x;
@@ -737,7 +739,7 @@ __global__ void __launch_bounds__(kBlock) HASTY_CONNECT_ATTR kConnect(JobSet js,
     pb.L[slot] = f4(f4xyz(L4) + mul(beta, mul(Li, t2)), 0.0f);
     });
   }
-  if (countStats) {
+  if (COUNT) {
     waveAddStat(&pb.stats[kStatShadowRays], rays);
     waveAddStat(&pb.stats[kStatNodes], nodes);
     waveAddStat(&pb.stats[kStatTris], tris);
@@ -939,7 +941,8 @@ void launchChunk(const JobSet& js, const PathBuffers& pbIn, const LaunchConfig& 
     kFillCommit<<<1, 64, 0, stream>>>(pb.counters);
     t1(kKMisc);
     t0(kKExtend);
-    kExtend<MULTI><<<cfg.traceBlocks, kBlock, 0, stream>>>(js, pb, countStats);
+    if (countStats) kExtend<MULTI, true><<<cfg.traceBlocks, kBlock, 0, stream>>>(js, pb);
+    else kExtend<MULTI, false><<<cfg.traceBlocks, kBlock, 0, stream>>>(js, pb);
     t1(kKExtend);
     if (GUIDED) {
       t0(kKGuideInfer);
@@ -950,7 +953,8 @@ void launchChunk(const JobSet& js, const PathBuffers& pbIn, const LaunchConfig& 
     kShade<MULTI, GUIDED><<<cfg.shadeBlocks, kBlock, 0, stream>>>(js, pb);
     t1(kKShade);
     t0(kKConnect);
-    kConnect<MULTI><<<cfg.traceBlocks, kBlock, 0, stream>>>(js, pb, countStats);
+    if (countStats) kConnect<MULTI, true><<<cfg.traceBlocks, kBlock, 0, stream>>>(js, pb);
+    else kConnect<MULTI, false><<<cfg.traceBlocks, kBlock, 0, stream>>>(js, pb);
     t1(kKConnect);
     t0(kKRetire);
     kRetire<GUIDED><<<cfg.shadeBlocks, kBlock, 0, stream>>>(js, pb);
