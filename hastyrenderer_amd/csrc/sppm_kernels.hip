@@ -455,35 +455,104 @@ __device__ inline uint2 cellLookup(const SppmDev& g, unsigned long long key) {
 // PointHashCells::radiusNeighbors (HashCells.h:71-98): cells x -> y -> z,
 // photons of a cell in emission order, |x - c|^2 <= r^2; the BSDF is the
 // diffuse lobe only with outside IOR 1.
-__device__ inline V3 gather(const SppmDev& g, const DevMaterial& m, const Interaction& it, V3 wo, float r, int* M) {
-  V3 est = v3(0.0f, 0.0f, 0.0f);
-  float cs = *g.cellSize;
-  float r2 = r * r;
-  V3 c = it.x;
-  int3 lo = cellOf(c - v3(r, r, r), cs), hi = cellOf(c + v3(r, r, r), cs);
-  lo = make_int3(max(lo.x, g.bbox[0]), max(lo.y, g.bbox[1]), max(lo.z, g.bbox[2]));
-  hi = make_int3(min(hi.x, g.bbox[3]), min(hi.y, g.bbox[4]), min(hi.z, g.bbox[5]));
-  V3 ns = shadingNormal(it, wo);
-  int found = 0;
-  for (int i = lo.x; i <= hi.x; ++i)
-    for (int j = lo.y; j <= hi.y; ++j)
-      for (int k = lo.z; k <= hi.z; ++k) {
-        uint2 cell = cellLookup(g, packKey(make_int3(i, j, k)));
-        for (uint32_t q = cell.x; q < cell.x + cell.y; ++q) {
-          float4 x4 = g.sX[q];
-          V3 dlt = v3(x4.x, x4.y, x4.z) - c;
-          if (!(dot(dlt, dlt) <= r2)) continue;
-          V3 wi = f4xyz(g.sWi[q]);
-          V3 fl = f4xyz(g.sF[q]);
-          float dotNsWi = dot(ns, wi);
-          V3 fD = evalDiffuse(dotNsWi, m, normalScale(wi, ns, it.ng));
-          float fc = __builtin_fabsf(dot(wi, it.ng)) / (1e-7f + __builtin_fabsf(x4.w));
-          est = est + mul(fl * fc, fD + v3(0.0f, 0.0f, 0.0f));
-          found++;
+//
+// When some lane of the wave has many candidate photons (dense maps, large
+// radii), the lanes that reach the call together serve each other's
+// queries one owner at a time, every lane testing a share of the owner's
+// photons (rank k of n takes photons k, k + n, ... of each cell, so a
+// cell's run is read in whole cache lines); the owner adds the lanes'
+// partial sums in rank order.  The set and the count of photons are the
+// reference's; only the FP32 association of the flux sum differs, the same
+// way on every run.  Otherwise each lane sums its own photons in the
+// reference's order.  `partials` is this wave's 64-entry LDS scratch.
+__device__ inline void photonTerm(const SppmDev& g, uint32_t q, V3 c, float r2, V3 ns, V3 ng, const DevMaterial& m,
+                                  V3* acc, int* found) {
+  const float4 x4 = g.sX[q];
+  const V3 dlt = v3(x4.x, x4.y, x4.z) - c;
+  if (!(dot(dlt, dlt) <= r2)) return;
+  const V3 wi = f4xyz(g.sWi[q]);
+  const V3 fl = f4xyz(g.sF[q]);
+  const V3 fD = evalDiffuse(dot(ns, wi), m, normalScale(wi, ns, ng));
+  const float fc = __builtin_fabsf(dot(wi, ng)) / (1e-7f + __builtin_fabsf(x4.w));
+  *acc = *acc + mul(fl * fc, fD + v3(0.0f, 0.0f, 0.0f));
+  *found += 1;
+}
+// Lanes whose cells hold fewer candidates than this gather on their own (a
+// shared gather costs every lane each owner's cell lookups).
+constexpr uint32_t kCoopCandidates = 2048;
+__device__ inline V3 gatherCoop(const SppmDev& g, const DevScene& s, const Interaction& it, V3 wo, float r, int* M,
+                                float4* partials) {
+  {
+    // candidates of this lane's own query; if no lane of the wave has many,
+    // every lane gathers alone in the reference's order
+    const float cs = *g.cellSize;
+    int3 lo = cellOf(it.x - v3(r, r, r), cs), hi = cellOf(it.x + v3(r, r, r), cs);
+    lo = make_int3(max(lo.x, g.bbox[0]), max(lo.y, g.bbox[1]), max(lo.z, g.bbox[2]));
+    hi = make_int3(min(hi.x, g.bbox[3]), min(hi.y, g.bbox[4]), min(hi.z, g.bbox[5]));
+    uint32_t cand = 0;
+    for (int i = lo.x; i <= hi.x; ++i)
+      for (int j = lo.y; j <= hi.y; ++j)
+        for (int k = lo.z; k <= hi.z; ++k) cand += cellLookup(g, packKey(make_int3(i, j, k))).y;
+    if (__ballot(cand > kCoopCandidates) == 0ull) {
+      const DevMaterial m = ldgStruct(s.mats + it.mat);
+      const V3 ns = shadingNormal(it, wo);
+      V3 est = v3(0.0f, 0.0f, 0.0f);
+      int found = 0;
+      for (int i = lo.x; i <= hi.x; ++i)
+        for (int j = lo.y; j <= hi.y; ++j)
+          for (int k = lo.z; k <= hi.z; ++k) {
+            const uint2 cell = cellLookup(g, packKey(make_int3(i, j, k)));
+            for (uint32_t q = cell.x; q < cell.x + cell.y; ++q) photonTerm(g, q, it.x, r * r, ns, it.ng, m, &est, &found);
+          }
+      *M = found;
+      return est;
+    }
+  }
+  const uint64_t active = __ballot(1);
+  const uint32_t lane = __lane_id();
+  const int rank = __popcll(active & ((1ull << lane) - 1ull));
+  const int nAct = __popcll(active);
+  const V3 nsMine = shadingNormal(it, wo);
+  const float cs = *g.cellSize;
+  V3 myEst = v3(0.0f, 0.0f, 0.0f);
+  int myM = 0;
+  uint64_t todo = active;
+  while (todo) {
+    const int owner = __ffsll((unsigned long long)todo) - 1;
+    todo &= todo - 1ull;
+    const V3 c = v3(__shfl(it.x.x, owner), __shfl(it.x.y, owner), __shfl(it.x.z, owner));
+    const V3 ns = v3(__shfl(nsMine.x, owner), __shfl(nsMine.y, owner), __shfl(nsMine.z, owner));
+    const V3 ng = v3(__shfl(it.ng.x, owner), __shfl(it.ng.y, owner), __shfl(it.ng.z, owner));
+    const float rr = __shfl(r, owner);
+    const DevMaterial m = ldgStruct(s.mats + __shfl(it.mat, owner));
+    const float r2 = rr * rr;
+    int3 lo = cellOf(c - v3(rr, rr, rr), cs), hi = cellOf(c + v3(rr, rr, rr), cs);
+    lo = make_int3(max(lo.x, g.bbox[0]), max(lo.y, g.bbox[1]), max(lo.z, g.bbox[2]));
+    hi = make_int3(min(hi.x, g.bbox[3]), min(hi.y, g.bbox[4]), min(hi.z, g.bbox[5]));
+    V3 acc = v3(0.0f, 0.0f, 0.0f);
+    int found = 0;
+    for (int i = lo.x; i <= hi.x; ++i)
+      for (int j = lo.y; j <= hi.y; ++j)
+        for (int k = lo.z; k <= hi.z; ++k) {
+          const uint2 cell = cellLookup(g, packKey(make_int3(i, j, k)));
+          for (uint32_t q = cell.x + (uint32_t)rank; q < cell.x + cell.y; q += (uint32_t)nAct)
+            photonTerm(g, q, c, r2, ns, ng, m, &acc, &found);
         }
+    partials[rank] = make_float4(acc.x, acc.y, acc.z, __int_as_float(found));
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    if ((int)lane == owner) {
+      for (int t = 0; t < nAct; ++t) {
+        const float4 pp = partials[t];
+        myEst = myEst + v3(pp.x, pp.y, pp.z);
+        myM += __float_as_int(pp.w);
       }
-  *M = found;
-  return est;
+    }
+    __builtin_amdgcn_wave_barrier();  // partials are rewritten for the next owner
+  }
+  *M = myM;
+  return myEst;
 }
 
 // Occupancy target of the camera megakernel (waves per SIMD; 0: the
@@ -508,6 +577,7 @@ __global__ void __launch_bounds__(kB) HASTY_SPPM_CAMERA_ATTR kSppmCamera(DevScen
                                                   float alpha, int emittedInt, float* color, float* normalOut,
                                                   float* albedoOut) {
   __shared__ uint2 stack[kStackLds * kB];
+  __shared__ float4 gatherPartials[kB];
   uint32_t P = s.width * s.height;
   uint32_t gtid = xcdRemap() * kB + threadIdx.x;
   uint2* ovf = g.ovf + (size_t)gtid * kStackOvf;
@@ -564,7 +634,7 @@ __global__ void __launch_bounds__(kB) HASTY_SPPM_CAMERA_ATTR kSppmCamera(DevScen
       }
       if (m.hasDiffuseLobe) {
         int M = 0;
-        V3 est = gather(g, m, it, wo, radius, &M);
+        V3 est = gatherCoop(g, s, it, wo, radius, &M, gatherPartials + (threadIdx.x & ~63u));
         Mtot += M;
         F0 = F0 + mul(WR, mul(Tb, est));
       }
