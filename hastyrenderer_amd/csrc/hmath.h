@@ -109,6 +109,22 @@ HM_HD double dcos(double x) {
 }
 HM_HD float sinf(float x) { return (float)dsin((double)x); }
 HM_HD float cosf(float x) { return (float)dcos((double)x); }
+// sinf(x) and cosf(x), bit for bit, from one reduction.
+HM_HD void sincosf(float x, float* s, float* c) {
+  if (x != x) {
+    *s = *c = x;
+    return;
+  }
+  int q;
+  const double r = reduce_pio2((double)x, &q);
+  const double sp = sin_poly(r), cp = cos_poly(r);
+  switch (q) {
+    case 0: *s = (float)sp, *c = (float)cp; break;
+    case 1: *s = (float)cp, *c = (float)-sp; break;
+    case 2: *s = (float)-sp, *c = (float)-cp; break;
+    default: *s = (float)-cp, *c = (float)sp; break;
+  }
+}
 HM_HD float tanf(float x) {
   double d = (double)x;
   return (float)(dsin(d) / dcos(d));

@@ -521,8 +521,8 @@ __device__ inline V3 sampleCosHemisphere(hrng::Stream& rng, V3 n, float* pdf) {
     float phi = 2.0f * (float)kPi * xi2;
     costheta = hm::fsqrt(xi1);
     float sintheta = hm::fsqrt(1.0f - xi1);
-    float cosphi = hm::cosf(phi);
-    float sinphi = hm::sinf(phi);
+    float sinphi, cosphi;
+    hm::sincosf(phi, &sinphi, &cosphi);
     V3 rv = v3(sintheta * cosphi, sintheta * sinphi, costheta);
     dir = normalize(applyRot(rot, rv));
     costheta = dot(dir, n);
@@ -554,8 +554,10 @@ __device__ inline V3 sampleVNDF(V3 w, float ax, float ay, float xi1, float xi2) 
   float r = hm::fsqrt(xi1);
   float a = 1.0f / (1.0f + Vh.z);
   float phi = (float)kPi * ((xi2 < a) ? (xi2 / a) : (1.0f + (xi2 - a) / (1.0f - a)));
-  float t1 = r * hm::cosf(phi);
-  float t2 = r * hm::sinf(phi) * (xi2 < a ? 1.0f : Vh.z);
+  float sphi, cphi;
+  hm::sincosf(phi, &sphi, &cphi);
+  float t1 = r * cphi;
+  float t2 = r * sphi * (xi2 < a ? 1.0f : Vh.z);
   float t3 = hm::fsqrt(fmaxf(0.0f, 1.0f - t1 * t1 - t2 * t2));
   V3 Nh = T1 * t1 + T2 * t2 + Vh * t3;
   return normalize(v3(ax * Nh.x, ay * Nh.y, fmaxf(0.0f, Nh.z)));
@@ -956,9 +958,9 @@ __device__ inline V3 envSample(const DevScene& s, hrng::Stream& rng, float* dens
     dens = widthF * heightF * pxy;
     float phi = u * 2.0f * (float)kPi;
     float theta = (1.0f - v) * (float)kPi;
-    float cosT = hm::cosf(theta);
-    sinT = hm::sinf(theta);
-    float sinp = hm::sinf(phi), cosp = hm::cosf(phi);
+    float cosT, sinp, cosp;
+    hm::sincosf(theta, &sinT, &cosT);
+    hm::sincosf(phi, &sinp, &cosp);
     dir = v3(sinT * cosp, cosT, sinT * sinp);
   } while (sinT == 0.0f);
   *density = dens / (2.0f * (float)kPi * (float)kPi * sinT);
@@ -1015,14 +1017,18 @@ __device__ inline void sampleNGon(hrng::Stream& rng, int numBlades, float bladeR
   do {
     float alpha = rng.uniform01f() * (float)(2.0 * kPi);
     float r0 = hm::fsqrt(rng.uniform01f());
-    sx = r0 * hm::cosf(alpha);
-    sy = r0 * hm::sinf(alpha);
+    float sa, ca;
+    hm::sincosf(alpha, &sa, &ca);
+    sx = r0 * ca;
+    sy = r0 * sa;
     float r = hm::fsqrt(sx * sx + sy * sy);
     float angle = hm::atan2f(sy, sx);
     angle -= bladeRotation;
     angle += (float)(2.0 * kPi) * (1.0f + kFltEps);
     angle = hm::fmodf(angle, anglePerSector);
-    float px = r * hm::cosf(angle), py = r * hm::sinf(angle);
+    float sang, cang;
+    hm::sincosf(angle, &sang, &cang);
+    float px = r * cang, py = r * sang;
     outside = (nx * px + ny * py + d) > 0.0f;
   } while (outside);
   *ox = sx;
