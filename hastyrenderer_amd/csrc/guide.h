@@ -295,7 +295,11 @@ __device__ inline void forwardAll(const Net& net, Acts& a, f32x4* raw) {
 // ---------------------------------------------- LDS-staged inference ----
 // Inference kernels copy the BF16 W^T and the biases into LDS once per block
 // (42,176 B) and loop over many 16-query tiles, so that each MFMA's weight
-// operand is an LDS read instead of a global load per tile.
+// operand is an LDS read instead of a global load per tile.  The copy is kept
+// in operand order: the A operand of k-step ks of output tile mt of layer l
+// is 64 lanes x 4 BF16 at wqOffset(l) + 4 (64 (KS mt + ks) + lane), so a
+// wave's read is 512 contiguous bytes (rows of W^T 128 B apart would put the
+// 16 lanes of a lane group in one bank).
 constexpr int kBiasLds = 4 * kHidden + kOutPad;
 __host__ __device__ constexpr int biasLds(int l) { return l * kHidden; }
 struct LdsNet {
@@ -313,17 +317,17 @@ __device__ inline int lobeMajorRow(int p) {
 // lobeMajor: the last layer's rows in lobeMajorRow order (kGuideInfer);
 // otherwise the network's own order.
 __device__ inline void stageNet(const Net& net, LdsNet* L, bool lobeMajor = false) {
-  const uint4* src = reinterpret_cast<const uint4*>(net.wT);
-  uint4* dst = reinterpret_cast<uint4*>(L->wT);
-  constexpr int lastRowVecs = kHidden / 8;  // uint4 per row of the last layer
-  constexpr int lastBase = wqOffset(kLayers - 1) / 8;
-  for (int i = threadIdx.x; i < kWeights / 8; i += blockDim.x) {
-    int from = i;
-    if (lobeMajor && i >= lastBase) {
-      const int row = (i - lastBase) / lastRowVecs, col = (i - lastBase) % lastRowVecs;
-      from = lastBase + lobeMajorRow(row) * lastRowVecs + col;
-    }
-    dst[i] = src[from];
+  const uint2* src = reinterpret_cast<const uint2*>(net.wT);
+  uint2* dst = reinterpret_cast<uint2*>(L->wT);
+  for (int i = threadIdx.x; i < kWeights / 4; i += blockDim.x) {
+    int l = 0;
+    while (l < kLayers - 1 && i >= wqOffset(l + 1) / 4) ++l;
+    const int din = layerIn(l), KS = din / 16;
+    const int local = i - wqOffset(l) / 4, lane = local & 63, t = local >> 6;
+    const int mt = t / KS, ks = t - KS * mt;
+    int row = 16 * mt + (lane & 15);
+    if (lobeMajor && l == kLayers - 1) row = lobeMajorRow(row);
+    dst[i] = src[(wqOffset(l) + row * din + 16 * ks + 4 * (lane >> 4)) / 4];
   }
   for (int i = threadIdx.x; i < kBiasLds; i += blockDim.x) {
     const int l = i < 4 * kHidden ? i / kHidden : 4;
@@ -335,14 +339,13 @@ __device__ inline void stageNet(const Net& net, LdsNet* L, bool lobeMajor = fals
 }
 template <int KS, int MT>
 __device__ inline void layerFwdLds(const LdsNet* L, int l, const bf16x4* x, f32x4* z) {
-  const int lane = threadIdx.x & 63, r = lane & 15, q = lane >> 4;
-  const int din = layerIn(l);
+  const int lane = threadIdx.x & 63, q = lane >> 4;
 #pragma unroll
   for (int mt = 0; mt < MT; ++mt) {
     f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
-    const short* row = L->wT + wqOffset(l) + (16 * mt + r) * din + 4 * q;
+    const short* a = L->wT + wqOffset(l) + 4 * (64 * KS * mt + lane);
 #pragma unroll
-    for (int ks = 0; ks < KS; ++ks) acc = mfma(load4(row + 16 * ks), x[ks], acc);
+    for (int ks = 0; ks < KS; ++ks) acc = mfma(load4(a + 256 * ks), x[ks], acc);
     const float* b = L->bias + biasLds(l) + 16 * mt + 4 * q;
     acc.x += b[0];
     acc.y += b[1];

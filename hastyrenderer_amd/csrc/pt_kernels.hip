@@ -815,58 +815,77 @@ __global__ void __launch_bounds__(kBlock) kAccumulate(PathBuffers pb, PassInfo p
 }
 
 // ------------------------------------------------------------ guide infer ----
-// Raw guide outputs for every path in qA that hit a surface: inputs are the
-// hit point, the geometric normal facing wo, and wo; 16 paths per wave.
-// One 1024-thread block per CU (16 waves, 4 per SIMD, sharing one LDS copy
-// of the network): each block stages the network once, then its waves loop
-// over 16-path tiles.
-constexpr uint32_t kGuideInferBlocks = 256, kGuideInferThreads = 1024;
-__global__ void __launch_bounds__(kGuideInferThreads) kGuideInfer(JobSet js, PathBuffers pb) {
+// Raw guide outputs for every path in qA that hit a guided surface: inputs
+// are the hit point, the geometric normal facing wo, and wo.  A wave takes 64
+// paths, one per lane, for the gathers (hit, material, triangle, ray), then
+// runs the network on its four 16-path tiles in turn; the tile's lanes fetch
+// their path's inputs from the gathering lane by shuffles.  Blocks of 512
+// threads, three per CU (the LDS copy of the network is 41 KiB per block).
+#ifndef HASTY_GUIDE_INFER_WAVES
+#define HASTY_GUIDE_INFER_WAVES 6
+#endif
+constexpr uint32_t kGuideInferThreads = 512, kGuideInferBlocks = 256 * 3;
+__global__ void __launch_bounds__(kGuideInferThreads) HASTY_WAVES_ATTR_(HASTY_GUIDE_INFER_WAVES)
+    kGuideInfer(JobSet js, PathBuffers pb) {
   __shared__ guide::LdsNet ldsNet;
   guide::stageNet(js.g.net, &ldsNet, true);
   const DevScene& s = js.s;
   const uint32_t n = pb.counters[kCntA];
   const int lane = threadIdx.x & 63, r = lane & 15, q = lane >> 4;
-  uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  uint32_t waves = (gridDim.x * blockDim.x) >> 6;
-  for (uint32_t t = wave; t * 16 < n; t += waves) {
-    uint32_t e = t * 16 + r;
-    bool valid = e < n;
-    uint32_t slot = pb.qA[valid ? e : n - 1];
-    float4 h4 = pb.hit[slot];
-    int32_t tri = __float_as_int(h4.w);
-    // only hits on guided materials read the outputs; a wave whose 16 paths
-    // have none skips the network (the test is uniform over the wave)
-    bool want = valid && tri >= 0;
-    if (want) {
-      const DevMaterial m = ldgStruct(s.mats + ldg(s.triMat + tri));
-      want = !m.isLight && m.alpha >= kGuideMinAlpha && m.transmission == 0.0f;
+  const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const uint32_t waves = (gridDim.x * blockDim.x) >> 6;
+  for (uint32_t g = wave; g * 64 < n; g += waves) {
+    const uint32_t e = g * 64 + lane;
+    uint32_t slot = 0u;
+    bool want = false;
+    V3 x = v3(0.0f, 0.0f, 0.0f), ng = v3(0.0f, 0.0f, 1.0f), wo = v3(0.0f, 0.0f, 1.0f);
+    if (e < n) {
+      slot = pb.qA[e];
+      const float4 h4 = pb.hit[slot];
+      const int32_t tri = __float_as_int(h4.w);
+      if (tri >= 0) {
+        // only hits on guided materials read the outputs
+        const DevMaterial m = ldgStruct(s.mats + ldg(s.triMat + tri));
+        want = !m.isLight && m.alpha >= kGuideMinAlpha && m.transmission == 0.0f;
+        if (want) {
+          hitPointNormal(s, tri, h4.y, h4.z, &x, &ng);
+          wo = -f4xyz(pb.rayD[slot]);
+          if (dot(ng, wo) < 0.0f) ng = -ng;
+          x = guide::normPos(js.g.net, x);
+        }
+      }
     }
-    if (__ballot(want) == 0ull) continue;
-    V3 x = v3(0.0f, 0.0f, 0.0f), ng = v3(0.0f, 0.0f, 1.0f);
-    if (tri >= 0) hitPointNormal(s, tri, h4.y, h4.z, &x, &ng);
-    V3 wo = -f4xyz(pb.rayD[slot]);
-    if (dot(ng, wo) < 0.0f) ng = -ng;
-    guide::bf16x4 h0[5];
-    guide::inputOperands(guide::normPos(js.g.net, x), ng, wo, h0);
-    guide::f32x4 o[3];
-    guide::forwardLds(&ldsNet, h0, o);
-    // lane group q holds lobes 2q and 2q + 1 (raw outputs 10q .. 10q + 9);
-    // the softmax over the 8 logits runs across the query's four lanes
-    const float r10[10] = {o[0].x, o[0].y, o[0].z, o[0].w, o[1].x, o[1].y, o[1].z, o[1].w, o[2].x, o[2].y};
-    float amax = fmaxf(r10[4], r10[9]);
-    amax = fmaxf(amax, __shfl_xor(amax, 16));
-    amax = fmaxf(amax, __shfl_xor(amax, 32));
-    const float e0 = __expf(r10[4] - amax), e1 = __expf(r10[9] - amax);
-    float sum = e0 + e1;
-    sum += __shfl_xor(sum, 16);
-    sum += __shfl_xor(sum, 32);
-    const float w0 = e0 / sum, w1 = e1 / sum;
-    if (want) {
-      float4* out = js.g.raw + (size_t)slot * 10;
-      out[2 * q] = guide::lobeEntry(r10, w0);
-      out[2 * q + 1] = guide::lobeEntry(r10 + 5, w1);
-      reinterpret_cast<float2*>(out + guide::kLobes)[q] = make_float2(w0, w1);
+    const unsigned long long wantMask = __ballot(want);
+    for (int j = 0; j < 4; ++j) {
+      // a tile whose 16 paths have no guided hit skips the network (the test
+      // is uniform over the wave)
+      if (((wantMask >> (16 * j)) & 0xFFFFull) == 0ull) continue;
+      const int src = 16 * j + r;
+      const V3 xq = v3(__shfl(x.x, src), __shfl(x.y, src), __shfl(x.z, src));
+      const V3 nq = v3(__shfl(ng.x, src), __shfl(ng.y, src), __shfl(ng.z, src));
+      const V3 wq = v3(__shfl(wo.x, src), __shfl(wo.y, src), __shfl(wo.z, src));
+      const uint32_t sq = __shfl(slot, src);
+      guide::bf16x4 h0[5];
+      guide::inputOperands(xq, nq, wq, h0);
+      guide::f32x4 o[3];
+      guide::forwardLds(&ldsNet, h0, o);
+      // lane group q holds lobes 2q and 2q + 1 (raw outputs 10q .. 10q + 9);
+      // the softmax over the 8 logits runs across the path's four lanes
+      const float r10[10] = {o[0].x, o[0].y, o[0].z, o[0].w, o[1].x, o[1].y, o[1].z, o[1].w, o[2].x, o[2].y};
+      float amax = fmaxf(r10[4], r10[9]);
+      amax = fmaxf(amax, __shfl_xor(amax, 16));
+      amax = fmaxf(amax, __shfl_xor(amax, 32));
+      const float e0 = __expf(r10[4] - amax), e1 = __expf(r10[9] - amax);
+      float sum = e0 + e1;
+      sum += __shfl_xor(sum, 16);
+      sum += __shfl_xor(sum, 32);
+      const float w0 = e0 / sum, w1 = e1 / sum;
+      if ((wantMask >> src) & 1ull) {
+        float4* out = js.g.raw + (size_t)sq * 10;
+        out[2 * q] = guide::lobeEntry(r10, w0);
+        out[2 * q + 1] = guide::lobeEntry(r10 + 5, w1);
+        reinterpret_cast<float2*>(out + guide::kLobes)[q] = make_float2(w0, w1);
+      }
     }
   }
 }
