@@ -464,10 +464,12 @@ __device__ inline uint2 cellLookup(const SppmDev& g, unsigned long long key) {
 // partial sums in rank order.  The set and the count of photons are the
 // reference's; only the FP32 association of the flux sum differs, the same
 // way on every run.  Otherwise each lane sums its own photons in the
-// reference's order.  `partials` is this wave's 64-entry LDS scratch.
-__device__ inline void photonTerm(const SppmDev& g, uint32_t q, V3 c, float r2, V3 ns, V3 ng, const DevMaterial& m,
-                                  V3* acc, int* found) {
-  const float4 x4 = g.sX[q];
+// reference's order.  In the shared gather the owner's cell lookups are
+// spread over the lanes (one hash probe each, all in flight together) and
+// the cells' photon runs read back from LDS in cell order.  `partials` and
+// `runs` are this wave's 64-entry LDS scratch.
+__device__ inline void photonTerm(const SppmDev& g, uint32_t q, float4 x4, V3 c, float r2, V3 ns, V3 ng,
+                                  const DevMaterial& m, V3* acc, int* found) {
   const V3 dlt = v3(x4.x, x4.y, x4.z) - c;
   if (!(dot(dlt, dlt) <= r2)) return;
   const V3 wi = f4xyz(g.sWi[q]);
@@ -477,11 +479,32 @@ __device__ inline void photonTerm(const SppmDev& g, uint32_t q, V3 c, float r2, 
   *acc = *acc + mul(fl * fc, fD + v3(0.0f, 0.0f, 0.0f));
   *found += 1;
 }
+// Photons first, first + stride, ... below end, in that order, with
+// kPhotonLoads position loads in flight at a time.
+#ifndef HASTY_PHOTON_LOADS
+#define HASTY_PHOTON_LOADS 4
+#endif
+constexpr uint32_t kPhotonLoads = HASTY_PHOTON_LOADS;
+__device__ inline void photonRun(const SppmDev& g, uint32_t first, uint32_t end, uint32_t stride, V3 c, float r2,
+                                 V3 ns, V3 ng, const DevMaterial& m, V3* acc, int* found) {
+  uint32_t q = first;
+  for (; q + (kPhotonLoads - 1u) * stride < end; q += kPhotonLoads * stride) {
+    float4 x[kPhotonLoads];
+#pragma unroll
+    for (uint32_t u = 0; u < kPhotonLoads; ++u) x[u] = g.sX[q + u * stride];
+#pragma unroll
+    for (uint32_t u = 0; u < kPhotonLoads; ++u) photonTerm(g, q + u * stride, x[u], c, r2, ns, ng, m, acc, found);
+  }
+  for (; q < end; q += stride) photonTerm(g, q, g.sX[q], c, r2, ns, ng, m, acc, found);
+}
 // Lanes whose cells hold fewer candidates than this gather on their own (a
 // shared gather costs every lane each owner's cell lookups).
-constexpr uint32_t kCoopCandidates = 2048;
+#ifndef HASTY_COOP_CANDIDATES
+#define HASTY_COOP_CANDIDATES 1024
+#endif
+constexpr uint32_t kCoopCandidates = HASTY_COOP_CANDIDATES;
 __device__ inline V3 gatherCoop(const SppmDev& g, const DevScene& s, const Interaction& it, V3 wo, float r, int* M,
-                                float4* partials) {
+                                float4* partials, uint2* runs) {
   {
     // candidates of this lane's own query; if no lane of the wave has many,
     // every lane gathers alone in the reference's order
@@ -502,7 +525,7 @@ __device__ inline V3 gatherCoop(const SppmDev& g, const DevScene& s, const Inter
         for (int j = lo.y; j <= hi.y; ++j)
           for (int k = lo.z; k <= hi.z; ++k) {
             const uint2 cell = cellLookup(g, packKey(make_int3(i, j, k)));
-            for (uint32_t q = cell.x; q < cell.x + cell.y; ++q) photonTerm(g, q, it.x, r * r, ns, it.ng, m, &est, &found);
+            photonRun(g, cell.x, cell.x + cell.y, 1u, it.x, r * r, ns, it.ng, m, &est, &found);
           }
       *M = found;
       return est;
@@ -531,13 +554,28 @@ __device__ inline V3 gatherCoop(const SppmDev& g, const DevScene& s, const Inter
     hi = make_int3(min(hi.x, g.bbox[3]), min(hi.y, g.bbox[4]), min(hi.z, g.bbox[5]));
     V3 acc = v3(0.0f, 0.0f, 0.0f);
     int found = 0;
-    for (int i = lo.x; i <= hi.x; ++i)
-      for (int j = lo.y; j <= hi.y; ++j)
-        for (int k = lo.z; k <= hi.z; ++k) {
-          const uint2 cell = cellLookup(g, packKey(make_int3(i, j, k)));
-          for (uint32_t q = cell.x + (uint32_t)rank; q < cell.x + cell.y; q += (uint32_t)nAct)
-            photonTerm(g, q, c, r2, ns, ng, m, &acc, &found);
-        }
+    // the cells of the box in x -> y -> z order, nAct at a time: lane rank
+    // looks up cell base + rank
+    const int ny = max(hi.y - lo.y + 1, 0), nz = max(hi.z - lo.z + 1, 0);
+    const int nyz = ny * nz, nc = max(hi.x - lo.x + 1, 0) * nyz;
+    for (int base = 0; base < nc; base += nAct) {
+      const int ci = base + rank;
+      uint2 run = make_uint2(0u, 0u);
+      if (ci < nc) {
+        const int ix = ci / nyz, rem = ci - ix * nyz, iy = rem / nz, iz = rem - iy * nz;
+        run = cellLookup(g, packKey(make_int3(lo.x + ix, lo.y + iy, lo.z + iz)));
+      }
+      runs[rank] = run;
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      const int nb = min(nAct, nc - base);
+      for (int t = 0; t < nb; ++t) {
+        const uint2 cell = runs[t];
+        photonRun(g, cell.x + (uint32_t)rank, cell.x + cell.y, (uint32_t)nAct, c, r2, ns, ng, m, &acc, &found);
+      }
+      __builtin_amdgcn_wave_barrier();  // runs are rewritten for the next cells
+    }
     partials[rank] = make_float4(acc.x, acc.y, acc.z, __int_as_float(found));
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
@@ -556,10 +594,11 @@ __device__ inline V3 gatherCoop(const SppmDev& g, const DevScene& s, const Inter
 }
 
 // Occupancy target of the camera megakernel (waves per SIMD; 0: the
-// compiler's choice, 208 VGPRs = 2 waves).  At 3 waves it spills 128 B per
-// lane and is still ~15% faster on the 1920x1080 caustic box.
+// compiler's choice, 208 VGPRs = 2 waves).  At 4 waves it spills ~350 B per
+// lane (path state held across the gathers) and is still the fastest on the
+// 1920x1080 caustic box: the gathers wait on memory.
 #ifndef HASTY_SPPM_CAMERA_WAVES
-#define HASTY_SPPM_CAMERA_WAVES 3
+#define HASTY_SPPM_CAMERA_WAVES 4
 #endif
 #if HASTY_SPPM_CAMERA_WAVES > 0
 #define HASTY_SPPM_CAMERA_ATTR __attribute__((amdgpu_waves_per_eu(HASTY_SPPM_CAMERA_WAVES)))
@@ -578,6 +617,7 @@ __global__ void __launch_bounds__(kB) HASTY_SPPM_CAMERA_ATTR kSppmCamera(DevScen
                                                   float* albedoOut) {
   __shared__ uint2 stack[kStackLds * kB];
   __shared__ float4 gatherPartials[kB];
+  __shared__ uint2 gatherRuns[kB];
   uint32_t P = s.width * s.height;
   uint32_t gtid = xcdRemap() * kB + threadIdx.x;
   uint2* ovf = g.ovf + (size_t)gtid * kStackOvf;
@@ -634,7 +674,8 @@ __global__ void __launch_bounds__(kB) HASTY_SPPM_CAMERA_ATTR kSppmCamera(DevScen
       }
       if (m.hasDiffuseLobe) {
         int M = 0;
-        V3 est = gatherCoop(g, s, it, wo, radius, &M, gatherPartials + (threadIdx.x & ~63u));
+        V3 est = gatherCoop(g, s, it, wo, radius, &M, gatherPartials + (threadIdx.x & ~63u),
+                           gatherRuns + (threadIdx.x & ~63u));
         Mtot += M;
         F0 = F0 + mul(WR, mul(Tb, est));
       }

@@ -87,6 +87,30 @@ def test_sppm_cornell_matches_oracle(cornell_job):
         assert rel_ok(c, rc).mean() >= 0.999
 
 
+def test_sppm_shared_gather_matches_oracle(caustic_small_job):
+    """Dense photon maps (16 x W H photons stored, radius 20x the footprint):
+    waves with a query of more than 1024 candidate photons take the shared
+    gather (lanes split the owner's cell lookups and photons); the image and
+    the photon counts still follow the oracle's sequential gather."""
+    w, h = 96, 54
+    kw = dict(alpha=0.4, footprint_scale=20.0, store_fraction=16.0, emit_fraction=4000.0)
+    sc, g = _gpu_sppm(caustic_small_job, seed=3, w=w, h=h, **kw)
+    ref = po.OracleSppm(caustic_small_job, seed=3, width=w, height=h, **kw)
+    for k in range(2):
+        c, n, a = g.pass_(k)
+        rc, rn, ra = ref.run_pass(k, w, h)
+        gi, ri = g.info(), ref.info()
+        assert gi["emitted"] == ri["emitted"] and gi["stored"] == ri["stored"]
+        ok = rel_ok(c, rc)
+        print(f"pass {k}: {gi['stored']} photons, {ok.mean():.5f} of pixels within 1e-3", flush=True)
+        assert ok.mean() >= 0.999
+        if k == 0:
+            found = ref.pixels(w, h)[:, 1] / kw["alpha"]
+            print(f"photons found per pixel: mean {found.mean():.0f}, max {found.max():.0f}", flush=True)
+            assert (found > 2048).mean() > 0.05  # many waves take the shared gather
+            np.testing.assert_allclose(g.pixels()["N"].ravel(), ref.pixels(w, h)[:, 1], rtol=1e-6)
+
+
 def test_sppm_caustic_closed_form(tmp_path):
     """Mirror caustic on a diffuse ceiling: rho / pi * E(x) within 1%."""
     w = h = 64
