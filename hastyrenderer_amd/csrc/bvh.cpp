@@ -205,7 +205,7 @@ Bvh4 buildBvh4(const float* triPos, uint32_t numTris, const BvhBuildOptions& opt
     for (int k = 0; k < 4; ++k) {
       for (int c = 0; c < 3; ++c) {
         n[(2 * c) * 4 + k] = INFINITY;
-        n[(2 * c + 1) * 4 + k] = -INFINITY;
+        n[(2 * c + 1) * 4 + k] = INFINITY;
       }
       uint32_t ref = kLeafFlag;
       memcpy(&n[24 + k], &ref, 4);
@@ -248,7 +248,7 @@ Bvh4 buildBvh4(const float* triPos, uint32_t numTris, const BvhBuildOptions& opt
         refs[k] = kLeafFlag;
         for (int c = 0; c < 3; ++c) {
           lo[k][c] = INFINITY;
-          hi[k][c] = -INFINITY;
+          hi[k][c] = INFINITY;
         }
         continue;
       }

@@ -9,7 +9,9 @@
 //   [7] reserved
 // child ref >= 0: index of an inner node
 // child ref <  0: 0x80000000 | count << 24 | first leaf triangle (count 1..127);
-//                 count 0 marks an empty slot (its box is empty: lo=+inf, hi=-inf)
+//                 count 0 marks an empty slot (all six planes +inf: a slab
+//                 test gives it entry distance +inf or exit distance -inf,
+//                 so the traversal never enters it and needs no ref test)
 // Triangle record (3 x 16 bytes): v0.xyz + global triangle index (bits),
 // v1.xyz + 0, v2.xyz + 0.
 //

@@ -310,7 +310,9 @@ __device__ inline Hit traverse(const DevScene& s, const RayPre& r, float tBound,
       const float tb = (CLOSEST && best.tri < 0) ? __builtin_inff() : best.t * kCullSlack;
       // Entry and exit distances of a child box; the exit is scaled by 1 + 6u
       // (Ize, "Robust BVH ray traversal") so that rounding never culls a box
-      // the exact ray enters.
+      // the exact ray enters.  An empty slot's planes are all +inf, so its
+      // entry is +inf or its exit -inf: the closest-hit test needs no branch
+      // on the child ref (the occlusion test keeps it: faster there).
       if constexpr (CLOSEST) {
         const float4* n = s.nodes + (size_t)ref * 8;
         const float4 lox = ldg4(n), hix = ldg4(n + 1), loy = ldg4(n + 2), hiy = ldg4(n + 3), loz = ldg4(n + 4),
@@ -323,7 +325,7 @@ __device__ inline Hit traverse(const DevScene& s, const RayPre& r, float tBound,
     float t0z = (LZ - r.o.z) * r.inv.z, t1z = (HZ - r.o.z) * r.inv.z;                        \
     float tmin = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fminf(t0z, t1z));           \
     float tmax = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fmaxf(t0z, t1z)) * kTmaxScale; \
-    bool h = (uint32_t)(RF) != kLeafFlag && tmin <= tmax && tmax > 0.0f && tmin <= tb;      \
+    bool h = (tmin <= tmax) & (tmax > 0.0f) & (tmin <= tb);                                      \
     TOUT = h ? tmin : __builtin_inff();                                                      \
   }
         HDEV_SLAB(lox.x, hix.x, loy.x, hiy.x, loz.x, hiz.x, refs.x, e0)
