@@ -356,9 +356,9 @@ __device__ inline Hit traverse(const DevScene& s, const RayPre& r, float tBound,
 #undef HDEV_SLAB
       }
       uint32_t r0 = (uint32_t)refs.x, r1 = (uint32_t)refs.y, r2 = (uint32_t)refs.z, r3 = (uint32_t)refs.w;
-      int cnt = (e0 != __builtin_inff()) + (e1 != __builtin_inff()) + (e2 != __builtin_inff()) +
-                (e3 != __builtin_inff());
-      if (cnt > 0) {
+      // children entered have finite entry distances; after the sort they
+      // come first
+      if (fminf(fminf(e0, e1), fminf(e2, e3)) != __builtin_inff()) {
         cswap(e0, r0, e1, r1);
         cswap(e2, r2, e3, r3);
         cswap(e0, r0, e2, r2);
@@ -374,9 +374,9 @@ __device__ inline Hit traverse(const DevScene& s, const RayPre& r, float tBound,
     else ovf[sp - LDS_DEPTH] = e_;                             \
     ++sp;                                                      \
   }
-        if (cnt > 3) HDEV_PUSH(r3, e3)
-        if (cnt > 2) HDEV_PUSH(r2, e2)
-        if (cnt > 1) HDEV_PUSH(r1, e1)
+        if (e3 != __builtin_inff()) HDEV_PUSH(r3, e3)
+        if (e2 != __builtin_inff()) HDEV_PUSH(r2, e2)
+        if (e1 != __builtin_inff()) HDEV_PUSH(r1, e1)
 #undef HDEV_PUSH
         ref = r0;
         continue;
