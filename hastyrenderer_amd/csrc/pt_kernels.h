@@ -136,6 +136,7 @@ enum KernelId {
   kKSppmPhotons,
   kKSppmBuild,
   kKSppmCamera,
+  kKSppmGather,
   kNumKernels
 };
 const char* kernelName(int id);

@@ -45,6 +45,7 @@ const char* kernelName(int id) {
     case kKSppmPhotons: return "sppm_photons";
     case kKSppmBuild: return "sppm_build";
     case kKSppmCamera: return "sppm_camera";
+    case kKSppmGather: return "sppm_gather";
     default: return "pt_misc";
   }
 }

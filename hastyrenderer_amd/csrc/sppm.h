@@ -65,6 +65,11 @@ class SppmRenderer {
   // per pixel
   float* dPixel_ = nullptr;  // 10 planes
   float* dRadii_ = nullptr;
+  // gather queries of the pass's camera paths (maxDepth x P records of 64
+  // bytes), queries per pixel, path radiance (3 planes)
+  float4* dQuery_ = nullptr;
+  uint32_t* dNq_ = nullptr;
+  float* dLpath_ = nullptr;
   // cell size reductions
   float* dPartial_ = nullptr;
   float* dCand_ = nullptr;

@@ -95,6 +95,12 @@ struct SppmDev {
   const float* cellSize;
   uint32_t* counters;
   uint2* ovf;
+  // gather queries of the pass's camera paths: record (v, p) of pixel p's
+  // v-th diffuse vertex at query[4 (v P + p)]: (x, material), (shading
+  // normal, Tb.x), (geometric normal, Tb.y), (W_R, Tb.z)
+  float4* query;
+  uint32_t* nq;  // queries per pixel
+  float* Lpath;  // 3P: LL + LR + LS of the pass's camera path
 };
 
 enum Counter { kCtrStored = 0, kCtrEmitted = 1, kCtrBatchEmit = 2, kCtrKeyOverflow = 3, kNumCtr = 4 };
@@ -503,59 +509,62 @@ __device__ inline void photonRun(const SppmDev& g, uint32_t first, uint32_t end,
 #define HASTY_COOP_CANDIDATES 1024
 #endif
 constexpr uint32_t kCoopCandidates = HASTY_COOP_CANDIDATES;
-__device__ inline V3 gatherCoop(const SppmDev& g, const DevScene& s, const Interaction& it, V3 wo, float r, int* M,
-                                float4* partials, uint2* runs) {
+__device__ inline V3 gatherQuery(const SppmDev& g, const DevScene& s, bool has, V3 x, V3 ns, V3 ng, int mat,
+                                 float r, int* M, float4* partials, uint2* runs) {
+  const float cs = *g.cellSize;
   {
     // candidates of this lane's own query; if no lane of the wave has many,
     // every lane gathers alone in the reference's order
-    const float cs = *g.cellSize;
-    int3 lo = cellOf(it.x - v3(r, r, r), cs), hi = cellOf(it.x + v3(r, r, r), cs);
-    lo = make_int3(max(lo.x, g.bbox[0]), max(lo.y, g.bbox[1]), max(lo.z, g.bbox[2]));
-    hi = make_int3(min(hi.x, g.bbox[3]), min(hi.y, g.bbox[4]), min(hi.z, g.bbox[5]));
     uint32_t cand = 0;
-    for (int i = lo.x; i <= hi.x; ++i)
-      for (int j = lo.y; j <= hi.y; ++j)
-        for (int k = lo.z; k <= hi.z; ++k) cand += cellLookup(g, packKey(make_int3(i, j, k))).y;
-    if (__ballot(cand > kCoopCandidates) == 0ull) {
-      const DevMaterial m = ldgStruct(s.mats + it.mat);
-      const V3 ns = shadingNormal(it, wo);
-      V3 est = v3(0.0f, 0.0f, 0.0f);
-      int found = 0;
+    int3 lo = make_int3(0, 0, 0), hi = make_int3(-1, -1, -1);
+    if (has) {
+      lo = cellOf(x - v3(r, r, r), cs), hi = cellOf(x + v3(r, r, r), cs);
+      lo = make_int3(max(lo.x, g.bbox[0]), max(lo.y, g.bbox[1]), max(lo.z, g.bbox[2]));
+      hi = make_int3(min(hi.x, g.bbox[3]), min(hi.y, g.bbox[4]), min(hi.z, g.bbox[5]));
       for (int i = lo.x; i <= hi.x; ++i)
         for (int j = lo.y; j <= hi.y; ++j)
-          for (int k = lo.z; k <= hi.z; ++k) {
-            const uint2 cell = cellLookup(g, packKey(make_int3(i, j, k)));
-            photonRun(g, cell.x, cell.x + cell.y, 1u, it.x, r * r, ns, it.ng, m, &est, &found);
-          }
+          for (int k = lo.z; k <= hi.z; ++k) cand += cellLookup(g, packKey(make_int3(i, j, k))).y;
+    }
+    if (__ballot(cand > kCoopCandidates) == 0ull) {
+      V3 est = v3(0.0f, 0.0f, 0.0f);
+      int found = 0;
+      if (has) {
+        const DevMaterial m = ldgStruct(s.mats + mat);
+        for (int i = lo.x; i <= hi.x; ++i)
+          for (int j = lo.y; j <= hi.y; ++j)
+            for (int k = lo.z; k <= hi.z; ++k) {
+              const uint2 cell = cellLookup(g, packKey(make_int3(i, j, k)));
+              photonRun(g, cell.x, cell.x + cell.y, 1u, x, r * r, ns, ng, m, &est, &found);
+            }
+      }
       *M = found;
       return est;
     }
   }
-  const uint64_t active = __ballot(1);
+  // shared gather: every lane of the wave is a rank (the caller keeps the
+  // whole wave here), so a query's partial sums do not depend on which
+  // other queries share its wave
   const uint32_t lane = __lane_id();
-  const int rank = __popcll(active & ((1ull << lane) - 1ull));
-  const int nAct = __popcll(active);
-  const V3 nsMine = shadingNormal(it, wo);
-  const float cs = *g.cellSize;
+  const int rank = (int)lane, nAct = 64;
   V3 myEst = v3(0.0f, 0.0f, 0.0f);
   int myM = 0;
-  uint64_t todo = active;
+  uint64_t todo = __ballot(has);
   while (todo) {
     const int owner = __ffsll((unsigned long long)todo) - 1;
     todo &= todo - 1ull;
-    const V3 c = v3(__shfl(it.x.x, owner), __shfl(it.x.y, owner), __shfl(it.x.z, owner));
-    const V3 ns = v3(__shfl(nsMine.x, owner), __shfl(nsMine.y, owner), __shfl(nsMine.z, owner));
-    const V3 ng = v3(__shfl(it.ng.x, owner), __shfl(it.ng.y, owner), __shfl(it.ng.z, owner));
+    const V3 c = v3(__shfl(x.x, owner), __shfl(x.y, owner), __shfl(x.z, owner));
+    const V3 nso = v3(__shfl(ns.x, owner), __shfl(ns.y, owner), __shfl(ns.z, owner));
+    const V3 ngo = v3(__shfl(ng.x, owner), __shfl(ng.y, owner), __shfl(ng.z, owner));
     const float rr = __shfl(r, owner);
-    const DevMaterial m = ldgStruct(s.mats + __shfl(it.mat, owner));
+    const DevMaterial m = ldgStruct(s.mats + __shfl(mat, owner));
     const float r2 = rr * rr;
     int3 lo = cellOf(c - v3(rr, rr, rr), cs), hi = cellOf(c + v3(rr, rr, rr), cs);
     lo = make_int3(max(lo.x, g.bbox[0]), max(lo.y, g.bbox[1]), max(lo.z, g.bbox[2]));
     hi = make_int3(min(hi.x, g.bbox[3]), min(hi.y, g.bbox[4]), min(hi.z, g.bbox[5]));
     V3 acc = v3(0.0f, 0.0f, 0.0f);
     int found = 0;
-    // the cells of the box in x -> y -> z order, nAct at a time: lane rank
-    // looks up cell base + rank
+    // the cells of the box in x -> y -> z order, 64 at a time: lane rank
+    // looks up cell base + rank (all probes in flight together)
     const int ny = max(hi.y - lo.y + 1, 0), nz = max(hi.z - lo.z + 1, 0);
     const int nyz = ny * nz, nc = max(hi.x - lo.x + 1, 0) * nyz;
     for (int base = 0; base < nc; base += nAct) {
@@ -572,7 +581,7 @@ __device__ inline V3 gatherCoop(const SppmDev& g, const DevScene& s, const Inter
       const int nb = min(nAct, nc - base);
       for (int t = 0; t < nb; ++t) {
         const uint2 cell = runs[t];
-        photonRun(g, cell.x + (uint32_t)rank, cell.x + cell.y, (uint32_t)nAct, c, r2, ns, ng, m, &acc, &found);
+        photonRun(g, cell.x + (uint32_t)rank, cell.x + cell.y, (uint32_t)nAct, c, r2, nso, ngo, m, &acc, &found);
       }
       __builtin_amdgcn_wave_barrier();  // runs are rewritten for the next cells
     }
@@ -616,8 +625,6 @@ __global__ void __launch_bounds__(kB) HASTY_SPPM_CAMERA_ATTR kSppmCamera(DevScen
                                                   float alpha, int emittedInt, float* color, float* normalOut,
                                                   float* albedoOut) {
   __shared__ uint2 stack[kStackLds * kB];
-  __shared__ float4 gatherPartials[kB];
-  __shared__ uint2 gatherRuns[kB];
   uint32_t P = s.width * s.height;
   uint32_t gtid = xcdRemap() * kB + threadIdx.x;
   uint2* ovf = g.ovf + (size_t)gtid * kStackOvf;
@@ -627,13 +634,10 @@ __global__ void __launch_bounds__(kB) HASTY_SPPM_CAMERA_ATTR kSppmCamera(DevScen
     hrng::Stream rng = hrng::make_stream(s.seed, p, pass, hrng::kDomainPath);
     V3 ro, rd;
     cameraRay(s, rng, (float)i + offX, (float)j + offY, &ro, &rd);
-    float pxRadius = g.radius[p];
-    if (pxRadius == -1.0f) pxRadius = g.radii[p];
-    float radius = pxRadius;
     V3 nrm = v3(0.0f, 0.0f, 0.0f), alb = v3(0.0f, 0.0f, 0.0f);
-    V3 LL0 = v3(0.0f, 0.0f, 0.0f), LR0 = v3(0.0f, 0.0f, 0.0f), LS0 = v3(0.0f, 0.0f, 0.0f), F0 = v3(0.0f, 0.0f, 0.0f);
+    V3 LL0 = v3(0.0f, 0.0f, 0.0f), LR0 = v3(0.0f, 0.0f, 0.0f), LS0 = v3(0.0f, 0.0f, 0.0f);
     V3 WS = v3(1.0f, 1.0f, 1.0f), WR = v3(1.0f, 1.0f, 1.0f), WLR = v3(0.0f, 0.0f, 0.0f);
-    int Mtot = 0;
+    uint32_t nq = 0;
     Media md;
     md.n = 0;
     md.overflow = false;
@@ -673,11 +677,15 @@ __global__ void __launch_bounds__(kB) HASTY_SPPM_CAMERA_ATTR kSppmCamera(DevScen
         break;
       }
       if (m.hasDiffuseLobe) {
-        int M = 0;
-        V3 est = gatherCoop(g, s, it, wo, radius, &M, gatherPartials + (threadIdx.x & ~63u),
-                           gatherRuns + (threadIdx.x & ~63u));
-        Mtot += M;
-        F0 = F0 + mul(WR, mul(Tb, est));
+        // photon gather of this vertex: recorded for kSppmGather (at most
+        // one per depth, so nq < maxDepth)
+        const V3 ns = shadingNormal(it, wo);
+        float4* qr = g.query + 4 * ((size_t)nq * P + p);
+        qr[0] = make_float4(it.x.x, it.x.y, it.x.z, __int_as_float(it.mat));
+        qr[1] = make_float4(ns.x, ns.y, ns.z, Tb.x);
+        qr[2] = make_float4(it.ng.x, it.ng.y, it.ng.z, Tb.y);
+        qr[3] = make_float4(WR.x, WR.y, WR.z, Tb.z);
+        ++nq;
       }
       float rrQ = depth >= s.rrStart ? s.rrQ : 0.0f;
       if (rng.uniform01f() <= rrQ) break;
@@ -783,8 +791,69 @@ __global__ void __launch_bounds__(kB) HASTY_SPPM_CAMERA_ATTR kSppmCamera(DevScen
       rd = bs.dir;
       h = h2;
     }
+    const V3 L = LL0 + LR0 + LS0;
+    g.Lpath[p] = L.x, g.Lpath[P + p] = L.y, g.Lpath[2 * P + p] = L.z;
+    g.nq[p] = nq;
+    normalOut[p * 3] = nrm.x, normalOut[p * 3 + 1] = nrm.y, normalOut[p * 3 + 2] = nrm.z;
+    albedoOut[p * 3] = alb.x, albedoOut[p * 3 + 1] = alb.y, albedoOut[p * 3 + 2] = alb.z;
+  }
+}
+
+// Occupancy target of the gather kernel (waves per SIMD; 0: the compiler's
+// choice).
+#ifndef HASTY_SPPM_GATHER_WAVES
+#define HASTY_SPPM_GATHER_WAVES 0
+#endif
+#if HASTY_SPPM_GATHER_WAVES > 0
+#define HASTY_SPPM_GATHER_ATTR __attribute__((amdgpu_waves_per_eu(HASTY_SPPM_GATHER_WAVES)))
+#else
+#define HASTY_SPPM_GATHER_ATTR
+#endif
+// The photon gathers of the pass's camera paths (photonMapFluxEstimate,
+// PhotonMapping.cpp:271-296, at each diffuse vertex of
+// pathTraceWithCausticsMap) and the pixel statistics update.  A wave holds 64
+// consecutive pixels; round v gathers every pixel's v-th query, and all 64
+// lanes stay in the loop so that the shared gather always has 64 ranks.
+// F_R = sum over the vertices in path order of W_R (Tb est), as the camera
+// path would have summed it.
+__global__ void __launch_bounds__(kB) HASTY_SPPM_GATHER_ATTR kSppmGather(DevScene s, SppmDev g, float alpha,
+                                                                        int emittedInt, float* color) {
+  __shared__ float4 gatherPartials[kB];
+  __shared__ uint2 gatherRuns[kB];
+  const uint32_t P = s.width * s.height;
+  for (uint32_t base = blockIdx.x * kB; base < P; base += gridDim.x * kB) {
+    const uint32_t p = base + threadIdx.x;
+    const bool valid = p < P;
+    float pxRadius = 0.0f;
+    uint32_t nq = 0;
+    if (valid) {
+      pxRadius = g.radius[p];
+      if (pxRadius == -1.0f) pxRadius = g.radii[p];
+      nq = g.nq[p];
+    }
+    float radius = pxRadius;
+    uint32_t rounds = nq;
+    for (int off = 32; off > 0; off >>= 1) rounds = max(rounds, (uint32_t)__shfl_xor((int)rounds, off));
+    V3 F0 = v3(0.0f, 0.0f, 0.0f);
+    int Mtot = 0;
+    for (uint32_t v = 0; v < rounds; ++v) {
+      const bool has = v < nq;
+      float4 q0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), q1 = q0, q2 = q0, q3 = q0;
+      if (has) {
+        const float4* qr = g.query + 4 * ((size_t)v * P + p);
+        q0 = qr[0], q1 = qr[1], q2 = qr[2], q3 = qr[3];
+      }
+      int M = 0;
+      const V3 est = gatherQuery(g, s, has, f4xyz(q0), f4xyz(q1), f4xyz(q2), __float_as_int(q0.w), radius, &M,
+                                 gatherPartials + (threadIdx.x & ~63u), gatherRuns + (threadIdx.x & ~63u));
+      if (has) {
+        Mtot += M;
+        F0 = F0 + mul(f4xyz(q3), mul(v3(q1.w, q2.w, q3.w), est));
+      }
+    }
+    if (!valid) continue;
     // PMRenderJob::unlockAndUpdateStatistic (PhotonMapping.h:63-105)
-    V3 L = LL0 + LR0 + LS0;
+    const V3 L = v3(g.Lpath[p], g.Lpath[P + p], g.Lpath[2 * P + p]);
     bool bad = L.x != L.x || L.y != L.y || L.z != L.z || F0.x != F0.x || F0.y != F0.y || F0.z != F0.z;
     V3 LSum = v3(g.LSum[p], g.LSum[P + p], g.LSum[2 * P + p]);
     int Nd = g.Nd[p];
@@ -826,8 +895,6 @@ __global__ void __launch_bounds__(kB) HASTY_SPPM_CAMERA_ATTR kSppmCamera(DevScen
     }
     g.radii[p] = radius;
     color[p * 3] = Lnew.x, color[p * 3 + 1] = Lnew.y, color[p * 3 + 2] = Lnew.z;
-    normalOut[p * 3] = nrm.x, normalOut[p * 3 + 1] = nrm.y, normalOut[p * 3 + 2] = nrm.z;
-    albedoOut[p * 3] = alb.x, albedoOut[p * 3 + 1] = alb.y, albedoOut[p * 3 + 2] = alb.z;
   }
 }
 
@@ -868,6 +935,10 @@ SppmRenderer::SppmRenderer(const DevScene& scene, const SppmParams& params, hipS
   size_t P = (size_t)W_ * H_;
   dPixel_ = alloc<float>(P * 10);
   dRadii_ = alloc<float>(P);
+  // gather queries: at most one per path depth and pixel
+  dQuery_ = alloc<float4>((size_t)4 * std::max(1, s_.maxDepth) * P);
+  dNq_ = alloc<uint32_t>(P);
+  dLpath_ = alloc<float>(3 * P);
   dCounters_ = alloc<uint32_t>(8);
   dCellSize_ = alloc<float>(4);
   dCand_ = alloc<float>(16);
@@ -1009,6 +1080,9 @@ void SppmRenderer::pass(uint32_t passIndex, float offX, float offY, float* dColo
   g.cellSize = dCellSize_;
   g.counters = dCounters_;
   g.ovf = dOverflow_;
+  g.query = dQuery_;
+  g.nq = dNq_;
+  g.Lpath = dLpath_;
 
   // 1. cell size
   if (timer) timer->begin(kKSppmCellSize, stream_);
@@ -1086,12 +1160,18 @@ void SppmRenderer::pass(uint32_t passIndex, float offX, float offY, float* dColo
   HC(hipGetLastError());
   if (timer) timer->end(kKSppmBuild, stream_);
 
-  // 4. camera paths, gathers, statistics
+  // 4. camera paths (recording their gathers), then the gathers and the
+  // statistics
   if (timer) timer->begin(kKSppmCamera, stream_);
   hipLaunchKernelGGL(kSppmCamera, dim3(traceBlocks_), dim3(kB), 0, stream_, s_, g, passIndex, offX, offY, p_.alpha,
                      (int)emitted, dColor, dNormal, dAlbedo);
   HC(hipGetLastError());
   if (timer) timer->end(kKSppmCamera, stream_);
+  if (timer) timer->begin(kKSppmGather, stream_);
+  hipLaunchKernelGGL(kSppmGather, dim3((uint32_t)std::min<size_t>((P + kB - 1) / kB, 16384)), dim3(kB), 0, stream_,
+                     s_, g, p_.alpha, (int)emitted, dColor);
+  HC(hipGetLastError());
+  if (timer) timer->end(kKSppmGather, stream_);
   uint32_t c[4];
   HC(hipMemcpyAsync(c, dCounters_, sizeof(c), hipMemcpyDeviceToHost, stream_));
   float cs;
