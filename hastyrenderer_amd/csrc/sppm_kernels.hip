@@ -602,10 +602,9 @@ __device__ inline V3 gatherQuery(const SppmDev& g, const DevScene& s, bool has, 
   return myEst;
 }
 
-// Occupancy target of the camera megakernel (waves per SIMD; 0: the
-// compiler's choice, 208 VGPRs = 2 waves).  At 4 waves it spills ~350 B per
-// lane (path state held across the gathers) and is still the fastest on the
-// 1920x1080 caustic box: the gathers wait on memory.
+// Occupancy target of the camera kernel (waves per SIMD; 0: the compiler's
+// choice).  At 4 waves it spills ~200 B per lane and is still the fastest
+// on the 1920x1080 caustic box (3, 5 and 6 measured slower).
 #ifndef HASTY_SPPM_CAMERA_WAVES
 #define HASTY_SPPM_CAMERA_WAVES 4
 #endif
