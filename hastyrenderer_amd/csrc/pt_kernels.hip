@@ -1071,7 +1071,10 @@ void renderJobs(const std::vector<RenderJobDesc>& jobs, const PathBuffers& pb, c
       if (g) (void)hipGraphDestroy(g);
     }
   } graphFree{graph, exec};
-  if (cfg.useGraph) {
+  // Per-kernel event timing needs eager launches (HIP gives no elapsed time
+  // for events recorded inside a captured graph), so a timed render does not
+  // replay graphs.
+  if (cfg.useGraph && !timer) {
     hipOk(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
     if (nJobs > 1) launchChunk<true, false>(js, pb, cfg, chunk, stream, nullptr);
     else if (guide) launchChunk<false, true>(js, pb, cfg, chunk, stream, nullptr);

@@ -72,6 +72,7 @@ def test_graph_replay_equals_eager(cornell_job):
     sc = ha.Scene(cornell_job)
     sc.set_settings(width=128, height=96)
     sc.set_launch(path_slots=4000)  # many chunks: the graph is replayed many times
+    sc.set_launch_ex(use_graph=0)
     eager = _alone(sc, 3, 0, 12)
     sc.set_launch_ex(use_graph=1)
     g1 = _alone(sc, 3, 0, 12)
