@@ -225,8 +225,9 @@ __device__ inline void startPath(const DevScene& s, const PathBuffers& pb, const
 // register budget each may use is 512 / waves VGPRs.  0 leaves it to the
 // compiler.  Extend at 8 waves (64 VGPRs) is ~7% faster than at the
 // compiler's 7; shade at 3 keeps the guided variant (which would take 190
-// VGPRs) at the plain one's occupancy without spills; squeezing connect or
-// shade further loses.
+// VGPRs) at the plain one's occupancy without spills (4 and 5 lose); connect
+// at 5 (96 VGPRs, 28 B of spills) is ~1% faster than the compiler's 4, at 6
+// slower.
 #ifndef HASTY_EXTEND_WAVES
 #define HASTY_EXTEND_WAVES 8
 #endif
@@ -234,7 +235,7 @@ __device__ inline void startPath(const DevScene& s, const PathBuffers& pb, const
 #define HASTY_SHADE_WAVES 3
 #endif
 #ifndef HASTY_CONNECT_WAVES
-#define HASTY_CONNECT_WAVES 0
+#define HASTY_CONNECT_WAVES 5
 #endif
 #define HASTY_WAVES_ATTR_(n) __attribute__((amdgpu_waves_per_eu(n)))
 #if HASTY_EXTEND_WAVES > 0
