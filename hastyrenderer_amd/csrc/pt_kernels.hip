@@ -333,8 +333,11 @@ __device__ inline uint32_t blockPrefix(bool pred, uint32_t* ldsWave, uint32_t* t
 // slots, takes offsets for all of them with one atomic on the fill cursor
 // and a qA range for those it can start with one more, then starts them in
 // slot order.  (Two atomics per block, instead of two per 256 slots: same-
-// address atomics from every block were what bounded this kernel.)
-constexpr uint32_t kFillBlocks = 2048;
+// address atomics from every block were what bounded this kernel.)  8192
+// blocks (1024 slots each at 2^23 slots) measured fastest: 2048 and 4096
+// leave too few waves to hide the mark reads, 16384 and more pay for the
+// atomics again.
+constexpr uint32_t kFillBlocks = 8192;
 template <bool MULTI>
 __global__ void __launch_bounds__(kBlock) kFill(JobSet js, PathBuffers pb) {
   __shared__ uint32_t ldsWave[kBlock / 64];
