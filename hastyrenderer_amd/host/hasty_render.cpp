@@ -3,8 +3,8 @@
 //   hasty_render [--sppm] <renderJobFile> <outputDirectory>
 // starts numThreads render threads with seeds 0..numThreads-1, waits for
 // them, and writes output.pfm, outputNormal.pfm and outputAlbedo.pfm (the
-// reference writes EXR through OpenImageIO and then denoises; neither
-// library is part of this build).
+// reference writes EXR through OpenImageIO and then denoises; this renderer
+// links neither library).
 #include <stdio.h>
 #include <string.h>
 #include <sys/stat.h>
