@@ -111,6 +111,22 @@ def test_sppm_shared_gather_matches_oracle(caustic_small_job):
             np.testing.assert_allclose(g.pixels()["N"].ravel(), ref.pixels(w, h)[:, 1], rtol=1e-6)
 
 
+def test_sppm_ragged_image_matches_oracle(caustic_small_job):
+    """97 x 53 pixels (not a multiple of 64): the gather kernel's last wave
+    has lanes without a pixel, which still serve the shared gathers."""
+    w, h = 97, 53
+    kw = dict(alpha=0.4, footprint_scale=20.0, store_fraction=16.0, emit_fraction=4000.0)
+    sc, g = _gpu_sppm(caustic_small_job, seed=5, w=w, h=h, **kw)
+    ref = po.OracleSppm(caustic_small_job, seed=5, width=w, height=h, **kw)
+    c, n, a = g.pass_(0)
+    rc, rn, ra = ref.run_pass(0, w, h)
+    assert g.info()["stored"] == ref.info()["stored"]
+    ok = rel_ok(c, rc)
+    print(f"{ok.mean():.5f} of pixels within 1e-3", flush=True)
+    assert ok.mean() >= 0.999
+    np.testing.assert_allclose(g.pixels()["N"].ravel(), ref.pixels(w, h)[:, 1], rtol=1e-6)
+
+
 def test_sppm_caustic_closed_form(tmp_path):
     """Mirror caustic on a diffuse ceiling: rho / pi * E(x) within 1%."""
     w = h = 64
