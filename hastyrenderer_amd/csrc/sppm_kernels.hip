@@ -506,7 +506,7 @@ __device__ inline void photonRun(const SppmDev& g, uint32_t first, uint32_t end,
 // Lanes whose cells hold fewer candidates than this gather on their own (a
 // shared gather costs every lane each owner's cell lookups).
 #ifndef HASTY_COOP_CANDIDATES
-#define HASTY_COOP_CANDIDATES 1024
+#define HASTY_COOP_CANDIDATES 512
 #endif
 constexpr uint32_t kCoopCandidates = HASTY_COOP_CANDIDATES;
 __device__ inline V3 gatherQuery(const SppmDev& g, const DevScene& s, bool has, V3 x, V3 ns, V3 ng, int mat,

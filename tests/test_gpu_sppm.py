@@ -89,7 +89,7 @@ def test_sppm_cornell_matches_oracle(cornell_job):
 
 def test_sppm_shared_gather_matches_oracle(caustic_small_job):
     """Dense photon maps (16 x W H photons stored, radius 20x the footprint):
-    waves with a query of more than 1024 candidate photons take the shared
+    waves with a query of more than 512 candidate photons take the shared
     gather (lanes split the owner's cell lookups and photons); the image and
     the photon counts still follow the oracle's sequential gather."""
     w, h = 96, 54
