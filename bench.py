@@ -506,8 +506,11 @@ def main():
         render()
         log("warmup %d: %.3f s" % (i, time.perf_counter() - t))
 
+    # timed steps: HIP events around the extension launches only (the
+    # roofline's kernel), so that the events of the other launches add
+    # nothing to the timed region
     scene.reset_stats()
-    scene.set_profiling(events=True, counters=False)
+    scene.set_profiling(events=True, counters=False, extend_only=True)
     hip.sync()
     t_start = time.perf_counter()
     for i in range(args.steps):
@@ -516,6 +519,11 @@ def main():
         log("step %d: %.3f s" % (i, time.perf_counter() - t))
     hip.sync()
     elapsed = time.perf_counter() - t_start
+    kstats_timed = scene.kernel_stats()
+    # per-kernel breakdown: one more, untimed render with every launch timed
+    scene.set_profiling(events=True, counters=False)
+    scene.reset_stats()
+    render()
     kstats = scene.kernel_stats()
     scene.set_profiling(events=False, counters=True)
     scene.reset_stats()
@@ -536,7 +544,7 @@ def main():
         log("cpu baseline %.4g paths/s" % base["value"])
 
     roofline = None
-    launches, ext_ms = kstats.get("pt_extend", (0, 0.0))
+    launches, ext_ms = kstats_timed.get("pt_extend", (0, 0.0))
     if canon and launches:
         b_ray = (64.0 * canon["node_visits"] + 48.0 * canon["tri_tests"]) / canon["rays"]
         ext_rays_per_render = counters["extension_rays"]

@@ -329,8 +329,11 @@ class Scene:
                                        tri.ctypes.data_as(C.POINTER(C.c_int32)), _fptr(tuv)))
         return gp, tri, tuv
 
-    def set_profiling(self, events=True, counters=False):
-        _err(lib().hasty_set_profiling(self._h, (1 if events else 0) | (2 if counters else 0)))
+    def set_profiling(self, events=True, counters=False, extend_only=False):
+        """HIP-event timing of every launch (of the extension launches only
+        with extend_only) and/or device counters."""
+        _err(lib().hasty_set_profiling(self._h, (1 if events else 0) | (2 if counters else 0) |
+                                       (4 if extend_only else 0)))
 
     def reset_stats(self):
         _err(lib().hasty_reset_stats(self._h))

@@ -81,12 +81,15 @@ struct EventTimer : hasty::KernelTimer {
     HIPCHECK(hipEventCreate(&e));
     return e;
   }
+  int only = -1;  // >= 0: time only this kernel id
   void begin(int k, hipStream_t s) override {
+    if (only >= 0 && k != only) return;
     hipEvent_t e = get();
     HIPCHECK(hipEventRecord(e, s));
     open[k] = e;
   }
   void end(int k, hipStream_t s) override {
+    if (only >= 0 && k != only) return;
     hipEvent_t e = get();
     HIPCHECK(hipEventRecord(e, s));
     pending.push_back({k, open[k], e});
@@ -758,6 +761,10 @@ int hasty_set_profiling(HastyScene* s, int flags) {
   if (!s) return 1;
   s->profiling = (flags & 1) != 0;
   s->counting = (flags & 2) != 0;
+  if (s->profiling) {
+    if (!s->timer) s->timer = std::make_unique<EventTimer>();
+    s->timer->only = (flags & 4) != 0 ? hasty::kKExtend : -1;
+  }
   return 0;
 }
 

@@ -91,8 +91,10 @@ int hasty_render_pt_device(HastyScene* scene, uint32_t seed, uint32_t first_samp
 int hasty_trace_closest(HastyScene* scene, const float* rays, uint32_t n, uint32_t* geom_prim, int32_t* tri,
                         float* tuv);
 
-/* Per-kernel timing with HIP events on the scene's stream (flags bit 0) and
- * device counters (flags bit 1, see hasty_counters). */
+/* Per-kernel timing with HIP events on the scene's stream (flags bit 0;
+ * with bit 2 only the closest-hit extension launches are timed, which keeps
+ * the events' own cost out of a timed render) and device counters (flags
+ * bit 1, see hasty_counters). */
 typedef struct HastyKernelStat {
   char name[32];
   uint64_t launches;
