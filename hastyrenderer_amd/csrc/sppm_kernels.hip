@@ -463,17 +463,16 @@ __device__ inline uint2 cellLookup(const SppmDev& g, unsigned long long key) {
 // diffuse lobe only with outside IOR 1.
 //
 // When some lane of the wave has many candidate photons (dense maps, large
-// radii), the lanes that reach the call together serve each other's
-// queries one owner at a time, every lane testing a share of the owner's
-// photons (rank k of n takes photons k, k + n, ... of each cell, so a
-// cell's run is read in whole cache lines); the owner adds the lanes'
-// partial sums in rank order.  The set and the count of photons are the
-// reference's; only the FP32 association of the flux sum differs, the same
-// way on every run.  Otherwise each lane sums its own photons in the
-// reference's order.  In the shared gather the owner's cell lookups are
-// spread over the lanes (one hash probe each, all in flight together) and
-// the cells' photon runs read back from LDS in cell order.  `partials` and
-// `runs` are this wave's 64-entry LDS scratch.
+// radii), the 64 lanes of the wave serve its queries one owner at a time,
+// every lane testing a share of the owner's photons (rank k of 64 takes
+// photons k, k + 64, ... of each cell, so a cell's run is read in whole
+// cache lines), and the lanes' partial sums are added by a butterfly over
+// the wave.  The set and the count of photons are the reference's; only the
+// FP32 association of the flux sum differs, the same way on every run.
+// Otherwise each lane sums its own photons in the reference's order.  In the
+// shared gather the owner's cell lookups are spread over the lanes (one hash
+// probe each, all in flight together) and the cells' photon runs read back
+// from LDS in cell order; `runs` is this wave's 64-entry LDS scratch.
 __device__ inline void photonTerm(const SppmDev& g, uint32_t q, float4 x4, V3 c, float r2, V3 ns, V3 ng,
                                   const DevMaterial& m, V3* acc, int* found) {
   const V3 dlt = v3(x4.x, x4.y, x4.z) - c;
@@ -510,7 +509,7 @@ __device__ inline void photonRun(const SppmDev& g, uint32_t first, uint32_t end,
 #endif
 constexpr uint32_t kCoopCandidates = HASTY_COOP_CANDIDATES;
 __device__ inline V3 gatherQuery(const SppmDev& g, const DevScene& s, bool has, V3 x, V3 ns, V3 ng, int mat,
-                                 float r, int* M, float4* partials, uint2* runs) {
+                                 float r, int* M, uint2* runs) {
   const float cs = *g.cellSize;
   {
     // candidates of this lane's own query; if no lane of the wave has many,
@@ -585,18 +584,19 @@ __device__ inline V3 gatherQuery(const SppmDev& g, const DevScene& s, bool has, 
       }
       __builtin_amdgcn_wave_barrier();  // runs are rewritten for the next cells
     }
-    partials[rank] = make_float4(acc.x, acc.y, acc.z, __int_as_float(found));
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    if ((int)lane == owner) {
-      for (int t = 0; t < nAct; ++t) {
-        const float4 pp = partials[t];
-        myEst = myEst + v3(pp.x, pp.y, pp.z);
-        myM += __float_as_int(pp.w);
-      }
+    // the 64 partial sums by a butterfly: each step adds the same two
+    // operands on both lanes of a pair (in either order), so every lane ends
+    // with the same total
+    for (int off = 32; off > 0; off >>= 1) {
+      acc.x += __shfl_xor(acc.x, off);
+      acc.y += __shfl_xor(acc.y, off);
+      acc.z += __shfl_xor(acc.z, off);
+      found += __shfl_xor(found, off);
     }
-    __builtin_amdgcn_wave_barrier();  // partials are rewritten for the next owner
+    if ((int)lane == owner) {
+      myEst = acc;
+      myM = found;
+    }
   }
   *M = myM;
   return myEst;
@@ -817,7 +817,6 @@ __global__ void __launch_bounds__(kB) HASTY_SPPM_CAMERA_ATTR kSppmCamera(DevScen
 // path would have summed it.
 __global__ void __launch_bounds__(kB) HASTY_SPPM_GATHER_ATTR kSppmGather(DevScene s, SppmDev g, float alpha,
                                                                         int emittedInt, float* color) {
-  __shared__ float4 gatherPartials[kB];
   __shared__ uint2 gatherRuns[kB];
   const uint32_t P = s.width * s.height;
   for (uint32_t base = blockIdx.x * kB; base < P; base += gridDim.x * kB) {
@@ -844,7 +843,7 @@ __global__ void __launch_bounds__(kB) HASTY_SPPM_GATHER_ATTR kSppmGather(DevScen
       }
       int M = 0;
       const V3 est = gatherQuery(g, s, has, f4xyz(q0), f4xyz(q1), f4xyz(q2), __float_as_int(q0.w), radius, &M,
-                                 gatherPartials + (threadIdx.x & ~63u), gatherRuns + (threadIdx.x & ~63u));
+                                 gatherRuns + (threadIdx.x & ~63u));
       if (has) {
         Mtot += M;
         F0 = F0 + mul(f4xyz(q3), mul(v3(q1.w, q2.w, q3.w), est));
