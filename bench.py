@@ -82,6 +82,31 @@ class Hip:
         self.lib.hipFree(C.c_void_p(p))
 
 
+def host_cpus():
+    """What the host offers this process: the CPU model (/proc/cpuinfo), the
+    logical CPUs of the machine, those this process may run on (affinity),
+    and the worker count the environment allots (OMP_NUM_THREADS, else the
+    affinity count)."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        affinity = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        affinity = os.cpu_count() or 1
+    share = affinity
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.isdigit() and int(env) > 0:
+        share = min(share, int(env))
+    return {"model": model, "logical_cpus": os.cpu_count() or 1, "affinity_cpus": affinity, "share": share}
+
+
 def cpu_baseline(job_path, seed, threads, spp):
     """Oracle timing on the host (bounded sample: `spp` passes of the 1024^2
     film) and the canonical BVH2 counts (2^16 paths: 128^2 film of the same
@@ -96,6 +121,7 @@ def cpu_baseline(job_path, seed, threads, spp):
     paths = info["width"] * info["height"] * spp
     oj.set(width=128, height=128)
     _, cnt = oj.render_counted(0, 4)
+    hc = host_cpus()
     return {
         "value": paths / dt,
         "unit": "paths/s",
@@ -103,12 +129,18 @@ def cpu_baseline(job_path, seed, threads, spp):
         "kind": "port",
         "sample": "%d spp of the %dx%d film (%d paths) in %.2f s, %d host threads" % (
             spp, info["width"], info["height"], paths, dt, threads),
+        "cpu_model": hc["model"],
+        "host_logical_cpus": hc["logical_cpus"],
+        "affinity_cpus": hc["affinity_cpus"],
     }, cnt
 
 
 def measured_traffic():
     """HBM bytes per pt_extend launch from the committed rocprofv3 summary
-    (line 'traffic_bytes_per_launch: N'), or None."""
+    (line 'traffic_bytes_per_launch: N'), or None.  rocprofv3 counters need a
+    run of their own, so this figure comes from the profiles/ summary of an
+    earlier counter run of the same command, not from this run; the bench line
+    says so beside it."""
     try:
         with open(TRAFFIC_SUMMARY) as f:
             for line in f:
@@ -409,7 +441,9 @@ def main():
     ap.add_argument("--spp", type=int, default=256)
     ap.add_argument("--scene-dir", default=None)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
+    ap.add_argument("--cpu-threads", type=int, default=0,
+                    help="host threads of the CPU baseline (0: the CPUs this process may use, capped by "
+                         "OMP_NUM_THREADS when set)")
     ap.add_argument("--cpu-spp", type=int, default=48)
     ap.add_argument("--workload", choices=("ggx", "sppm", "batch", "guide"), default="ggx")
     ap.add_argument("--tune-spp", type=int, default=0,
@@ -437,6 +471,8 @@ def main():
                     help="constant initial radius (0: 10x pixel footprint as the reference)")
     args = ap.parse_args()
 
+    if args.cpu_threads <= 0:
+        args.cpu_threads = host_cpus()["share"]
     if args.gpus != 1:
         print("bench.py measures one GPU only: this path tracer renders on a single MI355X "
               "(multi-GPU sharding is a design note, not a measured configuration); run with --gpus 1.",
@@ -559,6 +595,8 @@ def main():
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": measured_traffic(),
+            "traffic_source": "rocprofv3 FETCH_SIZE/WRITE_SIZE of an earlier run of this command, read from "
+                              "profiles/pt_extend_ggx_grid_1024_traffic.md (not measured in this run)",
             "kernel": "pt_extend",
             "bytes_per_ray_canonical": round(b_ray, 1),
             "rays_per_launch": round(rays_per_launch, 1),
@@ -588,6 +626,9 @@ def main():
             "environment": "2048x1024 lat-long, importance sampled",
             "camera": "thin lens f/2.8, 8 blades",
             "launch": tuned,
+            # the timed steps record HIP events around the extension launches,
+            # and a render with events launches eagerly (no hipGraph replay)
+            "launch_timed": dict(tuned, use_graph=0),
             "parallelism": "single-gpu",
         },
         "rays_per_s": value * rays_per_path,

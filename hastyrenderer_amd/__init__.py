@@ -209,8 +209,19 @@ def _err(rc):
 
 
 def _fptr(a: np.ndarray):
-    assert a.dtype == np.float32 and a.flags["C_CONTIGUOUS"]
+    """float* of a C-contiguous float32 array (ValueError otherwise: the C side
+    reads and writes through it without knowing its size)."""
+    if not isinstance(a, np.ndarray) or a.dtype != np.float32 or not a.flags["C_CONTIGUOUS"]:
+        raise ValueError("expected a C-contiguous float32 numpy array")
     return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def _check_films(films, h, w):
+    if len(films) != 3:
+        raise ValueError("expected three films (color, normal, albedo)")
+    for f in films:
+        if not isinstance(f, np.ndarray) or f.shape != (h, w, 3):
+            raise ValueError("film shape %s, expected %s" % (getattr(f, "shape", None), (h, w, 3)))
 
 
 class Scene:
@@ -274,8 +285,7 @@ class Scene:
         h, w = info["height"], info["width"]
         if films is None:
             films = tuple(np.zeros((h, w, 3), np.float32) for _ in range(3))
-        for f in films:
-            assert f.shape == (h, w, 3)
+        _check_films(films, h, w)
         _err(lib().hasty_render_pt(self._h, seed, first_sample, num_samples, *(_fptr(f) for f in films)))
         return films
 
@@ -451,7 +461,8 @@ class Guide:
 
     def set_params(self, p):
         p = np.ascontiguousarray(p, np.float32)
-        assert p.size == self.n_params
+        if p.size != self.n_params:
+            raise ValueError("expected %d parameters, got %d" % (self.n_params, p.size))
         _err(lib().hasty_guide_set_params(self._h, _fptr(p)))
 
     def get_params(self):
@@ -506,8 +517,7 @@ class Sppm:
         h, w = self.height, self.width
         if films is None:
             films = tuple(np.zeros((h, w, 3), np.float32) for _ in range(3))
-        for f in films:
-            assert f.shape == (h, w, 3)
+        _check_films(films, h, w)
         _err(lib().hasty_sppm_pass(self._h, index, *(_fptr(f) for f in films)))
         return films
 

@@ -29,6 +29,29 @@ void check(hipError_t e, const char* what) {
 }
 #define HIPCHECK(x) check((x), #x)
 
+// Grow-only device buffer owned by a scene (freed with it).  Callers
+// synchronise their stream before a call that may grow it again.
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  void* ensure(size_t n) {
+    if (n > bytes) {
+      if (p) (void)hipFree(p);
+      p = nullptr;
+      bytes = 0;
+      HIPCHECK(hipMalloc(&p, n));
+      bytes = n;
+    }
+    return p;
+  }
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+};
+
 // Reference Halton point generator (core/include/Hasty/Random.h:14-63).
 struct Halton {
   uint32_t b, n = 0, d = 1;
@@ -160,6 +183,11 @@ struct HastyScene {
   // trace entry point
   uint2* traceOverflow = nullptr;
   uint32_t traceOverflowThreads = 0;
+  DevBuf traceIO;  // rays in, hits out of the host-pointer trace call
+
+  // Halton offsets of a render call and the job tables of a batch
+  DevBuf offsets;
+  DevBuf jobTables;
 
   // guided rendering
   HastyGuide* guide = nullptr;
@@ -326,28 +354,15 @@ static void renderBatch(HastyScene* owner, const BatchItem* items, uint32_t nIte
   for (uint32_t i = 0; i < nItems; ++i)
     allPaths += (uint64_t)items[i].n * items[i].scene->dev.width * items[i].scene->dev.height;
   owner->ensurePathBuffers(allPaths);
-  std::vector<void*> tmp;
-  struct FreeAll {
-    std::vector<void*>& v;
-    ~FreeAll() {
-      for (void* p : v) (void)hipFree(p);
-    }
-  } freeAll{tmp};
-  std::vector<hasty::RenderJobDesc> group;
-  uint64_t groupPaths = 0;
-  const uint64_t maxPaths = 0x7FFFFFFFull;
-  owner->launch.countStats = owner->counting;
-  if (owner->profiling && !owner->timer) owner->timer = std::make_unique<EventTimer>();
-  hasty::KernelTimer* timer = owner->profiling ? owner->timer.get() : nullptr;
-  auto flush = [&]() {
-    if (group.empty()) return;
-    hasty::renderJobs(group, owner->pb, owner->launch, owner->stream, timer);
-    HIPCHECK(hipGetLastError());
-    group.clear();
-    groupPaths = 0;
-    for (void* p : tmp) (void)hipFree(p);
-    tmp.clear();
+  // Split every item into calls of < 2^31 paths and lay out all Halton
+  // offsets in one upload to the owner's offset buffer.
+  struct Piece {
+    uint32_t item, first, n;
+    size_t off;
   };
+  std::vector<Piece> pieces;
+  std::vector<float2> offs;
+  const uint64_t maxPaths = 0x7FFFFFFFull;
   for (uint32_t i = 0; i < nItems; ++i) {
     const BatchItem& it = items[i];
     HastyScene* sc = it.scene;
@@ -358,26 +373,45 @@ static void renderBatch(HastyScene* owner, const BatchItem* items, uint32_t nIte
     uint32_t perCall = (uint32_t)std::max<uint64_t>(1, maxPaths / P);
     for (uint32_t done = 0; done < it.n; done += perCall) {
       uint32_t k = std::min(perCall, it.n - done);
-      if (groupPaths + (uint64_t)k * P > maxPaths) flush();
-      std::vector<float2> offs = haltonOffsets(it.first + done, k);
-      void* d = nullptr;
-      HIPCHECK(hipMalloc(&d, sizeof(float2) * std::max<uint32_t>(1, k)));
-      tmp.push_back(d);
-      HIPCHECK(hipMemcpy(d, offs.data(), sizeof(float2) * k, hipMemcpyHostToDevice));
-      hasty::RenderJobDesc jd;
-      jd.scene = sc->dev;
-      jd.scene.seed = it.seed;
-      jd.first = it.first + done;
-      jd.n = k;
-      jd.dOffsets = (const float2*)d;
-      jd.dTileOrder = sc->dTileOrder;
-      jd.dRankOf = sc->rankOf();
-      jd.dColor = it.color;
-      jd.dNormal = it.normal;
-      jd.dAlbedo = it.albedo;
-      group.push_back(jd);
-      groupPaths += (uint64_t)k * P;
+      pieces.push_back(Piece{i, it.first + done, k, offs.size()});
+      std::vector<float2> o = haltonOffsets(it.first + done, k);
+      offs.insert(offs.end(), o.begin(), o.end());
     }
+  }
+  float2* dOffs = (float2*)owner->offsets.ensure(sizeof(float2) * std::max<size_t>(1, offs.size()));
+  if (!offs.empty())
+    HIPCHECK(hipMemcpyAsync(dOffs, offs.data(), sizeof(float2) * offs.size(), hipMemcpyHostToDevice, owner->stream));
+  void* tables = owner->jobTables.ensure(hasty::jobTableBytes((uint32_t)std::max<size_t>(1, pieces.size())));
+  std::vector<hasty::RenderJobDesc> group;
+  uint64_t groupPaths = 0;
+  owner->launch.countStats = owner->counting;
+  if (owner->profiling && !owner->timer) owner->timer = std::make_unique<EventTimer>();
+  hasty::KernelTimer* timer = owner->profiling ? owner->timer.get() : nullptr;
+  auto flush = [&]() {
+    if (group.empty()) return;
+    hasty::renderJobs(group, owner->pb, owner->launch, owner->stream, timer, nullptr, tables);
+    HIPCHECK(hipGetLastError());
+    group.clear();
+    groupPaths = 0;
+  };
+  for (const Piece& pc : pieces) {
+    const BatchItem& it = items[pc.item];
+    HastyScene* sc = it.scene;
+    uint64_t P = (uint64_t)sc->dev.width * sc->dev.height;
+    if (groupPaths + (uint64_t)pc.n * P > maxPaths) flush();
+    hasty::RenderJobDesc jd;
+    jd.scene = sc->dev;
+    jd.scene.seed = it.seed;
+    jd.first = pc.first;
+    jd.n = pc.n;
+    jd.dOffsets = dOffs + pc.off;
+    jd.dTileOrder = sc->dTileOrder;
+    jd.dRankOf = sc->rankOf();
+    jd.dColor = it.color;
+    jd.dNormal = it.normal;
+    jd.dAlbedo = it.albedo;
+    group.push_back(jd);
+    groupPaths += (uint64_t)pc.n * P;
   }
   flush();
   HIPCHECK(hipStreamSynchronize(owner->stream));
@@ -453,32 +487,21 @@ static void renderGuided(HastyScene* s, const BatchItem& it) {
   s->launch.countStats = s->counting;
   if (s->profiling && !s->timer) s->timer = std::make_unique<EventTimer>();
   hasty::KernelTimer* timer = s->profiling ? s->timer.get() : nullptr;
-  void* dOff = nullptr;
-  HIPCHECK(hipMalloc(&dOff, sizeof(float2)));
-  struct FreeOff {
-    void* p;
-    ~FreeOff() { (void)hipFree(p); }
-  } freeOff{dOff};
   const uint64_t P = (uint64_t)s->dev.width * s->dev.height;
   const uint64_t groupPaths = training ? HastyScene::kGuideTrainPaths : 0x7FFFFFFFull;
   const uint32_t perCall = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(it.n, groupPaths / P));
-  if (perCall > 1) {
-    (void)hipFree(dOff);
-    freeOff.p = nullptr;
-    HIPCHECK(hipMalloc(&dOff, sizeof(float2) * perCall));
-    freeOff.p = dOff;
-  }
+  float2* dOff = (float2*)s->offsets.ensure(sizeof(float2) * perCall);
   for (uint32_t k = 0; k < it.n; k += perCall) {
     const uint32_t cnt = std::min(perCall, it.n - k);
     std::vector<float2> off = haltonOffsets(it.first + k, cnt);
-    HIPCHECK(hipMemcpy(dOff, off.data(), sizeof(float2) * cnt, hipMemcpyHostToDevice));
+    HIPCHECK(hipMemcpyAsync(dOff, off.data(), sizeof(float2) * cnt, hipMemcpyHostToDevice, s->stream));
     HIPCHECK(hipMemsetAsync(s->guideCount, 0, sizeof(uint32_t), s->stream));
     hasty::RenderJobDesc jd;
     jd.scene = s->dev;
     jd.scene.seed = it.seed;
     jd.first = it.first + k;
     jd.n = cnt;
-    jd.dOffsets = (const float2*)dOff;
+    jd.dOffsets = dOff;
     jd.dTileOrder = s->dTileOrder;
     jd.dRankOf = s->rankOf();
     jd.dColor = it.color;
@@ -506,10 +529,16 @@ static void renderGuided(HastyScene* s, const BatchItem& it) {
   if (s->profiling) s->timer->collect();
 }
 
+// The wavefront path state keeps a path's depth in 8 bits.
+static void checkSettings(const hasty::RenderSettings& st) {
+  if (st.maxDepth > 255) throw std::runtime_error("max_depth above 255 is not supported");
+}
+
 static void buildScene(HastyScene& S, const char* path) {
   hasty::loadJob(path, S.job);
   hasty::Scene& sc = S.job.scene;
   const hasty::RenderSettings& st = S.job.settings;
+  checkSettings(st);
   if (sc.shapeNames.size() >= 65536) throw std::runtime_error("more than 65535 shapes");
   if (sc.materials.size() >= 65536) throw std::runtime_error("more than 65535 materials");
   std::vector<hdev::DevMaterial> mats(sc.materials.size());
@@ -653,7 +682,7 @@ int hasty_scene_set_settings(HastyScene* s, const HastyRenderSettings* st) {
     g_lastError = "null argument";
     return 1;
   }
-  hasty::RenderSettings& r = s->job.settings;
+  hasty::RenderSettings r = s->job.settings;
   r.rouletteStartDepth = st->roulette_start_depth;
   r.rouletteQ = st->roulette_q;
   r.width = st->width;
@@ -662,10 +691,12 @@ int hasty_scene_set_settings(HastyScene* s, const HastyRenderSettings* st) {
   r.maxDepth = st->max_depth;
   r.numSamples = st->num_samples;
   r.useMIS = st->use_mis != 0;
-  if (r.maxDepth > 255) {
-    g_lastError = "max_depth above 255 is not supported";
-    return 1;
+  try {
+    checkSettings(r);  // a rejected call leaves the scene unchanged
+  } catch (const std::exception& e) {
+    return fail(e, 1);
   }
+  s->job.settings = r;
   s->applySettings();
   return 0;
 }
@@ -728,21 +759,17 @@ int hasty_trace_closest(HastyScene* s, const float* rays, uint32_t n, uint32_t* 
       HIPCHECK(hipMalloc(&s->traceOverflow, sizeof(uint2) * threads * hasty::kStackOvf));
       s->traceOverflowThreads = threads;
     }
-    float* dRays = nullptr;
-    int32_t* dTri = nullptr;
-    float* dTuv = nullptr;
-    HIPCHECK(hipMalloc(&dRays, sizeof(float) * 6 * n));
-    HIPCHECK(hipMalloc(&dTri, sizeof(int32_t) * n));
-    HIPCHECK(hipMalloc(&dTuv, sizeof(float) * 3 * n));
+    // rays, triangle ids and (t, u, v) share the scene's trace buffer
+    char* io = (char*)s->traceIO.ensure((sizeof(float) * 6 + sizeof(int32_t) + sizeof(float) * 3) * (size_t)n);
+    float* dRays = (float*)io;
+    int32_t* dTri = (int32_t*)(io + sizeof(float) * 6 * (size_t)n);
+    float* dTuv = (float*)(io + (sizeof(float) * 6 + sizeof(int32_t)) * (size_t)n);
     HIPCHECK(hipMemcpy(dRays, rays, sizeof(float) * 6 * n, hipMemcpyHostToDevice));
     hasty::launchTraceClosest(s->dev, dRays, n, dTri, dTuv, s->traceOverflow, s->traceOverflowThreads, s->stream);
     HIPCHECK(hipGetLastError());
     HIPCHECK(hipStreamSynchronize(s->stream));
     HIPCHECK(hipMemcpy(tri, dTri, sizeof(int32_t) * n, hipMemcpyDeviceToHost));
     HIPCHECK(hipMemcpy(tuv, dTuv, sizeof(float) * 3 * n, hipMemcpyDeviceToHost));
-    (void)hipFree(dRays);
-    (void)hipFree(dTri);
-    (void)hipFree(dTuv);
     for (uint32_t i = 0; i < n; ++i) {
       if (tri[i] < 0) {
         geomPrim[2 * i] = geomPrim[2 * i + 1] = 0xFFFFFFFFu;

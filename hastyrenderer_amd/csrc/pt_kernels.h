@@ -162,11 +162,15 @@ struct RenderJobDesc {
   float* dAlbedo;
 };
 
+// Device bytes of the scene and pass tables of an n-job render call.
+size_t jobTableBytes(uint32_t nJobs);
+
 // Renders every job through one shared pool of path slots (paths of all jobs
 // in flight together; at most 2^31 paths per call).  Synchronises with the
-// stream once per cfg.chunk bounces.
+// stream once per cfg.chunk bounces.  With more than one job the per-job
+// tables go to `jobTables` (device memory of jobTableBytes(n) bytes at least).
 void renderJobs(const std::vector<RenderJobDesc>& jobs, const PathBuffers& pb, const LaunchConfig& cfg,
-                hipStream_t stream, KernelTimer* timer, const GuideRun* guide = nullptr);
+                hipStream_t stream, KernelTimer* timer, const GuideRun* guide = nullptr, void* jobTables = nullptr);
 
 // Pixels of a width x height film in 8x8-tile order (row-major inside a
 // tile), and the inverse map.

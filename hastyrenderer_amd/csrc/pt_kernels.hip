@@ -994,8 +994,13 @@ void launchChunk(const JobSet& js, const PathBuffers& pbIn, const LaunchConfig& 
 
 }  // namespace
 
+size_t jobTableBytes(uint32_t nJobs) {
+  const size_t a = (sizeof(DevScene) * nJobs + 255) & ~(size_t)255;
+  return a + sizeof(PassInfo) * nJobs;
+}
+
 void renderJobs(const std::vector<RenderJobDesc>& jobs, const PathBuffers& pb, const LaunchConfig& cfg,
-                hipStream_t stream, KernelTimer* timer, const GuideRun* guide) {
+                hipStream_t stream, KernelTimer* timer, const GuideRun* guide, void* jobTables) {
   const uint32_t nJobs = (uint32_t)jobs.size();
   if (nJobs == 0) return;
   std::vector<DevScene> scenes(nJobs);
@@ -1027,19 +1032,11 @@ void renderJobs(const std::vector<RenderJobDesc>& jobs, const PathBuffers& pb, c
     if (nJobs != 1) throw std::runtime_error("renderJobs: guided rendering takes one job");
     js.g = *guide;
   }
-  void* dScenes = nullptr;
-  void* dPasses = nullptr;
-  struct Free {
-    void*& a;
-    void*& b;
-    ~Free() {
-      if (a) (void)hipFree(a);
-      if (b) (void)hipFree(b);
-    }
-  } freeOnExit{dScenes, dPasses};
   if (nJobs > 1) {
-    hipOk(hipMalloc(&dScenes, sizeof(DevScene) * nJobs), "hipMalloc");
-    hipOk(hipMalloc(&dPasses, sizeof(PassInfo) * nJobs), "hipMalloc");
+    if (!jobTables) throw std::runtime_error("renderJobs: no job table buffer for a batch");
+    char* base = (char*)jobTables;
+    void* dScenes = base;
+    void* dPasses = base + ((sizeof(DevScene) * nJobs + 255) & ~(size_t)255);
     hipOk(hipMemcpyAsync(dScenes, scenes.data(), sizeof(DevScene) * nJobs, hipMemcpyHostToDevice, stream), "copy");
     hipOk(hipMemcpyAsync(dPasses, passes.data(), sizeof(PassInfo) * nJobs, hipMemcpyHostToDevice, stream), "copy");
     js.scenes = (const DevScene*)dScenes;

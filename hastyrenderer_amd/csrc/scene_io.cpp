@@ -490,10 +490,19 @@ void parseObj(const std::string& path, const std::string& mtlSearchDir, std::vec
           ix.v = resolveIndex(atoi(t.substr(0, s1).c_str()), vertices.size() / 3);
           size_t s2 = t.find('/', s1 + 1);
           std::string ts = t.substr(s1 + 1, s2 == std::string::npos ? std::string::npos : s2 - s1 - 1);
-          if (!ts.empty()) ix.t = resolveIndex(atoi(ts.c_str()), texcoords.size() / 2);
+          // "f 1/5" or "f 1//9" naming more vt / vn lines than the file has
+          if (!ts.empty()) {
+            ix.t = resolveIndex(atoi(ts.c_str()), texcoords.size() / 2);
+            if (ix.t < 0 || (size_t)ix.t >= texcoords.size() / 2)
+              throw std::runtime_error("TinyObjReader error invalid texcoord index in " + path);
+          }
           if (s2 != std::string::npos) {
             std::string ns = t.substr(s2 + 1);
-            if (!ns.empty()) ix.n = resolveIndex(atoi(ns.c_str()), normals.size() / 3);
+            if (!ns.empty()) {
+              ix.n = resolveIndex(atoi(ns.c_str()), normals.size() / 3);
+              if (ix.n < 0 || (size_t)ix.n >= normals.size() / 3)
+                throw std::runtime_error("TinyObjReader error invalid normal index in " + path);
+            }
           }
         }
         if (ix.v < 0 || (size_t)ix.v >= vertices.size() / 3)

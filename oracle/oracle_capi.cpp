@@ -153,6 +153,43 @@ void oracle_camera_rays(void* jp, uint32_t sample, float* out) {
     }
 }
 
+// One camera ray through the continuous pixel position (px, py) with the
+// lens stream (seed, 0, 0, 97): origin xyz, direction xyz.
+void oracle_camera_ray(void* jp, float px, float py, uint32_t seed, float* out) {
+  PTJob* job = (PTJob*)jp;
+  RNG rng = hrng::make_stream(seed, 0, 0, 97);
+  Ray r = job->scene.computeRay(rng, Vec2f(px, py), (float)job->settings.width, (float)job->settings.height);
+  for (int c = 0; c < 3; ++c) {
+    out[c] = r.o[c];
+    out[3 + c] = r.d[c];
+  }
+}
+// Environment radiance seen along n directions (Scene.cpp:482-485).
+void oracle_env_eval(void* jp, const float* dirs, uint32_t n, float* out) {
+  for (uint32_t i = 0; i < n; ++i) {
+    const float* d = dirs + 3 * (size_t)i;
+    Ray r(Vec3f(0.0f, 0.0f, 0.0f), Vec3f(d[0], d[1], d[2]));
+    Vec3f L = ((PTJob*)jp)->scene.evalEnvironment(r);
+    for (int c = 0; c < 3; ++c) out[3 * (size_t)i + c] = L[c];
+  }
+}
+// Environment sampling density at n directions (Environment.h:158-165).
+void oracle_env_pdf_n(void* jp, const float* dirs, uint32_t n, float* out) {
+  for (uint32_t i = 0; i < n; ++i) {
+    const float* d = dirs + 3 * (size_t)i;
+    out[i] = ((PTJob*)jp)->scene.envEvalSamplingDensity(Vec3f(d[0], d[1], d[2]));
+  }
+}
+// Positions of a loaded triangle (global index): p0, p1, p2.
+int oracle_tri_pos(void* jp, uint32_t tri, float* out) {
+  const PTJob* job = (const PTJob*)jp;
+  if (tri >= job->scene.data.numTris) return 1;
+  memcpy(out, &job->scene.data.triPos[(size_t)tri * 9], 9 * sizeof(float));
+  return 0;
+}
+// Sets a planted mistake (see oracle::Mutation); 0 restores the restatement.
+void oracle_set_mutation(int m) { g_mutation = m; }
+
 // ---- leaf functions for unit tests -------------------------------------
 void oracle_halton(uint32_t k, float* out) {
   Vec2f v = haltonOffset(k);

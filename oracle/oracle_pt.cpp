@@ -7,6 +7,9 @@
 
 namespace oracle {
 
+int g_mutation = kMutNone;
+
+
 Vec2f haltonOffset(uint32_t k) {
   HaltonSequence s1(2), s2(3);
   Vec2f r;
@@ -668,6 +671,7 @@ Vec3f Scene::envEvaluateUV(const Vec2f& uv) const {
   int col = (int)(u * data.env.width);
   int row = (int)(oneMinusV * data.env.height);
   if (col > (int)data.env.width - 1) col = (int)data.env.width - 1;
+  if (g_mutation == kMutEnvRowFromBottom) row = (int)data.env.height - 1 - row;
   const float* px = &data.env.rgb[((size_t)row * data.env.width + col) * 3];
   return Vec3f(px[0], px[1], px[2]);
 }
@@ -720,6 +724,7 @@ float Scene::envEvalSamplingDensity(const Vec3f& dir) const {
   Vec2f c = sphericalFromDirection(dir);
   Vec2f uv(c[1] * float(0.5 * InvPi), 1.0f - c[0] * float(InvPi));
   float sinTheta = hm::sinf(c[0]);
+  if (g_mutation == kMutEnvPdfNoSinTheta) sinTheta = 1.0f;
   return envEvalDensityUV(uv) / std::max(1e-7f, 2.0f * float(Pi * Pi) * sinTheta);
 }
 // Scene.cpp:506-526
@@ -804,12 +809,15 @@ Ray Scene::computeRay(RNG& rng, const Vec2f& p, float frameWidth, float frameHei
   float q = hm::tanf(fovRadians * 0.5f);
   float lx = (x - 0.5f * frameWidth) / frameWidth * 2.0f;
   float ly = (y - 0.5f * frameHeight) / frameWidth * 2.0f;
+  if (g_mutation == kMutCameraLyByHeight) ly = (y - 0.5f * frameHeight) / frameHeight * 2.0f;
   Vec3f direction = fwd + right * (q * lx) - up * (q * ly);
+  if (g_mutation == kMutCameraUpSign) direction = fwd + right * (q * lx) + up * (q * ly);
   direction = normalize(direction);
   if (c.aperture > 0.0f) {
     float density;
     Vec2f offset = sampleCircularNGonUniformly(rng, c.numBlades, c.bladeRotation, density);
     float t = c.focalDistance * dot(direction, fwd);
+    if (g_mutation == kMutLensFocalAlongRay) t = c.focalDistance;
     Vec3f focalPoint = pos + t * direction;
     Vec3f origin = pos + c.aperture * offset[0] * up + c.aperture * offset[1] * right;
     direction = normalize(focalPoint - origin);
@@ -837,6 +845,7 @@ float Scene::computeRayAngle(const Vec2f& p, float frameWidth, float frameHeight
 // PathTracing.h:46-59
 Vec3f beersLaw(const Vec3f& transmittance, float d) {
   if (transmittance == Vec3f::Ones()) return Vec3f::Ones();
+  if (g_mutation == kMutBeerNoDistance) return transmittance;
   Vec3f result;
   float dRef = 1.0f;
   float invdRef = 1.0f / dRef;

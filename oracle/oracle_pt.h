@@ -22,6 +22,19 @@
 
 namespace oracle {
 
+// Planted mistakes for the pin tests (tests/ set one, run a check that must
+// then FAIL, and reset it to 0).  0 = the faithful restatement.
+enum Mutation {
+  kMutNone = 0,
+  kMutEnvPdfNoSinTheta = 1,   // envEvalSamplingDensity without its 1 / sin(theta)
+  kMutEnvRowFromBottom = 2,   // environment lookup counts rows from the bottom
+  kMutCameraLyByHeight = 3,   // computeRay normalises ly by the height
+  kMutCameraUpSign = 4,       // computeRay adds the up term instead of subtracting it
+  kMutBeerNoDistance = 5,     // beersLaw returns T instead of T^d
+  kMutLensFocalAlongRay = 6,  // thin lens refocuses at focalDistance along the ray
+};
+extern int g_mutation;
+
 using RNG = hrng::Stream;
 inline float uniform01f(RNG& rng) { return rng.uniform01f(); }
 

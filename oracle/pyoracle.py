@@ -38,6 +38,11 @@ def lib():
         L.oracle_material.argtypes = [P, C.c_int, fp]
         L.oracle_light.argtypes = [P, C.c_int, C.POINTER(C.c_int32), C.POINTER(C.c_float)]
         L.oracle_camera_rays.argtypes = [P, C.c_uint32, fp]
+        L.oracle_camera_ray.argtypes = [P, C.c_float, C.c_float, C.c_uint32, fp]
+        L.oracle_env_eval.argtypes = [P, fp, C.c_uint32, fp]
+        L.oracle_env_pdf_n.argtypes = [P, fp, C.c_uint32, fp]
+        L.oracle_tri_pos.argtypes = [P, C.c_uint32, fp]
+        L.oracle_set_mutation.argtypes = [C.c_int]
         L.oracle_halton.argtypes = [C.c_uint32, fp]
         L.oracle_philox.argtypes = [up, up, up]
         L.oracle_sample_vndf.argtypes = [fp, C.c_uint32, fp]
@@ -166,6 +171,32 @@ class OracleJob:
         lib().oracle_camera_rays(self.h, sample, out)
         return out
 
+    def camera_ray(self, px, py, seed=0):
+        """Ray through the continuous pixel position (px, py): origin, direction."""
+        out = np.zeros(6, np.float32)
+        lib().oracle_camera_ray(self.h, float(px), float(py), seed, out)
+        return out[:3].copy(), out[3:].copy()
+
+    def env_eval(self, dirs):
+        """Environment radiance along directions (n, 3) -> (n, 3)."""
+        d = f32(dirs).reshape(-1, 3)
+        out = np.zeros((d.shape[0], 3), np.float32)
+        lib().oracle_env_eval(self.h, d, d.shape[0], out)
+        return out
+
+    def env_pdf_n(self, dirs):
+        """Environment sampling density at directions (n, 3) -> (n,)."""
+        d = f32(dirs).reshape(-1, 3)
+        out = np.zeros(d.shape[0], np.float32)
+        lib().oracle_env_pdf_n(self.h, d, d.shape[0], out)
+        return out
+
+    def tri_pos(self, tri):
+        out = np.zeros(9, np.float32)
+        if lib().oracle_tri_pos(self.h, int(tri), out) != 0:
+            raise IndexError(tri)
+        return out.reshape(3, 3)
+
     def env_sample(self, seed, count):
         out = np.zeros((count, 5), np.float32)
         lib().oracle_env_sample(self.h, seed, count, out)
@@ -173,6 +204,29 @@ class OracleJob:
 
     def env_pdf(self, d):
         return float(lib().oracle_env_pdf(self.h, f32(d)))
+
+
+class mutation:
+    """Context manager planting one of the oracle's documented mistakes
+    (oracle::Mutation) for the duration of a check that must then fail."""
+
+    ENV_PDF_NO_SIN_THETA = 1
+    ENV_ROW_FROM_BOTTOM = 2
+    CAMERA_LY_BY_HEIGHT = 3
+    CAMERA_UP_SIGN = 4
+    BEER_NO_DISTANCE = 5
+    LENS_FOCAL_ALONG_RAY = 6
+
+    def __init__(self, m):
+        self.m = m
+
+    def __enter__(self):
+        lib().oracle_set_mutation(self.m)
+        return self
+
+    def __exit__(self, *exc):
+        lib().oracle_set_mutation(0)
+        return False
 
 
 def sampler(kind, seed, count, normal=(0, 0, 1), wo=(0, 0, 1), param=0.0):

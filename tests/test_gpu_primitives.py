@@ -8,7 +8,7 @@ own pt_device.h) on the same inputs as the oracle and require equal bits.
 import numpy as np
 import pytest
 
-from conftest import load_testkernels as testkernels
+from conftest import load_testkernels as _kernels
 from oracle import pyoracle as po
 
 pytestmark = pytest.mark.gpu
@@ -39,7 +39,7 @@ def test_math_bitexact(fn, lo, hi, blo, bhi):
     # include edge values
     a[:4] = np.array([lo, hi, 0.0, -0.0], np.float32).clip(lo, hi)
     out = np.zeros(n, np.float32)
-    assert testkernels().tk_math(fn, a, b, n, out) == 0
+    assert _kernels().tk_math(fn, a, b, n, out) == 0
     ref = po.hm_vec(fn, a, b)
     same = (_bits(out) == _bits(ref)) | (np.isnan(out) & np.isnan(ref))
     assert same.all(), (fn, a[~same][:5], out[~same][:5], ref[~same][:5])
@@ -52,7 +52,7 @@ def test_philox_bitexact():
     ctr[1] = [0xFFFFFFFF] * 6
     ctr[2] = [0x243F6A88, 0x85A308D3, 0x13198A2E, 0x03707344, 0xA4093822, 0x299F31D0]
     out = np.zeros((n, 4), np.uint32)
-    assert testkernels().tk_philox(np.ascontiguousarray(ctr), n, out) == 0
+    assert _kernels().tk_philox(np.ascontiguousarray(ctr), n, out) == 0
     # Random123 known-answer vectors for Philox-4x32-10
     assert list(out[0]) == [0x6627E8D5, 0xE169C58D, 0xBC57AC4C, 0x9B00DBD8]
     assert list(out[1]) == [0x408F276D, 0x41C83B0E, 0xA20BC7C6, 0x6D5451FD]
@@ -70,7 +70,7 @@ def test_vndf_bitexact():
     w[:, 2] = np.abs(w[:, 2])
     rows[:, 4:7] = w / np.linalg.norm(w, axis=1, keepdims=True)
     out = np.zeros((n, 3), np.float32)
-    assert testkernels().tk_vndf(rows, n, out) == 0
+    assert _kernels().tk_vndf(rows, n, out) == 0
     ref = po.sample_vndf(rows)
     assert (_bits(out) == _bits(ref)).all()
 
@@ -83,7 +83,7 @@ def test_sampler_sequences_bitexact(kind, param):
     wo = np.array([0.1, 0.9, 0.3], np.float32)
     wo /= np.linalg.norm(wo)
     out = np.zeros((count, 5), np.float32)
-    assert testkernels().tk_sampler(kind, 7, count, nrm, wo, param, out) == 0
+    assert _kernels().tk_sampler(kind, 7, count, nrm, wo, param, out) == 0
     ref = po.sampler(kind, 7, count, nrm, wo, param)
     assert (_bits(out) == _bits(ref)).all()
 
@@ -110,7 +110,7 @@ def test_bsdf_eval_bitexact(mi):
     d[:, 3:] /= np.linalg.norm(d[:, 3:], axis=1, keepdims=True)
     d = d.astype(np.float32)
     out = np.zeros((count, 7), np.float32)
-    assert testkernels().tk_bsdf_eval(p, n, d, count, out) == 0
+    assert _kernels().tk_bsdf_eval(p, n, d, count, out) == 0
     ref = po.bsdf_eval(p, n, d)
     same = (_bits(out) == _bits(ref)) | (np.isnan(out) & np.isnan(ref))
     assert same.all()
@@ -124,7 +124,7 @@ def test_bsdf_sample_bitexact(mi):
     wo /= np.linalg.norm(wo)
     count = 4000
     out = np.zeros((count, 7), np.float32)
-    assert testkernels().tk_bsdf_sample(p, n, wo, 11, count, out) == 0
+    assert _kernels().tk_bsdf_sample(p, n, wo, 11, count, out) == 0
     ref = po.bsdf_sample(p, n, wo, 11, count)
     same = (_bits(out) == _bits(ref)) | (np.isnan(out) & np.isnan(ref))
     assert same.all()

@@ -7,8 +7,9 @@ statistical, apps/Tests.cpp:40-232), so the oracle is pinned by:
     integral of cos/pdf, refraction round trip, Fresnel symmetry);
   * known answers (Random123's Philox-4x32-10 vectors, Halton points);
   * closed forms (Fresnel at normal incidence, Lambertian throughput, the
-    cosine-hemisphere integral, the environment pdf integrating to one, the
-    VNDF sample at alpha = 1, octagon area, uniform triangle moments);
+    cosine-hemisphere integral, the VNDF sample at alpha = 1, octagon area,
+    uniform triangle moments; the environment map, camera and media closed
+    forms are in test_oracle_pins_cpu.py);
   * brute force (BVH2 closest hit vs every triangle) and exact estimator
     cases (a diffuse plane under a constant sky returns its albedo).
 """
