@@ -30,12 +30,15 @@ _lib = None
 # Every symbol declared in include/hasty_amd.h.
 EXPORTS = (
     "hasty_scene_load",
+    "hasty_scene_create",
     "hasty_scene_destroy",
     "hasty_scene_info",
     "hasty_scene_set_settings",
     "hasty_render_pt",
     "hasty_render_pt_device",
     "hasty_trace_closest",
+    "hasty_trace_hits",
+    "hasty_trace_hits_device",
     "hasty_set_profiling",
     "hasty_kernel_stats",
     "hasty_reset_stats",
@@ -74,6 +77,11 @@ class HastyError(RuntimeError):
     pass
 
 
+# HastyHit (include/hasty_amd.h) as a numpy record.
+HIT_DTYPE = np.dtype([("geom_id", "<u4"), ("prim_id", "<u4"), ("t", "<f4"), ("u", "<f4"), ("v", "<f4"),
+                      ("ng", "<f4", (3,))])
+
+
 class RenderSettingsC(C.Structure):
     _fields_ = [
         ("roulette_start_depth", C.c_int32),
@@ -84,6 +92,64 @@ class RenderSettingsC(C.Structure):
         ("max_depth", C.c_int32),
         ("num_samples", C.c_int32),
         ("use_mis", C.c_int32),
+    ]
+
+
+class MaterialDescC(C.Structure):
+    _fields_ = [
+        ("albedo", C.c_float * 3),
+        ("roughness", C.c_float),
+        ("metallic", C.c_float),
+        ("specular", C.c_float),
+        ("ior", C.c_float),
+        ("anisotropy", C.c_float),
+        ("transmission", C.c_float),
+        ("vary_ior", C.c_int32),
+        ("emission", C.c_float * 3),
+        ("tinted_emitter", C.c_int32),
+    ]
+
+
+class CameraDescC(C.Structure):
+    _fields_ = [
+        ("position", C.c_float * 3),
+        ("forward", C.c_float * 3),
+        ("right", C.c_float * 3),
+        ("up", C.c_float * 3),
+        ("fov_degree", C.c_float),
+        ("aperture", C.c_float),
+        ("focal_distance", C.c_float),
+        ("blade_rotation", C.c_float),
+        ("num_blades", C.c_int32),
+        ("exposure", C.c_float),
+    ]
+
+
+class SceneDescC(C.Structure):
+    _fields_ = [
+        ("settings", RenderSettingsC),
+        ("camera", CameraDescC),
+        ("num_vertices", C.c_uint32),
+        ("vertices", C.c_void_p),
+        ("num_normals", C.c_uint32),
+        ("normals", C.c_void_p),
+        ("num_texcoords", C.c_uint32),
+        ("texcoords", C.c_void_p),
+        ("num_triangles", C.c_uint32),
+        ("vertex_indices", C.c_void_p),
+        ("normal_indices", C.c_void_p),
+        ("texcoord_indices", C.c_void_p),
+        ("geom_ids", C.c_void_p),
+        ("prim_ids", C.c_void_p),
+        ("material_ids", C.c_void_p),
+        ("smooth", C.c_void_p),
+        ("num_materials", C.c_uint32),
+        ("materials", C.c_void_p),
+        ("has_environment", C.c_int32),
+        ("env_width", C.c_uint32),
+        ("env_height", C.c_uint32),
+        ("env_rgb", C.c_void_p),
+        ("background", C.c_float * 3),
     ]
 
 
@@ -159,6 +225,7 @@ def lib():
     P = C.c_void_p
     fp = C.POINTER(C.c_float)
     L.hasty_scene_load.argtypes = [C.c_char_p, C.POINTER(P)]
+    L.hasty_scene_create.argtypes = [C.POINTER(SceneDescC), C.POINTER(P)]
     L.hasty_scene_destroy.argtypes = [P]
     L.hasty_scene_destroy.restype = None
     L.hasty_scene_info.argtypes = [P, C.POINTER(SceneInfoC)]
@@ -166,6 +233,8 @@ def lib():
     L.hasty_render_pt.argtypes = [P, C.c_uint32, C.c_uint32, C.c_uint32, fp, fp, fp]
     L.hasty_render_pt_device.argtypes = [P, C.c_uint32, C.c_uint32, C.c_uint32, P, P, P]
     L.hasty_trace_closest.argtypes = [P, fp, C.c_uint32, C.POINTER(C.c_uint32), C.POINTER(C.c_int32), fp]
+    L.hasty_trace_hits.argtypes = [P, C.c_void_p, C.c_uint32, C.c_void_p]
+    L.hasty_trace_hits_device.argtypes = [P, C.c_void_p, C.c_uint32, C.c_void_p]
     L.hasty_set_profiling.argtypes = [P, C.c_int]
     L.hasty_kernel_stats.argtypes = [P, C.POINTER(KernelStatC), C.c_int]
     L.hasty_reset_stats.argtypes = [P]
@@ -231,6 +300,65 @@ class Scene:
         h = C.c_void_p()
         _err(lib().hasty_scene_load(os.fsencode(render_job_path), C.byref(h)))
         self._h = h
+
+    @classmethod
+    def create(cls, settings: dict, camera: dict, vertices, vertex_indices, geom_ids, prim_ids, material_ids,
+               materials, normals=None, normal_indices=None, texcoords=None, texcoord_indices=None, smooth=None,
+               env_rgb=None, background=(0.0, 0.0, 0.0)):
+        """hasty_scene_create from arrays (what Scene::Scene holds after
+        tinyobj / OIIO): settings as RenderSettingsC fields, camera as
+        CameraDescC fields, materials as a list of MaterialDescC-field dicts,
+        per-triangle arrays of 3 indices (or one id) per triangle, env_rgb an
+        (H, W, 3) image with row 0 at the top or None."""
+        keep = []
+
+        def arr(a, dtype, cols):
+            if a is None:
+                return None, 0
+            a = np.ascontiguousarray(a, dtype=dtype)
+            if a.size % cols:
+                raise ValueError("array size %d is not a multiple of %d" % (a.size, cols))
+            keep.append(a)
+            return a.ctypes.data, a.size // cols
+
+        d = SceneDescC()
+        for k, v in settings.items():
+            setattr(d.settings, k, v)
+        for k, v in camera.items():
+            if k in ("position", "forward", "right", "up"):
+                setattr(d.camera, k, (C.c_float * 3)(*v))
+            else:
+                setattr(d.camera, k, v)
+        d.vertices, d.num_vertices = arr(vertices, np.float32, 3)
+        d.normals, d.num_normals = arr(normals, np.float32, 3)
+        d.texcoords, d.num_texcoords = arr(texcoords, np.float32, 2)
+        d.vertex_indices, d.num_triangles = arr(vertex_indices, np.int32, 3)
+        d.normal_indices, _ = arr(normal_indices, np.int32, 3)
+        d.texcoord_indices, _ = arr(texcoord_indices, np.int32, 3)
+        for name, a, dt in (("geom_ids", geom_ids, np.uint32), ("prim_ids", prim_ids, np.uint32),
+                            ("material_ids", material_ids, np.int32), ("smooth", smooth, np.uint8)):
+            ptr, n = arr(a, dt, 1)
+            if ptr is not None and n != d.num_triangles:
+                raise ValueError("%s has %d entries for %d triangles" % (name, n, d.num_triangles))
+            setattr(d, name, ptr)
+        mats = (MaterialDescC * max(1, len(materials)))()
+        for i, m in enumerate(materials):
+            for k, v in m.items():
+                setattr(mats[i], k, (C.c_float * 3)(*v) if k in ("albedo", "emission") else v)
+        d.num_materials = len(materials)
+        d.materials = C.cast(mats, C.c_void_p)
+        if env_rgb is not None:
+            env = np.ascontiguousarray(env_rgb, np.float32)
+            keep.append(env)
+            d.has_environment = 1
+            d.env_height, d.env_width = env.shape[0], env.shape[1]
+            d.env_rgb = env.ctypes.data
+        d.background = (C.c_float * 3)(*background)
+        h = C.c_void_p()
+        _err(lib().hasty_scene_create(C.byref(d), C.byref(h)))
+        obj = cls.__new__(cls)
+        obj._h = h
+        return obj
 
     def close(self):
         if getattr(self, "_h", None) is not None and _lib is not None:
@@ -338,6 +466,18 @@ class Scene:
         _err(lib().hasty_trace_closest(self._h, _fptr(rays), n, gp.ctypes.data_as(C.POINTER(C.c_uint32)),
                                        tri.ctypes.data_as(C.POINTER(C.c_int32)), _fptr(tuv)))
         return gp, tri, tuv
+
+    def trace_hits(self, rays: np.ndarray) -> np.ndarray:
+        """Full hit records (HastyHit) of rays (n, 6): a structured array with
+        geom_id, prim_id, t, u, v, ng (3)."""
+        rays = np.ascontiguousarray(rays, dtype=np.float32).reshape(-1, 6)
+        hits = np.zeros(rays.shape[0], HIT_DTYPE)
+        _err(lib().hasty_trace_hits(self._h, rays.ctypes.data, rays.shape[0], hits.ctypes.data))
+        return hits
+
+    def trace_hits_device(self, d_rays: int, n: int, d_hits: int):
+        """Rays (n x HastyRay) and hits (n x HastyHit) in device memory."""
+        _err(lib().hasty_trace_hits_device(self._h, C.c_void_p(d_rays), n, C.c_void_p(d_hits)))
 
     def set_profiling(self, events=True, counters=False, extend_only=False):
         """HIP-event timing of every launch (of the extension launches only

@@ -1,0 +1,26 @@
+// Internal view of a HastyScene for entry points compiled in other
+// translation units of the library (not part of the public ABI).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+
+#include "../../include/hasty_amd.h"
+#include "pt_device.h"
+
+namespace hasty {
+
+struct SceneView {
+  hdev::DevScene dev;  // device pointers and constants of the loaded scene
+  hipStream_t stream;  // the scene's stream
+  int device;          // the device the scene lives on
+  int bvhDepth;        // inner levels of the 4-wide BVH
+};
+
+// Fills *out; false for a null scene.
+bool sceneView(const HastyScene* scene, SceneView* out);
+
+// Sets the message hasty_last_error() returns on the calling thread.
+void setLastError(const std::string& message);
+
+}  // namespace hasty

@@ -14,6 +14,7 @@
 #include <vector>
 
 #include "bvh.h"
+#include "capi_internal.h"
 #include "pt_kernels.h"
 #include "rng.h"
 #include "scene_io.h"
@@ -534,8 +535,108 @@ static void checkSettings(const hasty::RenderSettings& st) {
   if (st.maxDepth > 255) throw std::runtime_error("max_depth above 255 is not supported");
 }
 
+// Scene::Scene's inputs from a descriptor (what tinyobj and OIIO hand it).
+static void jobFromDesc(const HastySceneDesc& d, hasty::Job& job) {
+  hasty::RenderSettings& st = job.settings;
+  st.rouletteStartDepth = d.settings.roulette_start_depth;
+  st.rouletteQ = d.settings.roulette_q;
+  st.width = d.settings.width;
+  st.height = d.settings.height;
+  st.numThreads = d.settings.num_threads;
+  st.maxDepth = d.settings.max_depth;
+  st.numSamples = d.settings.num_samples;
+  st.useMIS = d.settings.use_mis != 0;
+  hasty::Scene& sc = job.scene;
+  hasty::Camera& c = sc.camera;
+  for (int i = 0; i < 3; ++i) {
+    c.position[i] = d.camera.position[i];
+    c.forward[i] = d.camera.forward[i];
+    c.right[i] = d.camera.right[i];
+    c.up[i] = d.camera.up[i];
+  }
+  c.fovDegree = d.camera.fov_degree;
+  c.aperture = d.camera.aperture;
+  c.focalDistance = d.camera.focal_distance;
+  c.bladeRotation = d.camera.blade_rotation;
+  c.numBlades = d.camera.num_blades;
+  c.exposure = d.camera.exposure;
+  if (c.aperture > 0.0f && c.numBlades < 3) throw std::runtime_error("camera num_blades must be >= 3");
+  auto need = [](const void* p, uint32_t n, const char* what) {
+    if (n > 0 && !p) throw std::runtime_error(std::string("descriptor: ") + what + " is null");
+  };
+  need(d.vertices, d.num_vertices, "vertices");
+  need(d.normals, d.num_normals, "normals");
+  need(d.texcoords, d.num_texcoords, "texcoords");
+  need(d.materials, d.num_materials, "materials");
+  for (const void* p : {(const void*)d.vertex_indices, (const void*)d.geom_ids, (const void*)d.prim_ids,
+                        (const void*)d.material_ids})
+    need(p, d.num_triangles, "a per-triangle array");
+  sc.vertices.assign(d.vertices, d.vertices + 3 * (size_t)d.num_vertices);
+  sc.normals.assign(d.normals, d.normals + 3 * (size_t)d.num_normals);
+  sc.texcoords.assign(d.texcoords, d.texcoords + 2 * (size_t)d.num_texcoords);
+  std::vector<uint8_t> tinted;
+  for (uint32_t m = 0; m < d.num_materials; ++m) {
+    const HastyMaterialDesc& md = d.materials[m];
+    hasty::Material out;
+    out.name = "material" + std::to_string(m);
+    for (int i = 0; i < 3; ++i) {
+      out.albedo[i] = md.albedo[i];
+      out.emission[i] = md.emission[i];
+    }
+    out.roughness = md.roughness;
+    out.metallic = md.metallic;
+    out.specular = md.specular;
+    out.ior = md.ior;
+    out.anisotropy = md.anisotropy;
+    out.transmission = md.transmission;
+    out.varyIOR = md.vary_ior != 0;
+    sc.materials.push_back(out);
+    tinted.push_back(md.tinted_emitter != 0);
+  }
+  // shapes from (geom_id, prim_id): shape by shape, faces in order
+  std::vector<hasty::ShapeFaces> shapes;
+  for (uint32_t i = 0; i < d.num_triangles; ++i) {
+    const uint32_t g = d.geom_ids[i];
+    if (!shapes.empty() && g < shapes.size() - 1) throw std::runtime_error("descriptor: geom_ids decrease");
+    if (g >= (1u << 16)) throw std::runtime_error("more than 65535 shapes");
+    while (shapes.size() <= g) {
+      shapes.emplace_back();
+      shapes.back().name = "shape" + std::to_string(shapes.size() - 1);
+    }
+    hasty::ShapeFaces& sh = shapes[g];
+    if (d.prim_ids[i] != sh.materialIds.size()) throw std::runtime_error("descriptor: prim_ids must count faces from 0");
+    for (int v = 0; v < 3; ++v) {
+      hasty::FaceIndex ix;
+      ix.v = d.vertex_indices[3 * (size_t)i + v];
+      ix.n = d.normal_indices ? d.normal_indices[3 * (size_t)i + v] : -1;
+      ix.t = d.texcoord_indices ? d.texcoord_indices[3 * (size_t)i + v] : -1;
+      sh.indices.push_back(ix);
+    }
+    sh.materialIds.push_back(d.material_ids[i]);
+    sh.smoothing.push_back(d.smooth && d.smooth[i] ? 1u : 0u);
+  }
+  if (d.has_environment) {
+    need(d.env_rgb, d.env_width * d.env_height, "env_rgb");
+    hasty::ImageRGB img;
+    img.width = d.env_width;
+    img.height = d.env_height;
+    img.rgb.assign(d.env_rgb, d.env_rgb + (size_t)d.env_width * d.env_height * 3);
+    hasty::setEnvironment(sc, std::move(img));
+  } else {
+    sc.hasEnvTexture = false;
+    for (int i = 0; i < 3; ++i) sc.backgroundColor[i] = d.background[i];
+  }
+  hasty::buildTriangles(sc, shapes, tinted);
+}
+
+static void uploadScene(HastyScene& S);
+
 static void buildScene(HastyScene& S, const char* path) {
   hasty::loadJob(path, S.job);
+  uploadScene(S);
+}
+
+static void uploadScene(HastyScene& S) {
   hasty::Scene& sc = S.job.scene;
   const hasty::RenderSettings& st = S.job.settings;
   checkSettings(st);
@@ -586,6 +687,7 @@ static void buildScene(HastyScene& S, const char* path) {
   d.triNrm = S.upload(sc.triNrm.data(), sc.triNrm.size());
   d.triMat = S.upload(sc.triMaterial.data(), sc.triMaterial.size());
   d.triGeom = S.upload(sc.triGeomID.data(), sc.triGeomID.size());
+  d.shapeTriOffset = S.upload(sc.shapeTriOffset.data(), sc.shapeTriOffset.size());
   d.triSmooth = S.upload(sc.triSmooth.data(), sc.triSmooth.size());
   d.mats = S.upload(mats.data(), mats.size());
   d.lightTris = S.upload(sc.lightTris.data(), sc.lightTris.size());
@@ -626,6 +728,27 @@ static void buildScene(HastyScene& S, const char* path) {
   S.applySettings();
 }
 
+// The scene's traversal stack overflow buffer (256 blocks of threads).
+static void ensureTraceOverflow(HastyScene* s) {
+  if (!s->traceOverflow) {
+    const uint32_t threads = 256 * hasty::kBlock;
+    HIPCHECK(hipMalloc(&s->traceOverflow, sizeof(uint2) * threads * hasty::kStackOvf));
+    s->traceOverflowThreads = threads;
+  }
+}
+
+namespace hasty {
+bool sceneView(const HastyScene* s, SceneView* out) {
+  if (!s || !out) return false;
+  out->dev = s->dev;
+  out->stream = s->stream;
+  out->device = s->device;
+  out->bvhDepth = s->bvh.depth;
+  return true;
+}
+void setLastError(const std::string& message) { g_lastError = message; }
+}  // namespace hasty
+
 static int fail(const std::exception& e, int code) {
   g_lastError = e.what();
   return code;
@@ -643,6 +766,22 @@ int hasty_scene_load(const char* path, HastyScene** out) {
   try {
     auto S = std::make_unique<HastyScene>();
     buildScene(*S, path);
+    *out = S.release();
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e, 2);
+  }
+}
+
+int hasty_scene_create(const HastySceneDesc* desc, HastyScene** out) {
+  if (!desc || !out) {
+    g_lastError = "null argument";
+    return 1;
+  }
+  try {
+    auto S = std::make_unique<HastyScene>();
+    jobFromDesc(*desc, S->job);
+    uploadScene(*S);
     *out = S.release();
     return 0;
   } catch (const std::exception& e) {
@@ -754,11 +893,7 @@ int hasty_trace_closest(HastyScene* s, const float* rays, uint32_t n, uint32_t* 
   if (n == 0) return 0;
   try {
     HIPCHECK(hipSetDevice(s->device));
-    const uint32_t threads = 256 * hasty::kBlock;
-    if (!s->traceOverflow) {
-      HIPCHECK(hipMalloc(&s->traceOverflow, sizeof(uint2) * threads * hasty::kStackOvf));
-      s->traceOverflowThreads = threads;
-    }
+    ensureTraceOverflow(s);
     // rays, triangle ids and (t, u, v) share the scene's trace buffer
     char* io = (char*)s->traceIO.ensure((sizeof(float) * 6 + sizeof(int32_t) + sizeof(float) * 3) * (size_t)n);
     float* dRays = (float*)io;
@@ -778,6 +913,53 @@ int hasty_trace_closest(HastyScene* s, const float* rays, uint32_t n, uint32_t* 
         geomPrim[2 * i + 1] = s->job.scene.triPrimID[tri[i]];
       }
     }
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e, 3);
+  }
+}
+
+static_assert(sizeof(HastyHit) == sizeof(hasty::HitRecord), "HastyHit layout");
+static_assert(sizeof(HastyRay) == 6 * sizeof(float), "HastyRay layout");
+
+int hasty_trace_hits_device(HastyScene* s, const HastyRay* dRays, uint32_t n, HastyHit* dHits) {
+  if (!s || (n > 0 && (!dRays || !dHits))) {
+    g_lastError = "null argument";
+    return 1;
+  }
+  if (((uintptr_t)dHits & 15u) != 0 || ((uintptr_t)dRays & 3u) != 0) {
+    g_lastError = "hits must be 16-byte aligned and rays 4-byte aligned";
+    return 1;
+  }
+  if (n == 0) return 0;
+  try {
+    HIPCHECK(hipSetDevice(s->device));
+    ensureTraceOverflow(s);
+    hasty::launchTraceHits(s->dev, (const float*)dRays, n, (hasty::HitRecord*)dHits, s->traceOverflow,
+                           s->traceOverflowThreads, s->stream);
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipStreamSynchronize(s->stream));
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e, 3);
+  }
+}
+
+int hasty_trace_hits(HastyScene* s, const HastyRay* rays, uint32_t n, HastyHit* hits) {
+  if (!s || (n > 0 && (!rays || !hits))) {
+    g_lastError = "null argument";
+    return 1;
+  }
+  if (n == 0) return 0;
+  try {
+    HIPCHECK(hipSetDevice(s->device));
+    // rays then hits in the scene's trace buffer (hits 256-byte aligned)
+    const size_t rayBytes = (sizeof(HastyRay) * (size_t)n + 255) & ~(size_t)255;
+    char* io = (char*)s->traceIO.ensure(rayBytes + sizeof(HastyHit) * (size_t)n);
+    HIPCHECK(hipMemcpy(io, rays, sizeof(HastyRay) * (size_t)n, hipMemcpyHostToDevice));
+    int rc = hasty_trace_hits_device(s, (const HastyRay*)io, n, (HastyHit*)(io + rayBytes));
+    if (rc != 0) return rc;
+    HIPCHECK(hipMemcpy(hits, io + rayBytes, sizeof(HastyHit) * (size_t)n, hipMemcpyDeviceToHost));
     return 0;
   } catch (const std::exception& e) {
     return fail(e, 3);

@@ -150,6 +150,7 @@ struct DevScene {
   const float* triNrm;
   const int32_t* triMat;
   const uint32_t* triGeom;
+  const uint32_t* shapeTriOffset;  // first global triangle of each shape (primID = tri - offset)
   const uint8_t* triSmooth;
   const DevMaterial* mats;
   const uint32_t* lightTris;

@@ -176,6 +176,22 @@ void renderJobs(const std::vector<RenderJobDesc>& jobs, const PathBuffers& pb, c
 // tile), and the inverse map.
 void tileOrder(uint32_t width, uint32_t height, uint32_t* order, uint32_t* rankOf);
 
+// Hit record of a batched closest-hit query: the layout of HastyHit
+// (include/hasty_amd.h): geomID, primID (0xFFFFFFFF on a miss), t, u, v,
+// unit geometric normal.
+struct HitRecord {
+  uint32_t geomID, primID;
+  float t, u, v;
+  float ng[3];
+};
+static_assert(sizeof(HitRecord) == 32, "HitRecord is 32 bytes");
+
+// Closest-hit records for n rays (6 floats each).  `overflow` holds
+// kStackOvf stack entries for each of the overflowThreads threads the grid
+// may use (a multiple of kBlock).
+void launchTraceHits(const hdev::DevScene& scene, const float* dRays, uint32_t n, HitRecord* dHits, uint2* dOverflow,
+                     uint32_t overflowThreads, hipStream_t stream);
+
 // Closest hit for n rays (6 floats each): tri, t, u, v.
 void launchTraceClosest(const hdev::DevScene& scene, const float* dRays, uint32_t n, int32_t* dTri, float* dTuv,
                         uint2* dOverflow, uint32_t overflowThreads, hipStream_t stream);

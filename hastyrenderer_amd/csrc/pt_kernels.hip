@@ -923,6 +923,43 @@ __global__ void __launch_bounds__(kBlock) kTraceRays(DevScene s, const float* ra
   }
 }
 
+// Closest-hit records (HitRecord) of n rays, grid-stride over the rays.
+__global__ void __launch_bounds__(kBlock) kTraceHits(DevScene s, const float* rays, uint32_t n, HitRecord* hits,
+                                                     uint2* overflow) {
+  __shared__ uint2 stack[kStackLds * kBlock];
+  uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t stride = gridDim.x * blockDim.x;
+  uint2* ovf = overflow + (size_t)gtid * kStackOvf;
+  for (uint32_t q = gtid; q < n; q += stride) {
+    const float* r = rays + (size_t)q * 6;
+    Hit h = traceClosest<kStackLds, kStackOvf>(s, v3(r[0], r[1], r[2]), v3(r[3], r[4], r[5]), stack + threadIdx.x,
+                                                kBlock, ovf, nullptr, nullptr);
+    HitRecord rec;
+    if (h.tri < 0) {
+      rec.geomID = rec.primID = 0xFFFFFFFFu;
+      rec.t = __builtin_inff();
+      rec.u = rec.v = 0.0f;
+      rec.ng[0] = rec.ng[1] = rec.ng[2] = 0.0f;
+    } else {
+      const uint32_t g = ldg(s.triGeom + h.tri);
+      rec.geomID = g;
+      rec.primID = (uint32_t)h.tri - ldg(s.shapeTriOffset + g);
+      rec.t = h.t;
+      rec.u = h.u;
+      rec.v = h.v;
+      const float* p = s.triPos + (size_t)h.tri * 9;
+      const V3 p0 = load3g(p), p1 = load3g(p + 3), p2 = load3g(p + 6);
+      const V3 ng = normalize(cross(p1 - p0, p2 - p0));  // Scene.cpp:297, Embree's Ng orientation
+      rec.ng[0] = ng.x;
+      rec.ng[1] = ng.y;
+      rec.ng[2] = ng.z;
+    }
+    float4* out = reinterpret_cast<float4*>(hits + q);
+    out[0] = make_float4(__uint_as_float(rec.geomID), __uint_as_float(rec.primID), rec.t, rec.u);
+    out[1] = make_float4(rec.v, rec.ng[0], rec.ng[1], rec.ng[2]);
+  }
+}
+
 }  // namespace
 
 void tileOrder(uint32_t width, uint32_t height, uint32_t* order, uint32_t* rankOf) {
@@ -1129,8 +1166,20 @@ void renderJobs(const std::vector<RenderJobDesc>& jobs, const PathBuffers& pb, c
   if (front != nJobs) throw std::runtime_error("renderJobs: not every sample was accumulated");
 }
 
+void launchTraceHits(const DevScene& s, const float* dRays, uint32_t n, HitRecord* dHits, uint2* dOverflow,
+                     uint32_t overflowThreads, hipStream_t stream) {
+  // the grid-stride index q + stride stays below 2^32
+  if (n >= 0x80000000u) throw std::runtime_error("more than 2^31 - 1 rays in one query");
+  uint32_t blocks = (n + kBlock - 1) / kBlock;
+  uint32_t maxBlocks = overflowThreads / kBlock;
+  if (blocks > maxBlocks) blocks = maxBlocks;
+  if (blocks == 0) return;
+  kTraceHits<<<blocks, kBlock, 0, stream>>>(s, dRays, n, dHits, dOverflow);
+}
+
 void launchTraceClosest(const DevScene& s, const float* dRays, uint32_t n, int32_t* dTri, float* dTuv, uint2* dOverflow,
                         uint32_t overflowThreads, hipStream_t stream) {
+  if (n >= 0x80000000u) throw std::runtime_error("more than 2^31 - 1 rays in one query");
   uint32_t blocks = (n + kBlock - 1) / kBlock;
   uint32_t maxBlocks = overflowThreads / kBlock;
   if (blocks > maxBlocks) blocks = maxBlocks;

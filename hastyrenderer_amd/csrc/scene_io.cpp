@@ -371,15 +371,8 @@ void writePFM(const std::string& path, uint32_t width, uint32_t height, const fl
 // ============================================================ OBJ / MTL ===
 namespace {
 
-struct ObjIndex {
-  int v, t, n;
-};
-struct ObjShape {
-  std::string name;
-  std::vector<ObjIndex> indices;
-  std::vector<int> materialIds;
-  std::vector<uint32_t> smoothing;
-};
+using ObjIndex = FaceIndex;
+using ObjShape = ShapeFaces;
 
 struct RawMaterial {
   std::string name;
@@ -552,65 +545,35 @@ std::vector<double> buildDiscreteCdf(const float* w, size_t n) {
   return cdf;
 }
 
-// Scene.cpp:12-195
-void loadScene(const std::string& sceneJsonPath, Scene& sc) {
-  std::string folder = parentDir(sceneJsonPath);
-  JsonValue j = readJsonFile(sceneJsonPath);
-  cameraFromJson(j.at("camera"), sc.camera);
-  const JsonValue& world = j.at("world");
-  if (world.contains("background_color")) {
-    sc.hasEnvTexture = false;
-    vec3FromJson(world.at("background_color"), sc.backgroundColor);
-  } else {
-    sc.hasEnvTexture = true;
-    sc.env = readPFM(joinPath(folder, world.at("environment_texture").asString()));
-    // Environment.h:105-118: weights norm(pixel) * sin(theta_row)
-    uint32_t W = sc.env.width, H = sc.env.height;
-    float heightF = (float)H;
-    sc.envPdf.resize((size_t)W * H);
-    for (uint32_t y = 0; y < H; ++y) {
-      float theta = (((float)(int)y + 0.5f) / heightF) * (float)hm::kPi;
-      float st = hm::sinf(theta);
-      for (uint32_t x = 0; x < W; ++x) {
-        const float* px = &sc.env.rgb[((size_t)y * W + x) * 3];
-        float n = hm::fsqrt(px[0] * px[0] + px[1] * px[1] + px[2] * px[2]);
-        sc.envPdf[(size_t)y * W + x] = n * st;
-      }
+// Environment.h:105-118: weights norm(pixel) * sin(theta_row)
+void setEnvironment(Scene& sc, ImageRGB env) {
+  sc.hasEnvTexture = true;
+  sc.env = std::move(env);
+  uint32_t W = sc.env.width, H = sc.env.height;
+  if (W == 0 || H == 0 || sc.env.rgb.size() != (size_t)W * H * 3)
+    throw std::runtime_error("environment texture has no pixels");
+  float heightF = (float)H;
+  sc.envPdf.assign((size_t)W * H, 0.0f);
+  for (uint32_t y = 0; y < H; ++y) {
+    float theta = (((float)(int)y + 0.5f) / heightF) * (float)hm::kPi;
+    float st = hm::sinf(theta);
+    for (uint32_t x = 0; x < W; ++x) {
+      const float* px = &sc.env.rgb[((size_t)y * W + x) * 3];
+      float n = hm::fsqrt(px[0] * px[0] + px[1] * px[1] + px[2] * px[2]);
+      sc.envPdf[(size_t)y * W + x] = n * st;
     }
-    float s = 0.0f;
-    for (float v : sc.envPdf) s += v;  // std::accumulate(..., 0.f)
-    sc.envValueSum = s;
-    if (!(s > 0.0f)) throw std::runtime_error("environment texture has no energy");
-    sc.envCdf = buildDiscreteCdf(sc.envPdf.data(), sc.envPdf.size());
   }
-  sc.varyGlassIOR = world.contains("vary_glass_ior") ? world.at("vary_glass_ior").asBool() : false;
+  float s = 0.0f;
+  for (float v : sc.envPdf) s += v;  // std::accumulate(..., 0.f)
+  sc.envValueSum = s;
+  if (!(s > 0.0f)) throw std::runtime_error("environment texture has no energy");
+  sc.envCdf = buildDiscreteCdf(sc.envPdf.data(), sc.envPdf.size());
+}
 
-  std::string meshPath = joinPath(folder, j.at("mesh").asString());
-  std::vector<ObjShape> shapes;
-  std::vector<RawMaterial> raw;
-  parseObj(meshPath, folder, sc.vertices, sc.normals, sc.texcoords, shapes, raw);
-
-  // Scene.cpp:65-118
-  for (const RawMaterial& m : raw) {
-    if (!m.bumpTex.empty() || !m.diffuseTex.empty() || !m.specularHighlightTex.empty() || !m.metallicTex.empty())
-      throw std::runtime_error("material '" + m.name + "': texture maps are not supported by this renderer");
-    Material out;
-    out.name = m.name;
-    for (int i = 0; i < 3; ++i) out.albedo[i] = m.diffuse[i];
-    out.transmission = 1.0f - m.dissolve;
-    out.varyIOR = sc.varyGlassIOR && out.transmission > 0.0f;
-    float nSpecular = m.shininess;
-    out.roughness = 1.0f - hm::fsqrt(nSpecular / 1000.0f);
-    out.metallic = m.ambient[0];
-    out.specular = m.specular[0];
-    out.ior = m.ior;
-    out.anisotropy = m.anisotropy;
-    for (int i = 0; i < 3; ++i) out.emission[i] = m.emission[i];
-    sc.materials.push_back(out);
-  }
-
+void buildTriangles(Scene& sc, const std::vector<ShapeFaces>& shapes, const std::vector<uint8_t>& emitterTinted) {
   size_t total = 0;
   for (auto& s : shapes) total += s.materialIds.size();
+  if (total >= 0x80000000ull) throw std::runtime_error("more than 2^31 - 1 triangles");
   sc.numTris = (uint32_t)total;
   sc.triPos.resize(total * 9);
   sc.triNrm.resize(total * 9);
@@ -620,19 +583,33 @@ void loadScene(const std::string& sceneJsonPath, Scene& sc) {
   sc.triMaterial.resize(total);
   sc.triGeomID.resize(total);
   sc.triPrimID.resize(total);
+  sc.shapeNames.clear();
+  sc.shapeTriOffset.clear();
+  sc.lightTris.clear();
+  sc.lightAreas.clear();
+  const size_t nv = sc.vertices.size() / 3, nn = sc.normals.size() / 3, nt = sc.texcoords.size() / 2;
   uint32_t g = 0;
   for (size_t s = 0; s < shapes.size(); ++s) {
-    const ObjShape& sh = shapes[s];
+    const ShapeFaces& sh = shapes[s];
     sc.shapeNames.push_back(sh.name);
     sc.shapeTriOffset.push_back(g);
     size_t nf = sh.materialIds.size();
+    if (sh.indices.size() != 3 * nf || sh.smoothing.size() != nf)
+      throw std::runtime_error("shape '" + sh.name + "': index, material and smoothing counts disagree");
     for (size_t f = 0; f < nf; ++f, ++g) {
       int matId = sh.materialIds[f];
       if (matId == -1) throw std::runtime_error(" error face has no material ");
+      if (matId < 0 || (size_t)matId >= sc.materials.size()) throw std::runtime_error("material index out of range");
+      for (int v = 0; v < 3; ++v) {
+        const FaceIndex& ix = sh.indices[3 * f + v];
+        if (ix.v < 0 || (size_t)ix.v >= nv) throw std::runtime_error("vertex index out of range");
+        if (ix.n >= 0 && (size_t)ix.n >= nn) throw std::runtime_error("normal index out of range");
+        if (ix.t >= 0 && (size_t)ix.t >= nt) throw std::runtime_error("texcoord index out of range");
+      }
       // collectTriangle (tinyobj.cpp:7-28)
       float p[3][3];
       for (int v = 0; v < 3; ++v) {
-        const ObjIndex& ix = sh.indices[3 * f + v];
+        const FaceIndex& ix = sh.indices[3 * f + v];
         for (int c = 0; c < 3; ++c) p[v][c] = sc.vertices[3 * (size_t)ix.v + c];
       }
       for (int v = 0; v < 3; ++v)
@@ -643,7 +620,7 @@ void loadScene(const std::string& sceneJsonPath, Scene& sc) {
         if (sh.indices[3 * f + v].n < 0) allNormals = false;
       if (allNormals) {
         for (int v = 0; v < 3; ++v) {
-          const ObjIndex& ix = sh.indices[3 * f + v];
+          const FaceIndex& ix = sh.indices[3 * f + v];
           for (int c = 0; c < 3; ++c) sc.triNrm[(size_t)g * 9 + v * 3 + c] = sc.normals[3 * (size_t)ix.n + c];
         }
       } else {
@@ -670,11 +647,8 @@ void loadScene(const std::string& sceneJsonPath, Scene& sc) {
       sc.triPrimID[g] = (uint32_t)f;
 
       // Scene.cpp:159-181
-      const RawMaterial& mat = raw[matId];
-      bool isLight = mat.emission[0] > 0.0f || mat.emission[1] > 0.0f || mat.emission[2] > 0.0f;
-      if (isLight) {
-        if (mat.diffuse[0] > 0.0f || mat.diffuse[1] > 0.0f || mat.diffuse[2] > 0.0f || mat.specular[0] > 0.0f ||
-            mat.specular[1] > 0.0f || mat.specular[2] > 0.0f)
+      if (sc.materials[matId].isLight()) {
+        if ((size_t)matId < emitterTinted.size() && emitterTinted[matId])
           throw std::runtime_error(" don't support non black diffuse materials with emission ");
         sc.lightTris.push_back(g);
         float e1[3] = {p[1][0] - p[0][0], p[1][1] - p[0][1], p[1][2] - p[0][2]};
@@ -688,7 +662,51 @@ void loadScene(const std::string& sceneJsonPath, Scene& sc) {
   float la = 0.0f;
   for (float a : sc.lightAreas) la += a;  // std::accumulate(..., 0.f)
   sc.totalLightArea = la;
+  sc.lightCdf.clear();
   if (!sc.lightAreas.empty()) sc.lightCdf = buildDiscreteCdf(sc.lightAreas.data(), sc.lightAreas.size());
+}
+
+// Scene.cpp:12-195
+void loadScene(const std::string& sceneJsonPath, Scene& sc) {
+  std::string folder = parentDir(sceneJsonPath);
+  JsonValue j = readJsonFile(sceneJsonPath);
+  cameraFromJson(j.at("camera"), sc.camera);
+  const JsonValue& world = j.at("world");
+  if (world.contains("background_color")) {
+    sc.hasEnvTexture = false;
+    vec3FromJson(world.at("background_color"), sc.backgroundColor);
+  } else {
+    setEnvironment(sc, readPFM(joinPath(folder, world.at("environment_texture").asString())));
+  }
+  sc.varyGlassIOR = world.contains("vary_glass_ior") ? world.at("vary_glass_ior").asBool() : false;
+
+  std::string meshPath = joinPath(folder, j.at("mesh").asString());
+  std::vector<ObjShape> shapes;
+  std::vector<RawMaterial> raw;
+  parseObj(meshPath, folder, sc.vertices, sc.normals, sc.texcoords, shapes, raw);
+
+  // Scene.cpp:65-118
+  std::vector<uint8_t> tinted;
+  for (const RawMaterial& m : raw) {
+    if (!m.bumpTex.empty() || !m.diffuseTex.empty() || !m.specularHighlightTex.empty() || !m.metallicTex.empty())
+      throw std::runtime_error("material '" + m.name + "': texture maps are not supported by this renderer");
+    Material out;
+    out.name = m.name;
+    for (int i = 0; i < 3; ++i) out.albedo[i] = m.diffuse[i];
+    out.transmission = 1.0f - m.dissolve;
+    out.varyIOR = sc.varyGlassIOR && out.transmission > 0.0f;
+    float nSpecular = m.shininess;
+    out.roughness = 1.0f - hm::fsqrt(nSpecular / 1000.0f);
+    out.metallic = m.ambient[0];
+    out.specular = m.specular[0];
+    out.ior = m.ior;
+    out.anisotropy = m.anisotropy;
+    for (int i = 0; i < 3; ++i) out.emission[i] = m.emission[i];
+    sc.materials.push_back(out);
+    tinted.push_back(m.diffuse[0] > 0.0f || m.diffuse[1] > 0.0f || m.diffuse[2] > 0.0f || m.specular[0] > 0.0f ||
+                     m.specular[1] > 0.0f || m.specular[2] > 0.0f);
+  }
+  buildTriangles(sc, shapes, tinted);
 }
 
 // RenderJob.cpp:21-28

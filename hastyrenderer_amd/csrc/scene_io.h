@@ -148,6 +148,29 @@ struct Job {
   std::string jobPath;
 };
 
+// Faces of one shape as tinyobj lists them (indices into the attribute
+// arrays, -1 = absent), one material id and smoothing group per triangle.
+struct FaceIndex {
+  int v, t, n;
+};
+struct ShapeFaces {
+  std::string name;
+  std::vector<FaceIndex> indices;  // 3 per triangle
+  std::vector<int> materialIds;
+  std::vector<uint32_t> smoothing;
+};
+
+// The environment map and its sampling tables (Environment.h:105-118,
+// 52-58); throws if the image has no energy.
+void setEnvironment(Scene& sc, ImageRGB env);
+
+// Flattens the shapes' triangles into the Scene arrays (collectTriangle*,
+// tinyobj.cpp:7-88) and collects the surface lights (Scene.cpp:159-181,
+// 193-194).  sc.vertices / normals / texcoords / materials must be set;
+// emitterTinted[m] marks an emissive material with a non-black Kd or Ks,
+// which the reference refuses when a face uses it.
+void buildTriangles(Scene& sc, const std::vector<ShapeFaces>& shapes, const std::vector<uint8_t>& emitterTinted);
+
 // Throws std::runtime_error with the reference's messages on malformed input.
 void loadScene(const std::string& sceneJsonPath, Scene& out);
 void loadJob(const std::string& renderJobPath, Job& out);

@@ -58,6 +58,83 @@ typedef struct HastySceneInfo {
  * material, an emissive face with a non-black Kd/Ks, ...). */
 int hasty_scene_load(const char* render_job_path, HastyScene** out_scene);
 
+/* ---- scene from data the caller already holds ----
+ * What Scene::Scene (core/src/Scene.cpp:12-195) has after tinyobj and OIIO
+ * loaded the files, handed over as flat arrays instead of re-parsed, so an
+ * app that builds its own Scene keeps exactly its inputs. */
+
+/* PrincipledBRDF parameters after the MTL mapping of Scene.cpp:65-118:
+ * albedo = Kd, roughness = 1 - sqrt(Ns / 1000), metallic = Ka[0],
+ * specular = Ks[0], ior = Ni, anisotropy = aniso, transmission = 1 - d,
+ * emission = Ke (radiance Ke / pi).  tinted_emitter: the MTL's Kd or Ks is
+ * non-black (the reference refuses a face using an emissive material that
+ * is, Scene.cpp:166-173). */
+typedef struct HastyMaterialDesc {
+  float albedo[3];
+  float roughness;
+  float metallic;
+  float specular;
+  float ior;
+  float anisotropy;
+  float transmission;
+  int32_t vary_ior;  /* world.vary_glass_ior && transmission > 0 */
+  float emission[3];
+  int32_t tinted_emitter;
+} HastyMaterialDesc;
+
+/* Camera (core/include/Hasty/Camera.h:14-60) with its frame as setLookAt
+ * leaves it (Camera.cpp:31-39). */
+typedef struct HastyCameraDesc {
+  float position[3];
+  float forward[3];
+  float right[3];
+  float up[3];
+  float fov_degree;  /* 45 in the reference */
+  float aperture;
+  float focal_distance;
+  float blade_rotation;
+  int32_t num_blades;
+  float exposure;
+} HastyCameraDesc;
+
+/* Triangles are listed shape by shape, faces in order: geom_ids[i] (the
+ * tinyobj shape) never decreases and prim_ids[i] counts the faces of its
+ * shape from 0, so (geom_id, prim_id) are the ids Embree would report.
+ * Index arrays hold 3 entries per triangle into the attribute arrays
+ * (tinyobj::attrib_t), -1 = absent; normal_indices / texcoord_indices may be
+ * NULL.  env_rgb is width*height RGB float32 with row 0 at the top (OIIO's
+ * order); without an environment the background colour is used. */
+typedef struct HastySceneDesc {
+  HastyRenderSettings settings;
+  HastyCameraDesc camera;
+  uint32_t num_vertices;
+  const float* vertices;   /* 3 per vertex */
+  uint32_t num_normals;
+  const float* normals;    /* 3 per normal */
+  uint32_t num_texcoords;
+  const float* texcoords;  /* 2 per texcoord */
+  uint32_t num_triangles;
+  const int32_t* vertex_indices;
+  const int32_t* normal_indices;
+  const int32_t* texcoord_indices;
+  const uint32_t* geom_ids;
+  const uint32_t* prim_ids;
+  const int32_t* material_ids;
+  const uint8_t* smooth;   /* smoothing group != 0 (NULL: none) */
+  uint32_t num_materials;
+  const HastyMaterialDesc* materials;
+  int32_t has_environment;
+  uint32_t env_width;
+  uint32_t env_height;
+  const float* env_rgb;
+  float background[3];
+} HastySceneDesc;
+
+/* Builds the BVH on the host and uploads the scene to the current device,
+ * like hasty_scene_load after parsing.  The descriptor's arrays are copied;
+ * the caller keeps ownership. */
+int hasty_scene_create(const HastySceneDesc* desc, HastyScene** out_scene);
+
 /* Frees the scene and its device memory (Scene::~Scene, Scene.cpp:196-200). */
 void hasty_scene_destroy(HastyScene* scene);
 
@@ -90,6 +167,36 @@ int hasty_render_pt_device(HastyScene* scene, uint32_t seed, uint32_t first_samp
  * tuv[3i..] = t, u, v with barycentrics (1-u-v, u, v).  Host pointers. */
 int hasty_trace_closest(HastyScene* scene, const float* rays, uint32_t n, uint32_t* geom_prim, int32_t* tri,
                         float* tuv);
+
+/* One ray of a batched query: origin and direction (tnear 1e-6, tfar +inf,
+ * the values Scene::rayHit sets, core/src/Scene.cpp:281-291). */
+typedef struct HastyRay {
+  float org[3];
+  float dir[3];
+} HastyRay;
+
+/* The RTCHit fields Scene::rayHit reads (core/src/Scene.cpp:293-302,
+ * core/include/Hasty/Scene.h:20-46): geom_id (OBJ shape index) and prim_id
+ * (face within the shape), both 0xFFFFFFFF for a miss (RTC_INVALID_GEOMETRY_ID);
+ * the hit distance t (+inf for a miss); barycentrics (1-u-v, u, v); and the
+ * geometric normal normalize((v1 - v0) x (v2 - v0)), zero for a miss. */
+typedef struct HastyHit {
+  uint32_t geom_id;
+  uint32_t prim_id;
+  float t;
+  float u;
+  float v;
+  float ng[3];
+} HastyHit;
+
+/* Batched Scene::rayHit (core/src/Scene.cpp:281-303, declared
+ * core/include/Hasty/Scene.h:68) with the full hit record, host pointers. */
+int hasty_trace_hits(HastyScene* scene, const HastyRay* rays, uint32_t n, HastyHit* hits);
+
+/* Same with rays and hits in device memory of the scene's device.  The
+ * traversal's stack overflow buffer is the scene's (allocated on first use).
+ * Synchronous on the scene's stream. */
+int hasty_trace_hits_device(HastyScene* scene, const HastyRay* d_rays, uint32_t n, HastyHit* d_hits);
 
 /* Per-kernel timing with HIP events on the scene's stream (flags bit 0;
  * with bit 2 only the closest-hit extension launches are timed, which keeps

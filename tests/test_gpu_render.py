@@ -46,6 +46,37 @@ def test_closest_hit_matches_oracle(which, cornell_job, ggx_small_job):
     assert (gp[~hit] == 0xFFFFFFFF).all()
 
 
+@pytest.mark.parametrize("which", ["cornell", "ggx_small"])
+def test_trace_hits_records_match_oracle(which, cornell_job, ggx_small_job):
+    """hasty_trace_hits: (geomID, primID, t, u, v) bit-equal to the oracle's
+    BVH2 hit, Ng = normalize((v1 - v0) x (v2 - v0)) of the hit triangle
+    computed in float64 from the oracle's positions (within 2 ulp-ish), and
+    the Embree miss convention (0xFFFFFFFF ids, t = +inf, Ng = 0)."""
+    job = cornell_job if which == "cornell" else ggx_small_job
+    rng = np.random.default_rng(8)
+    scene = ha.Scene(job)
+    oj = po.OracleJob(job)
+    rays = _random_rays(30000, -1.2, 2.2, rng)
+    hits = scene.trace_hits(rays)
+    otri, otuv, _ = oj.trace(rays)
+    hit = otri >= 0
+    assert hit.sum() > 1000 and (~hit).sum() > 100
+    assert (hits["t"][hit].view(np.uint32) == otuv[hit, 0].view(np.uint32)).all()
+    assert (hits["u"][hit].view(np.uint32) == otuv[hit, 1].view(np.uint32)).all()
+    assert (hits["v"][hit].view(np.uint32) == otuv[hit, 2].view(np.uint32)).all()
+    ids = np.array([oj.tri_ids(t) for t in otri[hit]], np.uint32)
+    assert (hits["geom_id"][hit] == ids[:, 0]).all() and (hits["prim_id"][hit] == ids[:, 1]).all()
+    for i in np.nonzero(hit)[0][:400]:
+        p = oj.tri_pos(otri[i]).astype(np.float64)
+        n = np.cross(p[1] - p[0], p[2] - p[0])
+        assert np.allclose(hits["ng"][i], n / np.linalg.norm(n), atol=3e-7)
+    assert (hits["geom_id"][~hit] == 0xFFFFFFFF).all() and (hits["prim_id"][~hit] == 0xFFFFFFFF).all()
+    assert np.isinf(hits["t"][~hit]).all() and (hits["ng"][~hit] == 0).all()
+    # the record agrees with the older (tri, t, u, v) query
+    _, tri, tuv = scene.trace_closest(rays)
+    assert (tri == otri).all()
+
+
 def test_edge_rays_watertight(cornell_job):
     """Rays aimed exactly at shared vertices/edges of the walls never slip through."""
     scene = ha.Scene(cornell_job)
