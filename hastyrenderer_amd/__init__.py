@@ -39,6 +39,7 @@ EXPORTS = (
     "hasty_trace_closest",
     "hasty_trace_hits",
     "hasty_trace_hits_device",
+    "hasty_trace_hits_device_noalloc",
     "hasty_set_profiling",
     "hasty_kernel_stats",
     "hasty_reset_stats",
@@ -235,6 +236,7 @@ def lib():
     L.hasty_trace_closest.argtypes = [P, fp, C.c_uint32, C.POINTER(C.c_uint32), C.POINTER(C.c_int32), fp]
     L.hasty_trace_hits.argtypes = [P, C.c_void_p, C.c_uint32, C.c_void_p]
     L.hasty_trace_hits_device.argtypes = [P, C.c_void_p, C.c_uint32, C.c_void_p]
+    L.hasty_trace_hits_device_noalloc.argtypes = [P, C.c_void_p, C.c_uint32, C.c_void_p]
     L.hasty_set_profiling.argtypes = [P, C.c_int]
     L.hasty_kernel_stats.argtypes = [P, C.POINTER(KernelStatC), C.c_int]
     L.hasty_reset_stats.argtypes = [P]
@@ -478,6 +480,11 @@ class Scene:
     def trace_hits_device(self, d_rays: int, n: int, d_hits: int):
         """Rays (n x HastyRay) and hits (n x HastyHit) in device memory."""
         _err(lib().hasty_trace_hits_device(self._h, C.c_void_p(d_rays), n, C.c_void_p(d_hits)))
+
+    def trace_hits_device_noalloc(self, d_rays: int, n: int, d_hits: int):
+        """trace_hits_device without any device allocation (the library's
+        static scratch array holds the traversal stack overflow)."""
+        _err(lib().hasty_trace_hits_device_noalloc(self._h, C.c_void_p(d_rays), n, C.c_void_p(d_hits)))
 
     def set_profiling(self, events=True, counters=False, extend_only=False):
         """HIP-event timing of every launch (of the extension launches only

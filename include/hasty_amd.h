@@ -198,6 +198,16 @@ int hasty_trace_hits(HastyScene* scene, const HastyRay* rays, uint32_t n, HastyH
  * Synchronous on the scene's stream. */
 int hasty_trace_hits_device(HastyScene* scene, const HastyRay* d_rays, uint32_t n, HastyHit* d_hits);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under hipGraph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): the query of hasty_trace_hits_device with its traversal
+ * stack overflow in a static 9 MiB device array of the library instead of a
+ * scene buffer, on a fixed grid of 64 blocks.  Allocates nothing (no
+ * hipMalloc / hipFree); n at most 2^31 - 1; hits 16-byte aligned.  Every call
+ * on one GPU shares the array: calls that overlap on one GPU race.
+ * Synchronous on the scene's stream. */
+int hasty_trace_hits_device_noalloc(HastyScene* scene, const HastyRay* d_rays, uint32_t n, HastyHit* d_hits);
+
 /* Per-kernel timing with HIP events on the scene's stream (flags bit 0;
  * with bit 2 only the closest-hit extension launches are timed, which keeps
  * the events' own cost out of a timed render) and device counters (flags
