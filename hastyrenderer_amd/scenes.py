@@ -37,6 +37,51 @@ def _write_job(folder, settings, scene_name="scene.json"):
         json.dump({"settings": settings, "scene": scene_name}, f, indent=2)
 
 
+_SETTINGS_DEFAULTS = {  # RenderJob.cpp:10-20, RenderJob.h:21-29
+    "roulette_start_depth": 1, "roulette_q": 0.3, "width": 720, "height": 720, "num_threads": 4,
+    "max_depth": 30, "num_samples": 32, "use_mis": True,
+}
+
+
+def _finish(folder, obj, specs, scene, settings, env_img=None, as_desc=False):
+    """Writes renderJob.json, scene.json, scene.obj / scene.mtl (+ sky.pfm
+    when env_img is given) and returns the job path; with as_desc, writes
+    nothing and returns the hastyrenderer_amd.Scene.create arguments the
+    loader would make of those files."""
+    if not as_desc:
+        obj.write(os.path.join(folder, "scene.obj"), "scene.mtl")
+        with open(os.path.join(folder, "scene.mtl"), "w") as f:
+            f.write("\n".join(_mtl(n, **kw) for n, kw in specs))
+        if env_img is not None:
+            write_pfm(os.path.join(folder, "sky.pfm"), env_img)
+        with open(os.path.join(folder, "scene.json"), "w") as f:
+            json.dump(scene, f, indent=2)
+        _write_job(folder, settings)
+        return os.path.join(folder, "renderJob.json")
+    world = scene["world"]
+    vary = bool(world.get("vary_glass_ior", False))
+    names = [n for n, _ in specs]
+    kw = obj.arrays(names)
+    kw["materials"] = [mtl_material(vary=vary, **m) for _, m in specs]
+    cj = scene["camera"]
+    pos = cj["position"]
+    look = cj.get("look_at", [pos[0] + 0.0, pos[1] + 0.0, pos[2] - 1.0])
+    p, fwd, right, up = look_at_frame(pos, look)
+    kw["camera"] = {"position": p, "forward": fwd, "right": right, "up": up, "fov_degree": float(F32(cj.get("fov", 45.0))),
+                    "aperture": float(F32(cj.get("aperture", 0.0))),
+                    "focal_distance": float(F32(cj.get("focal_distance", 1.0))),
+                    "blade_rotation": float(F32(cj.get("blade_rotation", 0.0))),
+                    "num_blades": int(cj.get("num_blades", 8)), "exposure": float(F32(cj.get("exposure", 0.0)))}
+    st = dict(_SETTINGS_DEFAULTS)
+    st.update(settings)
+    kw["settings"] = {k: (float(v) if k == "roulette_q" else int(v)) for k, v in st.items()}
+    if "background_color" in world:
+        kw["background"] = tuple(float(F32(x)) for x in world["background_color"])
+    else:
+        kw["env_rgb"] = np.ascontiguousarray(env_img, np.float32)
+    return kw
+
+
 def write_pfm(path, rgb_top_first: np.ndarray):
     """Colour PFM, rows bottom-to-top as the format specifies."""
     h, w, _ = rgb_top_first.shape
@@ -61,48 +106,99 @@ def read_pfm(path) -> np.ndarray:
 
 
 class _Obj:
+    """An OBJ file under construction.  Records the statements in order and
+    renders them either as OBJ text (write) or as the arrays the loader would
+    make of that text (arrays), so large scenes need no text round trip."""
+
     def __init__(self):
-        self.lines = []
+        self.ops = []
         self.nv = 0
         self.nn = 0
 
     def shape(self, name):
-        self.lines.append("o %s" % name)
+        self.ops.append(("o", name))
 
     def material(self, name):
-        self.lines.append("usemtl %s" % name)
+        self.ops.append(("usemtl", name))
 
     def smooth(self, on):
-        self.lines.append("s %s" % ("1" if on else "off"))
+        self.ops.append(("s", bool(on)))
 
     def vertices(self, v):
         base = self.nv
-        for p in np.asarray(v, dtype=F32):
-            self.lines.append("v %s %s %s" % (_fmt(p[0]), _fmt(p[1]), _fmt(p[2])))
+        v = np.asarray(v, dtype=F32).reshape(-1, 3)
+        self.ops.append(("v", v))
         self.nv += len(v)
         return base
 
     def normals(self, n):
         base = self.nn
-        for p in np.asarray(n, dtype=F32):
-            self.lines.append("vn %s %s %s" % (_fmt(p[0]), _fmt(p[1]), _fmt(p[2])))
+        n = np.asarray(n, dtype=F32).reshape(-1, 3)
+        self.ops.append(("vn", n))
         self.nn += len(n)
         return base
 
     def faces(self, tris, vbase, nbase=None):
-        for t in tris:
-            a, b, c = (int(i) + vbase + 1 for i in t)
-            if nbase is None:
-                self.lines.append("f %d %d %d" % (a, b, c))
-            else:
-                na, nb, nc = (int(i) + nbase + 1 for i in t)
-                self.lines.append("f %d//%d %d//%d %d//%d" % (a, na, b, nb, c, nc))
+        t = np.asarray(tris, dtype=np.int64).reshape(-1, 3)
+        self.ops.append(("f", t + vbase, None if nbase is None else t + nbase))
 
     def write(self, path, mtllib):
+        lines = ["mtllib %s" % mtllib]
+        for op in self.ops:
+            k = op[0]
+            if k == "o":
+                lines.append("o %s" % op[1])
+            elif k == "usemtl":
+                lines.append("usemtl %s" % op[1])
+            elif k == "s":
+                lines.append("s %s" % ("1" if op[1] else "off"))
+            elif k in ("v", "vn"):
+                lines.extend("%s %s %s %s" % (k, _fmt(p[0]), _fmt(p[1]), _fmt(p[2])) for p in op[1])
+            else:
+                v1 = op[1] + 1
+                if op[2] is None:
+                    lines.extend("f %d %d %d" % tuple(r) for r in v1.tolist())
+                else:
+                    n1 = op[2] + 1
+                    lines.extend("f %d//%d %d//%d %d//%d" % (a, na, b, nb, c, nc)
+                                 for (a, b, c), (na, nb, nc) in zip(v1.tolist(), n1.tolist()))
         with open(path, "w") as f:
-            f.write("mtllib %s\n" % mtllib)
-            f.write("\n".join(self.lines))
+            f.write("\n".join(lines))
             f.write("\n")
+
+    def arrays(self, material_names):
+        """The loader's reading of this file (Scene.cpp:32-58, tinyobj
+        shapes split at o/g, empty shapes dropped): vertices, normals and
+        per-triangle index / id arrays in Scene.create's form."""
+        vs = [op[1] for op in self.ops if op[0] == "v"]
+        ns = [op[1] for op in self.ops if op[0] == "vn"]
+        tri_v, tri_n, gid, pid, mid, smooth = [], [], [], [], [], []
+        shape_faces = 0
+        g = -1
+        cur_mat, cur_s = -1, False
+        for op in self.ops:
+            if op[0] == "o":  # a new shape: its first face takes the next geomID
+                shape_faces = 0
+            elif op[0] == "usemtl":
+                cur_mat = material_names.index(op[1]) if op[1] in material_names else -1
+            elif op[0] == "s":
+                cur_s = op[1]
+            elif op[0] == "f":
+                if shape_faces == 0:
+                    g += 1
+                k = len(op[1])
+                tri_v.append(op[1])
+                tri_n.append(op[2] if op[2] is not None else np.full((k, 3), -1, np.int64))
+                gid.append(np.full(k, g, np.uint32))
+                pid.append(np.arange(shape_faces, shape_faces + k, dtype=np.uint32))
+                mid.append(np.full(k, cur_mat, np.int32))
+                smooth.append(np.full(k, 1 if cur_s else 0, np.uint8))
+                shape_faces += k
+        cat = lambda a, dt, shape: np.concatenate(a).astype(dt) if a else np.zeros(shape, dt)
+        return dict(vertices=cat(vs, np.float32, (0, 3)), normals=cat(ns, np.float32, (0, 3)),
+                    vertex_indices=cat(tri_v, np.int32, (0, 3)), normal_indices=cat(tri_n, np.int32, (0, 3)),
+                    geom_ids=cat(gid, np.uint32, 0), prim_ids=cat(pid, np.uint32, 0),
+                    material_ids=cat(mid, np.int32, 0), smooth=cat(smooth, np.uint8, 0))
 
 
 def _mtl(name, kd=(0, 0, 0), ka=(0, 0, 0), ks=(0, 0, 0), ke=(0, 0, 0), ns=0.0, ni=1.0, d=1.0):
@@ -311,15 +407,10 @@ def _random_rotations(rng, n):
     ], 1)
 
 
-def write_cuboid_scene(folder, n_cuboids, width, height, spp, seed, max_depth=8):
-    """One batching job: `n_cuboids` random cuboids (positions uniform in the
-    unit cube, edge lengths uniform in [0.002, 0.05], uniform random
-    rotations) split evenly at random over three material classes drawn once
-    per job -- Lambertian, GGX conductor with alpha in U[0.05, 0.8], smooth
-    glass with IOR in U[1.3, 1.8] -- merged into one shape per class (12
-    triangles per cuboid), one 0.3 x 0.3 area light above the cube, dim grey
-    background, pinhole camera looking at the cube."""
-    os.makedirs(folder, exist_ok=True)
+def _cuboid_parts(n_cuboids, seed):
+    """The random content of one batching job (see write_cuboid_scene):
+    material parameters as _mtl arguments and, per material class, the
+    (k, 8, 3) float32 corners of its cuboids."""
     rng = np.random.default_rng(seed)
     alpha = rng.uniform(0.05, 0.8)
     ior = rng.uniform(1.3, 1.8)
@@ -333,64 +424,159 @@ def write_cuboid_scene(folder, n_cuboids, width, height, spp, seed, max_depth=8)
     albedo = rng.uniform(0.3, 0.9, 3)
     f0 = rng.uniform(0.5, 1.0, 3)
     ns = 1000.0 * (1.0 - math.sqrt(alpha)) ** 2  # roughness = 1 - sqrt(Ns / 1000), alpha = roughness^2
-    mtl = [
-        _mtl("lambert", kd=tuple(albedo)),
-        _mtl("ggx", kd=tuple(f0), ka=(1.0, 1.0, 1.0), ns=ns),
-        _mtl("glass", kd=(1.0, 1.0, 1.0), ns=1000.0, ni=ior, d=0.0),
-        _mtl("light", ke=(20.0, 20.0, 20.0)),
+    mats = [
+        ("lambert", dict(kd=tuple(albedo))),
+        ("ggx", dict(kd=tuple(f0), ka=(1.0, 1.0, 1.0), ns=ns)),
+        ("glass", dict(kd=(1.0, 1.0, 1.0), ns=1000.0, ni=ior, d=0.0)),
+        ("light", dict(ke=(20.0, 20.0, 20.0))),
     ]
-    lines = ["mtllib scene.mtl"]
-    vbase = 0
+    groups = []
     for c, name in enumerate(("lambert", "ggx", "glass")):
         sel = np.nonzero(cls == c)[0]
-        if len(sel) == 0:
+        groups.append((name, verts[sel].astype(np.float32)))
+    return mats, groups
+
+
+_CUBOID_LIGHT = np.array([[0.35, 1.6, 0.35], [0.35, 1.6, 0.65], [0.65, 1.6, 0.65], [0.65, 1.6, 0.35]], np.float32)
+_CUBOID_CAMERA = {"position": [0.5, 0.9, 2.6], "look_at": [0.5, 0.5, 0.5], "fov": 40.0}
+
+
+def _cuboid_settings(width, height, spp, max_depth):
+    return {"width": int(width), "height": int(height), "num_samples": int(spp), "max_depth": max_depth,
+            "roulette_start_depth": 3, "roulette_q": 0.3, "use_mis": True, "num_threads": 1}
+
+
+def write_cuboid_scene(folder, n_cuboids, width, height, spp, seed, max_depth=8):
+    """One batching job: `n_cuboids` random cuboids (positions uniform in the
+    unit cube, edge lengths uniform in [0.002, 0.05], uniform random
+    rotations) split evenly at random over three material classes drawn once
+    per job -- Lambertian, GGX conductor with alpha in U[0.05, 0.8], smooth
+    glass with IOR in U[1.3, 1.8] -- merged into one shape per class (12
+    triangles per cuboid), one 0.3 x 0.3 area light above the cube, dim grey
+    background, pinhole camera looking at the cube."""
+    os.makedirs(folder, exist_ok=True)
+    mats, groups = _cuboid_parts(n_cuboids, seed)
+    mtl = [_mtl(name, **kw) for name, kw in mats]
+    lines = ["mtllib scene.mtl"]
+    vbase = 0
+    for name, corners in groups:
+        if len(corners) == 0:
             continue
         lines.append("o %s" % name)
         lines.append("s off")
         lines.append("usemtl %s" % name)
-        v = verts[sel].reshape(-1, 3).astype(np.float32)
+        v = corners.reshape(-1, 3)
         lines.append("\n".join("v %.9g %.9g %.9g" % tuple(r) for r in v.tolist()))
-        f = (_CUBE_TRIS[None, :, :] + 8 * np.arange(len(sel))[:, None, None]).reshape(-1, 3) + vbase + 1
+        f = (_CUBE_TRIS[None, :, :] + 8 * np.arange(len(corners))[:, None, None]).reshape(-1, 3) + vbase + 1
         lines.append("\n".join("f %d %d %d" % tuple(r) for r in f.tolist()))
         vbase += len(v)
     lines.append("o light")
     lines.append("s off")
     lines.append("usemtl light")
-    lines.append("v 0.35 1.6 0.35\nv 0.35 1.6 0.65\nv 0.65 1.6 0.65\nv 0.65 1.6 0.35")
+    lines.append("v 0.35 1.6 0.35\nv 0.35 1.6 0.65\nv 0.65 1.6 0.65\nv 0.65 1.6 0.35")  # _CUBOID_LIGHT
     lines.append("f %d %d %d\nf %d %d %d" % (vbase + 1, vbase + 2, vbase + 3, vbase + 1, vbase + 3, vbase + 4))
     with open(os.path.join(folder, "scene.obj"), "w") as fh:
         fh.write("\n".join(lines) + "\n")
     with open(os.path.join(folder, "scene.mtl"), "w") as fh:
         fh.write("\n".join(mtl))
-    scene = {
-        "camera": {"position": [0.5, 0.9, 2.6], "look_at": [0.5, 0.5, 0.5], "fov": 40.0},
-        "world": {"background_color": [0.05, 0.05, 0.05]},
-        "mesh": "scene.obj",
-    }
+    scene = {"camera": dict(_CUBOID_CAMERA), "world": {"background_color": [0.05, 0.05, 0.05]}, "mesh": "scene.obj"}
     with open(os.path.join(folder, "scene.json"), "w") as fh:
         json.dump(scene, fh, indent=2)
-    _write_job(folder, {"width": int(width), "height": int(height), "num_samples": int(spp),
-                        "max_depth": max_depth, "roulette_start_depth": 3, "roulette_q": 0.3, "use_mis": True,
-                        "num_threads": 1})
+    _write_job(folder, _cuboid_settings(width, height, spp, max_depth))
     return os.path.join(folder, "renderJob.json")
 
 
-def batch_jobs(folder, n_jobs=32, seed=2, max_pixels=1024 * 1024, min_pixels=128 * 128, spp_range=(16, 256),
-               cuboid_range=(1000, 160000)):
-    """The batching workload: `n_jobs` cuboid scenes with resolution (pixel
-    count) log-uniform in [min_pixels, max_pixels] (square films), spp
-    log-uniform in spp_range and cuboid count log-uniform in cuboid_range.
-    Returns the job paths."""
+def mtl_material(kd=(0, 0, 0), ka=(0, 0, 0), ks=(0, 0, 0), ke=(0, 0, 0), ns=0.0, ni=1.0, d=1.0, vary=False):
+    """The material the loader makes of _mtl(name, ...) (the MTL mapping of
+    Scene.cpp:65-118 on the float32 values the file holds), as a
+    hastyrenderer_amd.Scene.create material dict."""
+    f = [F32(x) for x in kd], [F32(x) for x in ka], [F32(x) for x in ks], [F32(x) for x in ke]
+    kd32, ka32, ks32, ke32 = f
+    transmission = F32(1.0) - F32(d)
+    return {"albedo": [float(x) for x in kd32], "roughness": float(F32(1.0) - np.sqrt(F32(ns) / F32(1000.0))),
+            "metallic": float(ka32[0]), "specular": float(ks32[0]), "ior": float(F32(ni)), "anisotropy": 0.0,
+            "transmission": float(transmission), "vary_ior": int(bool(vary) and transmission > 0),
+            "emission": [float(x) for x in ke32],
+            "tinted_emitter": int(any(x > 0 for x in kd32) or any(x > 0 for x in ks32))}
+
+
+def look_at_frame(pos, look):
+    """Camera::setLookAt (Camera.cpp:31-39) in float32 with the loader's
+    operation order: position, forward, right, up."""
+    pos = [F32(x) for x in pos]
+    look = [F32(x) for x in look]
+    f = [look[i] - pos[i] for i in range(3)]
+    n = np.sqrt(f[0] * f[0] + f[1] * f[1] + f[2] * f[2])
+    fwd = [f[i] / n for i in range(3)]
+    u = [F32(0.0), F32(1.0), F32(0.0)]
+    dd = u[0] * fwd[0] + u[1] * fwd[1] + u[2] * fwd[2]
+    u = [u[i] - fwd[i] * dd for i in range(3)]
+    un = np.sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2])
+    up = [u[i] / un for i in range(3)]
+    right = [fwd[1] * up[2] - fwd[2] * up[1], fwd[2] * up[0] - fwd[0] * up[2], fwd[0] * up[1] - fwd[1] * up[0]]
+    return [[float(x) for x in v] for v in (pos, fwd, right, up)]
+
+
+def cuboid_scene_desc(n_cuboids, width, height, spp, seed, max_depth=8):
+    """The scene write_cuboid_scene writes, as hastyrenderer_amd.Scene.create
+    arguments (no files): the same float32 positions, material parameters,
+    camera frame and settings the loader reads from those files."""
+    mats, groups = _cuboid_parts(n_cuboids, seed)
+    names = [m[0] for m in mats]
+    materials = [mtl_material(**kw) for _, kw in mats]
+    vparts, iparts, gids, pids, mids = [], [], [], [], []
+    vbase = 0
+    g = 0
+    for name, corners in groups + [("light", None)]:
+        if corners is None:
+            v = _CUBOID_LIGHT
+            t = np.array([[0, 1, 2], [0, 2, 3]])
+        elif len(corners) == 0:
+            continue
+        else:
+            v = corners.reshape(-1, 3)
+            t = (_CUBE_TRIS[None, :, :] + 8 * np.arange(len(corners))[:, None, None]).reshape(-1, 3)
+        vparts.append(v)
+        iparts.append(t + vbase)
+        gids.append(np.full(len(t), g, np.uint32))
+        pids.append(np.arange(len(t), dtype=np.uint32))
+        mids.append(np.full(len(t), names.index(name), np.int32))
+        vbase += len(v)
+        g += 1
+    pos, fwd, right, up = look_at_frame(_CUBOID_CAMERA["position"], _CUBOID_CAMERA["look_at"])
+    camera = {"position": pos, "forward": fwd, "right": right, "up": up, "fov_degree": _CUBOID_CAMERA["fov"],
+              "aperture": 0.0, "focal_distance": 1.0, "blade_rotation": 0.0, "num_blades": 8, "exposure": 0.0}
+    st = _cuboid_settings(width, height, spp, max_depth)
+    settings = {k: (float(v) if k == "roulette_q" else int(v)) for k, v in st.items()}
+    return dict(settings=settings, camera=camera, vertices=np.concatenate(vparts).astype(np.float32),
+                vertex_indices=np.concatenate(iparts).astype(np.int32), geom_ids=np.concatenate(gids),
+                prim_ids=np.concatenate(pids), material_ids=np.concatenate(mids), materials=materials,
+                background=(0.05, 0.05, 0.05))
+
+
+def batch_job_specs(n_jobs=32, seed=2, max_pixels=1024 * 1024, min_pixels=128 * 128, spp_range=(16, 256),
+                    cuboid_range=(1000, 160000)):
+    """(side, spp, cuboids, scene seed) of each batching job: pixel count
+    log-uniform in [min_pixels, max_pixels] (square films), spp log-uniform in
+    spp_range, cuboid count log-uniform in cuboid_range."""
     rng = np.random.default_rng(seed)
-    paths = []
-    for j in range(n_jobs):
+    specs = []
+    for _ in range(n_jobs):
         px = math.exp(rng.uniform(math.log(min_pixels), math.log(max_pixels)))
         side = max(8, int(round(math.sqrt(px))))
         spp = int(round(math.exp(rng.uniform(math.log(spp_range[0]), math.log(spp_range[1])))))
         n = int(round(math.exp(rng.uniform(math.log(cuboid_range[0]), math.log(cuboid_range[1])))))
-        paths.append(write_cuboid_scene(os.path.join(folder, "job%02d" % j), n, side, side, spp,
-                                        seed=int(rng.integers(1 << 31))))
-    return paths
+        specs.append((side, spp, n, int(rng.integers(1 << 31))))
+    return specs
+
+
+def batch_jobs(folder, n_jobs=32, seed=2, max_pixels=1024 * 1024, min_pixels=128 * 128, spp_range=(16, 256),
+               cuboid_range=(1000, 160000)):
+    """The batching workload: one cuboid scene per batch_job_specs entry,
+    written under `folder`.  Returns the job paths."""
+    specs = batch_job_specs(n_jobs, seed, max_pixels, min_pixels, spp_range, cuboid_range)
+    return [write_cuboid_scene(os.path.join(folder, "job%02d" % j), n, side, side, spp, seed=s)
+            for j, (side, spp, n, s) in enumerate(specs)]
 
 
 def icosphere(subdiv):
@@ -450,19 +636,21 @@ def sky_texture(width=2048, height=1024, sun_elev_deg=35.0, sun_azim_deg=60.0, s
 
 
 def write_ggx_grid(folder, width=1024, height=1024, spp=256, max_depth=16, seed=1, env_size=(2048, 1024),
-                   subdiv=4):
+                   subdiv=4, as_desc=False):
     """7 x 7 icospheres (subdivision 4: 5,120 triangles each) on a
     2-triangle ground plane; GGX alpha linear in [0.02, 1] across columns;
     rows alternate conductor (metallic, F0 ~ U[0.5, 1]^3) and opaque
     dielectric (IOR 1.5, specular 0.08 -> F0 = 0.04); importance-sampled
     2048 x 1024 sky with a 5e4 sun of 0.5 degrees; thin lens f/2.8
     (36 mm film at the fixed 45 degree FOV -> f = 43.5 mm, aperture radius
-    7.8 mm, 1 unit = 1 m), 8 blades."""
-    os.makedirs(folder, exist_ok=True)
+    7.8 mm, 1 unit = 1 m), 8 blades.  as_desc: no files, Scene.create
+    arguments instead (see _finish)."""
+    if not as_desc:
+        os.makedirs(folder, exist_ok=True)
     rng = np.random.default_rng(seed)
     verts, faces = icosphere(subdiv)
     obj = _Obj()
-    mtl = [_mtl("ground", kd=(0.5, 0.5, 0.5), ns=0.0)]
+    specs = [("ground", dict(kd=(0.5, 0.5, 0.5), ns=0.0))]
     obj.shape("spheres")
     obj.smooth(True)
     radius = 0.4
@@ -473,10 +661,10 @@ def write_ggx_grid(folder, width=1024, height=1024, spp=256, max_depth=16, seed=
             name = "sphere_%d_%d" % (r, c)
             if r % 2 == 0:
                 f0 = rng.uniform(0.5, 1.0, 3)
-                mtl.append(_mtl(name, kd=tuple(f0), ka=(1, 1, 1), ns=ns))
+                specs.append((name, dict(kd=tuple(f0), ka=(1, 1, 1), ns=ns)))
             else:
                 kd = rng.uniform(0.2, 0.8, 3)
-                mtl.append(_mtl(name, kd=tuple(kd), ks=(0.08, 0.08, 0.08), ns=ns, ni=1.5))
+                specs.append((name, dict(kd=tuple(kd), ks=(0.08, 0.08, 0.08), ns=ns, ni=1.5)))
             center = np.array([(c - 3) * 1.0, radius, (r - 3) * 1.0])
             obj.material(name)
             vb = obj.vertices(verts * radius + center)
@@ -488,11 +676,7 @@ def write_ggx_grid(folder, width=1024, height=1024, spp=256, max_depth=16, seed=
     g = 12.0
     vb = obj.vertices([[-g, 0, -g], [-g, 0, g], [g, 0, g], [g, 0, -g]])
     obj.faces([(0, 1, 2), (0, 2, 3)], vb)  # normal +y
-    obj.write(os.path.join(folder, "scene.obj"), "scene.mtl")
-    with open(os.path.join(folder, "scene.mtl"), "w") as f:
-        f.write("\n".join(mtl))
     img, _ = sky_texture(*env_size)
-    write_pfm(os.path.join(folder, "sky.pfm"), img)
     pos = np.array([0.0, 4.0, 8.5])
     look = np.array([0.0, 0.4, 0.0])
     focal_mm = 18.0 / math.tan(math.radians(22.5))
@@ -507,11 +691,9 @@ def write_ggx_grid(folder, width=1024, height=1024, spp=256, max_depth=16, seed=
         "world": {"environment_texture": "sky.pfm"},
         "mesh": "scene.obj",
     }
-    with open(os.path.join(folder, "scene.json"), "w") as f:
-        json.dump(scene, f, indent=2)
-    _write_job(folder, {"width": width, "height": height, "num_samples": spp, "max_depth": max_depth,
-                        "use_mis": True, "num_threads": 1})
-    return os.path.join(folder, "renderJob.json")
+    settings = {"width": width, "height": height, "num_samples": spp, "max_depth": max_depth, "use_mis": True,
+                "num_threads": 1}
+    return _finish(folder, obj, specs, scene, settings, env_img=img, as_desc=as_desc)
 
 
 def torus(major_r, minor_r, n_major, n_minor):
@@ -532,20 +714,22 @@ def torus(major_r, minor_r, n_major, n_minor):
 
 
 def write_caustic_scene(folder, width=1920, height=1080, passes=100, n_major=2000, n_minor=500, max_depth=16,
-                        light_radiance=40.0):
+                        light_radiance=40.0, as_desc=False):
     """Caustic box: a triangulated glass prism (IOR 1.5, RGB dispersion via
     world.vary_glass_ior) and a GGX metal torus ring (R = 1, r = 0.08,
     alpha = 0.05; 2 * n_major * n_minor triangles, 2M at the defaults) inside
-    a closed diffuse box lit by a square light covering 1% of the ceiling."""
-    os.makedirs(folder, exist_ok=True)
+    a closed diffuse box lit by a square light covering 1% of the ceiling.
+    as_desc: no files, Scene.create arguments instead (see _finish)."""
+    if not as_desc:
+        os.makedirs(folder, exist_ok=True)
     obj = _Obj()
     alpha = 0.05
-    mtl = [
-        _mtl("wall", kd=(0.7, 0.7, 0.7)),
-        _mtl("floor", kd=(0.6, 0.55, 0.5)),
-        _mtl("glass", kd=(1.0, 1.0, 1.0), ns=1000.0, ni=1.5, d=0.0),
-        _mtl("gold", kd=(1.0, 0.78, 0.34), ka=(1, 1, 1), ns=1000.0 * (1.0 - math.sqrt(alpha)) ** 2),
-        _mtl("light", ke=tuple(math.pi * c for c in (light_radiance, light_radiance * 0.92, light_radiance * 0.8))),
+    specs = [
+        ("wall", dict(kd=(0.7, 0.7, 0.7))),
+        ("floor", dict(kd=(0.6, 0.55, 0.5))),
+        ("glass", dict(kd=(1.0, 1.0, 1.0), ns=1000.0, ni=1.5, d=0.0)),
+        ("gold", dict(kd=(1.0, 0.78, 0.34), ka=(1, 1, 1), ns=1000.0 * (1.0 - math.sqrt(alpha)) ** 2)),
+        ("light", dict(ke=tuple(math.pi * c for c in (light_radiance, light_radiance * 0.92, light_radiance * 0.8)))),
     ]
     X, Y, Z = 2.0, 3.0, 2.0
     quad = [(0, 1, 2), (0, 2, 3)]
@@ -589,16 +773,11 @@ def write_caustic_scene(folder, width=1920, height=1080, passes=100, n_major=200
     obj.material("light")
     ls = 0.5 * math.sqrt(0.01 * (2 * X) * (2 * Z))
     obj.faces(quad, obj.vertices([[-ls, Y - 0.001, -ls], [ls, Y - 0.001, -ls], [ls, Y - 0.001, ls], [-ls, Y - 0.001, ls]]))
-    obj.write(os.path.join(folder, "scene.obj"), "scene.mtl")
-    with open(os.path.join(folder, "scene.mtl"), "w") as f:
-        f.write("\n".join(mtl))
     scene = {
         "camera": {"position": [0.0, 1.7, 1.95], "look_at": [0.0, 0.2, -0.2]},
         "world": {"background_color": [0.0, 0.0, 0.0], "vary_glass_ior": True},
         "mesh": "scene.obj",
     }
-    with open(os.path.join(folder, "scene.json"), "w") as f:
-        json.dump(scene, f, indent=2)
-    _write_job(folder, {"width": width, "height": height, "num_samples": passes, "max_depth": max_depth,
-                        "use_mis": True, "num_threads": 1})
-    return os.path.join(folder, "renderJob.json")
+    settings = {"width": width, "height": height, "num_samples": passes, "max_depth": max_depth, "use_mis": True,
+                "num_threads": 1}
+    return _finish(folder, obj, specs, scene, settings, as_desc=as_desc)

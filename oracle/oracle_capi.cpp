@@ -370,11 +370,12 @@ int oracle_sppm_create(const char* path, uint32_t seed, const float* params, voi
     job->pt.settings.numSamples = j.settings.numSamples;
     job->pt.settings.useMIS = j.settings.useMIS;
     job->pt.seed = seed;
-    if (params) {  // alpha, footprint scale, store fraction, emit fraction (<= 0 keeps the default)
+    if (params) {  // alpha, footprint scale, store fraction, emit fraction, initial radius (<= 0: default)
       if (params[0] > 0) job->params.alpha = params[0];
       if (params[1] > 0) job->params.footprintScale = params[1];
       if (params[2] > 0) job->params.storeFraction = params[2];
       if (params[3] > 0) job->params.emitFraction = params[3];
+      if (params[4] > 0) job->params.initialRadius = params[4];
     }
     *out = job.release();
     return 0;

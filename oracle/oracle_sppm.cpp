@@ -301,7 +301,8 @@ void sppmPass(SppmJob& job, uint32_t pass, float* color, float* normal, float* a
   uint32_t W = pt.settings.width, H = pt.settings.height;
   if (!job.initialised) {
     job.pixels.assign((size_t)W * H, PixelStat());
-    if (scene.hasSurfaceLight()) job.radii = computeInitialRadii(pt, job.params.footprintScale);
+    if (job.params.initialRadius > 0.0f) job.radii.assign((size_t)W * H, job.params.initialRadius);
+    else if (scene.hasSurfaceLight()) job.radii = computeInitialRadii(pt, job.params.footprintScale);
     else job.radii.assign((size_t)W * H, 1.0f);
     job.initialised = true;
   }

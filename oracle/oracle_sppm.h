@@ -32,6 +32,8 @@ struct SppmParams {
   float footprintScale = 10.0f; // PhotonMapping.cpp:840
   double storeFraction = 0.5;   // countThreshold = 0.5 * W * H (PhotonMapping.cpp:834)
   double emitFraction = 10.0;   // tracethreshold = 10 * W * H (PhotonMapping.cpp:835)
+  float initialRadius = 0.0f;   // > 0: this constant initial radius (no counterpart; PhotonMapping.cpp:838-841
+                                // uses footprintScale x the pixel footprint)
 };
 
 struct PixelStat {  // PMRenderJobPixel (PhotonMapping.h:12-31)

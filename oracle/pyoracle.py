@@ -285,9 +285,9 @@ class OracleSppm:
     """The photon mapper's oracle: one job, passes rendered in order."""
 
     def __init__(self, job_path: str, seed: int = 0, alpha=0.0, footprint_scale=0.0, store_fraction=0.0,
-                 emit_fraction=0.0, width=0, height=0):
+                 emit_fraction=0.0, width=0, height=0, initial_radius=0.0):
         h = C.c_void_p()
-        params = f32([alpha, footprint_scale, store_fraction, emit_fraction])
+        params = f32([alpha, footprint_scale, store_fraction, emit_fraction, initial_radius])
         if lib().oracle_sppm_create(job_path.encode(), seed, params, C.byref(h)) != 0:
             raise RuntimeError(lib().oracle_last_error().decode())
         self.h = h
