@@ -197,6 +197,9 @@ class SppmParamsC(C.Structure):
         ("store_fraction", C.c_double),
         ("emit_fraction", C.c_double),
         ("initial_radius", C.c_float),
+        ("fixed_point_flux", C.c_int32),
+        ("flux_frac_bits", C.c_int32),
+        ("gather_share_candidates", C.c_uint32),
     ]
 
 
@@ -208,6 +211,7 @@ class SppmInfoC(C.Structure):
         ("passes_done", C.c_uint32),
         ("emit_limit", C.c_uint32),
         ("store_limit", C.c_uint32),
+        ("flux_overflows", C.c_uint32),
     ]
 
 
@@ -642,9 +646,11 @@ class Sppm:
     statistics and the radius image stay on the device between passes."""
 
     def __init__(self, scene: Scene, seed: int, alpha=0.4, footprint_scale=10.0, store_fraction=0.5,
-                 emit_fraction=10.0, initial_radius=0.0):
+                 emit_fraction=10.0, initial_radius=0.0, fixed_point_flux=False, flux_frac_bits=0,
+                 gather_share_candidates=0):
         self.scene = scene  # keeps the scene alive
-        p = SppmParamsC(alpha, footprint_scale, store_fraction, emit_fraction, initial_radius)
+        p = SppmParamsC(alpha, footprint_scale, store_fraction, emit_fraction, initial_radius,
+                        1 if fixed_point_flux else 0, flux_frac_bits, gather_share_candidates)
         h = C.c_void_p()
         _err(lib().hasty_sppm_create(scene.handle, seed, C.byref(p), C.byref(h)))
         self._h = h
@@ -676,7 +682,7 @@ class Sppm:
         i = SppmInfoC()
         _err(lib().hasty_sppm_info(self._h, C.byref(i)))
         return {"emitted": i.emitted, "stored": i.stored, "cell_size": i.cell_size, "passes_done": i.passes_done,
-                "emit_limit": i.emit_limit, "store_limit": i.store_limit}
+                "emit_limit": i.emit_limit, "store_limit": i.store_limit, "flux_overflows": i.flux_overflows}
 
     def pixels(self) -> dict:
         """PMRenderJobPixel planes: radius, N, Ne, flux (h, w, 3), LSum (h, w, 3), Nd (int32)."""

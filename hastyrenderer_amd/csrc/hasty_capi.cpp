@@ -1183,7 +1183,11 @@ int hasty_sppm_create(HastyScene* s, uint32_t seed, const HastySppmParams* param
       p.initialRadius = params->initial_radius;
       p.storeFraction = params->store_fraction;
       p.emitFraction = params->emit_fraction;
+      p.fixedPointFlux = params->fixed_point_flux != 0;
+      if (params->flux_frac_bits) p.fluxFracBits = params->flux_frac_bits;
+      p.gatherShareCandidates = params->gather_share_candidates;
     }
+    if (p.fluxFracBits < 0 || p.fluxFracBits > 40) throw std::runtime_error("flux_frac_bits must be in [0, 40]");
     if (!(p.alpha > 0.0f && p.alpha <= 1.0f)) throw std::runtime_error("alpha must be in (0, 1]");
     hdev::DevScene d = s->dev;
     d.seed = seed;
@@ -1251,6 +1255,7 @@ int hasty_sppm_info(const HastySppm* h, HastySppmInfo* info) {
   info->passes_done = h->r->passesDone();
   info->emit_limit = h->r->emitLimit();
   info->store_limit = h->r->storeLimit();
+  info->flux_overflows = i.fluxOverflows;
   return 0;
 }
 

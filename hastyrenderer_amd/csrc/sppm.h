@@ -17,11 +17,15 @@ struct SppmParams {
   double storeFraction = 0.5;    // stop emitting once 0.5 W H photons are stored (PhotonMapping.cpp:834)
   double emitFraction = 10.0;    // ... or 10 W H photons were emitted (PhotonMapping.cpp:835)
   float initialRadius = 0.0f;    // > 0: this constant initial radius instead of the footprint estimate
+  bool fixedPointFlux = false;   // gathers sum photon terms in int64 units of 2^-fluxFracBits
+  int fluxFracBits = 24;
+  uint32_t gatherShareCandidates = 0;  // 0: default threshold for a wave's shared gather
 };
 
 struct SppmPassInfo {
   uint64_t emitted = 0, stored = 0;
   float cellSize = 0.0f;
+  uint32_t fluxOverflows = 0;  // gathers whose fixed-point sum saturated
 };
 
 // Device state of one progressive photon-mapping job: per-pixel statistics

@@ -101,9 +101,22 @@ struct SppmDev {
   float4* query;
   uint32_t* nq;  // queries per pixel
   float* Lpath;  // 3P: LL + LR + LS of the pass's camera path
+  // gathers: int64 fixed-point flux with fluxScale = 2^S units per 1
+  // (fluxInvScale = 2^-S), and the candidate count above which a wave
+  // gathers its queries together
+  float fluxScale;
+  double fluxInvScale;
+  uint32_t coopCandidates;
 };
 
-enum Counter { kCtrStored = 0, kCtrEmitted = 1, kCtrBatchEmit = 2, kCtrKeyOverflow = 3, kNumCtr = 4 };
+enum Counter {
+  kCtrStored = 0,
+  kCtrEmitted = 1,
+  kCtrBatchEmit = 2,
+  kCtrKeyOverflow = 3,
+  kCtrFluxOverflow = 4,  // gathers whose fixed-point flux sum saturated
+  kNumCtr = 5
+};
 
 // ------------------------------------------------------------ cell size ----
 // PhotonMapping.cpp:522-576 as three reductions.  Each block writes its
@@ -473,15 +486,88 @@ __device__ inline uint2 cellLookup(const SppmDev& g, unsigned long long key) {
 // shared gather the owner's cell lookups are spread over the lanes (one hash
 // probe each, all in flight together) and the cells' photon runs read back
 // from LDS in cell order; `runs` is this wave's 64-entry LDS scratch.
+// The sum of a gather's photon terms.  FloatSum: FP32 in the order the
+// terms arrive (the reference's, for a lane gathering alone).  FixedSum:
+// each term rounded to an integer number of 2^-S units (round to nearest
+// even) and summed in int64, so the sum does not depend on the order of the
+// terms -- a lane gathering alone, the 64 ranks of a shared gather, any cell
+// size, and (for several GPUs) any split of the photons give the same bits.
+// A term or a sum outside the int64 range saturates and sets bit 0 of
+// `flags` (the pass counts such gathers); a non-finite term sets bit 1 and
+// makes the estimate NaN, as the FP32 sum would be.
+struct FloatSum {
+  V3 v = V3{0.0f, 0.0f, 0.0f};
+  __device__ void add(V3 t, const SppmDev&) { v = v + t; }
+  __device__ void reduceWave() {
+    for (int off = 32; off > 0; off >>= 1) {
+      v.x += __shfl_xor(v.x, off);
+      v.y += __shfl_xor(v.y, off);
+      v.z += __shfl_xor(v.z, off);
+    }
+  }
+  __device__ V3 value(const SppmDev&) const { return v; }
+  __device__ bool saturated() const { return false; }
+};
+struct FixedSum {
+  long long x = 0, y = 0, z = 0;
+  int flags = 0;
+  __device__ static long long toFixed(float t, float scale, int& flags) {
+    const float s = t * scale;  // exact: scale is a power of two
+    if (!(s == s) || __builtin_isinf(t)) {
+      flags |= 2;
+      return 0;
+    }
+    if (!(__builtin_fabsf(s) < 0x1p62f)) {
+      flags |= 1;
+      return s < 0.0f ? -(1ll << 62) : (1ll << 62);
+    }
+    return __float2ll_rn(s);
+  }
+  __device__ static long long addSat(long long a, long long b, int& flags) {
+    long long r;
+    if (__builtin_add_overflow(a, b, &r)) {
+      flags |= 1;
+      return a < 0 ? (long long)0x8000000000000000ull : 0x7FFFFFFFFFFFFFFFll;
+    }
+    return r;
+  }
+  __device__ void add(V3 t, const SppmDev& g) {
+    x = addSat(x, toFixed(t.x, g.fluxScale, flags), flags);
+    y = addSat(y, toFixed(t.y, g.fluxScale, flags), flags);
+    z = addSat(z, toFixed(t.z, g.fluxScale, flags), flags);
+  }
+  __device__ static long long shfl64(long long v, int off) {
+    const int lo = __shfl_xor((int)(unsigned long long)v, off);
+    const int hi = __shfl_xor((int)((unsigned long long)v >> 32), off);
+    return (long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned long long)(unsigned)lo);
+  }
+  __device__ void reduceWave() {
+    for (int off = 32; off > 0; off >>= 1) {
+      const long long ox = shfl64(x, off), oy = shfl64(y, off), oz = shfl64(z, off);
+      flags |= __shfl_xor(flags, off);
+      x = addSat(x, ox, flags);
+      y = addSat(y, oy, flags);
+      z = addSat(z, oz, flags);
+    }
+  }
+  __device__ V3 value(const SppmDev& g) const {
+    if (flags & 2) return V3{__builtin_nanf(""), __builtin_nanf(""), __builtin_nanf("")};
+    return V3{(float)((double)x * g.fluxInvScale), (float)((double)y * g.fluxInvScale),
+              (float)((double)z * g.fluxInvScale)};
+  }
+  __device__ bool saturated() const { return (flags & 1) != 0; }
+};
+
+template <class Sum>
 __device__ inline void photonTerm(const SppmDev& g, uint32_t q, float4 x4, V3 c, float r2, V3 ns, V3 ng,
-                                  const DevMaterial& m, V3* acc, int* found) {
+                                  const DevMaterial& m, Sum* acc, int* found) {
   const V3 dlt = v3(x4.x, x4.y, x4.z) - c;
   if (!(dot(dlt, dlt) <= r2)) return;
   const V3 wi = f4xyz(g.sWi[q]);
   const V3 fl = f4xyz(g.sF[q]);
   const V3 fD = evalDiffuse(dot(ns, wi), m, normalScale(wi, ns, ng));
   const float fc = __builtin_fabsf(dot(wi, ng)) / (1e-7f + __builtin_fabsf(x4.w));
-  *acc = *acc + mul(fl * fc, fD + v3(0.0f, 0.0f, 0.0f));
+  acc->add(mul(fl * fc, fD + v3(0.0f, 0.0f, 0.0f)), g);
   *found += 1;
 }
 // Photons first, first + stride, ... below end, in that order, with
@@ -490,8 +576,9 @@ __device__ inline void photonTerm(const SppmDev& g, uint32_t q, float4 x4, V3 c,
 #define HASTY_PHOTON_LOADS 4
 #endif
 constexpr uint32_t kPhotonLoads = HASTY_PHOTON_LOADS;
+template <class Sum>
 __device__ inline void photonRun(const SppmDev& g, uint32_t first, uint32_t end, uint32_t stride, V3 c, float r2,
-                                 V3 ns, V3 ng, const DevMaterial& m, V3* acc, int* found) {
+                                 V3 ns, V3 ng, const DevMaterial& m, Sum* acc, int* found) {
   uint32_t q = first;
   for (; q + (kPhotonLoads - 1u) * stride < end; q += kPhotonLoads * stride) {
     float4 x[kPhotonLoads];
@@ -502,12 +589,11 @@ __device__ inline void photonRun(const SppmDev& g, uint32_t first, uint32_t end,
   }
   for (; q < end; q += stride) photonTerm(g, q, g.sX[q], c, r2, ns, ng, m, acc, found);
 }
-// Lanes whose cells hold fewer candidates than this gather on their own (a
-// shared gather costs every lane each owner's cell lookups).
-#ifndef HASTY_COOP_CANDIDATES
-#define HASTY_COOP_CANDIDATES 512
-#endif
-constexpr uint32_t kCoopCandidates = HASTY_COOP_CANDIDATES;
+// Lanes whose cells hold fewer candidates than g.coopCandidates (default
+// 512) gather on their own (a shared gather costs every lane each owner's
+// cell lookups).
+constexpr uint32_t kCoopCandidatesDefault = 512;
+template <class Sum>
 __device__ inline V3 gatherQuery(const SppmDev& g, const DevScene& s, bool has, V3 x, V3 ns, V3 ng, int mat,
                                  float r, int* M, uint2* runs) {
   const float cs = *g.cellSize;
@@ -524,8 +610,8 @@ __device__ inline V3 gatherQuery(const SppmDev& g, const DevScene& s, bool has, 
         for (int j = lo.y; j <= hi.y; ++j)
           for (int k = lo.z; k <= hi.z; ++k) cand += cellLookup(g, packKey(make_int3(i, j, k))).y;
     }
-    if (__ballot(cand > kCoopCandidates) == 0ull) {
-      V3 est = v3(0.0f, 0.0f, 0.0f);
+    if (__ballot(cand > g.coopCandidates) == 0ull) {
+      Sum est;
       int found = 0;
       if (has) {
         const DevMaterial m = ldgStruct(s.mats + mat);
@@ -535,9 +621,10 @@ __device__ inline V3 gatherQuery(const SppmDev& g, const DevScene& s, bool has, 
               const uint2 cell = cellLookup(g, packKey(make_int3(i, j, k)));
               photonRun(g, cell.x, cell.x + cell.y, 1u, x, r * r, ns, ng, m, &est, &found);
             }
+        if (est.saturated()) atomicAdd(&g.counters[kCtrFluxOverflow], 1u);
       }
       *M = found;
-      return est;
+      return est.value(g);
     }
   }
   // shared gather: every lane of the wave is a rank (the caller keeps the
@@ -560,7 +647,7 @@ __device__ inline V3 gatherQuery(const SppmDev& g, const DevScene& s, bool has, 
     int3 lo = cellOf(c - v3(rr, rr, rr), cs), hi = cellOf(c + v3(rr, rr, rr), cs);
     lo = make_int3(max(lo.x, g.bbox[0]), max(lo.y, g.bbox[1]), max(lo.z, g.bbox[2]));
     hi = make_int3(min(hi.x, g.bbox[3]), min(hi.y, g.bbox[4]), min(hi.z, g.bbox[5]));
-    V3 acc = v3(0.0f, 0.0f, 0.0f);
+    Sum acc;
     int found = 0;
     // the cells of the box in x -> y -> z order, 64 at a time: lane rank
     // looks up cell base + rank (all probes in flight together)
@@ -587,15 +674,12 @@ __device__ inline V3 gatherQuery(const SppmDev& g, const DevScene& s, bool has, 
     // the 64 partial sums by a butterfly: each step adds the same two
     // operands on both lanes of a pair (in either order), so every lane ends
     // with the same total
-    for (int off = 32; off > 0; off >>= 1) {
-      acc.x += __shfl_xor(acc.x, off);
-      acc.y += __shfl_xor(acc.y, off);
-      acc.z += __shfl_xor(acc.z, off);
-      found += __shfl_xor(found, off);
-    }
+    acc.reduceWave();
+    for (int off = 32; off > 0; off >>= 1) found += __shfl_xor(found, off);
     if ((int)lane == owner) {
-      myEst = acc;
+      myEst = acc.value(g);
       myM = found;
+      if (acc.saturated()) atomicAdd(&g.counters[kCtrFluxOverflow], 1u);
     }
   }
   *M = myM;
@@ -815,6 +899,7 @@ __global__ void __launch_bounds__(kB) HASTY_SPPM_CAMERA_ATTR kSppmCamera(DevScen
 // lanes stay in the loop so that the shared gather always has 64 ranks.
 // F_R = sum over the vertices in path order of W_R (Tb est), as the camera
 // path would have summed it.
+template <class Sum>
 __global__ void __launch_bounds__(kB) HASTY_SPPM_GATHER_ATTR kSppmGather(DevScene s, SppmDev g, float alpha,
                                                                         int emittedInt, float* color) {
   __shared__ uint2 gatherRuns[kB];
@@ -842,8 +927,8 @@ __global__ void __launch_bounds__(kB) HASTY_SPPM_GATHER_ATTR kSppmGather(DevScen
         q0 = qr[0], q1 = qr[1], q2 = qr[2], q3 = qr[3];
       }
       int M = 0;
-      const V3 est = gatherQuery(g, s, has, f4xyz(q0), f4xyz(q1), f4xyz(q2), __float_as_int(q0.w), radius, &M,
-                                 gatherRuns + (threadIdx.x & ~63u));
+      const V3 est = gatherQuery<Sum>(g, s, has, f4xyz(q0), f4xyz(q1), f4xyz(q2), __float_as_int(q0.w), radius,
+                                      &M, gatherRuns + (threadIdx.x & ~63u));
       if (has) {
         Mtot += M;
         F0 = F0 + mul(f4xyz(q3), mul(v3(q1.w, q2.w, q3.w), est));
@@ -1081,6 +1166,9 @@ void SppmRenderer::pass(uint32_t passIndex, float offX, float offY, float* dColo
   g.query = dQuery_;
   g.nq = dNq_;
   g.Lpath = dLpath_;
+  g.fluxScale = ldexpf(1.0f, p_.fluxFracBits);
+  g.fluxInvScale = ldexp(1.0, -p_.fluxFracBits);
+  g.coopCandidates = p_.gatherShareCandidates ? p_.gatherShareCandidates : kCoopCandidatesDefault;
 
   // 1. cell size
   if (timer) timer->begin(kKSppmCellSize, stream_);
@@ -1166,16 +1254,20 @@ void SppmRenderer::pass(uint32_t passIndex, float offX, float offY, float* dColo
   HC(hipGetLastError());
   if (timer) timer->end(kKSppmCamera, stream_);
   if (timer) timer->begin(kKSppmGather, stream_);
-  hipLaunchKernelGGL(kSppmGather, dim3((uint32_t)std::min<size_t>((P + kB - 1) / kB, 16384)), dim3(kB), 0, stream_,
-                     s_, g, p_.alpha, (int)emitted, dColor);
+  const dim3 gatherGrid((uint32_t)std::min<size_t>((P + kB - 1) / kB, 16384));
+  if (p_.fixedPointFlux)
+    hipLaunchKernelGGL(kSppmGather<FixedSum>, gatherGrid, dim3(kB), 0, stream_, s_, g, p_.alpha, (int)emitted, dColor);
+  else
+    hipLaunchKernelGGL(kSppmGather<FloatSum>, gatherGrid, dim3(kB), 0, stream_, s_, g, p_.alpha, (int)emitted, dColor);
   HC(hipGetLastError());
   if (timer) timer->end(kKSppmGather, stream_);
-  uint32_t c[4];
+  uint32_t c[kNumCtr];
   HC(hipMemcpyAsync(c, dCounters_, sizeof(c), hipMemcpyDeviceToHost, stream_));
   float cs;
   HC(hipMemcpyAsync(&cs, dCellSize_, sizeof(float), hipMemcpyDeviceToHost, stream_));
   HC(hipStreamSynchronize(stream_));
   info_.cellSize = cs;
+  info_.fluxOverflows = c[kCtrFluxOverflow];
   if (c[kCtrKeyOverflow] != 0)
     throw std::runtime_error("photon cell index outside the 21-bit key range (scene extent / cell size too large)");
   passes_ += 1;

@@ -331,6 +331,17 @@ typedef struct HastySppmParams {
   double emit_fraction;  /* ... or emit_fraction*W*H emitted photons (PhotonMapping.cpp:835), 10 */
   float initial_radius;  /* > 0: constant initial radius (no counterpart; the reference uses the
                             footprint estimate, PhotonMapping.cpp:838-841), default 0 */
+  /* No counterpart (the reference sums FP32, PhotonMapping.cpp:271-296):
+   * nonzero = each gather rounds its photon terms to int64 units of
+   * 2^-flux_frac_bits and sums them in integers, so the gathered flux does
+   * not depend on the order the photons are summed in (shared or per-lane
+   * gathers, cell size, a split of the photons over GPUs).  A saturated sum
+   * is counted in HastySppmInfo.flux_overflows.  Default 0 (FP32). */
+  int32_t fixed_point_flux;
+  int32_t flux_frac_bits;            /* 0: 24; at most 40 */
+  uint32_t gather_share_candidates;  /* 0: 512; a wave gathers its queries together when one of them has
+                                        more candidate photons than this (no effect on the result with
+                                        fixed-point flux) */
 } HastySppmParams;
 
 typedef struct HastySppmInfo {
@@ -340,6 +351,7 @@ typedef struct HastySppmInfo {
   uint32_t passes_done;
   uint32_t emit_limit;
   uint32_t store_limit;
+  uint32_t flux_overflows; /* gathers of the last pass whose fixed-point flux sum saturated */
 } HastySppmInfo;
 
 /* params may be NULL (reference defaults).  The scene must outlive the job. */

@@ -376,6 +376,8 @@ int oracle_sppm_create(const char* path, uint32_t seed, const float* params, voi
       if (params[2] > 0) job->params.storeFraction = params[2];
       if (params[3] > 0) job->params.emitFraction = params[3];
       if (params[4] > 0) job->params.initialRadius = params[4];
+      job->params.fixedPointFlux = params[5] != 0.0f;
+      if (params[6] > 0) job->params.fluxFracBits = (int)params[6];
     }
     *out = job.release();
     return 0;
@@ -408,6 +410,7 @@ void oracle_sppm_info(void* h, double* out) {
   out[1] = (double)job->lastStored;
   out[2] = job->lastCellSize;
   out[3] = job->passesDone;
+  out[4] = job->lastFluxOverflows;
 }
 // per pixel: radius, N, Ne, flux rgb, LSum rgb, Nd (10 floats)
 void oracle_sppm_pixels(void* h, float* out) {

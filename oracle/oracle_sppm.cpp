@@ -1,4 +1,5 @@
 // ORACLE (test infrastructure only).  See oracle_sppm.h.
+#include <climits>
 #include "oracle_sppm.h"
 
 #include <cmath>
@@ -75,11 +76,46 @@ static Vec3f photonMapFluxEstimate(RenderContext ctx, const PhotonMap& map, cons
                                    std::vector<uint32_t>& found) {
   Vec3f est = Vec3f::Zero();
   map.radiusNeighbors(hit.interaction.x, radius, found);
+  long long q[3] = {0, 0, 0};
+  int flags = 0;  // bit 0: saturated, bit 1: non-finite term
+  auto toFixed = [&](float t) -> long long {
+    const float s = t * map.fluxScale;
+    if (!(s == s) || std::isinf(t)) {
+      flags |= 2;
+      return 0;
+    }
+    if (!(std::fabs(s) < 0x1p62f)) {
+      flags |= 1;
+      return s < 0.0f ? -(1ll << 62) : (1ll << 62);
+    }
+    return std::llrint(s);  // round to nearest even
+  };
+  auto addSat = [&](long long a, long long b) -> long long {
+    long long r;
+    if (__builtin_add_overflow(a, b, &r)) {
+      flags |= 1;
+      return a < 0 ? LLONG_MIN : LLONG_MAX;
+    }
+    return r;
+  };
   for (uint32_t idx : found) {
     const Photon& ph = map.photons[idx];
     MaterialEvalResult ev = ctx.scene.evaluteBRDF(hit, hit.wFrom(), ph.wi, 1.0f, false, kDiffuse);
     float fluxCorrection = std::fabs(dot(ph.wi, hit.interaction.normalGeometric)) / (1e-7f + std::fabs(ph.ngwi));
-    est += cwiseProd(ph.flux * fluxCorrection, ev.fDiffuse + ev.fConcentrated);
+    Vec3f term = cwiseProd(ph.flux * fluxCorrection, ev.fDiffuse + ev.fConcentrated);
+    if (map.fixedFlux) {
+      for (int c = 0; c < 3; ++c) q[c] = addSat(q[c], toFixed(term[c]));
+    } else {
+      est += term;
+    }
+  }
+  if (map.fixedFlux) {
+    if (flags & 1) map.fluxOverflows.fetch_add(1);
+    if (flags & 2) {
+      est = Vec3f(NAN, NAN, NAN);
+    } else {
+      for (int c = 0; c < 3; ++c) est[c] = (float)((double)q[c] * map.fluxInvScale);
+    }
   }
   M = (int)found.size();
   return est;
@@ -311,6 +347,9 @@ void sppmPass(SppmJob& job, uint32_t pass, float* color, float* normal, float* a
   float cellSize = computeCellSize(job.radii, W, H);
   PhotonMap map;
   map.clear(cellSize);
+  map.fixedFlux = job.params.fixedPointFlux;
+  map.fluxScale = std::ldexp(1.0f, job.params.fluxFracBits);
+  map.fluxInvScale = std::ldexp(1.0, -job.params.fluxFracBits);
   if (scene.hasSurfaceLight()) {
     for (int i = 0; i < emitLimit && (int)map.photons.size() < storeLimit; ++i) {
       RNG rng = hrng::make_stream(pt.seed, (uint32_t)i, pass, hrng::kDomainPhoton);
@@ -358,6 +397,7 @@ void sppmPass(SppmJob& job, uint32_t pass, float* color, float* normal, float* a
   std::vector<std::thread> pool;
   for (int t = 0; t < threads; ++t) pool.emplace_back(worker, t);
   for (auto& th : pool) th.join();
+  job.lastFluxOverflows = map.fluxOverflows.load();
   job.passesDone += 1;
 }
 

@@ -14,6 +14,7 @@
 #include <stdint.h>
 
 #include <unordered_map>
+#include <atomic>
 #include <vector>
 
 #include "oracle_pt.h"
@@ -34,6 +35,8 @@ struct SppmParams {
   double emitFraction = 10.0;   // tracethreshold = 10 * W * H (PhotonMapping.cpp:835)
   float initialRadius = 0.0f;   // > 0: this constant initial radius (no counterpart; PhotonMapping.cpp:838-841
                                 // uses footprintScale x the pixel footprint)
+  bool fixedPointFlux = false;  // gathers sum photon terms as int64 multiples of 2^-fluxFracBits
+  int fluxFracBits = 24;
 };
 
 struct PixelStat {  // PMRenderJobPixel (PhotonMapping.h:12-31)
@@ -64,6 +67,12 @@ struct PhotonMap {
   std::vector<Vec3f> points;
   size_t emitted = 0;
   float cellSize = 1.0f;
+  // gathers in fixed point (no counterpart in the reference): terms rounded
+  // to int64 units of 1 / fluxScale, saturated sums counted
+  bool fixedFlux = false;
+  float fluxScale = 0.0f;
+  double fluxInvScale = 0.0;
+  mutable std::atomic<uint32_t> fluxOverflows{0};
   std::unordered_map<CellKey, std::vector<uint32_t>, CellKeyHash> cells;
 
   void clear(float cs) {
@@ -101,6 +110,7 @@ struct SppmJob {
   // last pass diagnostics
   uint64_t lastEmitted = 0, lastStored = 0;
   float lastCellSize = 0.0f;
+  uint32_t lastFluxOverflows = 0;
 };
 
 void tracePhoton(RenderContext ctx, PhotonMap& map, const Ray& ray, const Vec3f& flux);

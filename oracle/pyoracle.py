@@ -285,9 +285,11 @@ class OracleSppm:
     """The photon mapper's oracle: one job, passes rendered in order."""
 
     def __init__(self, job_path: str, seed: int = 0, alpha=0.0, footprint_scale=0.0, store_fraction=0.0,
-                 emit_fraction=0.0, width=0, height=0, initial_radius=0.0):
+                 emit_fraction=0.0, width=0, height=0, initial_radius=0.0, fixed_point_flux=False,
+                 flux_frac_bits=0):
         h = C.c_void_p()
-        params = f32([alpha, footprint_scale, store_fraction, emit_fraction, initial_radius])
+        params = f32([alpha, footprint_scale, store_fraction, emit_fraction, initial_radius,
+                      1.0 if fixed_point_flux else 0.0, flux_frac_bits])
         if lib().oracle_sppm_create(job_path.encode(), seed, params, C.byref(h)) != 0:
             raise RuntimeError(lib().oracle_last_error().decode())
         self.h = h
@@ -308,9 +310,10 @@ class OracleSppm:
         return films
 
     def info(self):
-        out = np.zeros(4, np.float64)
+        out = np.zeros(5, np.float64)
         lib().oracle_sppm_info(self.h, out)
-        return {"emitted": int(out[0]), "stored": int(out[1]), "cell_size": float(out[2]), "passes": int(out[3])}
+        return {"emitted": int(out[0]), "stored": int(out[1]), "cell_size": float(out[2]), "passes": int(out[3]),
+                "flux_overflows": int(out[4])}
 
     def pixels(self, w, h):
         out = np.zeros((h * w, 10), np.float32)

@@ -135,18 +135,19 @@ def full_caustic_scene():
 
 
 def test_config4_full_size_pass(full_caustic_scene):
-    """One pass at 1920 x 1080 with 4M stored photons: the photon budget is
-    met, every film value and pixel statistic is finite, radii stay at or
-    below r0, and a second job with the same seed reproduces the first
-    bit for bit."""
+    """One pass at 1920 x 1080 with 4M stored photons and int64 fixed-point
+    flux: the photon budget is met, no gather saturates, every film value and
+    pixel statistic is finite, radii stay at or below r0, and a second job
+    with the same seed reproduces the first bit for bit."""
     sc = full_caustic_scene
     assert sc.info()["num_triangles"] > 2_000_000
     outs = []
     for run in range(2):
         g = ha.Sppm(sc, 3, alpha=2.0 / 3.0, store_fraction=CONFIG_STORE, emit_fraction=CONFIG_EMIT,
-                    initial_radius=CONFIG_R0)
+                    initial_radius=CONFIG_R0, fixed_point_flux=True)
         films = g.pass_(0)
         info = g.info()
+        assert info["flux_overflows"] == 0  # the int64 guard at the 4M-photon budget
         print("[config4] full size run %d: emitted %d stored %d limit %d" % (run, info["emitted"], info["stored"],
                                                                            info["store_limit"]), flush=True)
         assert info["store_limit"] == _store_limit(CONFIG_STORE, 1920, 1080)

@@ -177,3 +177,64 @@ def test_sppm_bitexact_run_to_run(caustic_small_job):
         sc.close()
     for x, y in zip(outs[0], outs[1]):
         np.testing.assert_array_equal(x, y)
+
+
+# ------------------------------------------------------- fixed-point flux ----
+_FIXED_KW = dict(alpha=2.0 / 3.0, store_fraction=4e6 / (1920 * 1080), emit_fraction=100.0, initial_radius=0.064)
+
+
+def test_fixed_point_flux_matches_oracle(caustic_small_job):
+    """int64 fixed-point gathers on the GPU against the oracle's fixed-point
+    gathers: photon counts (N) identical, films within 1e-3."""
+    w, h = 192, 108
+    sc, g = _gpu_sppm(caustic_small_job, seed=8, fixed_point_flux=True, **_FIXED_KW)
+    ref = po.OracleSppm(caustic_small_job, seed=8, fixed_point_flux=True, **_FIXED_KW)
+    for k in range(2):
+        c, _, _ = g.pass_(k)
+        rc, _, _ = ref.run_pass(k, w, h)
+        gi = g.info()
+        assert gi["flux_overflows"] == 0 and ref.info()["flux_overflows"] == 0
+        ok = rel_ok(c, rc)
+        print("fixed point pass %d: %.5f of pixels within 1e-3" % (k, ok.mean()), flush=True)
+        assert ok.mean() >= 0.999
+    px, rp = g.pixels(), ref.pixels(w, h)
+    same_n = (px["N"].ravel() == rp[:, 1]).mean()
+    print("fixed point: N identical on %.5f of pixels" % same_n, flush=True)
+    assert same_n >= 0.999
+
+
+def test_fixed_point_flux_does_not_depend_on_gather_order(caustic_small_job):
+    """With fixed-point flux the films and every pixel statistic are the same
+    bits whether each lane gathers its own photons in the reference's order
+    or the 64 lanes of a wave share every gather (partial sums added by a
+    butterfly): integer sums do not depend on the order of the terms."""
+    outs = {}
+    for fixed in (True, False):
+        for share in (1, 1 << 30):  # 1: every wave shares its gathers; 2^30: never
+            sc, g = _gpu_sppm(caustic_small_job, seed=12, fixed_point_flux=fixed, gather_share_candidates=share,
+                              **_FIXED_KW)
+            films = [g.pass_(k)[0].copy() for k in range(3)]
+            px = g.pixels()
+            outs[(fixed, share)] = (films, px["flux"].copy(), px["radius"].copy(), px["N"].copy())
+    a, b = outs[(True, 1)], outs[(True, 1 << 30)]
+    for x, y in zip(a[0], b[0]):
+        assert np.array_equal(x.view(np.uint32), y.view(np.uint32))
+    for x, y in zip(a[1:], b[1:]):
+        assert np.array_equal(x.view(np.uint32), y.view(np.uint32))
+    fa, fb = outs[(False, 1)][1], outs[(False, 1 << 30)][1]
+    print("FP32 flux planes differ between gather orders on %.4f of pixels" % (fa != fb).any(-1).mean(), flush=True)
+    # fixed point agrees with FP32 to float rounding
+    assert rel_ok(a[0][-1], outs[(False, 1 << 30)][0][-1]).mean() >= 0.999
+
+
+def test_fixed_point_guard_counts_saturated_gathers(tmp_path):
+    """A 1e12 light saturates single photon terms: the pass completes and
+    counts the saturated gathers (no fault, no silent wrap)."""
+    job = scenes.write_caustic_scene(str(tmp_path / "hot"), width=32, height=18, passes=1, n_major=24, n_minor=8,
+                                     light_radiance=1e12)
+    sc, g = _gpu_sppm(job, seed=1, fixed_point_flux=True, store_fraction=2.0, emit_fraction=100.0,
+                      initial_radius=0.064)
+    g.pass_(0)
+    n = g.info()["flux_overflows"]
+    print("saturated gathers: %d" % n, flush=True)
+    assert n > 0

@@ -180,3 +180,36 @@ def test_hole_filler_skips_fourth_side():
     img3 = np.full((3, 3), -1.0, np.float32)
     img3[1, 0], img3[0, 2] = 1.0, 3.0  # pixels (0, 1) and (2, 0)
     assert po.get_close_non_negative(img3, 1, 1) == pytest.approx(3.0)
+
+
+# ------------------------------------------------------- fixed-point flux ----
+def test_fixed_point_gather_agrees_with_fp32(tmp_path):
+    """Gathers summed as int64 multiples of 2^-24 agree with the FP32 sums
+    of photonMapFluxEstimate (PhotonMapping.cpp:271-296) to float rounding:
+    each term is rounded by at most 2^-25 and the FP32 sum's own rounding is
+    larger.  Photon counts are identical (the set of photons is the same)."""
+    job = scenes.write_caustic_scene(str(tmp_path / "c"), width=48, height=27, passes=2, n_major=48, n_minor=12)
+    kw = dict(alpha=2.0 / 3.0, store_fraction=2.0, emit_fraction=100.0, initial_radius=0.064)
+    a = po.OracleSppm(job, seed=5, **kw)
+    b = po.OracleSppm(job, seed=5, fixed_point_flux=True, **kw)
+    for k in range(2):
+        ca, _, _ = a.run_pass(k, 48, 27)
+        cb, _, _ = b.run_pass(k, 48, 27)
+        assert b.info()["flux_overflows"] == 0
+        assert np.allclose(cb, ca, rtol=2e-5, atol=1e-7)
+    pa, pb = a.pixels(48, 27), b.pixels(48, 27)
+    np.testing.assert_array_equal(pa[:, 1], pb[:, 1])  # N: same photon counts M
+    np.testing.assert_array_equal(pa[:, 9], pb[:, 9])
+    assert np.allclose(pb[:, 3:6], pa[:, 3:6], rtol=2e-5, atol=1e-9)
+
+
+def test_fixed_point_guard_counts_saturated_gathers(tmp_path):
+    """A light of radiance 1e12 makes single photon terms exceed 2^62 units
+    of 2^-24: such gathers saturate and are counted, and the default light
+    counts none."""
+    job = scenes.write_caustic_scene(str(tmp_path / "hot"), width=32, height=18, passes=1, n_major=24, n_minor=8,
+                                     light_radiance=1e12)
+    o = po.OracleSppm(job, seed=1, store_fraction=2.0, emit_fraction=100.0, initial_radius=0.064,
+                      fixed_point_flux=True)
+    o.run_pass(0, 32, 18)
+    assert o.info()["flux_overflows"] > 0
