@@ -135,6 +135,32 @@ def cpu_baseline(job_path, seed, threads, spp):
     }, cnt
 
 
+def sppm_cpu_baseline(job_path, args):
+    """The oracle's photon mapper (a port of renderPhotonThread's pass) on a
+    bounded sample: one pass of the same job at half the width and height
+    (a quarter of the pixels, so also a quarter of the photon budget, which
+    the reference sets per pixel: PhotonMapping.cpp:834-835), scaled to the
+    full film as passes/s x 1/4.  Photons are emitted sequentially as in one
+    reference thread's pass; camera paths run on the host threads."""
+    from oracle import pyoracle
+
+    t0 = time.perf_counter()
+    o = pyoracle.OracleSppm(job_path, seed=3, alpha=args.sppm_alpha, store_fraction=args.sppm_store_fraction,
+                            emit_fraction=args.sppm_emit_fraction, initial_radius=args.sppm_initial_radius,
+                            width=960, height=540)
+    load = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    o.run_pass(0, 960, 540)
+    dt = time.perf_counter() - t0
+    hc = host_cpus()
+    info = o.info()
+    return {"value": 0.25 / dt, "unit": "passes/s", "cores": min(16, hc["logical_cpus"]), "kind": "port",
+            "sample": "1 pass at 960x540 (a quarter of the 1920x1080 pixels and photon budget: %d photons "
+                      "emitted, %d stored) in %.2f s after a %.1f s load; passes/s at 1920x1080 = 0.25 / that"
+                      % (info["emitted"], info["stored"], dt, load),
+            "cpu_model": hc["model"], "host_logical_cpus": hc["logical_cpus"]}
+
+
 def measured_traffic():
     """HBM bytes per pt_extend launch from the committed rocprofv3 summary
     (line 'traffic_bytes_per_launch: N'), or None.  rocprofv3 counters need a
@@ -197,13 +223,16 @@ def bench_sppm(args):
     scene.set_profiling(events=True, counters=False)
     hip.sync()
     t_start = time.perf_counter()
-    emitted = stored = 0
+    emitted = stored = queries = candidates = found = 0
     for _ in range(args.steps):
         t = time.perf_counter()
         sp.pass_device(k, *films)
         i = sp.info()
         emitted += i["emitted"]
         stored += i["stored"]
+        queries += i["gather_queries"]
+        candidates += i["gather_candidates"]
+        found += i["gather_found"]
         log("pass %d: %.3f s, emitted %d stored %d cell %.4g" % (k, time.perf_counter() - t, i["emitted"],
                                                               i["stored"], i["cell_size"]))
         k += 1
@@ -211,6 +240,23 @@ def bench_sppm(args):
     elapsed = time.perf_counter() - t_start
     kstats = scene.kernel_stats()
     scene.set_profiling(events=False, counters=False)
+    # roofline of the gather kernel: per gather 64 B of query record, 16 B
+    # per candidate photon position read in the cells it visits, 32 B
+    # (direction + flux) per photon within the radius
+    roofline = None
+    g_launches, g_ms = kstats.get("sppm_gather", (0, 0.0))
+    if g_launches and g_ms > 0:
+        gbytes = 64.0 * queries + 16.0 * candidates + 32.0 * found
+        achieved = gbytes / (g_ms / 1e3) / 1e9
+        roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": "sppm_gather",
+                    "bytes_per_launch": gbytes / g_launches, "avg_launch_ms": round(g_ms / g_launches, 4),
+                    "queries_per_pass": queries / max(1, args.steps),
+                    "candidates_per_query": candidates / max(1, queries), "found_per_query": found / max(1, queries)}
+    base = None
+    if not args.no_cpu_baseline:
+        base = sppm_cpu_baseline(job_path, args)
+        log("sppm cpu baseline %.4g passes/s at %dx%d" % (base["value"], W, H))
     line = {
         "metric": "photon-mapping passes/s at 1920x1080 (caustic box)",
         "value": args.steps / elapsed,
@@ -232,6 +278,8 @@ def bench_sppm(args):
         "photons_stored_per_pass": stored / max(1, args.steps),
         "load_s": load_s,
         "kernel_ms_per_pass": {k2: round(v[1] / args.steps, 3) for k2, v in kstats.items()},
+        "roofline": roofline,
+        "cpu_baseline": base,
     }
     for f in films:
         hip.free(f)

@@ -212,6 +212,9 @@ class SppmInfoC(C.Structure):
         ("emit_limit", C.c_uint32),
         ("store_limit", C.c_uint32),
         ("flux_overflows", C.c_uint32),
+        ("gather_queries", C.c_uint64),
+        ("gather_candidates", C.c_uint64),
+        ("gather_found", C.c_uint64),
     ]
 
 
@@ -682,7 +685,9 @@ class Sppm:
         i = SppmInfoC()
         _err(lib().hasty_sppm_info(self._h, C.byref(i)))
         return {"emitted": i.emitted, "stored": i.stored, "cell_size": i.cell_size, "passes_done": i.passes_done,
-                "emit_limit": i.emit_limit, "store_limit": i.store_limit, "flux_overflows": i.flux_overflows}
+                "emit_limit": i.emit_limit, "store_limit": i.store_limit, "flux_overflows": i.flux_overflows,
+                "gather_queries": i.gather_queries, "gather_candidates": i.gather_candidates,
+                "gather_found": i.gather_found}
 
     def pixels(self) -> dict:
         """PMRenderJobPixel planes: radius, N, Ne, flux (h, w, 3), LSum (h, w, 3), Nd (int32)."""

@@ -1256,6 +1256,9 @@ int hasty_sppm_info(const HastySppm* h, HastySppmInfo* info) {
   info->emit_limit = h->r->emitLimit();
   info->store_limit = h->r->storeLimit();
   info->flux_overflows = i.fluxOverflows;
+  info->gather_queries = i.gatherQueries;
+  info->gather_candidates = i.gatherCandidates;
+  info->gather_found = i.gatherFound;
   return 0;
 }
 

@@ -26,6 +26,7 @@ struct SppmPassInfo {
   uint64_t emitted = 0, stored = 0;
   float cellSize = 0.0f;
   uint32_t fluxOverflows = 0;  // gathers whose fixed-point sum saturated
+  uint64_t gatherQueries = 0, gatherCandidates = 0, gatherFound = 0;
 };
 
 // Device state of one progressive photon-mapping job: per-pixel statistics
@@ -81,6 +82,7 @@ class SppmRenderer {
   float* dCellSize_ = nullptr;
   int* dBox_ = nullptr;
   uint32_t* dCounters_ = nullptr;
+  unsigned long long* dGatherStats_ = nullptr;
   uint2* dOverflow_ = nullptr;
   // photon batches: per-thread slots, batch x cap
   uint32_t cap_ = 0, batchMax_ = 0;

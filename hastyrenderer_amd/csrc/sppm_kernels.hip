@@ -107,6 +107,9 @@ struct SppmDev {
   float fluxScale;
   double fluxInvScale;
   uint32_t coopCandidates;
+  // gathers of the pass: queries, candidate photons in the cells visited,
+  // photons found within the radius
+  unsigned long long* gatherStats;
 };
 
 enum Counter {
@@ -595,7 +598,7 @@ __device__ inline void photonRun(const SppmDev& g, uint32_t first, uint32_t end,
 constexpr uint32_t kCoopCandidatesDefault = 512;
 template <class Sum>
 __device__ inline V3 gatherQuery(const SppmDev& g, const DevScene& s, bool has, V3 x, V3 ns, V3 ng, int mat,
-                                 float r, int* M, uint2* runs) {
+                                 float r, int* M, uint2* runs, uint32_t* candidates) {
   const float cs = *g.cellSize;
   {
     // candidates of this lane's own query; if no lane of the wave has many,
@@ -610,6 +613,7 @@ __device__ inline V3 gatherQuery(const SppmDev& g, const DevScene& s, bool has, 
         for (int j = lo.y; j <= hi.y; ++j)
           for (int k = lo.z; k <= hi.z; ++k) cand += cellLookup(g, packKey(make_int3(i, j, k))).y;
     }
+    *candidates = cand;
     if (__ballot(cand > g.coopCandidates) == 0ull) {
       Sum est;
       int found = 0;
@@ -904,6 +908,7 @@ __global__ void __launch_bounds__(kB) HASTY_SPPM_GATHER_ATTR kSppmGather(DevScen
                                                                         int emittedInt, float* color) {
   __shared__ uint2 gatherRuns[kB];
   const uint32_t P = s.width * s.height;
+  unsigned long long nQueries = 0, nCand = 0, nFound = 0;
   for (uint32_t base = blockIdx.x * kB; base < P; base += gridDim.x * kB) {
     const uint32_t p = base + threadIdx.x;
     const bool valid = p < P;
@@ -920,6 +925,7 @@ __global__ void __launch_bounds__(kB) HASTY_SPPM_GATHER_ATTR kSppmGather(DevScen
     V3 F0 = v3(0.0f, 0.0f, 0.0f);
     int Mtot = 0;
     for (uint32_t v = 0; v < rounds; ++v) {
+      uint32_t cand = 0;
       const bool has = v < nq;
       float4 q0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), q1 = q0, q2 = q0, q3 = q0;
       if (has) {
@@ -928,8 +934,11 @@ __global__ void __launch_bounds__(kB) HASTY_SPPM_GATHER_ATTR kSppmGather(DevScen
       }
       int M = 0;
       const V3 est = gatherQuery<Sum>(g, s, has, f4xyz(q0), f4xyz(q1), f4xyz(q2), __float_as_int(q0.w), radius,
-                                      &M, gatherRuns + (threadIdx.x & ~63u));
+                                      &M, gatherRuns + (threadIdx.x & ~63u), &cand);
       if (has) {
+        nQueries += 1;
+        nCand += cand;
+        nFound += (unsigned long long)M;
         Mtot += M;
         F0 = F0 + mul(f4xyz(q3), mul(v3(q1.w, q2.w, q3.w), est));
       }
@@ -979,6 +988,17 @@ __global__ void __launch_bounds__(kB) HASTY_SPPM_GATHER_ATTR kSppmGather(DevScen
     g.radii[p] = radius;
     color[p * 3] = Lnew.x, color[p * 3 + 1] = Lnew.y, color[p * 3 + 2] = Lnew.z;
   }
+  // one atomic per wave and statistic
+  for (int off = 32; off > 0; off >>= 1) {
+    nQueries += __shfl_xor(nQueries, off);
+    nCand += __shfl_xor(nCand, off);
+    nFound += __shfl_xor(nFound, off);
+  }
+  if ((threadIdx.x & 63u) == 0) {
+    atomicAdd(&g.gatherStats[0], nQueries);
+    atomicAdd(&g.gatherStats[1], nCand);
+    atomicAdd(&g.gatherStats[2], nFound);
+  }
 }
 
 __global__ void __launch_bounds__(kB) kPixelInit(SppmDev g, uint32_t P) {
@@ -995,6 +1015,9 @@ __global__ void __launch_bounds__(kB) kPixelInit(SppmDev g, uint32_t P) {
 }
 __global__ void __launch_bounds__(kB) kFill(float* a, uint32_t n, float v) {
   for (uint32_t k = blockIdx.x * kB + threadIdx.x; k < n; k += gridDim.x * kB) a[k] = v;
+}
+__global__ void kZeroU64(unsigned long long* a, uint32_t n) {
+  for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) a[i] = 0ull;
 }
 __global__ void kZeroU32(uint32_t* a, uint32_t n) {
   if (threadIdx.x < n) a[threadIdx.x] = 0u;
@@ -1023,6 +1046,7 @@ SppmRenderer::SppmRenderer(const DevScene& scene, const SppmParams& params, hipS
   dNq_ = alloc<uint32_t>(P);
   dLpath_ = alloc<float>(3 * P);
   dCounters_ = alloc<uint32_t>(8);
+  dGatherStats_ = alloc<unsigned long long>(3);
   dCellSize_ = alloc<float>(4);
   dCand_ = alloc<float>(16);
   dState_ = alloc<float>(4);
@@ -1169,6 +1193,7 @@ void SppmRenderer::pass(uint32_t passIndex, float offX, float offY, float* dColo
   g.fluxScale = ldexpf(1.0f, p_.fluxFracBits);
   g.fluxInvScale = ldexp(1.0, -p_.fluxFracBits);
   g.coopCandidates = p_.gatherShareCandidates ? p_.gatherShareCandidates : kCoopCandidatesDefault;
+  g.gatherStats = dGatherStats_;
 
   // 1. cell size
   if (timer) timer->begin(kKSppmCellSize, stream_);
@@ -1189,6 +1214,7 @@ void SppmRenderer::pass(uint32_t passIndex, float offX, float offY, float* dColo
 
   // 2. photons
   hipLaunchKernelGGL(kZeroU32, dim3(1), dim3(64), 0, stream_, dCounters_, 8u);
+  hipLaunchKernelGGL(kZeroU64, dim3(1), dim3(64), 0, stream_, dGatherStats_, 3u);
   uint32_t stored = 0, emitted = 0, next = 0;
   if (s_.hasSurfaceLight) {
     uint32_t batch = std::min<uint32_t>(batchMax_, std::max<uint32_t>(65536, storeLimit_));
@@ -1263,11 +1289,16 @@ void SppmRenderer::pass(uint32_t passIndex, float offX, float offY, float* dColo
   if (timer) timer->end(kKSppmGather, stream_);
   uint32_t c[kNumCtr];
   HC(hipMemcpyAsync(c, dCounters_, sizeof(c), hipMemcpyDeviceToHost, stream_));
+  unsigned long long gs[3];
+  HC(hipMemcpyAsync(gs, dGatherStats_, sizeof(gs), hipMemcpyDeviceToHost, stream_));
   float cs;
   HC(hipMemcpyAsync(&cs, dCellSize_, sizeof(float), hipMemcpyDeviceToHost, stream_));
   HC(hipStreamSynchronize(stream_));
   info_.cellSize = cs;
   info_.fluxOverflows = c[kCtrFluxOverflow];
+  info_.gatherQueries = gs[0];
+  info_.gatherCandidates = gs[1];
+  info_.gatherFound = gs[2];
   if (c[kCtrKeyOverflow] != 0)
     throw std::runtime_error("photon cell index outside the 21-bit key range (scene extent / cell size too large)");
   passes_ += 1;

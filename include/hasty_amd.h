@@ -352,6 +352,11 @@ typedef struct HastySppmInfo {
   uint32_t emit_limit;
   uint32_t store_limit;
   uint32_t flux_overflows; /* gathers of the last pass whose fixed-point flux sum saturated */
+  /* gathers of the last pass (photonMapFluxEstimate calls, PhotonMapping.cpp:271-296): queries,
+   * candidate photons in the cells they visit, photons within the radius */
+  uint64_t gather_queries;
+  uint64_t gather_candidates;
+  uint64_t gather_found;
 } HastySppmInfo;
 
 /* params may be NULL (reference defaults).  The scene must outlive the job. */
