@@ -497,9 +497,9 @@ def main():
     ap.add_argument("--tune-spp", type=int, default=0,
                     help="samples per autotuner trial before timing (0: the library's default launch, which is "
                          "what the autotuner picks on this scene, without the run-to-run noise of short trials)")
-    ap.add_argument("--launch", default="", metavar="SLOTS,TRACE,SHADE,CHUNK,GRAPH",
+    ap.add_argument("--launch", default="", metavar="SLOTS,TRACE,SHADE,CHUNK,GRAPH[,WIDTH]",
                     help="explicit launch shape (path slots, trace blocks, shade blocks, bounces per check, "
-                         "hipGraph 0/1) instead of autotuning")
+                         "hipGraph 0/1, BVH node width 4/8) instead of autotuning")
     ap.add_argument("--guide-spp", type=int, default=None,
                     help="samples of both renders (default: 4096 on the ring, 256 on the room)")
     ap.add_argument("--guide-res", type=int, default=None, help="image side (default: 1024 ring, 512 room)")
@@ -568,7 +568,7 @@ def main():
     # launch shape: given (--launch), else what the autotuner picks from short
     # renders of this scene (untimed), else the default
     if args.launch:
-        keys = ("path_slots", "trace_blocks", "shade_blocks", "chunk", "use_graph")
+        keys = ("path_slots", "trace_blocks", "shade_blocks", "chunk", "use_graph", "bvh_width")
         scene.set_launch_ex(**dict(zip(keys, (int(x) for x in args.launch.split(",")))))
         tuned = scene.get_launch()
     elif args.tune_spp > 0:

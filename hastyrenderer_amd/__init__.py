@@ -165,6 +165,7 @@ class SceneInfoC(C.Structure):
         ("bvh_nodes", C.c_uint32),
         ("bvh_depth", C.c_int32),
         ("device_bytes", C.c_uint64),
+        ("bvh_width", C.c_int32),
     ]
 
 
@@ -175,6 +176,7 @@ class LaunchC(C.Structure):
         ("shade_blocks", C.c_uint32),
         ("chunk", C.c_uint32),
         ("use_graph", C.c_int32),
+        ("bvh_width", C.c_uint32),
     ]
 
 
@@ -401,6 +403,7 @@ class Scene:
             "has_environment": bool(i.has_environment),
             "bvh_nodes": i.bvh_nodes,
             "bvh_depth": i.bvh_depth,
+            "bvh_width": i.bvh_width,
             "device_bytes": i.device_bytes,
         }
 
@@ -454,7 +457,7 @@ class Scene:
         return {k: int(getattr(lc, k)) for k, _ in LaunchC._fields_}
 
     def set_launch_ex(self, **kw):
-        """path_slots, trace_blocks, shade_blocks, chunk, use_graph."""
+        """path_slots, trace_blocks, shade_blocks, chunk, use_graph, bvh_width."""
         cur = self.get_launch()
         cur.update(kw)
         lc = LaunchC(*(int(cur[k]) for k, _ in LaunchC._fields_))

@@ -164,9 +164,13 @@ void padBox(const Box& b, float* lo, float* hi) {
 
 }  // namespace
 
-Bvh4 buildBvh4(const float* triPos, uint32_t numTris, const BvhBuildOptions& opt) {
+Bvh4 buildBvh4(const float* triPos, uint32_t numTris, const BvhBuildOptions& opt, int width) {
+  if (width != 4 && width != 8) throw std::runtime_error("BVH width must be 4 or 8");
   Bvh4 out;
+  out.width = width;
   out.numTris = numTris;
+  const int W = width;
+  const size_t nodeFloats = 8 * (size_t)W;
   if (numTris >= (1u << 24)) throw std::runtime_error("scene has more than 2^24 triangles");
   Builder B(opt);
   B.triBox.resize(numTris);
@@ -184,32 +188,37 @@ Bvh4 buildBvh4(const float* triPos, uint32_t numTris, const BvhBuildOptions& opt
   if (numTris > 0) out.binaryDepth = B.build(0, 0, numTris, 0);
   out.binaryNodes = B.nodes.size();
 
-  // Collapse to 4-wide nodes, depth-first.
+  // Collapse to W-wide nodes, depth-first.
   const std::vector<BNode>& bn = B.nodes;
   std::vector<uint32_t> leafTris;
   leafTris.reserve(numTris);
   struct Work {
     int32_t bnode;
-    uint32_t slot;  // index of the 4-wide node to fill
+    uint32_t slot;  // index of the W-wide node to fill
     int depth;
   };
   std::vector<Work> stack;
   auto newNode = [&]() {
     uint32_t idx = out.numNodes++;
-    out.nodes.resize((size_t)out.numNodes * 32, 0.0f);
+    out.nodes.resize((size_t)out.numNodes * nodeFloats, 0.0f);
     return idx;
   };
+  // child k of node n: half k / 4 (32 floats), lane k % 4 in it
+  auto writeChild = [&](uint32_t node, int k, const float* lo, const float* hi, uint32_t ref) {
+    float* h = &out.nodes[(size_t)node * nodeFloats + 32 * (size_t)(k / 4)];
+    const int l = k % 4;
+    h[0 * 4 + l] = lo[0];
+    h[1 * 4 + l] = hi[0];
+    h[2 * 4 + l] = lo[1];
+    h[3 * 4 + l] = hi[1];
+    h[4 * 4 + l] = lo[2];
+    h[5 * 4 + l] = hi[2];
+    memcpy(&h[24 + l], &ref, 4);
+  };
+  const float emptyLo[3] = {INFINITY, INFINITY, INFINITY}, emptyHi[3] = {INFINITY, INFINITY, INFINITY};
   if (numTris == 0) {
     uint32_t r = newNode();
-    float* n = &out.nodes[(size_t)r * 32];
-    for (int k = 0; k < 4; ++k) {
-      for (int c = 0; c < 3; ++c) {
-        n[(2 * c) * 4 + k] = INFINITY;
-        n[(2 * c + 1) * 4 + k] = INFINITY;
-      }
-      uint32_t ref = kLeafFlag;
-      memcpy(&n[24 + k], &ref, 4);
-    }
+    for (int k = 0; k < W; ++k) writeChild(r, k, emptyLo, emptyHi, kLeafFlag);
     return out;
   }
   // The root of the 4-wide tree always is an inner node (a lone leaf becomes
@@ -225,7 +234,7 @@ Bvh4 buildBvh4(const float* triPos, uint32_t numTris, const BvhBuildOptions& opt
     } else {
       kids.push_back(bn[w.bnode].child[0]);
       kids.push_back(bn[w.bnode].child[1]);
-      while (kids.size() < 4) {
+      while ((int)kids.size() < W) {
         int best = -1;
         float bestA = -1.0f;
         for (size_t i = 0; i < kids.size(); ++i) {
@@ -241,39 +250,25 @@ Bvh4 buildBvh4(const float* triPos, uint32_t numTris, const BvhBuildOptions& opt
         kids.push_back(bn[b].child[1]);
       }
     }
-    uint32_t refs[4];
-    float lo[4][3], hi[4][3];
-    for (int k = 0; k < 4; ++k) {
+    for (int k = 0; k < W; ++k) {
       if (k >= (int)kids.size()) {
-        refs[k] = kLeafFlag;
-        for (int c = 0; c < 3; ++c) {
-          lo[k][c] = INFINITY;
-          hi[k][c] = INFINITY;
-        }
+        writeChild(w.slot, k, emptyLo, emptyHi, kLeafFlag);
         continue;
       }
       const BNode& kn = bn[kids[k]];
-      padBox(kn.box, lo[k], hi[k]);
+      float lo[3], hi[3];
+      padBox(kn.box, lo, hi);
+      uint32_t ref;
       if (kn.count > 0) {
         uint32_t firstOut = (uint32_t)leafTris.size();
         for (uint32_t i = 0; i < kn.count; ++i) leafTris.push_back(B.order[kn.first + i]);
         if (kn.count > 127) throw std::runtime_error("BVH leaf larger than 127 triangles");
-        refs[k] = makeLeafRef(firstOut, kn.count);
+        ref = makeLeafRef(firstOut, kn.count);
       } else {
-        uint32_t child = newNode();
-        refs[k] = child;
-        stack.push_back({kids[k], child, w.depth + 1});
+        ref = newNode();
+        stack.push_back({kids[k], ref, w.depth + 1});
       }
-    }
-    float* n = &out.nodes[(size_t)w.slot * 32];
-    for (int k = 0; k < 4; ++k) {
-      n[0 * 4 + k] = lo[k][0];
-      n[1 * 4 + k] = hi[k][0];
-      n[2 * 4 + k] = lo[k][1];
-      n[3 * 4 + k] = hi[k][1];
-      n[4 * 4 + k] = lo[k][2];
-      n[5 * 4 + k] = hi[k][2];
-      memcpy(&n[24 + k], &refs[k], 4);
+      writeChild(w.slot, k, lo, hi, ref);
     }
   }
   out.tris.resize((size_t)leafTris.size() * 12);

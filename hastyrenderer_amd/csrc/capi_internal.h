@@ -14,7 +14,8 @@ struct SceneView {
   hdev::DevScene dev;  // device pointers and constants of the loaded scene
   hipStream_t stream;  // the scene's stream
   int device;          // the device the scene lives on
-  int bvhDepth;        // inner levels of the 4-wide BVH
+  int bvhDepth;        // inner levels of the BVH in use
+  int bvhWidth;        // its children per node (4 or 8)
 };
 
 // Fills *out; false for a null scene.

@@ -159,9 +159,20 @@ struct BatchItem {
 };
 static void renderBatch(HastyScene* owner, const BatchItem* items, uint32_t nItems);
 
+// A BVH of one node width resident on the scene's device.
+struct DeviceBvh {
+  int width = 0;
+  void* nodes = nullptr;
+  void* tris = nullptr;
+  uint32_t numNodes = 0;
+  int depth = 0;
+  uint64_t bytes = 0;
+};
+
 struct HastyScene {
   hasty::Job job;
-  hasty::Bvh4 bvh;
+  std::vector<DeviceBvh> bvhs;  // built so far, one per width
+  int bvhIndex = 0;             // the one in use (dev.nodes / dev.tris)
   hdev::DevScene dev{};
   std::vector<void*> sceneAllocs;
   uint64_t sceneBytes = 0;
@@ -236,6 +247,10 @@ struct HastyScene {
   ~HastyScene() {
     if (timer) timer->collect();
     freePath();
+    for (DeviceBvh& b : bvhs) {
+      if (b.nodes) (void)hipFree(b.nodes);
+      if (b.tris) (void)hipFree(b.tris);
+    }
     for (void* p : sceneAllocs) (void)hipFree(p);
     if (dTileOrder) (void)hipFree(dTileOrder);
     for (void* p : {(void*)guideRaw, (void*)guideRec, (void*)guideSamples, (void*)guideCount})
@@ -322,6 +337,52 @@ struct HastyScene {
     tileH = dev.height;
   }
   const uint32_t* rankOf() const { return dTileOrder + (size_t)dev.width * dev.height; }
+
+  const DeviceBvh& bvh() const { return bvhs[bvhIndex]; }
+
+  // Makes the BVH of node width `width` (4 or 8) current, building and
+  // uploading it the first time.  Throws if its depth does not fit the
+  // traversal stack ((width - 1) * depth + 1 entries per lane).
+  void useBvhWidth(int width) {
+    for (size_t i = 0; i < bvhs.size(); ++i)
+      if (bvhs[i].width == width) {
+        bvhIndex = (int)i;
+        dev.nodes = (const float4*)bvhs[i].nodes;
+        dev.tris = (const float4*)bvhs[i].tris;
+        dev.bvhWidth = width;
+        return;
+      }
+    const hasty::Scene& sc = job.scene;
+    hasty::BvhBuildOptions opt;
+    hasty::Bvh4 b = hasty::buildBvh4(sc.triPos.data(), sc.numTris, opt, width);
+    if ((width - 1) * b.depth + 1 > hasty::kStackLds + hasty::kStackOvf)
+      throw std::runtime_error("BVH too deep for the traversal stack (width " + std::to_string(width) + ", depth " +
+                               std::to_string(b.depth) + ")");
+    if ((uint64_t)b.nodes.size() * sizeof(float) >= (1ull << 32))  // traversal uses 32-bit node offsets
+      throw std::runtime_error("BVH larger than 4 GiB (" + std::to_string(b.numNodes) + " nodes)");
+    DeviceBvh d;
+    d.width = width;
+    d.numNodes = b.numNodes;
+    d.depth = b.depth;
+    const size_t nb = b.nodes.size() * sizeof(float), tb = b.tris.size() * sizeof(float);
+    struct Guard {
+      DeviceBvh& d;
+      bool keep = false;
+      ~Guard() {
+        if (keep) return;
+        if (d.nodes) (void)hipFree(d.nodes);
+        if (d.tris) (void)hipFree(d.tris);
+      }
+    } guard{d};
+    HIPCHECK(hipMalloc(&d.nodes, std::max<size_t>(nb, 16)));
+    HIPCHECK(hipMalloc(&d.tris, std::max<size_t>(tb, 16)));
+    if (nb) HIPCHECK(hipMemcpy(d.nodes, b.nodes.data(), nb, hipMemcpyHostToDevice));
+    if (tb) HIPCHECK(hipMemcpy(d.tris, b.tris.data(), tb, hipMemcpyHostToDevice));
+    d.bytes = nb + tb;
+    guard.keep = true;
+    bvhs.push_back(d);
+    useBvhWidth(width);
+  }
 
   void renderDevice(uint32_t seed, uint32_t first, uint32_t nSamples, float* dC, float* dN, float* dA) {
     BatchItem it{this, seed, first, nSamples, dC, dN, dA};
@@ -671,18 +732,11 @@ static void uploadScene(HastyScene& S) {
   if (sc.hasLight() && st.useMIS && !sc.hasEnvTexture && sc.lightTris.empty())
     throw std::runtime_error("error: no light triangle found ");
 
-  hasty::BvhBuildOptions opt;
-  S.bvh = hasty::buildBvh4(sc.triPos.data(), sc.numTris, opt);
-  if (3 * S.bvh.depth + 1 > hasty::kStackLds + hasty::kStackOvf)
-    throw std::runtime_error("BVH too deep for the traversal stack (depth " + std::to_string(S.bvh.depth) + ")");
-  if ((uint64_t)S.bvh.nodes.size() * sizeof(float) >= (1ull << 32))  // traversal uses 32-bit node offsets
-    throw std::runtime_error("BVH larger than 4 GiB (" + std::to_string(S.bvh.numNodes) + " nodes)");
 
   HIPCHECK(hipGetDevice(&S.device));
   HIPCHECK(hipStreamCreate(&S.stream));
   hdev::DevScene& d = S.dev;
-  d.nodes = S.upload((const float4*)S.bvh.nodes.data(), S.bvh.nodes.size() / 4);
-  d.tris = S.upload((const float4*)S.bvh.tris.data(), S.bvh.tris.size() / 4);
+  S.useBvhWidth(4);
   d.triPos = S.upload(sc.triPos.data(), sc.triPos.size());
   d.triNrm = S.upload(sc.triNrm.data(), sc.triNrm.size());
   d.triMat = S.upload(sc.triMaterial.data(), sc.triMaterial.size());
@@ -743,7 +797,8 @@ bool sceneView(const HastyScene* s, SceneView* out) {
   out->dev = s->dev;
   out->stream = s->stream;
   out->device = s->device;
-  out->bvhDepth = s->bvh.depth;
+  out->bvhDepth = s->bvh().depth;
+  out->bvhWidth = s->bvh().width;
   return true;
 }
 void setLastError(const std::string& message) { g_lastError = message; }
@@ -810,9 +865,10 @@ int hasty_scene_info(const HastyScene* s, HastySceneInfo* info) {
   info->num_materials = (uint32_t)s->job.scene.materials.size();
   info->num_light_triangles = (uint32_t)s->job.scene.lightTris.size();
   info->has_environment = s->job.scene.hasEnvTexture ? 1 : 0;
-  info->bvh_nodes = s->bvh.numNodes;
-  info->bvh_depth = s->bvh.depth;
-  info->device_bytes = s->sceneBytes;
+  info->bvh_nodes = s->bvh().numNodes;
+  info->bvh_depth = s->bvh().depth;
+  info->device_bytes = s->sceneBytes + s->bvh().bytes;
+  info->bvh_width = s->bvh().width;
   return 0;
 }
 
@@ -1033,6 +1089,7 @@ int hasty_get_launch(const HastyScene* s, HastyLaunch* out) {
   out->shade_blocks = s->launch.shadeBlocks;
   out->chunk = s->launch.chunk;
   out->use_graph = s->launch.useGraph ? 1 : 0;
+  out->bvh_width = (uint32_t)s->bvh().width;
   return 0;
 }
 
@@ -1046,6 +1103,18 @@ int hasty_set_launch_ex(HastyScene* s, const HastyLaunch* l) {
   if (l->shade_blocks) s->launch.shadeBlocks = l->shade_blocks;
   if (l->chunk) s->launch.chunk = l->chunk;
   s->launch.useGraph = l->use_graph != 0;
+  if (l->bvh_width) {
+    if (l->bvh_width != 4 && l->bvh_width != 8) {
+      g_lastError = "bvh_width must be 4 or 8";
+      return 1;
+    }
+    try {
+      HIPCHECK(hipSetDevice(s->device));
+      s->useBvhWidth((int)l->bvh_width);
+    } catch (const std::exception& e) {
+      return fail(e, 3);
+    }
+  }
   return 0;
 }
 
@@ -1132,6 +1201,30 @@ static int autotuneItems(std::vector<BatchItem> items, uint32_t trialSamples, Ha
     tryKnob([](HastyLaunch& c, uint32_t v) { c.shade_blocks = v; }, shade);
     tryKnob([](HastyLaunch& c, uint32_t v) { c.chunk = v; }, chunks);
     tryKnob([](HastyLaunch& c, uint32_t v) { c.use_graph = (int32_t)v; }, {0u, 1u});
+    // node width: every job of the trial renders with the candidate width
+    // (a width whose tree is too deep for the traversal stack is skipped)
+    {
+      HastyLaunch base = best;
+      for (uint32_t wdt : {4u, 8u}) {
+        bool ok = true;
+        for (BatchItem& it : items) {
+          try {
+            it.scene->useBvhWidth((int)wdt);
+          } catch (const std::exception&) {
+            ok = false;
+          }
+        }
+        if (!ok) continue;
+        double t = timeIt();
+        if (t < bestT) {
+          bestT = t;
+          best = base;
+          best.bvh_width = wdt;
+        }
+      }
+      const int chosenWidth = best.bvh_width ? (int)best.bvh_width : 4;
+      for (BatchItem& it : items) it.scene->useBvhWidth(chosenWidth);
+    }
     owner->profiling = prof;
     owner->counting = cnt;
     hasty_get_launch(owner, &best);

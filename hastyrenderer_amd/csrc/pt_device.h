@@ -144,7 +144,8 @@ struct DevMaterial {
 };
 
 struct DevScene {
-  const float4* nodes;  // 8 float4 per node
+  const float4* nodes;  // 8 float4 per 4-wide node (or node half)
+  int bvhWidth;         // children per node: 4 or 8
   const float4* tris;   // 3 float4 per leaf triangle
   const float* triPos;  // 9 per global triangle
   const float* triNrm;
@@ -312,31 +313,22 @@ __device__ inline void walkInit(Walk& w, const RayPre& r, float tBound, int32_t 
 }
 // One step; true when the walk has ended (w.best then holds the result,
 // see walkHit).
-template <int LDS_DEPTH, int OVF_DEPTH, bool CLOSEST>
-__device__ inline bool walkStep(const DevScene& s, Walk& w, uint2* ldsGeneric, int laneStride, uint2* ovf) {
-  // The stack's LDS part through an LDS-qualified pointer: the compiler
-  // then keeps LDS and overflow accesses apart (ds_* and global_*) instead
-  // of merging them into one flat access, which would wait on both counters.
-  // Entries are (ref, entry distance) packed as ref | bits(t) << 32.
-  typedef __attribute__((address_space(3))) unsigned long long LdsEntry;
-  LdsEntry* lds = (LdsEntry*)ldsGeneric;
-  const RayPre& r = w.r;
-  {
-    if ((w.ref & kLeafFlag) == 0) {
-      float e0, e1, e2, e3;
-      int4 refs;
-      w.nNodes++;
-      const float tb = (CLOSEST && w.best.tri < 0) ? __builtin_inff() : w.best.t * kCullSlack;
-      // Entry and exit distances of a child box; the exit is scaled by 1 + 6u
-      // (Ize, "Robust BVH ray traversal") so that rounding never culls a box
-      // the exact ray enters.  An empty slot's planes are all +inf, so its
-      // entry is +inf or its exit -inf: the closest-hit test needs no branch
-      // on the child ref (the occlusion test keeps it: faster there).
-      if constexpr (CLOSEST) {
-        const float4* n = s.nodes + (size_t)w.ref * 8;
-        const float4 lox = ldg4(n), hix = ldg4(n + 1), loy = ldg4(n + 2), hiy = ldg4(n + 3), loz = ldg4(n + 4),
-                     hiz = ldg4(n + 5), rf = ldg4(n + 6);
-        refs = make_int4(__float_as_int(rf.x), __float_as_int(rf.y), __float_as_int(rf.z), __float_as_int(rf.w));
+// Entry distances of the four children of one 128-byte node half (a 4-wide
+// node, or half of an 8-wide one): +inf where the child is not entered
+// before tb.  The exit is scaled by 1 + 6u (Ize, "Robust BVH ray
+// traversal") so that rounding never culls a box the exact ray enters.  An
+// empty slot's planes are all +inf, so its entry is +inf or its exit -inf:
+// the closest-hit test needs no branch on the child ref (the occlusion test
+// keeps it: faster there).
+template <bool CLOSEST>
+__device__ inline void testHalf(const DevScene& s, const RayPre& r, uint32_t half, float tb, float& e0, float& e1,
+                                float& e2, float& e3, uint32_t& r0, uint32_t& r1, uint32_t& r2, uint32_t& r3) {
+  int4 refs;
+  if constexpr (CLOSEST) {
+    const float4* n = s.nodes + (size_t)half * 8;
+    const float4 lox = ldg4(n), hix = ldg4(n + 1), loy = ldg4(n + 2), hiy = ldg4(n + 3), loz = ldg4(n + 4),
+                 hiz = ldg4(n + 5), rf = ldg4(n + 6);
+    refs = make_int4(__float_as_int(rf.x), __float_as_int(rf.y), __float_as_int(rf.z), __float_as_int(rf.w));
 #define HDEV_SLAB(LX, HX, LY, HY, LZ, HZ, RF, TOUT)                                          \
   {                                                                                          \
     float t0x = (LX - r.o.x) * r.inv.x, t1x = (HX - r.o.x) * r.inv.x;                        \
@@ -347,19 +339,19 @@ __device__ inline bool walkStep(const DevScene& s, Walk& w, uint2* ldsGeneric, i
     bool h = (tmin <= tmax) & (tmax > 0.0f) & (tmin <= tb);                                      \
     TOUT = h ? tmin : __builtin_inff();                                                      \
   }
-        HDEV_SLAB(lox.x, hix.x, loy.x, hiy.x, loz.x, hiz.x, refs.x, e0)
-        HDEV_SLAB(lox.y, hix.y, loy.y, hiy.y, loz.y, hiz.y, refs.y, e1)
-        HDEV_SLAB(lox.z, hix.z, loy.z, hiy.z, loz.z, hiz.z, refs.z, e2)
-        HDEV_SLAB(lox.w, hix.w, loy.w, hiy.w, loz.w, hiz.w, refs.w, e3)
+    HDEV_SLAB(lox.x, hix.x, loy.x, hiy.x, loz.x, hiz.x, refs.x, e0)
+    HDEV_SLAB(lox.y, hix.y, loy.y, hiy.y, loz.y, hiz.y, refs.y, e1)
+    HDEV_SLAB(lox.z, hix.z, loy.z, hiy.z, loz.z, hiz.z, refs.z, e2)
+    HDEV_SLAB(lox.w, hix.w, loy.w, hiy.w, loz.w, hiz.w, refs.w, e3)
 #undef HDEV_SLAB
-      } else {
-        // 32-bit byte offsets from the node array (< 4 GiB of nodes)
-        const char* nb = reinterpret_cast<const char*>(s.nodes);
-        const uint32_t no = w.ref * 128u;
-        auto plane = [&](uint32_t off) { return ldg4(reinterpret_cast<const float4*>(nb + (no + off))); };
-        const float4 nx = plane(r.nearX), fx = plane(r.nearX ^ 16u), ny = plane(r.nearY), fy = plane(r.nearY ^ 16u),
-                     nz = plane(r.nearZ), fz = plane(r.nearZ ^ 16u), rf = plane(96u);
-        refs = make_int4(__float_as_int(rf.x), __float_as_int(rf.y), __float_as_int(rf.z), __float_as_int(rf.w));
+  } else {
+    // 32-bit byte offsets from the node array (< 4 GiB of nodes)
+    const char* nb = reinterpret_cast<const char*>(s.nodes);
+    const uint32_t no = half * 128u;
+    auto plane = [&](uint32_t off) { return ldg4(reinterpret_cast<const float4*>(nb + (no + off))); };
+    const float4 nx = plane(r.nearX), fx = plane(r.nearX ^ 16u), ny = plane(r.nearY), fy = plane(r.nearY ^ 16u),
+                 nz = plane(r.nearZ), fz = plane(r.nearZ ^ 16u), rf = plane(96u);
+    refs = make_int4(__float_as_int(rf.x), __float_as_int(rf.y), __float_as_int(rf.z), __float_as_int(rf.w));
 #define HDEV_SLAB(NX, FX, NY, FY, NZ, FZ, RF, TOUT)                                                \
   {                                                                                                \
     float tmin = fmaxf(fmaxf((NX - r.o.x) * r.binv.x, (NY - r.o.y) * r.binv.y), (NZ - r.o.z) * r.binv.z);  \
@@ -368,66 +360,124 @@ __device__ inline bool walkStep(const DevScene& s, Walk& w, uint2* ldsGeneric, i
     bool h = (uint32_t)(RF) != kLeafFlag && tmin <= tmax && tmax > 0.0f && tmin <= tb;            \
     TOUT = h ? tmin : __builtin_inff();                                                            \
   }
-        HDEV_SLAB(nx.x, fx.x, ny.x, fy.x, nz.x, fz.x, refs.x, e0)
-        HDEV_SLAB(nx.y, fx.y, ny.y, fy.y, nz.y, fz.y, refs.y, e1)
-        HDEV_SLAB(nx.z, fx.z, ny.z, fy.z, nz.z, fz.z, refs.z, e2)
-        HDEV_SLAB(nx.w, fx.w, ny.w, fy.w, nz.w, fz.w, refs.w, e3)
+    HDEV_SLAB(nx.x, fx.x, ny.x, fy.x, nz.x, fz.x, refs.x, e0)
+    HDEV_SLAB(nx.y, fx.y, ny.y, fy.y, nz.y, fz.y, refs.y, e1)
+    HDEV_SLAB(nx.z, fx.z, ny.z, fy.z, nz.z, fz.z, refs.z, e2)
+    HDEV_SLAB(nx.w, fx.w, ny.w, fy.w, nz.w, fz.w, refs.w, e3)
 #undef HDEV_SLAB
-      }
-      uint32_t r0 = (uint32_t)refs.x, r1 = (uint32_t)refs.y, r2 = (uint32_t)refs.z, r3 = (uint32_t)refs.w;
+  }
+  r0 = (uint32_t)refs.x, r1 = (uint32_t)refs.y, r2 = (uint32_t)refs.z, r3 = (uint32_t)refs.w;
+}
+
+// One step; true when the walk has ended (w.best then holds the result,
+// see walkHit).  W: children per node (4 or 8; an 8-wide node is two
+// 128-byte halves).
+template <int LDS_DEPTH, int OVF_DEPTH, bool CLOSEST, int W = 4>
+__device__ inline bool walkStep(const DevScene& s, Walk& w, uint2* ldsGeneric, int laneStride, uint2* ovf) {
+  static_assert(W == 4 || W == 8, "4- or 8-wide nodes");
+  // The stack's LDS part through an LDS-qualified pointer: the compiler
+  // then keeps LDS and overflow accesses apart (ds_* and global_*) instead
+  // of merging them into one flat access, which would wait on both counters.
+  // Entries are (ref, entry distance) packed as ref | bits(t) << 32.
+  typedef __attribute__((address_space(3))) unsigned long long LdsEntry;
+  LdsEntry* lds = (LdsEntry*)ldsGeneric;
+  const RayPre& r = w.r;
+  // push a child (farthest first); the BVH depth is checked at load so that
+  // (W - 1) * depth + 1 entries fit, the guard only keeps every write inside
+  // this lane's slice
+#define HDEV_PUSH(RR, EE)                                                                \
+  if ((EE) != __builtin_inff() && w.sp < LDS_DEPTH + OVF_DEPTH) {                       \
+    uint2 e_ = make_uint2(RR, __float_as_uint(EE));                                     \
+    if (w.sp < LDS_DEPTH) lds[w.sp * laneStride] = ((unsigned long long)e_.y << 32) | e_.x; \
+    else ovf[w.sp - LDS_DEPTH] = e_;                                                    \
+    ++w.sp;                                                                             \
+  }
+  if ((w.ref & kLeafFlag) == 0) {
+    w.nNodes++;
+    const float tb = (CLOSEST && w.best.tri < 0) ? __builtin_inff() : w.best.t * kCullSlack;
+    if constexpr (W == 4) {
+      float e0, e1, e2, e3;
+      uint32_t r0, r1, r2, r3;
+      testHalf<CLOSEST>(s, r, w.ref, tb, e0, e1, e2, e3, r0, r1, r2, r3);
       // children entered have finite entry distances; after the sort they
-      // come first
+      // come first; continue with the nearest
       if (fminf(fminf(e0, e1), fminf(e2, e3)) != __builtin_inff()) {
         cswap(e0, r0, e1, r1);
         cswap(e2, r2, e3, r3);
         cswap(e0, r0, e2, r2);
         cswap(e1, r1, e3, r3);
         cswap(e1, r1, e2, r2);
-        // push the farther children (farthest first), continue with the nearest
-        // The BVH depth is checked at load so that 3 * depth fits; the guard
-        // only keeps every write inside this lane's slice.
-#define HDEV_PUSH(RR, EE)                                       \
-  if (w.sp < LDS_DEPTH + OVF_DEPTH) {                            \
-    uint2 e_ = make_uint2(RR, __float_as_uint(EE));            \
-    if (w.sp < LDS_DEPTH) lds[w.sp * laneStride] = ((unsigned long long)e_.y << 32) | e_.x; \
-    else ovf[w.sp - LDS_DEPTH] = e_;                             \
-    ++w.sp;                                                      \
-  }
-        if (e3 != __builtin_inff()) HDEV_PUSH(r3, e3)
-        if (e2 != __builtin_inff()) HDEV_PUSH(r2, e2)
-        if (e1 != __builtin_inff()) HDEV_PUSH(r1, e1)
-#undef HDEV_PUSH
+        HDEV_PUSH(r3, e3)
+        HDEV_PUSH(r2, e2)
+        HDEV_PUSH(r1, e1)
         w.ref = r0;
         return false;
       }
-    } else if (w.ref != kLeafFlag) {
-      uint32_t count = (w.ref >> 24) & 0x7Fu;
-      uint32_t first = w.ref & 0x00FFFFFFu;
-      for (uint32_t k = 0; k < count; ++k) {
-        const float4* tr = s.tris + (size_t)(first + k) * 3;
-        float4 p0 = ldg4(tr), p1 = ldg4(tr + 1), p2 = ldg4(tr + 2);
-        int32_t id = (int32_t)__float_as_uint(p0.w);
-        w.nTris++;
-        float t, u, v;
-        if (intersectTri(r, p0, p1, p2, 1e-6f, &t, &u, &v)) {
-          if (CLOSEST) {
-            if (w.best.tri < 0 || t < w.best.t || (t == w.best.t && id < w.best.tri)) {
-              w.best.t = t;
-              w.best.u = u;
-              w.best.v = v;
-              w.best.tri = id;
-            }
-          } else if (t < w.tBound || (t == w.tBound && id < w.idBound)) {
+    } else {
+      float e0, e1, e2, e3, e4, e5, e6, e7;
+      uint32_t r0, r1, r2, r3, r4, r5, r6, r7;
+      testHalf<CLOSEST>(s, r, 2u * w.ref, tb, e0, e1, e2, e3, r0, r1, r2, r3);
+      testHalf<CLOSEST>(s, r, 2u * w.ref + 1u, tb, e4, e5, e6, e7, r4, r5, r6, r7);
+      if (fminf(fminf(fminf(e0, e1), fminf(e2, e3)), fminf(fminf(e4, e5), fminf(e6, e7))) != __builtin_inff()) {
+        // Batcher's odd-even merge sort of 8 (19 compare-exchanges)
+        cswap(e0, r0, e1, r1);
+        cswap(e2, r2, e3, r3);
+        cswap(e4, r4, e5, r5);
+        cswap(e6, r6, e7, r7);
+        cswap(e0, r0, e2, r2);
+        cswap(e1, r1, e3, r3);
+        cswap(e4, r4, e6, r6);
+        cswap(e5, r5, e7, r7);
+        cswap(e1, r1, e2, r2);
+        cswap(e5, r5, e6, r6);
+        cswap(e0, r0, e4, r4);
+        cswap(e1, r1, e5, r5);
+        cswap(e2, r2, e6, r6);
+        cswap(e3, r3, e7, r7);
+        cswap(e2, r2, e4, r4);
+        cswap(e3, r3, e5, r5);
+        cswap(e1, r1, e2, r2);
+        cswap(e3, r3, e4, r4);
+        cswap(e5, r5, e6, r6);
+        HDEV_PUSH(r7, e7)
+        HDEV_PUSH(r6, e6)
+        HDEV_PUSH(r5, e5)
+        HDEV_PUSH(r4, e4)
+        HDEV_PUSH(r3, e3)
+        HDEV_PUSH(r2, e2)
+        HDEV_PUSH(r1, e1)
+        w.ref = r0;
+        return false;
+      }
+    }
+  } else if (w.ref != kLeafFlag) {
+    uint32_t count = (w.ref >> 24) & 0x7Fu;
+    uint32_t first = w.ref & 0x00FFFFFFu;
+    for (uint32_t k = 0; k < count; ++k) {
+      const float4* tr = s.tris + (size_t)(first + k) * 3;
+      float4 p0 = ldg4(tr), p1 = ldg4(tr + 1), p2 = ldg4(tr + 2);
+      int32_t id = (int32_t)__float_as_uint(p0.w);
+      w.nTris++;
+      float t, u, v;
+      if (intersectTri(r, p0, p1, p2, 1e-6f, &t, &u, &v)) {
+        if (CLOSEST) {
+          if (w.best.tri < 0 || t < w.best.t || (t == w.best.t && id < w.best.tri)) {
             w.best.t = t;
             w.best.u = u;
             w.best.v = v;
             w.best.tri = id;
-            return true;
           }
+        } else if (t < w.tBound || (t == w.tBound && id < w.idBound)) {
+          w.best.t = t;
+          w.best.u = u;
+          w.best.v = v;
+          w.best.tri = id;
+          return true;
         }
       }
     }
   }
+#undef HDEV_PUSH
   // pop the nearest remaining entry that can still hold a closer hit
   while (w.sp > 0) {
     --w.sp;
@@ -455,31 +505,47 @@ __device__ inline Hit walkHit(const Walk& w) {
   return h;
 }
 
-template <int LDS_DEPTH, int OVF_DEPTH, bool CLOSEST>
-__device__ inline Hit traverse(const DevScene& s, const RayPre& r, float tBound, int32_t idBound, uint2* ldsGeneric,
-                               int laneStride, uint2* ovf, uint32_t* nodeCount, uint32_t* triCount) {
+template <int LDS_DEPTH, int OVF_DEPTH, bool CLOSEST, int W = 4>
+__device__ inline Hit traverseW(const DevScene& s, const RayPre& r, float tBound, int32_t idBound, uint2* ldsGeneric,
+                                int laneStride, uint2* ovf, uint32_t* nodeCount, uint32_t* triCount) {
   Walk w;
   walkInit(w, r, tBound, idBound);
   w.nNodes = w.nTris = 0;
-  while (!walkStep<LDS_DEPTH, OVF_DEPTH, CLOSEST>(s, w, ldsGeneric, laneStride, ovf)) {
+  while (!walkStep<LDS_DEPTH, OVF_DEPTH, CLOSEST, W>(s, w, ldsGeneric, laneStride, ovf)) {
   }
   if (nodeCount) *nodeCount += w.nNodes;
   if (triCount) *triCount += w.nTris;
   return walkHit<CLOSEST>(w);
 }
+// W = 0: the scene's node width (s.bvhWidth, the same for every lane).
+template <int LDS_DEPTH, int OVF_DEPTH, bool CLOSEST, int W = 0>
+__device__ inline Hit traverse(const DevScene& s, const RayPre& r, float tBound, int32_t idBound, uint2* ldsGeneric,
+                               int laneStride, uint2* ovf, uint32_t* nodeCount, uint32_t* triCount) {
+  if constexpr (W != 0) {
+    return traverseW<LDS_DEPTH, OVF_DEPTH, CLOSEST, W>(s, r, tBound, idBound, ldsGeneric, laneStride, ovf, nodeCount,
+                                                      triCount);
+  } else {
+    if (s.bvhWidth == 8)
+      return traverseW<LDS_DEPTH, OVF_DEPTH, CLOSEST, 8>(s, r, tBound, idBound, ldsGeneric, laneStride, ovf,
+                                                         nodeCount, triCount);
+    return traverseW<LDS_DEPTH, OVF_DEPTH, CLOSEST, 4>(s, r, tBound, idBound, ldsGeneric, laneStride, ovf, nodeCount,
+                                                       triCount);
+  }
+}
 
-template <int LDS_DEPTH, int OVF_DEPTH>
+template <int LDS_DEPTH, int OVF_DEPTH, int W = 0>
 __device__ inline Hit traceClosest(const DevScene& s, V3 o, V3 d, uint2* lds, int laneStride, uint2* ovf,
                                    uint32_t* nodeCount, uint32_t* triCount) {
   RayPre r = prepareRay(o, d);
-  return traverse<LDS_DEPTH, OVF_DEPTH, true>(s, r, __builtin_inff(), 0, lds, laneStride, ovf, nodeCount, triCount);
+  return traverse<LDS_DEPTH, OVF_DEPTH, true, W>(s, r, __builtin_inff(), 0, lds, laneStride, ovf, nodeCount,
+                                                 triCount);
 }
 
 // Is there a triangle whose hit beats (tBound, idBound)?  Returns its index or -1.
-template <int LDS_DEPTH, int OVF_DEPTH>
+template <int LDS_DEPTH, int OVF_DEPTH, int W = 0>
 __device__ inline int32_t findBlocker(const DevScene& s, const RayPre& r, float tBound, int32_t idBound, uint2* lds,
                                       int laneStride, uint2* ovf, uint32_t* nodeCount, uint32_t* triCount) {
-  Hit h = traverse<LDS_DEPTH, OVF_DEPTH, false>(s, r, tBound, idBound, lds, laneStride, ovf, nodeCount, triCount);
+  Hit h = traverse<LDS_DEPTH, OVF_DEPTH, false, W>(s, r, tBound, idBound, lds, laneStride, ovf, nodeCount, triCount);
   return h.tri;
 }
 

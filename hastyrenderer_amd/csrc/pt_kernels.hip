@@ -390,7 +390,8 @@ __global__ void kSetLimit(uint32_t* c, uint32_t limit) {
 // --------------------------------------------------------------- extend ----
 // COUNT: add rays, node visits and triangle tests to the statistics (the
 // counting is compiled out otherwise).
-template <bool MULTI, bool COUNT>
+// W: node width of the single job's scene (4 or 8), 0 = each job's own.
+template <bool MULTI, bool COUNT, int W>
 __global__ void __launch_bounds__(kBlock) HASTY_EXTEND_ATTR kExtend(JobSet js, PathBuffers pb) {
   __shared__ uint2 stack[kStackLds * kBlock];
   const ExtSet es = extensionSet(pb);
@@ -405,7 +406,8 @@ __global__ void __launch_bounds__(kBlock) HASTY_EXTEND_ATTR kExtend(JobSet js, P
     const uint32_t job = MULTI ? jobOf<MULTI>(js, pb.state[slot]) : 0u;
     Hit h;
     forJob<MULTI>(js, job, [&](const DevScene& sc, const PassInfo&) {
-      h = traceClosest<kStackLds, kStackOvf>(sc, f4xyz(o), f4xyz(d), stack + threadIdx.x, kBlock, ovf, &nodes, &tris);
+      h = traceClosest<kStackLds, kStackOvf, W>(sc, f4xyz(o), f4xyz(d), stack + threadIdx.x, kBlock, ovf, &nodes,
+                                                &tris);
     });
     pb.hit[slot] = make_float4(h.t, h.u, h.v, __int_as_float(h.tri));
     rays++;
@@ -664,7 +666,7 @@ __global__ void __launch_bounds__(kBlock) HASTY_SHADE_ATTR kShade(JobSet js, Pat
 }
 
 // -------------------------------------------------------------- connect ----
-template <bool MULTI, bool COUNT>
+template <bool MULTI, bool COUNT, int W>
 __global__ void __launch_bounds__(kBlock) HASTY_CONNECT_ATTR kConnect(JobSet js, PathBuffers pb) {
   __shared__ uint2 stack[kStackLds * kBlock];
   uint32_t n = pb.counters[kCntS];
@@ -695,18 +697,18 @@ __global__ void __launch_bounds__(kBlock) HASTY_CONNECT_ATTR kConnect(JobSet js,
       float tT, uT, vT;
       visible = intersectTri(rp, p0, p1, p2, 1e-6f, &tT, &uT, &vT);
       if (visible) {
-        visible = findBlocker<kStackLds, kStackOvf>(s, rp, tT, target, stack + threadIdx.x, kBlock, ovf,
+        visible = findBlocker<kStackLds, kStackOvf, W>(s, rp, tT, target, stack + threadIdx.x, kBlock, ovf,
                                                     &nodes, &tris) < 0;
         h.t = tT, h.u = uT, h.v = vT, h.tri = target;
       }
     } else if (!s.hasSurfaceLight) {
       // no emissive triangle: visible <=> nothing is hit at all
-      visible = findBlocker<kStackLds, kStackOvf>(s, rp, __builtin_inff(), 0x7FFFFFFF, stack + threadIdx.x, kBlock,
+      visible = findBlocker<kStackLds, kStackOvf, W>(s, rp, __builtin_inff(), 0x7FFFFFFF, stack + threadIdx.x, kBlock,
                                                   ovf, &nodes,
                                                   &tris) < 0;
       h.t = __builtin_inff(), h.u = h.v = 0.0f, h.tri = -1;
     } else {
-      h = traceClosest<kStackLds, kStackOvf>(s, so, sd, stack + threadIdx.x, kBlock, ovf,
+      h = traceClosest<kStackLds, kStackOvf, W>(s, so, sd, stack + threadIdx.x, kBlock, ovf,
                                              &nodes, &tris);
       bool hitLight = h.tri >= 0 && ldg(&s.mats[ldg(s.triMat + h.tri)].isLight);
       visible = h.tri < 0 || hitLight;
@@ -983,7 +985,7 @@ void hipOk(hipError_t e, const char* what) {
 // The bounces of one chunk: fill, extend, shade, connect, retire, advance.
 // `chunk` is even, so the queue pointers are back in place afterwards and
 // every chunk issues identical launches (what a captured graph replays).
-template <bool MULTI, bool GUIDED>
+template <bool MULTI, bool GUIDED, int W>
 void launchChunk(const JobSet& js, const PathBuffers& pbIn, const LaunchConfig& cfg, uint32_t chunk,
                  hipStream_t stream, KernelTimer* timer) {
   PathBuffers pb = pbIn;
@@ -1002,8 +1004,8 @@ void launchChunk(const JobSet& js, const PathBuffers& pbIn, const LaunchConfig& 
     kFillCommit<<<1, 64, 0, stream>>>(pb.counters);
     t1(kKMisc);
     t0(kKExtend);
-    if (countStats) kExtend<MULTI, true><<<cfg.traceBlocks, kBlock, 0, stream>>>(js, pb);
-    else kExtend<MULTI, false><<<cfg.traceBlocks, kBlock, 0, stream>>>(js, pb);
+    if (countStats) kExtend<MULTI, true, W><<<cfg.traceBlocks, kBlock, 0, stream>>>(js, pb);
+    else kExtend<MULTI, false, W><<<cfg.traceBlocks, kBlock, 0, stream>>>(js, pb);
     t1(kKExtend);
     if (GUIDED) {
       t0(kKGuideInfer);
@@ -1014,8 +1016,8 @@ void launchChunk(const JobSet& js, const PathBuffers& pbIn, const LaunchConfig& 
     kShade<MULTI, GUIDED><<<cfg.shadeBlocks, kBlock, 0, stream>>>(js, pb);
     t1(kKShade);
     t0(kKConnect);
-    if (countStats) kConnect<MULTI, true><<<cfg.traceBlocks, kBlock, 0, stream>>>(js, pb);
-    else kConnect<MULTI, false><<<cfg.traceBlocks, kBlock, 0, stream>>>(js, pb);
+    if (countStats) kConnect<MULTI, true, W><<<cfg.traceBlocks, kBlock, 0, stream>>>(js, pb);
+    else kConnect<MULTI, false, W><<<cfg.traceBlocks, kBlock, 0, stream>>>(js, pb);
     t1(kKConnect);
     t0(kKRetire);
     kRetire<GUIDED><<<cfg.shadeBlocks, kBlock, 0, stream>>>(js, pb);
@@ -1027,6 +1029,17 @@ void launchChunk(const JobSet& js, const PathBuffers& pbIn, const LaunchConfig& 
     pb.qA = pb.qB;
     pb.qB = tmp;
   }
+}
+
+// The chunk's kernels for a batch (each job's node width read by the
+// kernels) or for one job (its width compiled in).
+void launchChunkAny(const JobSet& js, const PathBuffers& pb, const LaunchConfig& cfg, uint32_t chunk,
+                    hipStream_t stream, KernelTimer* timer, bool multi, bool guided) {
+  if (multi) launchChunk<true, false, 0>(js, pb, cfg, chunk, stream, timer);
+  else if (guided && js.s.bvhWidth == 8) launchChunk<false, true, 8>(js, pb, cfg, chunk, stream, timer);
+  else if (guided) launchChunk<false, true, 4>(js, pb, cfg, chunk, stream, timer);
+  else if (js.s.bvhWidth == 8) launchChunk<false, false, 8>(js, pb, cfg, chunk, stream, timer);
+  else launchChunk<false, false, 4>(js, pb, cfg, chunk, stream, timer);
 }
 
 }  // namespace
@@ -1114,9 +1127,7 @@ void renderJobs(const std::vector<RenderJobDesc>& jobs, const PathBuffers& pb, c
   // replay graphs.
   if (cfg.useGraph && !timer) {
     hipOk(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
-    if (nJobs > 1) launchChunk<true, false>(js, pb, cfg, chunk, stream, nullptr);
-    else if (guide) launchChunk<false, true>(js, pb, cfg, chunk, stream, nullptr);
-    else launchChunk<false, false>(js, pb, cfg, chunk, stream, nullptr);
+    launchChunkAny(js, pb, cfg, chunk, stream, nullptr, nJobs > 1, guide != nullptr);
     hipOk(hipStreamEndCapture(stream, &graph), "hipStreamEndCapture");
     hipOk(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0), "hipGraphInstantiate");
   }
@@ -1126,12 +1137,8 @@ void renderJobs(const std::vector<RenderJobDesc>& jobs, const PathBuffers& pb, c
       t0(kKGraph);
       hipOk(hipGraphLaunch(exec, stream), "hipGraphLaunch");
       t1(kKGraph);
-    } else if (nJobs > 1) {
-      launchChunk<true, false>(js, pb, cfg, chunk, stream, timer);
-    } else if (guide) {
-      launchChunk<false, true>(js, pb, cfg, chunk, stream, timer);
     } else {
-      launchChunk<false, false>(js, pb, cfg, chunk, stream, timer);
+      launchChunkAny(js, pb, cfg, chunk, stream, timer, nJobs > 1, guide != nullptr);
     }
     uint32_t cnt[kNumCounters];
     hipOk(hipMemcpyAsync(cnt, pb.counters, sizeof(cnt), hipMemcpyDeviceToHost, stream), "counter readback");

@@ -72,8 +72,8 @@ extern "C" int hasty_trace_hits_device_noalloc(HastyScene* scene, const HastyRay
     return 1;
   }
   // A lane's stack slice holds kStackLds + kStackOvf entries; the traversal
-  // pushes at most 3 per inner level plus the root.
-  if (3 * v.bvhDepth + 1 > hasty::kStackLds + hasty::kStackOvf) {
+  // pushes at most width - 1 per inner level plus the root.
+  if ((v.bvhWidth - 1) * v.bvhDepth + 1 > hasty::kStackLds + hasty::kStackOvf) {
     hasty::setLastError("BVH too deep for the no-allocation traversal stack (depth " + std::to_string(v.bvhDepth) +
                         ")");
     return 1;

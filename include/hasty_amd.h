@@ -45,9 +45,10 @@ typedef struct HastySceneInfo {
   uint32_t num_materials;
   uint32_t num_light_triangles;
   int32_t has_environment;
-  uint32_t bvh_nodes;      /* 4-wide nodes, 128 bytes each */
-  int32_t bvh_depth;       /* inner levels of the 4-wide tree */
+  uint32_t bvh_nodes;      /* nodes of the BVH in use (32 x bvh_width bytes each) */
+  int32_t bvh_depth;       /* its inner levels */
   uint64_t device_bytes;   /* scene + BVH bytes resident in HBM */
+  int32_t bvh_width;       /* children per node: 4 or 8 */
 } HastySceneInfo;
 
 /* Replaces RenderJob::loadJSON (core/src/RenderJob.cpp:21-28) together with
@@ -235,14 +236,17 @@ int hasty_set_launch(HastyScene* scene, uint32_t path_slots, uint32_t trace_bloc
 
 /* Launch configuration of a scene's path pool (also used for batches whose
  * first job is this scene).  chunk = bounces between host checks (even);
- * use_graph = replay each chunk as one captured hipGraph.  Fields that are 0
- * keep their value (except use_graph). */
+ * use_graph = replay each chunk as one captured hipGraph; bvh_width =
+ * children per BVH node, 4 or 8 (the tree of that width is built and
+ * uploaded the first time it is asked for; images do not depend on it).
+ * Fields that are 0 keep their value (except use_graph). */
 typedef struct HastyLaunch {
   uint32_t path_slots;
   uint32_t trace_blocks;
   uint32_t shade_blocks;
   uint32_t chunk;
   int32_t use_graph;
+  uint32_t bvh_width;
 } HastyLaunch;
 int hasty_get_launch(const HastyScene* scene, HastyLaunch* out);
 int hasty_set_launch_ex(HastyScene* scene, const HastyLaunch* launch);
@@ -267,8 +271,9 @@ int hasty_render_batch_device(const HastyBatchJob* jobs, uint32_t n_jobs);
 
 /* Autotuner (no reference counterpart): times short renders (trial_samples
  * samples of every job into scratch films) over path slots, grid sizes,
- * chunk length and graph replay by coordinate descent, applies the fastest
- * configuration to jobs[0].scene and returns it in *chosen (may be NULL). */
+ * chunk length, graph replay and BVH node width by coordinate descent,
+ * applies the fastest configuration to jobs[0].scene (the width to every
+ * job's scene) and returns it in *chosen (may be NULL). */
 int hasty_autotune_batch(const HastyBatchJob* jobs, uint32_t n_jobs, uint32_t trial_samples, HastyLaunch* chosen);
 int hasty_autotune(HastyScene* scene, uint32_t seed, uint32_t trial_samples, HastyLaunch* chosen);
 

@@ -96,3 +96,43 @@ def test_autotune_changes_only_time(cornell_job, ggx_small_job):
     ref2 = _alone(sc2, 0, 0, 2)
     ha.autotune_batch([(sc2, 0, 0, 1, None), (sc, 1, 0, 1, None)], trial_samples=1)
     _same(ref2, _alone(sc2, 0, 0, 2))
+
+
+@pytest.mark.parametrize("which", ["cornell", "ggx_small", "caustic_small"])
+def test_bvh_width_changes_only_time(which, request, cuboid_job):
+    """4- and 8-wide BVHs of the same scene: identical hit records, films and
+    photon-map passes, bit for bit (conservative box tests and the
+    smallest-t / smallest-index tie rule make the closest hit independent of
+    the tree); a batch mixing widths equals its jobs alone."""
+    job = request.getfixturevalue(which + "_job")
+    sc = ha.Scene(job)
+    assert sc.info()["bvh_width"] == 4
+    rng = np.random.default_rng(3)
+    o = rng.uniform(-1.5, 2.5, (20000, 3))
+    d = rng.normal(size=(20000, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    rays = np.concatenate([o, d], 1).astype(np.float32)
+    hits4 = sc.trace_hits(rays)
+    f4 = _alone(sc, 2, 0, 3)
+    sc.set_launch_ex(bvh_width=8)
+    info = sc.info()
+    print("%s: 8-wide BVH %d nodes, depth %d" % (which, info["bvh_nodes"], info["bvh_depth"]), flush=True)
+    assert info["bvh_width"] == 8 and sc.get_launch()["bvh_width"] == 8
+    assert np.array_equal(sc.trace_hits(rays).view(np.uint8), hits4.view(np.uint8))
+    _same(_alone(sc, 2, 0, 3), f4)
+    if which == "caustic_small":
+        outs = []
+        for w in (4, 8):
+            sc.set_launch_ex(bvh_width=w)
+            g = ha.Sppm(sc, 5)
+            outs.append([g.pass_(k)[0].copy() for k in range(2)])
+        for x, y in zip(*outs):
+            assert np.array_equal(x.view(np.uint32), y.view(np.uint32))
+    # a batch whose jobs use different widths
+    other = ha.Scene(cuboid_job)
+    ref = _alone(other, 4, 0, 2)
+    sc.set_launch_ex(bvh_width=8)
+    films = [ha.DeviceFilms(s.info()["width"], s.info()["height"]) for s in (sc, other)]
+    ha.render_batch([(sc, 2, 0, 3, films[0]), (other, 4, 0, 2, films[1])])
+    _same(films[0].download(), f4)
+    _same(films[1].download(), ref)
