@@ -40,6 +40,8 @@ EXPORTS = (
     "hasty_trace_hits",
     "hasty_trace_hits_device",
     "hasty_trace_hits_device_noalloc",
+    "hasty_image_read",
+    "hasty_image_write_exr",
     "hasty_set_profiling",
     "hasty_kernel_stats",
     "hasty_reset_stats",
@@ -126,6 +128,15 @@ class CameraDescC(C.Structure):
     ]
 
 
+class TextureDescC(C.Structure):
+    _fields_ = [
+        ("width", C.c_uint32),
+        ("height", C.c_uint32),
+        ("channels", C.c_uint32),
+        ("pixels", C.c_void_p),
+    ]
+
+
 class SceneDescC(C.Structure):
     _fields_ = [
         ("settings", RenderSettingsC),
@@ -151,6 +162,9 @@ class SceneDescC(C.Structure):
         ("env_height", C.c_uint32),
         ("env_rgb", C.c_void_p),
         ("background", C.c_float * 3),
+        ("num_textures", C.c_uint32),
+        ("textures", C.c_void_p),
+        ("material_textures", C.c_void_p),
     ]
 
 
@@ -246,6 +260,9 @@ def lib():
     L.hasty_trace_hits.argtypes = [P, C.c_void_p, C.c_uint32, C.c_void_p]
     L.hasty_trace_hits_device.argtypes = [P, C.c_void_p, C.c_uint32, C.c_void_p]
     L.hasty_trace_hits_device_noalloc.argtypes = [P, C.c_void_p, C.c_uint32, C.c_void_p]
+    u32p = C.POINTER(C.c_uint32)
+    L.hasty_image_read.argtypes = [C.c_char_p, u32p, u32p, u32p, fp, C.c_uint64]
+    L.hasty_image_write_exr.argtypes = [C.c_char_p, C.c_uint32, C.c_uint32, fp]
     L.hasty_set_profiling.argtypes = [P, C.c_int]
     L.hasty_kernel_stats.argtypes = [P, C.POINTER(KernelStatC), C.c_int]
     L.hasty_reset_stats.argtypes = [P]
@@ -315,12 +332,15 @@ class Scene:
     @classmethod
     def create(cls, settings: dict, camera: dict, vertices, vertex_indices, geom_ids, prim_ids, material_ids,
                materials, normals=None, normal_indices=None, texcoords=None, texcoord_indices=None, smooth=None,
-               env_rgb=None, background=(0.0, 0.0, 0.0)):
+               env_rgb=None, background=(0.0, 0.0, 0.0), textures=None, material_textures=None):
         """hasty_scene_create from arrays (what Scene::Scene holds after
         tinyobj / OIIO): settings as RenderSettingsC fields, camera as
         CameraDescC fields, materials as a list of MaterialDescC-field dicts,
         per-triangle arrays of 3 indices (or one id) per triangle, env_rgb an
-        (H, W, 3) image with row 0 at the top or None."""
+        (H, W, 3) image with row 0 at the top or None.  textures: linear
+        float images, (H, W) for roughness / metallic or (H, W, 3) for albedo
+        / normal maps; material_textures: (num_materials, 4) indices (albedo,
+        roughness, metallic, normal map; -1 none)."""
         keep = []
 
         def arr(a, dtype, cols):
@@ -365,6 +385,25 @@ class Scene:
             d.env_height, d.env_width = env.shape[0], env.shape[1]
             d.env_rgb = env.ctypes.data
         d.background = (C.c_float * 3)(*background)
+        if textures:
+            tds = (TextureDescC * len(textures))()
+            for i, t in enumerate(textures):
+                t = np.ascontiguousarray(t, np.float32)
+                if t.ndim not in (2, 3) or (t.ndim == 3 and t.shape[2] != 3):
+                    raise ValueError("texture %d must be (H, W) or (H, W, 3)" % i)
+                keep.append(t)
+                tds[i].height, tds[i].width = t.shape[0], t.shape[1]
+                tds[i].channels = 1 if t.ndim == 2 else 3
+                tds[i].pixels = t.ctypes.data
+            keep.append(tds)
+            d.num_textures = len(textures)
+            d.textures = C.cast(tds, C.c_void_p)
+        if material_textures is not None:
+            mt = np.ascontiguousarray(material_textures, np.int32)
+            if mt.shape != (len(materials), 4):
+                raise ValueError("material_textures must be (num_materials, 4)")
+            keep.append(mt)
+            d.material_textures = mt.ctypes.data
         h = C.c_void_p()
         _err(lib().hasty_scene_create(C.byref(d), C.byref(h)))
         obj = cls.__new__(cls)
@@ -859,6 +898,24 @@ def renderPhotonThread(colorBuffer, normalBuffer, albedoBuffer, job: PMRenderJob
             b.copyBuffer()
         sample = job.fetchSample()
     threadData.stoppedFlag = True
+
+
+def read_image(path) -> np.ndarray:
+    """hasty_image_read: (H, W, C) float32 pixels, row 0 at the top."""
+    w, h, c = C.c_uint32(), C.c_uint32(), C.c_uint32()
+    p = os.fsencode(str(path))
+    _err(lib().hasty_image_read(p, C.byref(w), C.byref(h), C.byref(c), None, 0))
+    out = np.empty((h.value, w.value, c.value), np.float32)
+    _err(lib().hasty_image_read(p, C.byref(w), C.byref(h), C.byref(c), _fptr(out), out.size))
+    return out
+
+
+def write_exr(path, rgb) -> None:
+    """hasty_image_write_exr of an (H, W, 3) image."""
+    a = np.ascontiguousarray(rgb, np.float32)
+    if a.ndim != 3 or a.shape[2] != 3:
+        raise ValueError("write_exr takes an (H, W, 3) image")
+    _err(lib().hasty_image_write_exr(os.fsencode(str(path)), a.shape[1], a.shape[0], _fptr(a)))
 
 
 def computeEstimate(buffers) -> np.ndarray:

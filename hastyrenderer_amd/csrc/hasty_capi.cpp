@@ -17,6 +17,7 @@
 #include "capi_internal.h"
 #include "pt_kernels.h"
 #include "rng.h"
+#include "image_io.h"
 #include "scene_io.h"
 #include "sppm.h"
 #include "guide_kernels.h"
@@ -298,6 +299,7 @@ struct HastyScene {
     pb.L = allocPath<float4>(n);
     pb.state = allocPath<uint4>(n);
     pb.media = allocPath<uint4>(2 * n);
+    pb.mediaTx = allocPath<uint32_t>((size_t)hdev::kMaxMedia * n);
     pb.auxN = allocPath<float4>(n);
     pb.auxA = allocPath<float4>(n);
     pb.shO = allocPath<float4>(n);
@@ -635,6 +637,19 @@ static void jobFromDesc(const HastySceneDesc& d, hasty::Job& job) {
   sc.vertices.assign(d.vertices, d.vertices + 3 * (size_t)d.num_vertices);
   sc.normals.assign(d.normals, d.normals + 3 * (size_t)d.num_normals);
   sc.texcoords.assign(d.texcoords, d.texcoords + 2 * (size_t)d.num_texcoords);
+  need(d.textures, d.num_textures, "textures");
+  for (uint32_t t = 0; t < d.num_textures; ++t) {
+    const HastyTextureDesc& td = d.textures[t];
+    if (td.channels != 1 && td.channels != 3) throw std::runtime_error("descriptor: textures have 1 or 3 channels");
+    if (td.width == 0 || td.height == 0 || !td.pixels) throw std::runtime_error("descriptor: empty texture");
+    hasty::Texture tx;
+    tx.path = "texture" + std::to_string(t);
+    tx.width = td.width;
+    tx.height = td.height;
+    tx.channels = td.channels;
+    tx.px.assign(td.pixels, td.pixels + (size_t)td.width * td.height * td.channels);
+    sc.textures.push_back(std::move(tx));
+  }
   std::vector<uint8_t> tinted;
   for (uint32_t m = 0; m < d.num_materials; ++m) {
     const HastyMaterialDesc& md = d.materials[m];
@@ -651,6 +666,15 @@ static void jobFromDesc(const HastySceneDesc& d, hasty::Job& job) {
     out.anisotropy = md.anisotropy;
     out.transmission = md.transmission;
     out.varyIOR = md.vary_ior != 0;
+    if (d.material_textures) {
+      const int32_t* t = d.material_textures + 4 * (size_t)m;
+      for (int k = 0; k < 4; ++k)
+        if (t[k] < -1 || t[k] >= (int32_t)d.num_textures) throw std::runtime_error("descriptor: texture index out of range");
+      out.texAlbedo = t[0];
+      out.texRoughness = t[1];
+      out.texMetallic = t[2];
+      out.texNormal = t[3];
+    }
     sc.materials.push_back(out);
     tinted.push_back(md.tinted_emitter != 0);
   }
@@ -723,9 +747,32 @@ static void uploadScene(HastyScene& S) {
     d.varyIOR = m.varyIOR ? 1 : 0;
     d.isLight = m.isLight() ? 1 : 0;
     d.hasDiffuseLobe = ((1.0f - 0.5f * m.specular) * (1.0 - m.transmission) * (1.0 - m.metallic) > 0.0f) ? 1 : 0;
+    d.tex[0] = m.texAlbedo;
+    d.tex[1] = m.texRoughness;
+    d.tex[2] = m.texMetallic;
+    d.tex[3] = m.texNormal;
+    d.textured = m.textured() ? 1 : 0;
+    for (int t : {m.texAlbedo, m.texRoughness, m.texMetallic, m.texNormal})
+      if (t >= (int)sc.textures.size()) throw std::runtime_error("texture index out of range");
+    if (m.texAlbedo >= 0 && sc.textures[m.texAlbedo].channels != 3) throw std::runtime_error("albedo textures have 3 channels");
+    if (m.texNormal >= 0 && sc.textures[m.texNormal].channels != 3) throw std::runtime_error("normal maps have 3 channels");
+    if (m.texRoughness >= 0 && sc.textures[m.texRoughness].channels != 1)
+      throw std::runtime_error("roughness textures have 1 channel");
+    if (m.texMetallic >= 0 && sc.textures[m.texMetallic].channels != 1)
+      throw std::runtime_error("metallic textures have 1 channel");
     // Rough transmission throws "not implemented" when shaded (BRDF.cpp:386-390);
     // the GPU path rejects such a scene when it is loaded.
-    if (m.transmission > 0.0f && alpha != 0.0f) throw std::runtime_error("not implemented");
+    if (m.transmission > 0.0f && alpha != 0.0f && m.texRoughness < 0) throw std::runtime_error("not implemented");
+    if (m.transmission > 0.0f && m.texRoughness >= 0) {
+      // a roughness texture on a transmissive material: every texel must give
+      // alpha 0, and every face must have uv (roughness -1 without)
+      for (float r : sc.textures[m.texRoughness].px) {
+        float a = r * r;
+        if (a != 0.0f) throw std::runtime_error("not implemented");
+      }
+      for (uint32_t t = 0; t < sc.numTris; ++t)
+        if (sc.triMaterial[t] == (int32_t)i && !sc.triHasUV[t]) throw std::runtime_error("not implemented");
+    }
   }
   // Scene.cpp:508-512 exits the process when NEE needs a light triangle and
   // there is none; reject the scene instead.
@@ -744,6 +791,27 @@ static void uploadScene(HastyScene& S) {
   d.shapeTriOffset = S.upload(sc.shapeTriOffset.data(), sc.shapeTriOffset.size());
   d.triSmooth = S.upload(sc.triSmooth.data(), sc.triSmooth.size());
   d.mats = S.upload(mats.data(), mats.size());
+  d.texturedMedia = 0;
+  bool anyTexture = false;
+  for (const hasty::Material& m : sc.materials) {
+    anyTexture = anyTexture || m.textured();
+    if (m.texAlbedo >= 0) d.texturedMedia = 1;
+  }
+  if (anyTexture) {
+    std::vector<hdev::DevTexture> texs(sc.textures.size());
+    for (size_t t = 0; t < sc.textures.size(); ++t) {
+      const hasty::Texture& tx = sc.textures[t];
+      if (tx.width == 0 || tx.height == 0 || tx.px.size() != (size_t)tx.width * tx.height * tx.channels)
+        throw std::runtime_error("texture " + tx.path + " has no pixels");
+      if ((uint64_t)tx.width * tx.height >= 0xFFFFFFFFull) throw std::runtime_error("texture " + tx.path + " too large");
+      texs[t].px = S.upload(tx.px.data(), tx.px.size());
+      texs[t].w = tx.width;
+      texs[t].h = tx.height;
+    }
+    d.texs = S.upload(texs.data(), texs.size());
+    d.triUV = S.upload(sc.triUV.data(), sc.triUV.size());
+    d.triHasUV = S.upload(sc.triHasUV.data(), sc.triHasUV.size());
+  }
   d.lightTris = S.upload(sc.lightTris.data(), sc.lightTris.size());
   d.lightCdf = S.upload(sc.lightCdf.data(), sc.lightCdf.size());
   d.numLights = (uint32_t)sc.lightTris.size();
@@ -812,6 +880,40 @@ static int fail(const std::exception& e, int code) {
 extern "C" {
 
 const char* hasty_last_error(void) { return g_lastError.c_str(); }
+
+int hasty_image_read(const char* path, uint32_t* width, uint32_t* height, uint32_t* channels, float* pixels,
+                     uint64_t capacity) {
+  if (!path || !width || !height || !channels) {
+    g_lastError = "null argument";
+    return 1;
+  }
+  try {
+    hasty::FloatImage im = hasty::readImage(path);
+    *width = im.width;
+    *height = im.height;
+    *channels = im.channels;
+    if (pixels) {
+      if (capacity < im.px.size()) throw std::runtime_error("pixel buffer too small");
+      memcpy(pixels, im.px.data(), im.px.size() * sizeof(float));
+    }
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e, 2);
+  }
+}
+
+int hasty_image_write_exr(const char* path, uint32_t width, uint32_t height, const float* rgb) {
+  if (!path || (!rgb && (uint64_t)width * height > 0)) {
+    g_lastError = "null argument";
+    return 1;
+  }
+  try {
+    hasty::writeEXR(path, width, height, rgb);
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e, 2);
+  }
+}
 
 int hasty_scene_load(const char* path, HastyScene** out) {
   if (!path || !out) {
@@ -1269,6 +1371,10 @@ int hasty_sppm_create(HastyScene* s, uint32_t seed, const HastySppmParams* param
   try {
     HIPCHECK(hipSetDevice(s->device));
     if (s->dev.width == 0 || s->dev.height == 0) throw std::runtime_error("empty film");
+    // The photon mapper's visible points and photon records carry no texture
+    // coordinates; textured scenes render with the path tracer only.
+    for (const hasty::Material& m : s->job.scene.materials)
+      if (m.textured()) throw std::runtime_error("photon mapping does not support texture maps (material '" + m.name + "')");
     hasty::SppmParams p;
     if (params) {
       p.alpha = params->alpha;

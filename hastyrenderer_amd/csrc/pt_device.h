@@ -141,6 +141,15 @@ struct DevMaterial {
   float roughness, alpha, metallic, specular, ior, anisotropy, transmission;
   float emission[3];
   int varyIOR, isLight, hasDiffuseLobe;
+  // image textures, indices into DevScene::texs (-1 = the constant above):
+  // albedo, roughness, metallic, normal map; `textured` = any of them
+  int32_t tex[4];
+  int textured;
+};
+// Texels of one image texture: 1 or 3 floats each, row-major, row 0 = top.
+struct DevTexture {
+  const float* px;
+  uint32_t w, h;
 };
 
 struct DevScene {
@@ -154,6 +163,10 @@ struct DevScene {
   const uint32_t* shapeTriOffset;  // first global triangle of each shape (primID = tri - offset)
   const uint8_t* triSmooth;
   const DevMaterial* mats;
+  const DevTexture* texs;
+  const float* triUV;        // 6 per triangle (texture scenes only)
+  const uint8_t* triHasUV;   // (texture scenes only)
+  int texturedMedia;         // some material has an albedo texture
   const uint32_t* lightTris;
   const double* lightCdf;
   uint32_t numLights;
@@ -909,16 +922,100 @@ __device__ inline BsdfSample bsdfSample(const DevMaterial& m, hrng::Stream& rng,
   return res;
 }
 
+// ------------------------------------------------------------ textures ----
+__device__ inline int floorInt32(double x) {  // VMath.h:18-22
+  int i = (int)x;
+  return i - (i > x);
+}
+constexpr uint32_t kTexelNoUV = 0xFFFFFFFFu;
+// Texture::evaluateUV with Repeat addressing in u and v (Texture.h:119-140):
+// the nearest texel's index.  Two guards where the reference's result is
+// undefined: a coordinate that is not finite or not below 2^31 in magnitude
+// reads as 0, and a coordinate that rounds to 1 after wrapping reads the last
+// column / row.
+__device__ inline uint32_t texelOf(const DevTexture& t, float u, float v) {
+  if (!(__builtin_fabsf(u) < 2147483648.0f)) u = 0.0f;
+  if (!(__builtin_fabsf(v) < 2147483648.0f)) v = 0.0f;
+  float uu = u - (float)floorInt32(u);
+  float vv = 1.0f - v;
+  float om = vv - (float)floorInt32(vv);
+  int col = (int)(uu * (float)t.w);
+  int row = (int)(om * (float)t.h);
+  col = col < 0 ? 0 : (col > (int)t.w - 1 ? (int)t.w - 1 : col);
+  row = row < 0 ? 0 : (row > (int)t.h - 1 ? (int)t.h - 1 : row);
+  return (uint32_t)row * t.w + (uint32_t)col;
+}
+__device__ inline float alphaOf(float roughness) {  // PrincipledBRDF::computeAlpha, BRDF.cpp:253-263
+  float a = roughness * roughness;
+  if (a < 1e-2f && a != 0.0f) a = 1e-2f;
+  return a;
+}
+// The textured parameters of a hit (Texture.cpp:9-18 evaluated by
+// PrincipledBRDF, BRDF.cpp:604-750): albedo, roughness (with alpha),
+// metallic (with the diffuse-lobe flag) from the texels under the hit's uv =
+// uv0 b0 + uv1 b1 + uv2 b2 (Scene.cpp:243-248), and the normal map's normal
+// normalize(t x + b y + ng z) with (x, y, z) = 2 texel - 1 in place of the
+// default shading normal (getShadingNormal then applies its side test).  A
+// triangle without uv reads the NoUV values (1, 0, 0.7) and -1
+// (Texture.h:84-101).  *albedoTexel = the albedo texel (kTexelNoUV without
+// uv), which the media stack keeps for the entered surface.
+__device__ inline void applyTextures(const DevScene& s, DevMaterial& m, Interaction& it, float b0, float b1,
+                                     float b2, uint32_t* albedoTexel) {
+  const bool hasUV = ldg(s.triHasUV + it.tri) != 0;
+  float u = 0.0f, v = 0.0f;
+  if (hasUV) {
+    const float* q = s.triUV + (size_t)it.tri * 6;
+    u = ldg(q) * b0 + ldg(q + 2) * b1 + ldg(q + 4) * b2;
+    v = ldg(q + 1) * b0 + ldg(q + 3) * b1 + ldg(q + 5) * b2;
+  }
+  if (m.tex[0] >= 0) {
+    const DevTexture t = ldgStruct(s.texs + m.tex[0]);
+    const uint32_t k = hasUV ? texelOf(t, u, v) : kTexelNoUV;
+    const V3 a = hasUV ? load3g(t.px + (size_t)k * 3) : v3(1.0f, 0.0f, 0.7f);
+    m.albedo[0] = a.x, m.albedo[1] = a.y, m.albedo[2] = a.z;
+    *albedoTexel = k;
+  }
+  if (m.tex[1] >= 0) {
+    const DevTexture t = ldgStruct(s.texs + m.tex[1]);
+    m.roughness = hasUV ? ldg(t.px + texelOf(t, u, v)) : -1.0f;
+    m.alpha = alphaOf(m.roughness);
+  }
+  if (m.tex[2] >= 0) {
+    const DevTexture t = ldgStruct(s.texs + m.tex[2]);
+    m.metallic = hasUV ? ldg(t.px + texelOf(t, u, v)) : -1.0f;
+    m.hasDiffuseLobe = ((1.0f - 0.5f * m.specular) * (1.0 - (double)m.transmission) * (1.0 - (double)m.metallic) > 0.0)
+                           ? 1 : 0;
+  }
+  if (m.tex[3] >= 0) {
+    const DevTexture t = ldgStruct(s.texs + m.tex[3]);
+    const V3 c = hasUV ? load3g(t.px + (size_t)texelOf(t, u, v) * 3) : v3(1.0f, 0.0f, 0.7f);
+    const V3 tn = v3(2.0f * c.x - 1.0f, 2.0f * c.y - 1.0f, 2.0f * c.z - 1.0f);
+    it.ns = normalize(it.tangent * tn.x + it.bitangent * tn.y + it.ng * tn.z);
+  }
+}
+
 // ---------------------------------------------------------- media stack ----
 // LightRayInfo (LightRayInfo.cpp:8-125), entries = geomID << 16 | material.
+// An entry's transmittance is the albedo of the surface where the path
+// entered it (LightRayInfo.cpp:14).  For a material with an albedo texture
+// that is a texel: scenes with such materials keep each entry's texel index
+// in `txs` (kMaxMedia per path slot in HBM), and tx0 holds the first entry's.
 struct Media {
   uint32_t e[kMaxMedia];
   int n;
   bool overflow;
+  uint32_t tx0;
 };
+__device__ inline V3 mediaEntryAlbedo(const DevScene& s, uint32_t e, uint32_t texel) {
+  const DevMaterial* m = s.mats + (e & 0xFFFFu);
+  const int32_t ta = ldg(&m->tex[0]);
+  if (ta < 0) return albedoOf(*m);
+  if (texel == kTexelNoUV) return v3(1.0f, 0.0f, 0.7f);
+  return load3g(ldg(&s.texs[ta].px) + (size_t)texel * 3);
+}
 __device__ inline V3 mediaTransmittance(const DevScene& s, const Media& md) {
   if (md.n == 0) return v3(1.0f, 1.0f, 1.0f);
-  return albedoOf(s.mats[md.e[0] & 0xFFFFu]);
+  return mediaEntryAlbedo(s, md.e[0], md.tx0);
 }
 // Index of the first entry of shape geomID, or -1.
 __device__ inline int mediaFind(const Media& md, uint32_t geomID) {
@@ -928,8 +1025,10 @@ __device__ inline int mediaFind(const Media& md, uint32_t geomID) {
     if (idx < 0 && i < md.n && (md.e[i] >> 16) == geomID) idx = i;
   return idx;
 }
-// updateMedia; `wavelength` is the path's wavelength at the time of the call.
-__device__ inline void mediaUpdate(Media& md, const Interaction& it, V3 wFrom, V3 wTo) {
+// updateMedia; `texel` is the hit's albedo texel and `txs` the path's texel
+// slots (nullptr when no material has an albedo texture).
+__device__ inline void mediaUpdate(Media& md, const Interaction& it, V3 wFrom, V3 wTo, uint32_t texel,
+                                   uint32_t* txs) {
   float ndFrom = dot(it.ng, wFrom);
   float ndTo = dot(it.ng, wTo);
   // (entries are addressed with constant indices only, so the stack stays
@@ -940,6 +1039,8 @@ __device__ inline void mediaUpdate(Media& md, const Interaction& it, V3 wFrom, V
 #pragma unroll
       for (int i = 0; i < kMaxMedia; ++i)
         if (i == md.n) md.e[i] = v;
+      if (txs) txs[md.n] = texel;
+      if (md.n == 0) md.tx0 = texel;
       md.n += 1;
     } else {
       md.overflow = true;
@@ -950,9 +1051,29 @@ __device__ inline void mediaUpdate(Media& md, const Interaction& it, V3 wFrom, V
 #pragma unroll
       for (int j = 0; j < kMaxMedia - 1; ++j)
         if (j >= idx && j < md.n - 1) md.e[j] = md.e[j + 1];
+      if (txs) {
+        for (int j = idx; j < md.n - 1; ++j) txs[j] = txs[j + 1];
+        if (idx == 0 && md.n > 1) md.tx0 = txs[0];
+      }
       md.n -= 1;
     }
   }
+}
+// The transmittance the stack would have after mediaUpdate(md, it, wFrom,
+// wTo, texel, ...), without changing it (the light ray's copy of the state).
+__device__ inline V3 mediaTransmittanceAfter(const DevScene& s, const Media& md, const Interaction& it, V3 wFrom,
+                                             V3 wTo, uint32_t texel, const uint32_t* txs) {
+  float ndFrom = dot(it.ng, wFrom);
+  float ndTo = dot(it.ng, wTo);
+  if (ndFrom >= 0.0f && ndTo < 0.0f) {
+    if (md.n == 0) return mediaEntryAlbedo(s, (it.geomID << 16) | (uint32_t)it.mat, texel);
+  } else if (ndFrom < 0.0f && ndTo >= 0.0f) {
+    if (mediaFind(md, it.geomID) == 0) {
+      if (md.n == 1) return v3(1.0f, 1.0f, 1.0f);
+      return mediaEntryAlbedo(s, md.e[1], txs ? txs[1] : 0u);
+    }
+  }
+  return mediaTransmittance(s, md);
 }
 __device__ inline float mediaOutsideIOR(const DevScene& s, const Media& md, const Interaction& it, V3 wFrom,
                                         int wavelength) {
@@ -978,10 +1099,6 @@ __device__ inline void sphericalFromDir(V3 d, float* theta, float* phi) {  // VM
   *theta = hm::acosf(d.y);
   float p = hm::atan2f(d.z, d.x);
   *phi = p < 0.0f ? p + (float)(2.0 * kPi) : p;
-}
-__device__ inline int floorInt32(double x) {  // VMath.h:18-22
-  int i = (int)x;
-  return i - (i > x);
 }
 // Texture::evaluateUV with U repeat / V clamp (Texture.h:119-140)
 __device__ inline V3 envLookup(const DevScene& s, float u, float v) {

@@ -23,6 +23,7 @@ struct PathBuffers {
   float4* L;      // accumulated radiance
   uint4* state;   // x rng draws, y flags, z previous triangle, w path id
   uint4* media;   // 2 x uint4 per slot (8 entries)
+  uint32_t* mediaTx;  // 8 per slot: albedo texel of each media entry
   float4* auxN;   // depth-0 shading normal
   float4* auxA;   // depth-0 albedo
   float4* shO;    // shadow origin; w = BSDF pdf of the light direction

@@ -94,9 +94,11 @@ __device__ inline void waveAddStat(unsigned long long* c, uint32_t v) {
 
 // Entries beyond md.n are never read, so an empty stack is neither loaded
 // nor stored.
-__device__ inline void loadMedia(const PathBuffers& pb, uint32_t slot, uint32_t flags, Media& md) {
+__device__ inline void loadMedia(const PathBuffers& pb, uint32_t slot, uint32_t flags, const uint32_t* txs,
+                                 Media& md) {
   md.n = (int)((flags >> 12) & 0xFu);
   md.overflow = (flags >> 16) & 1u;
+  md.tx0 = 0u;
   if (md.n == 0) {
 #pragma unroll
     for (int i = 0; i < 8; ++i) md.e[i] = 0u;
@@ -105,6 +107,7 @@ __device__ inline void loadMedia(const PathBuffers& pb, uint32_t slot, uint32_t 
   uint4 a = pb.media[(size_t)slot * 2], b = pb.media[(size_t)slot * 2 + 1];
   md.e[0] = a.x, md.e[1] = a.y, md.e[2] = a.z, md.e[3] = a.w;
   md.e[4] = b.x, md.e[5] = b.y, md.e[6] = b.z, md.e[7] = b.w;
+  if (txs) md.tx0 = txs[0];
 }
 __device__ inline void storeMedia(const PathBuffers& pb, uint32_t slot, const Media& md) {
   pb.media[(size_t)slot * 2] = make_uint4(md.e[0], md.e[1], md.e[2], md.e[3]);
@@ -424,6 +427,12 @@ __global__ void __launch_bounds__(kBlock) HASTY_EXTEND_ATTR kExtend(JobSet js, P
 constexpr uint32_t kRecShift = 20;
 // Materials guided by the learned proposal: GGX alpha at least this.
 constexpr float kGuideMinAlpha = 0.25f;
+// Vertices that sample the guide: rough, non-emitting reflectors.  A
+// roughness texture makes the material eligible whatever its texels (the
+// one-sample mixture stays unbiased at glossy texels).
+__device__ inline bool guideEligible(const DevMaterial& m) {
+  return !m.isLight && m.transmission == 0.0f && (m.tex[1] >= 0 || m.alpha >= kGuideMinAlpha);
+}
 
 template <bool MULTI, bool GUIDED>
 __global__ void __launch_bounds__(kBlock) HASTY_SHADE_ATTR kShade(JobSet js, PathBuffers pb) {
@@ -442,8 +451,10 @@ __global__ void __launch_bounds__(kBlock) HASTY_SHADE_ATTR kShade(JobSet js, Pat
       const bool twoStrategies = s.hasLight && s.useMIS;
       int depth = (int)(st.y & 0xFFu);
       int wavelength = (int)((st.y >> 8) & 0x3u) - 1;
+      // per-entry albedo texels of the media stack (albedo-textured scenes)
+      uint32_t* txs = s.texturedMedia ? pb.mediaTx + (size_t)slot * kMaxMedia : nullptr;
       Media md;
-      loadMedia(pb, slot, st.y, md);
+      loadMedia(pb, slot, st.y, txs, md);
       int32_t prevTri = (int32_t)st.z;
       float4 ro4 = pb.rayO[slot], rd4 = pb.rayD[slot], h4 = pb.hit[slot];
       V3 ro = f4xyz(ro4), rd = f4xyz(rd4);
@@ -503,11 +514,15 @@ __global__ void __launch_bounds__(kBlock) HASTY_SHADE_ATTR kShade(JobSet js, Pat
           alive = false;
         } else {
           Interaction it = makeInteraction(s, tri, h4.y, h4.z);
-          const DevMaterial m = ldgStruct(s.mats + it.mat);
-          if (depth == 0) {
-            pb.auxN[slot] = f4(it.ns, 0.0f);
-            pb.auxA[slot] = make_float4(m.albedo[0], m.albedo[1], m.albedo[2], 0.0f);
-          }
+          DevMaterial m = ldgStruct(s.mats + it.mat);
+          // guided vertex: rough reflection (no delta lobe, and not so glossy
+          // that broad vMF lobes could only waste samples); decided on the
+          // material, before its textures, as kGuideInfer decides it
+          const bool guided = GUIDED && guideEligible(m);
+          if (depth == 0) pb.auxN[slot] = f4(it.ns, 0.0f);  // the default shading normal
+          uint32_t texel = 0u;
+          if (m.textured) applyTextures(s, m, it, 1.0f - h4.y - h4.z, h4.y, h4.z, &texel);
+          if (depth == 0) pb.auxA[slot] = make_float4(m.albedo[0], m.albedo[1], m.albedo[2], 0.0f);
           V3 Tb = beersLaw(mediaTransmittance(s, md), norm(it.x - ro));
           V3 wo = -rd;
           if (m.isLight) {
@@ -524,9 +539,6 @@ __global__ void __launch_bounds__(kBlock) HASTY_SHADE_ATTR kShade(JobSet js, Pat
               if (depth + 1 >= s.maxDepth) {
                 alive = false;  // every deeper estimate is zero
               } else {
-                // guided vertex: rough reflection (no delta lobe, and not so
-                // glossy that broad vMF lobes could only waste samples)
-                const bool guided = GUIDED && m.alpha >= kGuideMinAlpha && m.transmission == 0.0f;
                 // the vertex's mixture table, written by kGuideInfer
                 const float4* gm = GUIDED ? js.g.raw + (size_t)slot * 10 : nullptr;
                 const float gf = GUIDED ? js.g.fraction : 0.0f;
@@ -559,9 +571,7 @@ __global__ void __launch_bounds__(kBlock) HASTY_SHADE_ATTR kShade(JobSet js, Pat
                   float outsideIOR = mediaOutsideIOR(s, md, it, wo, wavelength);
                   float pB = bsdfPdf(m, it, wo, sd, outsideIOR);
                   if (GUIDED && guided) pB = gf * guide::mixturePdfTable(gm, sd) + (1.0f - gf) * pB;
-                  Media mc = md;
-                  mediaUpdate(mc, it, wo, sd);
-                  V3 Tc = mediaTransmittance(s, mc);
+                  V3 Tc = mediaTransmittanceAfter(s, md, it, wo, sd, texel, txs);
                   pb.shO[slot] = f4(so, pB);
                   pb.shD[slot] = f4(sd, __int_as_float(target));
                   pb.shFD[slot] = f4(ev.fD, __builtin_fabsf(dot(sd, it.ng)));
@@ -615,7 +625,7 @@ __global__ void __launch_bounds__(kBlock) HASTY_SHADE_ATTR kShade(JobSet js, Pat
                   V3 t2 = mul(thr, bs.tD + bs.tC);
                   if (!isZero(t2)) {
                     V3 no = offsetOrigin(it, bs.outside);
-                    mediaUpdate(md, it, wo, bs.dir);
+                    mediaUpdate(md, it, wo, bs.dir, texel, txs);
                     pb.rayO[slot] = f4(no, 0.0f);
                     pb.rayD[slot] = f4(bs.dir, bs.pdf);
                     pb.beta[slot] = f4(mul(beta, t2), 0.0f);
@@ -853,7 +863,7 @@ __global__ void __launch_bounds__(kGuideInferThreads) HASTY_WAVES_ATTR_(HASTY_GU
       if (tri >= 0) {
         // only hits on guided materials read the outputs
         const DevMaterial m = ldgStruct(s.mats + ldg(s.triMat + tri));
-        want = !m.isLight && m.alpha >= kGuideMinAlpha && m.transmission == 0.0f;
+        want = guideEligible(m);
         if (want) {
           hitPointNormal(s, tri, h4.y, h4.z, &x, &ng);
           wo = -f4xyz(pb.rayD[slot]);

@@ -5,11 +5,14 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <algorithm>
 #include <cmath>
+#include <limits>
 #include <fstream>
 #include <sstream>
 
 #include "hmath.h"
+#include "image_io.h"
 
 namespace hasty {
 
@@ -357,6 +360,45 @@ ImageRGB readPFM(const std::string& path) {
   return img;
 }
 
+ImageRGB readImageRGB(const std::string& path) {
+  FloatImage im = readImage(path);
+  if (im.channels != 3 && im.channels != 4)
+    throw std::runtime_error("could not read " + path + ", wrong number of channels, expected 3 or 4, found " +
+                             std::to_string(im.channels));
+  ImageRGB out;
+  out.width = im.width;
+  out.height = im.height;
+  out.rgb.resize((size_t)im.width * im.height * 3);
+  for (size_t i = 0; i < (size_t)im.width * im.height; ++i)
+    for (int c = 0; c < 3; ++c) out.rgb[3 * i + c] = im.px[i * im.channels + c];
+  return out;
+}
+
+Texture loadTexture(const std::string& path, uint32_t channels, bool srgb) {
+  Texture t;
+  t.path = path;
+  t.channels = channels;
+  if (channels == 3) {
+    ImageRGB im = readImageRGB(path);
+    t.width = im.width;
+    t.height = im.height;
+    t.px = std::move(im.rgb);
+  } else if (channels == 1) {
+    FloatImage im = readImage(path);
+    if (im.channels != 1)
+      throw std::runtime_error("could not read " + path + ", wrong number of channels, expected 1, found " +
+                               std::to_string(im.channels));
+    t.width = im.width;
+    t.height = im.height;
+    t.px = std::move(im.px);
+  } else {
+    throw std::runtime_error("textures have 1 or 3 channels");
+  }
+  if (srgb)  // clampAndPowInplace(image, 0, FLT_MAX, 2.2) (Image.h:320-326)
+    for (float& v : t.px) v = std::pow(std::min(std::max(v, 0.0f), std::numeric_limits<float>::max()), 2.2f);
+  return t;
+}
+
 void writePFM(const std::string& path, uint32_t width, uint32_t height, const float* rgb) {
   FILE* f = fopen(path.c_str(), "wb");
   if (!f) throw std::runtime_error("could not open file " + path + "for writing");
@@ -408,6 +450,31 @@ std::string restOfLine(const std::string& line, const std::string& key) {
   return line.substr(pos, end - pos);
 }
 
+// The file name of a texture statement: tinyobj's ParseTextureNameAndOption
+// skips the option flags (-bm 1.0, -o u v w, -clamp on, ...) and takes the
+// rest of the line as the name.
+std::string textureName(const std::string& line, const std::string& key) {
+  static const std::map<std::string, int> optionArgs = {
+      {"-blendu", 1}, {"-blendv", 1}, {"-clamp", 1}, {"-boost", 1}, {"-bm", 1}, {"-mm", 2},
+      {"-o", 3},      {"-s", 3},      {"-t", 3},     {"-texres", 1}, {"-imfchan", 1}, {"-type", 1},
+      {"-colorspace", 1}};
+  std::string rest = restOfLine(line, key);
+  for (;;) {
+    auto tok = splitWs(rest);
+    if (tok.empty()) return std::string();
+    auto it = optionArgs.find(tok[0]);
+    if (it == optionArgs.end()) return rest;
+    size_t skip = 1 + (size_t)it->second;
+    size_t pos = 0;
+    for (size_t k = 0; k < skip && pos < rest.size(); ++k) {
+      while (pos < rest.size() && (rest[pos] == ' ' || rest[pos] == '\t')) ++pos;
+      while (pos < rest.size() && rest[pos] != ' ' && rest[pos] != '\t') ++pos;
+    }
+    while (pos < rest.size() && (rest[pos] == ' ' || rest[pos] == '\t')) ++pos;
+    rest = rest.substr(pos);
+  }
+}
+
 void loadMtl(const std::string& path, std::vector<RawMaterial>& mats, std::map<std::string, int>& byName) {
   std::ifstream ifs(path);
   if (!ifs.is_open()) throw std::runtime_error("TinyObjReader error could not open material file " + path);
@@ -435,10 +502,10 @@ void loadMtl(const std::string& path, std::vector<RawMaterial>& mats, std::map<s
     else if (k == "d" && tok.size() > 1) cur->dissolve = parseFloat(tok[1]);
     else if (k == "Tr" && tok.size() > 1) cur->dissolve = 1.0f - parseFloat(tok[1]);
     else if (k == "aniso" && tok.size() > 1) cur->anisotropy = parseFloat(tok[1]);
-    else if (k == "map_Kd") cur->diffuseTex = restOfLine(line, "map_Kd");
-    else if (k == "map_Ns") cur->specularHighlightTex = restOfLine(line, "map_Ns");
-    else if (k == "map_Pm") cur->metallicTex = restOfLine(line, "map_Pm");
-    else if (k == "bump" || k == "map_bump" || k == "map_Bump") cur->bumpTex = restOfLine(line, k);
+    else if (k == "map_Kd") cur->diffuseTex = textureName(line, "map_Kd");
+    else if (k == "map_Ns") cur->specularHighlightTex = textureName(line, "map_Ns");
+    else if (k == "map_Pm") cur->metallicTex = textureName(line, "map_Pm");
+    else if (k == "bump" || k == "map_bump" || k == "map_Bump") cur->bumpTex = textureName(line, k);
   }
 }
 
@@ -676,7 +743,7 @@ void loadScene(const std::string& sceneJsonPath, Scene& sc) {
     sc.hasEnvTexture = false;
     vec3FromJson(world.at("background_color"), sc.backgroundColor);
   } else {
-    setEnvironment(sc, readPFM(joinPath(folder, world.at("environment_texture").asString())));
+    setEnvironment(sc, readImageRGB(joinPath(folder, world.at("environment_texture").asString())));
   }
   sc.varyGlassIOR = world.contains("vary_glass_ior") ? world.at("vary_glass_ior").asBool() : false;
 
@@ -685,12 +752,25 @@ void loadScene(const std::string& sceneJsonPath, Scene& sc) {
   std::vector<RawMaterial> raw;
   parseObj(meshPath, folder, sc.vertices, sc.normals, sc.texcoords, shapes, raw);
 
-  // Scene.cpp:65-118
+  // Scene.cpp:65-118.  Texture names are relative to the scene folder; a
+  // file used by several materials in the same role is loaded once.
+  std::map<std::string, int> loaded;
+  auto texture = [&](const std::string& name, uint32_t channels, bool srgb) -> int {
+    if (name.empty()) return -1;
+    const std::string path = joinPath(folder, name);
+    const std::string key = path + (channels == 1 ? "|1" : srgb ? "|srgb" : "|3");
+    auto it = loaded.find(key);
+    if (it != loaded.end()) return it->second;
+    sc.textures.push_back(loadTexture(path, channels, srgb));
+    return loaded[key] = (int)sc.textures.size() - 1;
+  };
   std::vector<uint8_t> tinted;
   for (const RawMaterial& m : raw) {
-    if (!m.bumpTex.empty() || !m.diffuseTex.empty() || !m.specularHighlightTex.empty() || !m.metallicTex.empty())
-      throw std::runtime_error("material '" + m.name + "': texture maps are not supported by this renderer");
     Material out;
+    out.texNormal = texture(m.bumpTex, 3, false);
+    out.texAlbedo = texture(m.diffuseTex, 3, true);
+    out.texRoughness = texture(m.specularHighlightTex, 1, false);
+    out.texMetallic = texture(m.metallicTex, 1, false);
     out.name = m.name;
     for (int i = 0; i < 3; ++i) out.albedo[i] = m.diffuse[i];
     out.transmission = 1.0f - m.dissolve;

@@ -1,5 +1,5 @@
-// Host-side scene loading: renderJob.json -> scene JSON -> .obj/.mtl (+ PFM
-// environment map), with the reference's key names, defaults and error
+// Host-side scene loading: renderJob.json -> scene JSON -> .obj/.mtl (+
+// environment map and image textures, read by image_io.h), with the reference's key names, defaults and error
 // behaviour.  Pure C++17 with no third-party dependency, so the same loader
 // runs on every box.  Both the HIP library and the CPU oracle consume the
 // resulting `Job`.
@@ -10,6 +10,7 @@
 //   Scene::Scene ..................... core/src/Scene.cpp:12-195
 //   tinyobj collectTriangle* ......... core/src/tinyobj.cpp:7-88
 //   EnvironmentTexture::load ......... core/include/Hasty/Environment.h:105-118
+//   Texture3f/1f::loadTexture ........ core/src/Texture.cpp:21-49, Image.cpp:74-138
 //   ImageDistribution::precompute .... core/include/Hasty/Environment.h:52-58
 #pragma once
 #include <stdint.h>
@@ -87,8 +88,26 @@ struct Material {
   float transmission = 0.0f;    // 1 - d
   bool varyIOR = false;         // world.vary_glass_ior && transmission > 0
   float emission[3] = {0, 0, 0};  // Ke
+  // Image textures (indices into Scene::textures, -1 = the constant above):
+  // map_Kd (sRGB, Texture3f), map_Ns (roughness, Texture1f), map_Pm
+  // (metallic, Texture1f) and bump (normal map, Texture3f), Scene.cpp:73-114.
+  int texAlbedo = -1, texRoughness = -1, texMetallic = -1, texNormal = -1;
   bool isLight() const { return emission[0] > 0.0f || emission[1] > 0.0f || emission[2] > 0.0f; }
+  bool textured() const { return texAlbedo >= 0 || texRoughness >= 0 || texMetallic >= 0 || texNormal >= 0; }
 };
+
+// An image texture after TextureTraits::loadTexture (Texture.cpp:21-49):
+// linear float texels, row 0 = top, 1 channel (Texture1f) or 3 (Texture3f;
+// the alpha of a 4-channel file is dropped as readImage3f drops it).
+struct Texture {
+  std::string path;
+  uint32_t width = 0, height = 0, channels = 0;
+  std::vector<float> px;
+};
+// Reads an image file as a texture of `channels` (1 or 3) channels, with the
+// reference's channel-count errors (Image.cpp:74-138); srgb applies
+// pow(clamp(p, 0, FLT_MAX), 2.2) per channel (Texture.cpp:28, 43).
+Texture loadTexture(const std::string& path, uint32_t channels, bool srgb);
 
 // --------------------------------------------------------------- image ----
 struct ImageRGB {
@@ -96,6 +115,9 @@ struct ImageRGB {
   std::vector<float> rgb;  // row-major, row 0 = top (OIIO in-memory order)
 };
 ImageRGB readPFM(const std::string& path);
+// Any image file readImage understands (PFM, PPM/PGM, PNG, EXR) with 3 or 4
+// channels, as readImage3f requires (Image.cpp:104-138); alpha is dropped.
+ImageRGB readImageRGB(const std::string& path);
 // Writes a colour PFM following the format's bottom-to-top row order.
 void writePFM(const std::string& path, uint32_t width, uint32_t height, const float* rgbTopRowFirst);
 
@@ -115,6 +137,7 @@ struct Scene {
   // tinyobj-equivalent attribute arrays and shapes.
   std::vector<float> vertices, normals, texcoords;
   std::vector<Material> materials;
+  std::vector<Texture> textures;
   std::vector<std::string> shapeNames;
   std::vector<uint32_t> shapeTriOffset;  // global triangle index = offset[geomID] + primID
 

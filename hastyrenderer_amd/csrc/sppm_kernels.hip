@@ -251,6 +251,7 @@ __global__ void __launch_bounds__(kB) kInitRadii(DevScene s, float footprintScal
             Media fresh;
             fresh.n = 0;
             fresh.overflow = false;
+            fresh.tx0 = 0u;
             float outsideIOR = mediaOutsideIOR(s, fresh, it, wo, wavelength);
             bs = bsdfSample(m, rng, it, wo, wavelength, outsideIOR, false, 2);
             bs.tD = bs.tD / sp;
@@ -316,6 +317,7 @@ __global__ void __launch_bounds__(kB) kTracePhotons(DevScene s, SppmDev g, uint3
         Media md;
         md.n = 0;
         md.overflow = false;
+        md.tx0 = 0u;
         int wavelength = -1;
         Interaction it = makeInteraction(s, h.tri, h.u, h.v);
         // empty media: transmittance is exactly one
@@ -346,7 +348,7 @@ __global__ void __launch_bounds__(kB) kTracePhotons(DevScene s, SppmDev g, uint3
           V3 o2 = offsetOrigin(it, bs.outside);
           Hit h2 = traceClosest<kStackLds, kStackOvf>(s, o2, bs.dir, stack + threadIdx.x, kB, ovf, nullptr, nullptr);
           if (h2.tri < 0 || h2.tri == it.tri) break;
-          mediaUpdate(md, it, wo, bs.dir);
+          mediaUpdate(md, it, wo, bs.dir, 0u, nullptr);  // photon mapping scenes have no textures
           Interaction it2 = makeInteraction(s, h2.tri, h2.u, h2.v);
           V3 t2 = beersLaw(mediaTransmittance(s, md), norm(it2.x - o2));
           thrThis = mul(thrThis, t2);
@@ -728,6 +730,7 @@ __global__ void __launch_bounds__(kB) HASTY_SPPM_CAMERA_ATTR kSppmCamera(DevScen
     Media md;
     md.n = 0;
     md.overflow = false;
+    md.tx0 = 0u;
     int wavelength = -1;
     Hit h = traceClosest<kStackLds, kStackOvf>(s, ro, rd, stack + threadIdx.x, kB, ovf, nullptr, nullptr);
     for (int depth = 0; depth < s.maxDepth; ++depth) {
@@ -818,10 +821,8 @@ __global__ void __launch_bounds__(kB) HASTY_SPPM_CAMERA_ATTR kSppmCamera(DevScen
               sum += powci<2>(pL);
               sum += powci<2>(pB);
               float w = powci<2>(pL) / sum;
-              Media mc = md;
-              mediaUpdate(mc, it, wo, sd);
               V3 tD = mul(thr, sD) * w, tC = mul(thr, sC) * w;
-              V3 Tc = mediaTransmittance(s, mc);
+              V3 Tc = mediaTransmittanceAfter(s, md, it, wo, sd, 0u, nullptr);
               if (hl.tri >= 0) {
                 const DevMaterial& lm = s.mats[s.triMat[hl.tri]];
                 V3 Le = dot(hn, -sd) <= 0.0f ? v3(0.0f, 0.0f, 0.0f)
@@ -868,7 +869,7 @@ __global__ void __launch_bounds__(kB) HASTY_SPPM_CAMERA_ATTR kSppmCamera(DevScen
         sum += powci<2>(pB);
         w = powci<2>(pB) / sum;
       }
-      mediaUpdate(md, it, wo, bs.dir);
+      mediaUpdate(md, it, wo, bs.dir, 0u, nullptr);  // photon mapping scenes have no textures
       wavelength = wl;
       V3 tD = mul(thr, bs.tD) * w, tC = mul(thr, bs.tC) * w;
       WLR = mul(WR, tD);

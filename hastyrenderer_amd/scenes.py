@@ -23,6 +23,9 @@ import json
 import math
 import os
 
+import struct
+import zlib
+
 import numpy as np
 
 F32 = np.float32
@@ -105,6 +108,118 @@ def read_pfm(path) -> np.ndarray:
     return np.ascontiguousarray(arr.astype(np.float32))
 
 
+def write_png(path, img, bits=8, palette=None):
+    """PNG writer (zlib, filter type 0 on every row): `img` is an integer
+    array (H, W) for grey, (H, W, 2) grey + alpha, (H, W, 3) RGB or
+    (H, W, 4) RGBA with samples < 2**bits (bits 8 or 16); with `palette`
+    ((N, 3) uint8), `img` (H, W) holds palette indices (8 bits)."""
+    img = np.asarray(img)
+    h, w = img.shape[:2]
+    ch = 1 if img.ndim == 2 else img.shape[2]
+    if palette is not None:
+        ctype, bits = 3, 8
+    else:
+        ctype = {1: 0, 2: 4, 3: 2, 4: 6}[ch]
+    dt = ">u2" if bits == 16 else "u1"
+    raw = np.ascontiguousarray(img.reshape(h, w * ch).astype(dt))
+    rows = b"".join(b"\x00" + raw[y].tobytes() for y in range(h))
+
+    def chunk(kind, data):
+        return struct.pack(">I", len(data)) + kind + data + struct.pack(">I", zlib.crc32(kind + data) & 0xFFFFFFFF)
+
+    out = b"\x89PNG\r\n\x1a\n" + chunk(b"IHDR", struct.pack(">IIBBBBB", w, h, bits, ctype, 0, 0, 0))
+    if palette is not None:
+        out += chunk(b"PLTE", np.asarray(palette, np.uint8).tobytes())
+    out += chunk(b"IDAT", zlib.compress(rows, 6)) + chunk(b"IEND", b"")
+    with open(path, "wb") as f:
+        f.write(out)
+
+
+def write_pnm(path, img, maxval=255, ascii=False):
+    """Binary (P5 / P6) or ASCII (P2 / P3) PGM / PPM of an integer image
+    (H, W) or (H, W, 3); 16-bit samples big-endian when maxval > 255."""
+    img = np.asarray(img)
+    h, w = img.shape[:2]
+    rgb = img.ndim == 3
+    magic = (b"P3" if rgb else b"P2") if ascii else (b"P6" if rgb else b"P5")
+    head = b"%s\n# written by hastyrenderer_amd.scenes\n%d %d\n%d\n" % (magic, w, h, maxval)
+    if ascii:
+        body = " ".join(str(int(v)) for v in img.ravel()).encode() + b"\n"
+    else:
+        body = np.ascontiguousarray(img.astype(">u2" if maxval > 255 else "u1")).tobytes()
+    with open(path, "wb") as f:
+        f.write(head + body)
+
+
+def _exr_predict(data: bytes) -> bytes:
+    """The RLE / ZIP pre-pass of OpenEXR: even bytes then odd bytes, then each
+    byte replaced by its difference to the previous one, plus 128."""
+    b = np.frombuffer(data, np.uint8)
+    t = np.concatenate([b[0::2], b[1::2]]).astype(np.int32)
+    d = t.copy()
+    d[1:] = (t[1:] - t[:-1] + 128 + 256) & 0xFF
+    return d.astype(np.uint8).tobytes()
+
+
+def _exr_rle(data: bytes) -> bytes:
+    """OpenEXR RLE coding: runs of 3 to 127 equal bytes as (n - 1, byte),
+    other stretches of up to 127 bytes as (-n, bytes)."""
+    out, i, n = bytearray(), 0, len(data)
+    while i < n:
+        j = i
+        while j < n and j - i < 127 and data[j] == data[i]:
+            j += 1
+        if j - i >= 3:
+            out += struct.pack("b", j - i - 1) + data[i:i + 1]
+            i = j
+            continue
+        j = i
+        while j < n and j - i < 127 and not (j + 2 < n and data[j] == data[j + 1] == data[j + 2]):
+            j += 1
+        out += struct.pack("b", -(j - i)) + data[i:j]
+        i = j
+    return bytes(out)
+
+
+def write_exr(path, rgb, compression="zip", pixel_type="half"):
+    """Single-part scanline OpenEXR file of an (H, W, 3) image, channels B, G,
+    R stored as HALF or FLOAT, compression "none", "rle", "zips" (1 line per
+    chunk) or "zip" (16 lines per chunk)."""
+    rgb = np.asarray(rgb, np.float32)
+    h, w, _ = rgb.shape
+    ptype, dt = (1, "<f2") if pixel_type == "half" else (2, "<f4")
+    comp, lines = {"none": (0, 1), "rle": (1, 1), "zips": (2, 1), "zip": (3, 16)}[compression]
+
+    def attr(name, kind, data):
+        return name.encode() + b"\0" + kind.encode() + b"\0" + struct.pack("<I", len(data)) + data
+
+    chl = b"".join(c + b"\0" + struct.pack("<iB3xii", ptype, 0, 1, 1) for c in (b"B", b"G", b"R")) + b"\0"
+    box = struct.pack("<iiii", 0, 0, w - 1, h - 1)
+    head = struct.pack("<II", 20000630, 2)
+    head += attr("channels", "chlist", chl) + attr("compression", "compression", bytes([comp]))
+    head += attr("dataWindow", "box2i", box) + attr("displayWindow", "box2i", box)
+    head += attr("lineOrder", "lineOrder", b"\0") + attr("pixelAspectRatio", "float", struct.pack("<f", 1.0))
+    head += attr("screenWindowCenter", "v2f", struct.pack("<ff", 0, 0))
+    head += attr("screenWindowWidth", "float", struct.pack("<f", 1.0)) + b"\0"
+    chunks = []
+    for y0 in range(0, h, lines):
+        block = b"".join(np.ascontiguousarray(rgb[y, :, c].astype(dt)).tobytes()
+                         for y in range(y0, min(h, y0 + lines)) for c in (2, 1, 0))
+        if comp:
+            pre = _exr_predict(block)
+            packed = _exr_rle(pre) if comp == 1 else zlib.compress(pre, 6)
+            if len(packed) < len(block):
+                block = packed
+        chunks.append(struct.pack("<iI", y0, len(block)) + block)
+    offset = len(head) + 8 * len(chunks)
+    table = b""
+    for c in chunks:
+        table += struct.pack("<Q", offset)
+        offset += len(c)
+    with open(path, "wb") as f:
+        f.write(head + table + b"".join(chunks))
+
+
 class _Obj:
     """An OBJ file under construction.  Records the statements in order and
     renders them either as OBJ text (write) or as the arrays the loader would
@@ -114,6 +229,7 @@ class _Obj:
         self.ops = []
         self.nv = 0
         self.nn = 0
+        self.nt = 0
 
     def shape(self, name):
         self.ops.append(("o", name))
@@ -138,9 +254,18 @@ class _Obj:
         self.nn += len(n)
         return base
 
-    def faces(self, tris, vbase, nbase=None):
+    def texcoords(self, uv):
+        base = self.nt
+        uv = np.asarray(uv, dtype=F32).reshape(-1, 2)
+        self.ops.append(("vt", uv))
+        self.nt += len(uv)
+        return base
+
+    def faces(self, tris, vbase, nbase=None, tbase=None):
+        """Faces indexing vertices (and normals / texcoords at the same
+        offsets within their blocks)."""
         t = np.asarray(tris, dtype=np.int64).reshape(-1, 3)
-        self.ops.append(("f", t + vbase, None if nbase is None else t + nbase))
+        self.ops.append(("f", t + vbase, None if nbase is None else t + nbase, None if tbase is None else t + tbase))
 
     def write(self, path, mtllib):
         lines = ["mtllib %s" % mtllib]
@@ -154,14 +279,21 @@ class _Obj:
                 lines.append("s %s" % ("1" if op[1] else "off"))
             elif k in ("v", "vn"):
                 lines.extend("%s %s %s %s" % (k, _fmt(p[0]), _fmt(p[1]), _fmt(p[2])) for p in op[1])
+            elif k == "vt":
+                lines.extend("vt %s %s" % (_fmt(p[0]), _fmt(p[1])) for p in op[1])
             else:
                 v1 = op[1] + 1
-                if op[2] is None:
-                    lines.extend("f %d %d %d" % tuple(r) for r in v1.tolist())
-                else:
-                    n1 = op[2] + 1
-                    lines.extend("f %d//%d %d//%d %d//%d" % (a, na, b, nb, c, nc)
-                                 for (a, b, c), (na, nb, nc) in zip(v1.tolist(), n1.tolist()))
+                n1 = None if op[2] is None else (op[2] + 1).tolist()
+                t1 = None if op[3] is None else (op[3] + 1).tolist()
+                for i, r in enumerate(v1.tolist()):
+                    if n1 is None and t1 is None:
+                        lines.append("f %d %d %d" % tuple(r))
+                    elif t1 is None:
+                        lines.append("f %d//%d %d//%d %d//%d" % (r[0], n1[i][0], r[1], n1[i][1], r[2], n1[i][2]))
+                    elif n1 is None:
+                        lines.append("f %d/%d %d/%d %d/%d" % (r[0], t1[i][0], r[1], t1[i][1], r[2], t1[i][2]))
+                    else:
+                        lines.append("f " + " ".join("%d/%d/%d" % (r[j], t1[i][j], n1[i][j]) for j in range(3)))
         with open(path, "w") as f:
             f.write("\n".join(lines))
             f.write("\n")
@@ -172,7 +304,8 @@ class _Obj:
         per-triangle index / id arrays in Scene.create's form."""
         vs = [op[1] for op in self.ops if op[0] == "v"]
         ns = [op[1] for op in self.ops if op[0] == "vn"]
-        tri_v, tri_n, gid, pid, mid, smooth = [], [], [], [], [], []
+        ts = [op[1] for op in self.ops if op[0] == "vt"]
+        tri_v, tri_n, tri_t, gid, pid, mid, smooth = [], [], [], [], [], [], []
         shape_faces = 0
         g = -1
         cur_mat, cur_s = -1, False
@@ -189,23 +322,31 @@ class _Obj:
                 k = len(op[1])
                 tri_v.append(op[1])
                 tri_n.append(op[2] if op[2] is not None else np.full((k, 3), -1, np.int64))
+                tri_t.append(op[3] if op[3] is not None else np.full((k, 3), -1, np.int64))
                 gid.append(np.full(k, g, np.uint32))
                 pid.append(np.arange(shape_faces, shape_faces + k, dtype=np.uint32))
                 mid.append(np.full(k, cur_mat, np.int32))
                 smooth.append(np.full(k, 1 if cur_s else 0, np.uint8))
                 shape_faces += k
         cat = lambda a, dt, shape: np.concatenate(a).astype(dt) if a else np.zeros(shape, dt)
-        return dict(vertices=cat(vs, np.float32, (0, 3)), normals=cat(ns, np.float32, (0, 3)),
+        out = dict(vertices=cat(vs, np.float32, (0, 3)), normals=cat(ns, np.float32, (0, 3)),
                     vertex_indices=cat(tri_v, np.int32, (0, 3)), normal_indices=cat(tri_n, np.int32, (0, 3)),
                     geom_ids=cat(gid, np.uint32, 0), prim_ids=cat(pid, np.uint32, 0),
                     material_ids=cat(mid, np.int32, 0), smooth=cat(smooth, np.uint8, 0))
+        if ts:
+            out["texcoords"] = cat(ts, np.float32, (0, 2))
+            out["texcoord_indices"] = cat(tri_t, np.int32, (0, 3))
+        return out
 
 
-def _mtl(name, kd=(0, 0, 0), ka=(0, 0, 0), ks=(0, 0, 0), ke=(0, 0, 0), ns=0.0, ni=1.0, d=1.0):
-    return (
+def _mtl(name, kd=(0, 0, 0), ka=(0, 0, 0), ks=(0, 0, 0), ke=(0, 0, 0), ns=0.0, ni=1.0, d=1.0, maps=None):
+    text = (
         "newmtl %s\nKa %s %s %s\nKd %s %s %s\nKs %s %s %s\nKe %s %s %s\nNs %s\nNi %s\nd %s\nillum 2\n"
         % (name, *map(_fmt, ka), *map(_fmt, kd), *map(_fmt, ks), *map(_fmt, ke), _fmt(ns), _fmt(ni), _fmt(d))
     )
+    for key, value in (maps or {}).items():  # e.g. {"map_Kd": "albedo.png", "bump": "-bm 1.0 normal.png"}
+        text += "%s %s\n" % (key, value)
+    return text
 
 
 def _box(center, size, angle_deg):
@@ -611,6 +752,100 @@ def icosphere(subdiv):
             new_faces += [(a, ab, ca), (b, bc, ab), (c, ca, bc), (ab, bc, ca)]
         faces = new_faces
     return np.array(verts), np.array(faces, dtype=np.int64)
+
+
+def write_textured_scene(folder, width=96, height=72, spp=8, max_depth=8, seed=5):
+    """A scene with every image-texture feature of the loader and the path
+    tracer (Scene.cpp:73-114, Texture.h:105-152, BRDF.cpp:604-628):
+
+    * floor quad, uv 0..3 (Repeat wrap): sRGB albedo checker (RGB PNG),
+      roughness (binary PGM), metallic (ASCII PGM with a space in its name);
+    * smooth icosphere, spherical uv: 16-bit RGBA PNG albedo and an RGB PNG
+      normal map given with a "-bm 1.0" option;
+    * glass cube (transmission 1, smooth) with a palette-PNG albedo, so media
+      absorb with the texel of the entry point;
+    * box without uv whose textures read the NoUV values;
+    * quad light, and a half-float ZIP-compressed EXR sky.
+    Returns the job path."""
+    os.makedirs(folder, exist_ok=True)
+    rng = np.random.default_rng(seed)
+    yy, xx = np.mgrid[0:8, 0:8]
+    checker = np.where(((xx // 2 + yy // 2) % 2)[..., None] == 0, [230, 200, 60], [40, 90, 200]).astype(np.uint8)
+    checker[0, 0] = (255, 0, 0)  # marks the top-left texel
+    write_png(os.path.join(folder, "checker.png"), checker)
+    write_pnm(os.path.join(folder, "rough.pgm"), rng.integers(40, 256, (8, 8)))
+    write_pnm(os.path.join(folder, "metal map.pgm"), np.where(rng.random((4, 4)) < 0.3, 255, 0), ascii=True)
+    write_png(os.path.join(folder, "globe.png"), rng.integers(0, 65536, (8, 16, 4)), bits=16)
+    sxy = rng.uniform(-0.6, 0.6, (16, 16, 2))
+    nrm = np.concatenate([sxy, np.ones((16, 16, 1))], 2)
+    nrm /= np.linalg.norm(nrm, axis=2, keepdims=True)
+    write_png(os.path.join(folder, "normal.png"), np.round((nrm + 1.0) * 0.5 * 255.0).astype(np.uint8))
+    write_png(os.path.join(folder, "tint.png"), rng.integers(0, 4, (4, 4)),
+              palette=[(250, 120, 120), (120, 250, 140), (140, 150, 250), (240, 240, 240)])
+    h, w = 32, 64
+    th = (np.arange(h) + 0.5) / h * math.pi
+    ph = (np.arange(w) + 0.5) / w * 2.0 * math.pi
+    sky = np.zeros((h, w, 3), np.float32)
+    sky[...] = (0.35 + 0.65 * np.cos(th)[:, None, None].clip(0, 1)) * np.array([0.5, 0.7, 1.0])
+    sky[6:9, 40:44] = (40.0, 36.0, 30.0)  # a small sun
+    sky += 0.05 * np.sin(ph)[None, :, None] ** 2
+    write_exr(os.path.join(folder, "sky.exr"), sky, compression="zip", pixel_type="half")
+
+    specs = [
+        ("floor", dict(kd=(0.5, 0.5, 0.5), ks=(0.5, 0.5, 0.5), ns=100.0,
+                       maps={"map_Kd": "checker.png", "map_Ns": "rough.pgm", "map_Pm": "metal map.pgm"})),
+        ("globe", dict(kd=(0.8, 0.8, 0.8), ks=(0.5, 0.5, 0.5), ns=300.0,
+                       maps={"map_Kd": "globe.png", "bump": "-bm 1.0 normal.png"})),
+        ("tint", dict(kd=(1, 1, 1), ns=1000.0, ni=1.5, d=0.0, maps={"map_Kd": "tint.png"})),
+        ("nouv", dict(kd=(0.5, 0.5, 0.5), ks=(0.5, 0.5, 0.5), ns=100.0,
+                      maps={"map_Kd": "checker.png", "map_Ns": "rough.pgm", "map_Pm": "metal map.pgm",
+                            "bump": "normal.png"})),
+        ("light", dict(ke=(8.0 * math.pi, 7.0 * math.pi, 6.0 * math.pi))),
+    ]
+    obj = _Obj()
+    obj.shape("floor")
+    obj.material("floor")
+    vb = obj.vertices([(-2, 0, 2), (2, 0, 2), (2, 0, -2), (-2, 0, -2)])
+    tb = obj.texcoords([(0, 0), (3, 0), (3, 3), (0, 3)])
+    obj.faces([(0, 1, 2), (0, 2, 3)], vb, tbase=tb)
+    obj.shape("globe")
+    obj.material("globe")
+    obj.smooth(True)
+    v, f = icosphere(3)
+    vb = obj.vertices(v * 0.45 + np.array([-0.55, 0.45, 0.0]))
+    nb = obj.normals(v)
+    uv = np.stack([np.arctan2(v[:, 2], v[:, 0]) / (2 * math.pi) + 0.5, np.arcsin(np.clip(v[:, 1], -1, 1)) / math.pi + 0.5], 1)
+    tb = obj.texcoords(uv)
+    obj.faces(f, vb, nbase=nb, tbase=tb)
+    obj.smooth(False)
+    obj.shape("tint")
+    obj.material("tint")
+    bv, bt = _box((0.6, 0.3, 0.25), (0.5, 0.6, 0.5), 25.0)
+    vb = obj.vertices(bv)
+    tb = obj.texcoords(np.stack([(bv[:, 0] - 0.35) * 2.0 + 0.1 * bv[:, 1], (bv[:, 2] + bv[:, 1]) * 1.3], 1))
+    obj.faces(bt, vb, tbase=tb)
+    obj.shape("nouv")
+    obj.material("nouv")
+    bv, bt = _box((0.1, 0.15, -0.8), (0.4, 0.3, 0.3), -30.0)
+    obj.faces(bt, obj.vertices(bv))
+    obj.shape("light")
+    obj.material("light")
+    vb = obj.vertices([(-0.3, 2.4, -0.3), (0.3, 2.4, -0.3), (0.3, 2.4, 0.3), (-0.3, 2.4, 0.3)])
+    obj.faces([(0, 1, 2), (0, 2, 3)], vb)
+    scene = {
+        "camera": {"position": [0.2, 1.3, 3.0], "look_at": [0.0, 0.35, 0.0]},
+        "world": {"environment_texture": "sky.exr"},
+        "mesh": "scene.obj",
+    }
+    settings = {"width": width, "height": height, "num_samples": spp, "max_depth": max_depth,
+                "roulette_start_depth": 3, "roulette_q": 0.3, "num_threads": 1, "use_mis": True}
+    obj.write(os.path.join(folder, "scene.obj"), "scene.mtl")
+    with open(os.path.join(folder, "scene.mtl"), "w") as fh:
+        fh.write("\n".join(_mtl(n, **kw) for n, kw in specs))
+    with open(os.path.join(folder, "scene.json"), "w") as fh:
+        json.dump(scene, fh, indent=2)
+    _write_job(folder, settings)
+    return os.path.join(folder, "renderJob.json")
 
 
 def sky_texture(width=2048, height=1024, sun_elev_deg=35.0, sun_azim_deg=60.0, sun_radiance=5e4,

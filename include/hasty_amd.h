@@ -98,6 +98,17 @@ typedef struct HastyCameraDesc {
   float exposure;
 } HastyCameraDesc;
 
+/* An image texture as TextureTraits::loadTexture leaves it
+ * (core/src/Texture.cpp:21-49): width*height texels of 1 float (roughness,
+ * metallic) or 3 floats (albedo, already decoded from sRGB; normal map),
+ * linear, row 0 at the top. */
+typedef struct HastyTextureDesc {
+  uint32_t width;
+  uint32_t height;
+  uint32_t channels; /* 1 or 3 */
+  const float* pixels;
+} HastyTextureDesc;
+
 /* Triangles are listed shape by shape, faces in order: geom_ids[i] (the
  * tinyobj shape) never decreases and prim_ids[i] counts the faces of its
  * shape from 0, so (geom_id, prim_id) are the ids Embree would report.
@@ -129,6 +140,13 @@ typedef struct HastySceneDesc {
   uint32_t env_height;
   const float* env_rgb;
   float background[3];
+  /* Image textures (may be 0 / NULL).  material_textures holds 4 texture
+   * indices per material -- albedo (map_Kd), roughness (map_Ns), metallic
+   * (map_Pm), normal map (bump), -1 = the material's constant
+   * (core/src/Scene.cpp:73-114) -- or is NULL for an untextured scene. */
+  uint32_t num_textures;
+  const HastyTextureDesc* textures;
+  const int32_t* material_textures;
 } HastySceneDesc;
 
 /* Builds the BVH on the host and uploads the scene to the current device,
@@ -380,6 +398,18 @@ int hasty_sppm_radii(HastySppm* job, float* out);
 /* Photons of the last pass in emission order, 10 floats each: position,
  * dot(n_g, w_i), w_i, flux (Photon, PhotonMapping.cpp:11-23). */
 int hasty_sppm_photons(HastySppm* job, float* out, uint64_t max_photons);
+
+/* Image files, on the host (no device work): reads a PFM, PPM / PGM, PNG or
+ * OpenEXR file into float pixels, row 0 at the top, channels interleaved --
+ * what OIIO hands readImage1f / readImage3f (core/src/Image.cpp:74-138).
+ * With pixels NULL only the size is returned; otherwise capacity must hold
+ * width*height*channels floats. */
+int hasty_image_read(const char* path, uint32_t* width, uint32_t* height, uint32_t* channels, float* pixels,
+                     uint64_t capacity);
+/* writeEXR (core/src/Image.cpp:140-152): an uncompressed OpenEXR file of
+ * FLOAT channels (stored B, G, R, the format's order) from width*height RGB
+ * floats, row 0 at the top. */
+int hasty_image_write_exr(const char* path, uint32_t width, uint32_t height, const float* rgb);
 
 const char* hasty_last_error(void);
 

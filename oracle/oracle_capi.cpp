@@ -121,6 +121,28 @@ int oracle_material(void* jp, int idx, float* out) {
   memcpy(out, v, sizeof(v));
   return 0;
 }
+// Texture indices of material idx: albedo, roughness, metallic, normal map.
+int oracle_material_textures(void* jp, int idx, int32_t* out4) {
+  PTJob* job = (PTJob*)jp;
+  if (idx < 0 || idx >= (int)job->scene.data.materials.size()) return 1;
+  const hasty::Material& m = job->scene.data.materials[idx];
+  out4[0] = m.texAlbedo;
+  out4[1] = m.texRoughness;
+  out4[2] = m.texMetallic;
+  out4[3] = m.texNormal;
+  return 0;
+}
+// Texture t as loaded: width, height, channels; texels when px is not null.
+int oracle_texture(void* jp, int t, uint32_t* whc, float* px) {
+  PTJob* job = (PTJob*)jp;
+  if (t < 0 || t >= (int)job->scene.data.textures.size()) return 1;
+  const hasty::Texture& tx = job->scene.data.textures[t];
+  whc[0] = tx.width;
+  whc[1] = tx.height;
+  whc[2] = tx.channels;
+  if (px) memcpy(px, tx.px.data(), tx.px.size() * sizeof(float));
+  return 0;
+}
 // Light table: global triangle index and area of light i.
 int oracle_light(void* jp, int i, int32_t* tri, float* area) {
   PTJob* job = (PTJob*)jp;

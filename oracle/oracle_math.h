@@ -44,6 +44,7 @@ struct Vec2f {
   float& operator[](int i) { return v[i]; }
   const float& operator[](int i) const { return v[i]; }
   Vec2f operator+(const Vec2f& o) const { return Vec2f(v[0] + o[0], v[1] + o[1]); }
+  Vec2f operator*(float s) const { return Vec2f(v[0] * s, v[1] * s); }
 };
 inline float dot(const Vec2f& a, const Vec2f& b) { return a[0] * b[0] + a[1] * b[1]; }
 inline float norm(const Vec2f& a) { return hm::fsqrt(dot(a, a)); }

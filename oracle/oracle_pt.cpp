@@ -302,10 +302,63 @@ float Principled::getIndexOfRefraction(const hasty::Material& m, int wavelength)
     default: return std::numeric_limits<float>::quiet_NaN();
   }
 }
-// BRDF.cpp:604-628 (no normal map)
+// ========================================================== Texture.h ===
+// Texture::evaluateUV, Repeat addressing in u and v (Texture.h:119-140).
+// Guards where the reference is undefined: a coordinate that is not finite or
+// not below 2^31 in magnitude reads as 0; a wrapped coordinate that rounds to
+// 1 reads the last column / row.
+static size_t texelIndex(const hasty::Texture& t, const Vec2f& uvIn) {
+  float uu = uvIn[0], vv = uvIn[1];
+  if (!(std::fabs(uu) < 2147483648.0f)) uu = 0.0f;
+  if (!(std::fabs(vv) < 2147483648.0f)) vv = 0.0f;
+  float u = fmod1p1(uu);
+  float oneMinusV = fmod1p1(g_mutation == kMutTexVNoFlip ? vv : 1.0f - vv);
+  int col = static_cast<int>(u * t.width);
+  int row = static_cast<int>(oneMinusV * t.height);
+  col = std::min(std::max(col, 0), (int)t.width - 1);
+  row = std::min(std::max(row, 0), (int)t.height - 1);
+  return (size_t)row * t.width + (size_t)col;
+}
+// Texture<T>::evaluate (Texture.cpp:9-18) with the NoUV values of
+// TextureTraits (Texture.h:84-101).
+static Vec3f evaluateTexture3(const hasty::Texture& t, const SurfaceInteraction& it) {
+  if (!it.hasUV) return Vec3f(1.0f, 0.0f, 0.7f);
+  const float* p = &t.px[texelIndex(t, it.uv) * 3];
+  return Vec3f(p[0], p[1], p[2]);
+}
+static float evaluateTexture1(const hasty::Texture& t, const SurfaceInteraction& it) {
+  if (!it.hasUV) return -1.0f;
+  return t.px[texelIndex(t, it.uv)];
+}
+// albedo / roughness / metallic ->evaluate(interaction) (BRDF.cpp:647-650,
+// 685-687, 708-710, 745): a copy of the material with the textured values.
+Scene::MatAt::MatAt(const Scene& scene, const SurfaceInteraction& it) {
+  const hasty::Material& m = scene.mat(it);
+  if (!m.textured()) {
+    p = &m;
+    return;
+  }
+  local = m;
+  if (m.texAlbedo >= 0) {
+    Vec3f a = evaluateTexture3(scene.data.textures[m.texAlbedo], it);
+    for (int c = 0; c < 3; ++c) local.albedo[c] = a[c];
+  }
+  if (m.texRoughness >= 0) local.roughness = evaluateTexture1(scene.data.textures[m.texRoughness], it);
+  if (m.texMetallic >= 0) local.metallic = evaluateTexture1(scene.data.textures[m.texMetallic], it);
+  p = &local;
+}
+
+// BRDF.cpp:604-628
 Vec3f Principled::getShadingNormal(const SurfaceInteraction& it, const Vec3f& wo, float dotNgWo) {
   float sideNgWo = signSide(dotNgWo);
-  Vec3f ns = it.normalShadingDefault;
+  Vec3f ns;
+  if (it.normalMap) {
+    Vec3f color = evaluateTexture3(*it.normalMap, it);
+    Vec3f texNormal = g_mutation == kMutNormalMapNoRemap ? color : color * 2.0f - Vec3f::Ones();
+    ns = normalize(it.tangent * texNormal[0] + it.bitangent * texNormal[1] + it.normalGeometric * texNormal[2]);
+  } else {
+    ns = it.normalShadingDefault;
+  }
   float sideNsWo = signSide(dot(ns, wo));
   if (sideNgWo != sideNsWo) ns = it.normalGeometric;
   ns *= sideNgWo;
@@ -547,12 +600,22 @@ float LightRayInfo::getOutsideIOR(const RayHit& hit) const {
 // ============================================================= Scene.cpp ===
 void Scene::init() {
   bvh.build(data.triPos.data(), data.numTris);
-  for (const hasty::Material& m : data.materials) {
-    if (m.transmission > 0.0f && Principled::computeAlpha(m.roughness) != 0.0f)
-      throw std::runtime_error("not implemented");  // BRDF.cpp:386-390, raised at load
+  for (size_t i = 0; i < data.materials.size(); ++i) {
+    const hasty::Material& m = data.materials[i];
+    // BRDF.cpp:386-390 throws when rough transmission is shaded; raised at
+    // load, for a roughness texture if any texel or a face without uv
+    // (roughness -1) gives alpha != 0
+    if (m.transmission > 0.0f && m.texRoughness < 0 && Principled::computeAlpha(m.roughness) != 0.0f)
+      throw std::runtime_error("not implemented");
+    if (m.transmission > 0.0f && m.texRoughness >= 0) {
+      for (float r : data.textures[m.texRoughness].px)
+        if (Principled::computeAlpha(r) != 0.0f) throw std::runtime_error("not implemented");
+      for (uint32_t t = 0; t < data.numTris; ++t)
+        if (data.triMaterial[t] == (int32_t)i && !data.triHasUV[t]) throw std::runtime_error("not implemented");
+    }
   }
 }
-// Scene.cpp:202-255 (uv only feeds image textures, which are not supported)
+// Scene.cpp:202-255
 SurfaceInteraction Scene::getInteraction(int32_t tri, const Vec3f& bary, const Vec3f& geomNormal) const {
   SurfaceInteraction it;
   const float* p = &data.triPos[(size_t)tri * 9];
@@ -579,6 +642,14 @@ SurfaceInteraction Scene::getInteraction(int32_t tri, const Vec3f& bary, const V
     it.normalShadingDefault = it.normalGeometric;
     it.h = 0.0f;
   }
+  if (data.triHasUV[tri]) {  // collectTriangleUV (tinyobj.cpp:73-88)
+    const float* q = &data.triUV[(size_t)tri * 6];
+    Vec2f uv[3] = {Vec2f(q[0], q[1]), Vec2f(q[2], q[3]), Vec2f(q[4], q[5])};
+    it.hasUV = true;
+    it.uv = uv[0] * bary[0] + uv[1] * bary[1] + uv[2] * bary[2];
+  }
+  const hasty::Material& m = data.materials[it.materialId];
+  if (m.texNormal >= 0) it.normalMap = &data.textures[m.texNormal];
   it.tangent = Vec3f(-it.x[2], 0.0f, it.x[0]);
   it.tangent = orthonormalizedOtherwiseAnyOrthonormal(it.normalGeometric, it.tangent);
   it.bitangent = cross(it.normalGeometric, it.tangent);
@@ -610,21 +681,24 @@ void Scene::rayHit(const Ray& ray, RayHit& hit) const {
     hit.interaction = getInteraction(hit.rtc.tri, bary, ng);
   }
 }
-Vec3f Scene::getAlbedo(const SurfaceInteraction& it) const {
-  const hasty::Material& m = mat(it);
-  return Vec3f(m.albedo[0], m.albedo[1], m.albedo[2]);
+Vec3f Scene::getAlbedo(const SurfaceInteraction& it) const {  // Scene.cpp:478-481
+  MatAt m(*this, it);
+  return Vec3f(m->albedo[0], m->albedo[1], m->albedo[2]);
 }
-bool Scene::hasBRDFDiffuseLobe(const RayHit& hit) const { return Principled::hasDiffuseLobe(mat(hit.interaction)); }
+bool Scene::hasBRDFDiffuseLobe(const RayHit& hit) const {
+  return Principled::hasDiffuseLobe(*MatAt(*this, hit.interaction));
+}
 float Scene::getIORInside(const RayHit& hit, int wavelength) const {
   return Principled::getIndexOfRefraction(mat(hit.interaction), wavelength);
 }
 MaterialEvalResult Scene::evaluteBRDF(const RayHit& hit, const Vec3f& wo, const Vec3f& wi, float iorOutside,
                                       bool adjoint, int flags) const {  // :323-326
-  return Principled::evaluate(mat(hit.interaction), hit.interaction, wo, wi, iorOutside, adjoint, flags);
+  return Principled::evaluate(*MatAt(*this, hit.interaction), hit.interaction, wo, wi, iorOutside, adjoint, flags);
 }
 // Scene.cpp:328-349
 SampleResult Scene::sampleBRDF(RNG& rng, LightRayInfo& lightRay, const RayHit& hit, bool adjoint, int flags) const {
-  const hasty::Material& m = mat(hit.interaction);
+  MatAt matAt(*this, hit.interaction);
+  const hasty::Material& m = *matAt;
   float sampleProbability;
   int wavelength = lightRay.sampleOrGetWavelength(rng, m.varyIOR, sampleProbability);
   float outsideIOR = lightRay.getOutsideIOR(hit);
@@ -635,7 +709,7 @@ SampleResult Scene::sampleBRDF(RNG& rng, LightRayInfo& lightRay, const RayHit& h
 }
 // Scene.cpp:350-353
 float Scene::evaluateSamplePDF(const RayHit& hit, const LightRayInfo& lightRay, const Vec3f& dir2) const {
-  return Principled::evaluateSamplePDF(mat(hit.interaction), hit.interaction, hit.wFrom(), dir2,
+  return Principled::evaluateSamplePDF(*MatAt(*this, hit.interaction), hit.interaction, hit.wFrom(), dir2,
                                        lightRay.getOutsideIOR(hit));
 }
 // Scene.cpp:494-500

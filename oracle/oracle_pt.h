@@ -32,6 +32,8 @@ enum Mutation {
   kMutCameraUpSign = 4,       // computeRay adds the up term instead of subtracting it
   kMutBeerNoDistance = 5,     // beersLaw returns T instead of T^d
   kMutLensFocalAlongRay = 6,  // thin lens refocuses at focalDistance along the ray
+  kMutTexVNoFlip = 7,         // evaluateUV reads row v * H instead of (1 - v) * H
+  kMutNormalMapNoRemap = 8,   // the normal map's texel is used as the normal, not 2 texel - 1
 };
 extern int g_mutation;
 
@@ -86,6 +88,11 @@ struct SurfaceInteraction {  // Scene.h:20-33
   Vec3f x, normalGeometric, normalShadingDefault, tangent, bitangent;
   int32_t materialId = -1;
   Vec3f barycentricCoordinates;
+  bool hasUV = false;  // std::optional<Vec2f> uv
+  Vec2f uv;
+  // The hit material's normal map (PrincipledBRDF::normalMap, BRDF.h:167),
+  // carried with the interaction so getShadingNormal can evaluate it.
+  const hasty::Texture* normalMap = nullptr;
 };
 struct RayHit {  // Scene.h:35-54
   HitRecord rtc;
@@ -151,8 +158,18 @@ class Scene {
   Ray constructRayEnd(const SurfaceInteraction& it, const Vec3f& end, bool outside) const;
   void rayHit(const Ray& ray, RayHit& hit) const;
 
-  // material access
+  // material access: mat() is the material as loaded; MatAt holds it with
+  // its image textures evaluated at an interaction
   const hasty::Material& mat(const SurfaceInteraction& it) const { return data.materials[it.materialId]; }
+  class MatAt {
+   public:
+    MatAt(const Scene& scene, const SurfaceInteraction& it);
+    const hasty::Material& operator*() const { return *p; }
+    const hasty::Material* operator->() const { return p; }
+   private:
+    hasty::Material local;
+    const hasty::Material* p;
+  };
   Vec3f getAlbedo(const SurfaceInteraction& it) const;
   bool hasBRDFDiffuseLobe(const RayHit& hit) const;
   float getIORInside(const RayHit& hit, int wavelength) const;

@@ -36,6 +36,8 @@ def lib():
         L.oracle_trace.argtypes = [P, fp, C.c_uint32, C.c_int, ip, fp, ip]
         L.oracle_tri_ids.argtypes = [P, C.c_int32, up]
         L.oracle_material.argtypes = [P, C.c_int, fp]
+        L.oracle_material_textures.argtypes = [P, C.c_int, C.POINTER(C.c_int32)]
+        L.oracle_texture.argtypes = [P, C.c_int, C.POINTER(C.c_uint32), C.c_void_p]
         L.oracle_light.argtypes = [P, C.c_int, C.POINTER(C.c_int32), C.POINTER(C.c_float)]
         L.oracle_camera_rays.argtypes = [P, C.c_uint32, fp]
         L.oracle_camera_ray.argtypes = [P, C.c_float, C.c_float, C.c_uint32, fp]
@@ -159,6 +161,23 @@ class OracleJob:
         return {"albedo": out[0:3], "roughness": out[3], "metallic": out[4], "specular": out[5], "ior": out[6],
                 "anisotropy": out[7], "transmission": out[8], "emission": out[9:12], "vary_ior": bool(out[12])}
 
+    def material_textures(self, idx):
+        """(albedo, roughness, metallic, normal map) texture indices, -1 = none."""
+        out = (C.c_int32 * 4)()
+        if lib().oracle_material_textures(self.h, idx, out) != 0:
+            raise IndexError(idx)
+        return tuple(out)
+
+    def texture(self, t):
+        """Texture t as the loader decoded it: (H, W) or (H, W, 3) float32."""
+        whc = (C.c_uint32 * 3)()
+        if lib().oracle_texture(self.h, t, whc, None) != 0:
+            raise IndexError(t)
+        w, h, c = whc
+        out = np.zeros((h, w, c), np.float32)
+        lib().oracle_texture(self.h, t, whc, out.ctypes.data)
+        return out[..., 0] if c == 1 else out
+
     def light(self, i):
         tri, area = C.c_int32(), C.c_float()
         if lib().oracle_light(self.h, i, C.byref(tri), C.byref(area)) != 0:
@@ -216,6 +235,8 @@ class mutation:
     CAMERA_UP_SIGN = 4
     BEER_NO_DISTANCE = 5
     LENS_FOCAL_ALONG_RAY = 6
+    TEX_V_NO_FLIP = 7
+    NORMAL_MAP_NO_REMAP = 8
 
     def __init__(self, m):
         self.m = m
