@@ -147,7 +147,7 @@ def sppm_cpu_baseline(job_path, args):
     t0 = time.perf_counter()
     o = pyoracle.OracleSppm(job_path, seed=3, alpha=args.sppm_alpha, store_fraction=args.sppm_store_fraction,
                             emit_fraction=args.sppm_emit_fraction, initial_radius=args.sppm_initial_radius,
-                            width=960, height=540)
+                            width=960, height=540, fixed_point_flux=args.sppm_fixed_flux)
     load = time.perf_counter() - t0
     t0 = time.perf_counter()
     o.run_pass(0, 960, 540)
@@ -211,7 +211,8 @@ def bench_sppm(args):
     W, H = info["width"], info["height"]
     log("scene loaded in %.2f s: %d triangles" % (load_s, info["num_triangles"]))
     sp = ha.Sppm(scene, 3, alpha=args.sppm_alpha, store_fraction=args.sppm_store_fraction,
-                 emit_fraction=args.sppm_emit_fraction, initial_radius=args.sppm_initial_radius)
+                 emit_fraction=args.sppm_emit_fraction, initial_radius=args.sppm_initial_radius,
+                 fixed_point_flux=args.sppm_fixed_flux)
     hip = Hip()
     nbytes = W * H * 3 * 4
     films = [hip.malloc(nbytes) for _ in range(3)]
@@ -219,25 +220,36 @@ def bench_sppm(args):
     for _ in range(args.warmup):
         sp.pass_device(k, *films)
         k += 1
-    scene.reset_stats()
-    scene.set_profiling(events=True, counters=False)
+    # timed passes: no event timing; each pass's counters are read after it
     hip.sync()
     t_start = time.perf_counter()
-    emitted = stored = queries = candidates = found = 0
+    emitted = stored = overflows = 0
     for _ in range(args.steps):
         t = time.perf_counter()
         sp.pass_device(k, *films)
         i = sp.info()
         emitted += i["emitted"]
         stored += i["stored"]
-        queries += i["gather_queries"]
-        candidates += i["gather_candidates"]
-        found += i["gather_found"]
+        overflows += i["flux_overflows"]
         log("pass %d: %.3f s, emitted %d stored %d cell %.4g" % (k, time.perf_counter() - t, i["emitted"],
                                                               i["stored"], i["cell_size"]))
         k += 1
     hip.sync()
     elapsed = time.perf_counter() - t_start
+    # per-kernel times and gather statistics from extra passes with HIP-event
+    # timing on every launch (outside the timed region)
+    prof_passes = max(1, min(3, args.steps))
+    scene.reset_stats()
+    scene.set_profiling(events=True, counters=False)
+    queries = candidates = found = 0
+    for _ in range(prof_passes):
+        sp.pass_device(k, *films)
+        i = sp.info()
+        queries += i["gather_queries"]
+        candidates += i["gather_candidates"]
+        found += i["gather_found"]
+        k += 1
+    hip.sync()
     kstats = scene.kernel_stats()
     scene.set_profiling(events=False, counters=False)
     # roofline of the gather kernel: per gather 64 B of query record, 16 B
@@ -251,7 +263,7 @@ def bench_sppm(args):
         roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": "sppm_gather",
                     "bytes_per_launch": gbytes / g_launches, "avg_launch_ms": round(g_ms / g_launches, 4),
-                    "queries_per_pass": queries / max(1, args.steps),
+                    "queries_per_pass": queries / prof_passes,
                     "candidates_per_query": candidates / max(1, queries), "found_per_query": found / max(1, queries)}
     base = None
     if not args.no_cpu_baseline:
@@ -273,11 +285,14 @@ def bench_sppm(args):
         "config": {"workload": "sppm_caustic_box", "width": W, "height": H, "triangles": info["num_triangles"],
                    "alpha": args.sppm_alpha, "store_fraction": args.sppm_store_fraction,
                    "emit_fraction": args.sppm_emit_fraction, "initial_radius": args.sppm_initial_radius or "footprint",
+                   "flux": "int64 fixed point" if args.sppm_fixed_flux else "fp32",
                    "parallelism": "single-gpu"},
         "photons_emitted_per_s": emitted / elapsed,
         "photons_stored_per_pass": stored / max(1, args.steps),
+        "flux_overflows": overflows,
         "load_s": load_s,
-        "kernel_ms_per_pass": {k2: round(v[1] / args.steps, 3) for k2, v in kstats.items()},
+        "kernel_ms_per_pass": {k2: round(v[1] / prof_passes, 3) for k2, v in kstats.items()},
+        "kernel_ms_source": "%d extra passes with HIP-event timing per launch, after the timed passes" % prof_passes,
         "roofline": roofline,
         "cpu_baseline": base,
     }
@@ -515,6 +530,8 @@ def main():
     ap.add_argument("--sppm-alpha", type=float, default=0.4)
     ap.add_argument("--sppm-store-fraction", type=float, default=0.5)
     ap.add_argument("--sppm-emit-fraction", type=float, default=10.0)
+    ap.add_argument("--sppm-fixed-flux", action="store_true",
+                    help="int64 fixed-point per-pixel flux (order-independent sums)")
     ap.add_argument("--sppm-initial-radius", type=float, default=0.0,
                     help="constant initial radius (0: 10x pixel footprint as the reference)")
     args = ap.parse_args()
