@@ -42,6 +42,15 @@ EXPORTS = (
     "hasty_trace_hits_device_noalloc",
     "hasty_image_read",
     "hasty_image_write_exr",
+    "hasty_estimate_device",
+    "hasty_present_device",
+    "hasty_films_create",
+    "hasty_films_destroy",
+    "hasty_films_render",
+    "hasty_films_snapshot",
+    "hasty_films_read",
+    "hasty_films_device",
+    "hasty_films_estimate",
     "hasty_set_profiling",
     "hasty_kernel_stats",
     "hasty_reset_stats",
@@ -180,6 +189,7 @@ class SceneInfoC(C.Structure):
         ("bvh_depth", C.c_int32),
         ("device_bytes", C.c_uint64),
         ("bvh_width", C.c_int32),
+        ("camera_exposure", C.c_float),
     ]
 
 
@@ -263,6 +273,18 @@ def lib():
     u32p = C.POINTER(C.c_uint32)
     L.hasty_image_read.argtypes = [C.c_char_p, u32p, u32p, u32p, fp, C.c_uint64]
     L.hasty_image_write_exr.argtypes = [C.c_char_p, C.c_uint32, C.c_uint32, fp]
+    L.hasty_estimate_device.argtypes = [P, C.POINTER(C.c_void_p), C.POINTER(C.c_uint32), C.c_uint32, C.c_float,
+                                        C.c_void_p]
+    L.hasty_present_device.argtypes = [P, C.c_void_p, C.c_void_p]
+    L.hasty_films_create.argtypes = [P, C.POINTER(P)]
+    L.hasty_films_destroy.argtypes = [P]
+    L.hasty_films_destroy.restype = None
+    L.hasty_films_render.argtypes = [P, C.c_uint32, C.c_uint32, C.c_uint32]
+    L.hasty_films_snapshot.argtypes = [P]
+    L.hasty_films_read.argtypes = [P, C.c_int, fp, fp, fp, C.POINTER(C.c_uint32)]
+    L.hasty_films_device.argtypes = [P, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), C.POINTER(C.c_void_p),
+                                     C.POINTER(C.c_uint32)]
+    L.hasty_films_estimate.argtypes = [C.POINTER(P), C.c_uint32, C.c_float, fp, fp, fp]
     L.hasty_set_profiling.argtypes = [P, C.c_int]
     L.hasty_kernel_stats.argtypes = [P, C.POINTER(KernelStatC), C.c_int]
     L.hasty_reset_stats.argtypes = [P]
@@ -443,6 +465,7 @@ class Scene:
             "bvh_nodes": i.bvh_nodes,
             "bvh_depth": i.bvh_depth,
             "bvh_width": i.bvh_width,
+            "camera_exposure": i.camera_exposure,
             "device_bytes": i.device_bytes,
         }
 
@@ -585,6 +608,15 @@ class DeviceFilms:
         for p in self.ptrs:
             if DeviceFilms._hip.hipMemset(C.c_void_p(p), 0, self.nbytes) != 0:
                 raise HastyError("hipMemset failed")
+
+    def upload(self, arrays):
+        """Copies three host (H, W, 3) float32 arrays into the films."""
+        for p, a in zip(self.ptrs, arrays):
+            a = np.ascontiguousarray(a, np.float32)
+            if a.nbytes != self.nbytes:
+                raise ValueError("film of %d bytes expected" % self.nbytes)
+            if DeviceFilms._hip.hipMemcpy(C.c_void_p(p), a.ctypes.data, self.nbytes, 1) != 0:  # host to device
+                raise HastyError("hipMemcpy failed")
 
     def download(self):
         out = []
@@ -898,6 +930,76 @@ def renderPhotonThread(colorBuffer, normalBuffer, albedoBuffer, job: PMRenderJob
             b.copyBuffer()
         sample = job.fetchSample()
     threadData.stoppedFlag = True
+
+
+def estimate_device(scene: Scene, films, num_samples, exposure: float, out_ptr: int) -> None:
+    """hasty_estimate_device: computeEstimate of device films (pointers) with
+    num_samples[k] samples each, scaled by 2^exposure, into out_ptr."""
+    n = len(films)
+    ptrs = (C.c_void_p * n)(*[C.c_void_p(int(p)) for p in films])
+    counts = (C.c_uint32 * n)(*[int(c) for c in num_samples])
+    _err(lib().hasty_estimate_device(scene.handle, ptrs, counts, n, float(exposure), C.c_void_p(int(out_ptr))))
+
+
+def present_device(scene: Scene, image_ptr: int, rgba_ptr: int) -> None:
+    """hasty_present_device: gamma 1/2.2 and RGBA8 packing of a device image."""
+    _err(lib().hasty_present_device(scene.handle, C.c_void_p(int(image_ptr)), C.c_void_p(int(rgba_ptr))))
+
+
+class Films:
+    """Device films of a scene with snapshots for progressive display (HastyFilms*)."""
+
+    def __init__(self, scene: Scene):
+        h = C.c_void_p()
+        _err(lib().hasty_films_create(scene.handle, C.byref(h)))
+        self._h = h
+        self.scene = scene
+        i = scene.info()
+        self.width, self.height = i["width"], i["height"]
+
+    def render(self, seed: int, first_sample: int, num_samples: int):
+        _err(lib().hasty_films_render(self._h, seed, first_sample, num_samples))
+
+    def snapshot(self):
+        _err(lib().hasty_films_snapshot(self._h))
+
+    def read(self, wait: bool = True, out=None):
+        """The last finished snapshot as (color, normal, albedo, samples), or
+        None when there is nothing new (or, without wait, it is in flight)."""
+        if out is None:
+            out = tuple(np.zeros((self.height, self.width, 3), np.float32) for _ in range(3))
+        _check_films(out, self.height, self.width)
+        n = C.c_uint32()
+        rc = lib().hasty_films_read(self._h, 1 if wait else 0, *(_fptr(f) for f in out), C.byref(n))
+        if rc == 4:
+            return None
+        _err(rc)
+        return (*out, n.value)
+
+    def device(self):
+        """(color, normal, albedo) device pointers and the samples added so far."""
+        c, nn, a = C.c_void_p(), C.c_void_p(), C.c_void_p()
+        k = C.c_uint32()
+        _err(lib().hasty_films_device(self._h, C.byref(c), C.byref(nn), C.byref(a), C.byref(k)))
+        return (c.value, nn.value, a.value), k.value
+
+    @staticmethod
+    def estimate(film_sets, exposure: float = 0.0):
+        """hasty_films_estimate: (color, normal, albedo) estimates of the film
+        sets (computeEstimate on the GPU, colour times 2^exposure)."""
+        f0 = film_sets[0]
+        out = tuple(np.zeros((f0.height, f0.width, 3), np.float32) for _ in range(3))
+        hs = (C.c_void_p * len(film_sets))(*[f._h for f in film_sets])
+        _err(lib().hasty_films_estimate(hs, len(film_sets), float(exposure), *(_fptr(o) for o in out)))
+        return out
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and _lib is not None:
+            _lib.hasty_films_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
 
 
 def read_image(path) -> np.ndarray:

@@ -24,4 +24,10 @@ bool sceneView(const HastyScene* scene, SceneView* out);
 // Sets the message hasty_last_error() returns on the calling thread.
 void setLastError(const std::string& message);
 
+// computeEstimate of `count` device films of `floats` values each into `out`
+// on `stream` (asynchronous): counts[k] = samples in film k, scale = the
+// exposure factor (output_kernels.hip).
+void launchEstimate(const float* const* films, const uint32_t* counts, int count, float scale, size_t floats,
+                    float* out, hipStream_t stream);
+
 }  // namespace hasty

@@ -8,7 +8,9 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cmath>
 #include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -971,8 +973,36 @@ int hasty_scene_info(const HastyScene* s, HastySceneInfo* info) {
   info->bvh_depth = s->bvh().depth;
   info->device_bytes = s->sceneBytes + s->bvh().bytes;
   info->bvh_width = s->bvh().width;
+  info->camera_exposure = s->job.scene.camera.exposure;
   return 0;
 }
+
+// ---- device films with snapshots for progressive display ----
+struct HastyFilms {
+  HastyScene* scene = nullptr;
+  size_t floats = 0;           // W*H*3 per film
+  float* film = nullptr;       // colour, normal, albedo (device)
+  float* snap = nullptr;       // the last snapshot's copy (device)
+  float* staging = nullptr;    // its copy in pinned host memory
+  hipStream_t copyStream = nullptr;
+  hipEvent_t rendered = nullptr;  // the last snapshot's device copy ended (scene stream)
+  hipEvent_t copied = nullptr;    // the last snapshot's device-to-host copy ended
+  float* estimate = nullptr;   // hasty_films_estimate's output (device, 3 films)
+  uint32_t samples = 0;         // samples added to the films
+  uint32_t snapSamples = 0;     // samples in the last snapshot
+  bool unread = false;          // a snapshot not yet handed out
+  std::mutex m;
+  ~HastyFilms() {
+    if (copyStream) (void)hipStreamSynchronize(copyStream);
+    if (copied) (void)hipEventDestroy(copied);
+    if (rendered) (void)hipEventDestroy(rendered);
+    if (copyStream) (void)hipStreamDestroy(copyStream);
+    if (staging) (void)hipHostFree(staging);
+    if (snap) (void)hipFree(snap);
+    if (film) (void)hipFree(film);
+    if (estimate) (void)hipFree(estimate);
+  }
+};
 
 int hasty_scene_set_settings(HastyScene* s, const HastyRenderSettings* st) {
   if (!s || !st) {
@@ -1011,6 +1041,161 @@ int hasty_render_pt_device(HastyScene* s, uint32_t seed, uint32_t first, uint32_
   } catch (const std::exception& e) {
     return fail(e, 3);
   }
+}
+
+int hasty_films_create(HastyScene* s, HastyFilms** out) {
+  if (!s || !out) {
+    g_lastError = "null argument";
+    return 1;
+  }
+  try {
+    HIPCHECK(hipSetDevice(s->device));
+    std::unique_ptr<HastyFilms> f(new HastyFilms());
+    f->scene = s;
+    f->floats = (size_t)s->dev.width * s->dev.height * 3;
+    if (f->floats == 0) throw std::runtime_error("empty film");
+    HIPCHECK(hipMalloc(&f->film, 3 * f->floats * sizeof(float)));
+    HIPCHECK(hipMalloc(&f->snap, 3 * f->floats * sizeof(float)));
+    HIPCHECK(hipHostMalloc(&f->staging, 3 * f->floats * sizeof(float), hipHostMallocDefault));
+    HIPCHECK(hipStreamCreate(&f->copyStream));
+    HIPCHECK(hipEventCreateWithFlags(&f->copied, hipEventDisableTiming));
+    HIPCHECK(hipEventCreateWithFlags(&f->rendered, hipEventDisableTiming));
+    HIPCHECK(hipMemsetAsync(f->film, 0, 3 * f->floats * sizeof(float), s->stream));
+    HIPCHECK(hipStreamSynchronize(s->stream));
+    *out = f.release();
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e, 3);
+  }
+}
+
+void hasty_films_destroy(HastyFilms* f) { delete f; }
+
+int hasty_films_render(HastyFilms* f, uint32_t seed, uint32_t first, uint32_t n) {
+  if (!f) {
+    g_lastError = "null argument";
+    return 1;
+  }
+  try {
+    HastyScene* s = f->scene;
+    HIPCHECK(hipSetDevice(s->device));
+    if ((size_t)s->dev.width * s->dev.height * 3 != f->floats)
+      throw std::runtime_error("the scene's film size changed after the films were created");
+    if (n > 0) s->renderDevice(seed, first, n, f->film, f->film + f->floats, f->film + 2 * f->floats);
+    std::lock_guard<std::mutex> lock(f->m);
+    f->samples += n;
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e, 3);
+  }
+}
+
+int hasty_films_snapshot(HastyFilms* f) {
+  if (!f) {
+    g_lastError = "null argument";
+    return 1;
+  }
+  try {
+    HastyScene* s = f->scene;
+    HIPCHECK(hipSetDevice(s->device));
+    std::lock_guard<std::mutex> lock(f->m);
+    const size_t bytes = 3 * f->floats * sizeof(float);
+    // the previous snapshot's readback must have left `snap` before it is
+    // overwritten; the copy follows the work queued on the scene's stream
+    HIPCHECK(hipStreamWaitEvent(s->stream, f->copied, 0));
+    HIPCHECK(hipMemcpyAsync(f->snap, f->film, bytes, hipMemcpyDeviceToDevice, s->stream));
+    HIPCHECK(hipEventRecord(f->rendered, s->stream));
+    HIPCHECK(hipStreamWaitEvent(f->copyStream, f->rendered, 0));
+    HIPCHECK(hipMemcpyAsync(f->staging, f->snap, bytes, hipMemcpyDeviceToHost, f->copyStream));
+    HIPCHECK(hipEventRecord(f->copied, f->copyStream));
+    f->snapSamples = f->samples;
+    f->unread = true;
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e, 3);
+  }
+}
+
+int hasty_films_read(HastyFilms* f, int wait, float* color, float* normal, float* albedo, uint32_t* numSamples) {
+  if (!f || !color || !normal || !albedo || !numSamples) {
+    g_lastError = "null argument";
+    return 1;
+  }
+  try {
+    HIPCHECK(hipSetDevice(f->scene->device));
+    std::lock_guard<std::mutex> lock(f->m);
+    if (!f->unread) return 4;
+    if (!wait) {
+      hipError_t q = hipEventQuery(f->copied);
+      if (q == hipErrorNotReady) return 4;
+      HIPCHECK(q);
+    }
+    HIPCHECK(hipEventSynchronize(f->copied));
+    memcpy(color, f->staging, f->floats * sizeof(float));
+    memcpy(normal, f->staging + f->floats, f->floats * sizeof(float));
+    memcpy(albedo, f->staging + 2 * f->floats, f->floats * sizeof(float));
+    *numSamples = f->snapSamples;
+    f->unread = false;
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e, 3);
+  }
+}
+
+int hasty_films_estimate(HastyFilms* const* films, uint32_t n, float exposure, float* color, float* normal,
+                         float* albedo) {
+  if (!films || n == 0 || !color || !normal || !albedo) {
+    g_lastError = "null argument";
+    return 1;
+  }
+  if (n > 64) {
+    g_lastError = "1 to 64 film sets per estimate";
+    return 1;
+  }
+  try {
+    HastyFilms* f0 = films[0];
+    if (!f0) throw std::runtime_error("null film set");
+    HastyScene* s = f0->scene;
+    HIPCHECK(hipSetDevice(s->device));
+    std::vector<const float*> planes[3];
+    std::vector<uint32_t> counts;
+    for (uint32_t k = 0; k < n; ++k) {
+      HastyFilms* f = films[k];
+      if (!f || f->scene != s) throw std::runtime_error("film sets of one scene");
+      std::lock_guard<std::mutex> lock(f->m);
+      for (int p = 0; p < 3; ++p) planes[p].push_back(f->film + p * f->floats);
+      counts.push_back(f->samples);
+    }
+    std::lock_guard<std::mutex> lock(f0->m);
+    const size_t floats = f0->floats;
+    if (!f0->estimate) HIPCHECK(hipMalloc(&f0->estimate, 3 * floats * sizeof(float)));
+    // PathTracingViewer.cpp:159-162: the colour estimate times 2^exposure
+    const float scale = exposure == 0.0f ? 1.0f : std::pow(2.0f, exposure);
+    for (int p = 0; p < 3; ++p)
+      hasty::launchEstimate(planes[p].data(), counts.data(), (int)n, p == 0 ? scale : 1.0f, floats,
+                            f0->estimate + p * floats, s->stream);
+    float* out[3] = {color, normal, albedo};
+    for (int p = 0; p < 3; ++p)
+      HIPCHECK(hipMemcpyAsync(out[p], f0->estimate + p * floats, floats * sizeof(float), hipMemcpyDeviceToHost,
+                              s->stream));
+    HIPCHECK(hipStreamSynchronize(s->stream));
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e, 3);
+  }
+}
+
+int hasty_films_device(HastyFilms* f, float** color, float** normal, float** albedo, uint32_t* numSamples) {
+  if (!f || !color || !normal || !albedo || !numSamples) {
+    g_lastError = "null argument";
+    return 1;
+  }
+  std::lock_guard<std::mutex> lock(f->m);
+  *color = f->film;
+  *normal = f->film + f->floats;
+  *albedo = f->film + 2 * f->floats;
+  *numSamples = f->samples;
+  return 0;
 }
 
 int hasty_render_pt(HastyScene* s, uint32_t seed, uint32_t first, uint32_t n, float* color, float* normal,

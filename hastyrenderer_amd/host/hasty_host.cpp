@@ -32,6 +32,7 @@ void RenderJob::loadJSON(const std::string& path) {
   renderSettings.maxDepth = info.settings.max_depth;
   renderSettings.numSamples = info.settings.num_samples;
   renderSettings.useMIS = info.settings.use_mis != 0;
+  cameraExposure = info.camera_exposure;
 }
 
 void RenderJob::fetchSamples(int count, int* first, int* n) {
@@ -50,30 +51,56 @@ PMRenderJob::~PMRenderJob() {
 
 // PathTracing.cpp:98-160 in batches of sample indices.
 void renderThread(Image3fAccDoubleBuffer& colorBuffer, Image3fAccDoubleBuffer& normalBuffer,
-                  Image3fAccDoubleBuffer& albedoBuffer, RenderJob& job, RenderThreadData& data, int batch) {
+                  Image3fAccDoubleBuffer& albedoBuffer, RenderJob& job, RenderThreadData& data, int batch,
+                  HastyFilms* films) {
   Image3fAccDoubleBuffer* bufs[3] = {&colorBuffer, &normalBuffer, &albedoBuffer};
   for (auto* b : bufs) {
     std::fill(b->getWriteBuffer().data.begin(), b->getWriteBuffer().data.end(), 0.0f);
     b->getWriteBuffer().numSamples = 0;
   }
+  HastyFilms* own = nullptr;
+  if (!films) {
+    std::lock_guard<std::mutex> lock(job.scene->mutex());
+    if (hasty_films_create(job.scene->handle(), &own) != 0) {
+      fprintf(stderr, "error: %s\n", hasty_last_error());
+      data.stoppedFlag = true;
+      return;
+    }
+    films = own;
+  }
+  // the newest finished snapshot, if any, into the write buffers -> readers
+  auto publish = [&](int wait) {
+    uint32_t k = 0;
+    int rc = hasty_films_read(films, wait, colorBuffer.getWriteBuffer().data.data(),
+                              normalBuffer.getWriteBuffer().data.data(), albedoBuffer.getWriteBuffer().data.data(), &k);
+    if (rc == 0) {
+      for (auto* b : bufs) {
+        b->getWriteBuffer().numSamples = (int)k;
+        b->copyBuffer();
+      }
+    } else if (rc != 4) {
+      fprintf(stderr, "error: %s\n", hasty_last_error());
+    }
+  };
   while (!job.stopFlag) {
     int first, n;
     job.fetchSamples(batch, &first, &n);
     if (n <= 0) break;
     {
       std::lock_guard<std::mutex> lock(job.scene->mutex());
-      int rc = hasty_render_pt(job.scene->handle(), data.seed, (uint32_t)first, (uint32_t)n,
-                               colorBuffer.getWriteBuffer().data.data(), normalBuffer.getWriteBuffer().data.data(),
-                               albedoBuffer.getWriteBuffer().data.data());
+      int rc = hasty_films_render(films, data.seed, (uint32_t)first, (uint32_t)n);
+      if (rc == 0) rc = hasty_films_snapshot(films);
       if (rc != 0) {
         fprintf(stderr, "error: %s\n", hasty_last_error());
         break;
       }
     }
-    for (auto* b : bufs) {
-      b->getWriteBuffer().numSamples += n;
-      b->copyBuffer();
-    }
+    publish(0);
+  }
+  publish(1);  // the last snapshot, once its copy has arrived
+  if (own) {
+    std::lock_guard<std::mutex> lock(job.scene->mutex());
+    hasty_films_destroy(own);
   }
   data.stoppedFlag = true;
 }
@@ -132,6 +159,10 @@ void computeEstimate(const std::vector<std::unique_ptr<Image3fAccDoubleBuffer>>&
   }
   float d = 1e-7f + float(total);
   for (float& x : image) x /= d;
+}
+
+void writeEXR(const std::string& path, const std::vector<float>& rgb, uint32_t width, uint32_t height) {
+  check(hasty_image_write_exr(path.c_str(), width, height, rgb.data()));
 }
 
 void writePFM(const std::string& path, const std::vector<float>& rgb, uint32_t width, uint32_t height) {

@@ -7,6 +7,7 @@
 //   RenderThreadData    core/include/Hasty/PathTracing.h:15-26
 //   DoubleBuffer / AccumulationBuffer  core/include/Hasty/Image.h:18-45, 302-316
 //   computeEstimate     apps/ViewerUtils.cpp:95-121
+//   writeEXR            core/src/Image.cpp:140-152
 // Images are row-major RGB float32 (W*H*3), row 0 at the top.
 #pragma once
 #include <stdint.h>
@@ -58,6 +59,7 @@ class RenderJob {
   virtual ~RenderJob() {}
   void loadJSON(const std::string& jsonFilepath);  // throws std::runtime_error like the reference
   RenderSettings renderSettings;
+  float cameraExposure = 0.0f;  // Camera::getExposure (Camera.h:34), the viewers' display scale 2^exposure
   std::shared_ptr<GpuScene> scene;
   std::atomic<int> sampleIdxGenerator{0};
   std::atomic<bool> stopFlag{false};
@@ -118,11 +120,16 @@ class DoubleBuffer {
 typedef DoubleBuffer<AccumulationBuffer> Image3fAccDoubleBuffer;
 
 // Samples are pulled from the job in batches of `batch` indices (0: all
-// that remain) and rendered in one GPU call each; the write buffers hold
-// sums and numSamples grows by the batch size, as if the reference's loop
-// had run once per index.
+// that remain) and rendered in one GPU call each into device films; after
+// each batch a snapshot of the films is read back asynchronously (the next
+// batch renders meanwhile) and, once it has arrived, published to the
+// buffers (copyBuffer).  The write buffers hold sums and numSamples counts
+// the samples in them, as if the reference's loop had run once per index.
+// With `films` given, the thread renders into that film set (which the
+// caller keeps, e.g. for hasty_films_estimate) instead of its own.
 void renderThread(Image3fAccDoubleBuffer& colorBuffer, Image3fAccDoubleBuffer& normalBuffer,
-                  Image3fAccDoubleBuffer& albedoBuffer, RenderJob& job, RenderThreadData& data, int batch = 0);
+                  Image3fAccDoubleBuffer& albedoBuffer, RenderJob& job, RenderThreadData& data, int batch = 0,
+                  HastyFilms* films = nullptr);
 
 // One photon-mapping pass per fetched sample index; the write buffers hold
 // the current estimate with numSamples = 1.
@@ -133,5 +140,8 @@ void computeEstimate(const std::vector<std::unique_ptr<Image3fAccDoubleBuffer>>&
 
 // Colour PFM (bottom row first, as the format stores it).
 void writePFM(const std::string& path, const std::vector<float>& rgb, uint32_t width, uint32_t height);
+
+// writeEXR (core/src/Image.cpp:140-152) through hasty_image_write_exr.
+void writeEXR(const std::string& path, const std::vector<float>& rgb, uint32_t width, uint32_t height);
 
 }  // namespace Hasty

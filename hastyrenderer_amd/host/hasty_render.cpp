@@ -2,14 +2,17 @@
 // (apps/PathTracingViewer.cpp:20-166, apps/PhotonMappingViewer.cpp):
 //   hasty_render [--sppm] <renderJobFile> <outputDirectory>
 // starts numThreads render threads with seeds 0..numThreads-1, waits for
-// them, and writes output.pfm, outputNormal.pfm and outputAlbedo.pfm (the
-// reference writes EXR through OpenImageIO and then denoises; this renderer
-// links neither library).
+// them, and writes output.exr (colour estimate times 2^exposure),
+// outputNormal.exr and outputAlbedo.exr as postProcessAndSaveToDisk does
+// (apps/ViewerUtils.cpp:137-152; denoising stays outside this renderer).  The
+// path tracer's estimate is computed on the GPU from the threads' device
+// films (hasty_films_estimate).
 #include <stdio.h>
 #include <string.h>
 #include <sys/stat.h>
 
 #include <chrono>
+#include <cmath>
 #include <memory>
 #include <thread>
 #include <vector>
@@ -55,22 +58,47 @@ int main(int argc, char** argv) {
     normal.emplace_back(std::make_unique<Image3fAccDoubleBuffer>(rs.width, rs.height));
     albedo.emplace_back(std::make_unique<Image3fAccDoubleBuffer>(rs.width, rs.height));
   }
+  std::vector<HastyFilms*> films(sppm ? 0 : nThreads, nullptr);
+  for (unsigned t = 0; t < films.size(); ++t) {
+    if (hasty_films_create(job->scene->handle(), &films[t]) != 0) {
+      printf("exception: %s\n", hasty_last_error());
+      return 2;
+    }
+  }
   for (unsigned t = 0; t < nThreads; ++t) {
     if (sppm)
       threads.emplace_back(renderPhotonThread, std::ref(*color[t]), std::ref(*normal[t]), std::ref(*albedo[t]),
                            std::ref(*static_cast<PMRenderJob*>(job.get())), std::ref(*data[t]));
     else
-      threads.emplace_back([&, t] { renderThread(*color[t], *normal[t], *albedo[t], *job, *data[t]); });
+      threads.emplace_back([&, t] { renderThread(*color[t], *normal[t], *albedo[t], *job, *data[t], 0, films[t]); });
   }
   for (auto& th : threads) th.join();
   double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-  std::vector<float> c, n, a;
-  computeEstimate(color, c);
-  computeEstimate(normal, n);
-  computeEstimate(albedo, a);
-  writePFM(out + "/output.pfm", c, rs.width, rs.height);
-  writePFM(out + "/outputNormal.pfm", n, rs.width, rs.height);
-  writePFM(out + "/outputAlbedo.pfm", a, rs.width, rs.height);
+  const size_t P = (size_t)rs.width * rs.height * 3;
+  std::vector<float> c(P), n(P), a(P);
+  if (sppm) {
+    computeEstimate(color, c);
+    const float scale = std::pow(2.0f, job->cameraExposure);  // PathTracingViewer.cpp:160
+    for (float& x : c) x *= scale;
+    computeEstimate(normal, n);
+    computeEstimate(albedo, a);
+  } else {
+    int rc = hasty_films_estimate(films.data(), (uint32_t)films.size(), job->cameraExposure, c.data(), n.data(),
+                                  a.data());
+    for (HastyFilms* f : films) hasty_films_destroy(f);
+    if (rc != 0) {
+      printf("exception: %s\n", hasty_last_error());
+      return 5;
+    }
+  }
+  try {
+    writeEXR(out + "/output.exr", c, rs.width, rs.height);
+    writeEXR(out + "/outputNormal.exr", n, rs.width, rs.height);
+    writeEXR(out + "/outputAlbedo.exr", a, rs.width, rs.height);
+  } catch (const std::exception& e) {
+    printf("exception: %s\n", e.what());
+    return 5;
+  }
   printf("rendered %d %s of %ux%u in %.3f s\n", rs.numSamples, sppm ? "passes" : "samples", rs.width, rs.height,
          secs);
   return 0;

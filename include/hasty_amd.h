@@ -49,11 +49,13 @@ typedef struct HastySceneInfo {
   int32_t bvh_depth;       /* its inner levels */
   uint64_t device_bytes;   /* scene + BVH bytes resident in HBM */
   int32_t bvh_width;       /* children per node: 4 or 8 */
+  float camera_exposure;   /* camera "exposure" (Camera.cpp:22): display scale 2^exposure */
 } HastySceneInfo;
 
 /* Replaces RenderJob::loadJSON (core/src/RenderJob.cpp:21-28) together with
  * Scene::Scene and its rtcCommitScene (core/src/Scene.cpp:12-195): parses
- * renderJob.json -> scene JSON -> .obj/.mtl (+ PFM environment map), builds
+ * renderJob.json -> scene JSON -> .obj/.mtl (+ environment map and image
+ * textures in PFM, PPM / PGM, PNG or OpenEXR), builds
  * the 4-wide BVH on the host and uploads everything to the current device.
  * Errors use the reference's messages (malformed input, a face without a
  * material, an emissive face with a non-black Kd/Ks, ...). */
@@ -398,6 +400,51 @@ int hasty_sppm_radii(HastySppm* job, float* out);
 /* Photons of the last pass in emission order, 10 floats each: position,
  * dot(n_g, w_i), w_i, flux (Photon, PhotonMapping.cpp:11-23). */
 int hasty_sppm_photons(HastySppm* job, float* out, uint64_t max_photons);
+
+/* ---- the output step (apps/ViewerUtils.cpp:15-152) on device films ----
+ * computeEstimate (ViewerUtils.cpp:95-121): the films, each W*H*3 floats of
+ * the scene's size holding sums of num_samples[k] samples, added in order
+ * from the first film with samples (the reference replaces its image until
+ * the running total is nonzero), divided by 1e-7 + the total, and scaled by
+ * 2^exposure as the viewers scale the colour estimate
+ * (PathTracingViewer.cpp:120, 160; pass 0 for normal / albedo films).
+ * Device pointers on the scene's device; 1 <= n_films <= 64. */
+int hasty_estimate_device(HastyScene* scene, const float* const* d_films, const uint32_t* num_samples,
+                          uint32_t n_films, float exposure, float* d_out);
+/* preparePresentableImage (ViewerUtils.cpp:15-65): each pixel of a W*H*3
+ * image as R << 24 | G << 16 | B << 8 | 255 with channel
+ * uint(255 * pow(clamp(v, 0, 1), 1/2.2)). */
+int hasty_present_device(HastyScene* scene, const float* d_image, uint32_t* d_rgba);
+
+/* Device films with snapshots for progressive display -- the renderThread /
+ * DoubleBuffer pattern (PathTracing.cpp:98-160, core/include/Hasty/Image.h:
+ * 18-45) with the GPU holding the sums:
+ *   hasty_films_render adds samples like hasty_render_pt_device;
+ *   hasty_films_snapshot copies the three films on the scene's stream after
+ *     the work queued so far and starts their copy to pinned host memory on
+ *     a second stream, so the next render overlaps the readback;
+ *   hasty_films_read hands the last finished snapshot to host buffers (the
+ *     DoubleBuffer's write buffer) with its sample count: 0 = copied, 4 =
+ *     nothing new, or (wait == 0) its copy has not finished;
+ *   hasty_films_device returns the films' device pointers (for
+ *     hasty_estimate_device) and the samples added so far.
+ * A film set belongs to one scene and must not outlive it; calls on one film
+ * set from several threads are serialised inside. */
+typedef struct HastyFilms HastyFilms;
+int hasty_films_create(HastyScene* scene, HastyFilms** out_films);
+void hasty_films_destroy(HastyFilms* films);
+int hasty_films_render(HastyFilms* films, uint32_t seed, uint32_t first_sample, uint32_t num_samples);
+int hasty_films_snapshot(HastyFilms* films);
+int hasty_films_read(HastyFilms* films, int wait, float* color, float* normal, float* albedo,
+                     uint32_t* num_samples);
+int hasty_films_device(HastyFilms* films, float** d_color, float** d_normal, float** d_albedo,
+                       uint32_t* num_samples);
+/* The viewers' last step (PathTracingViewer.cpp:157-162) over the film sets
+ * of one scene, one per render thread: computeEstimate of their colour,
+ * normal and albedo films on the GPU (colour times 2^exposure), copied to
+ * host buffers of W*H*3 floats (for writeEXR, hasty_image_write_exr). */
+int hasty_films_estimate(HastyFilms* const* films, uint32_t n_films, float exposure, float* color, float* normal,
+                         float* albedo);
 
 /* Image files, on the host (no device work): reads a PFM, PPM / PGM, PNG or
  * OpenEXR file into float pixels, row 0 at the top, channels interleaved --

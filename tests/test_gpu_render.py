@@ -233,15 +233,17 @@ def test_headless_cli_matches_library(cornell_job, caustic_small_job, tmp_path):
     out.mkdir()
     r = subprocess.run([cli, cornell_job, str(out)], capture_output=True, text=True, timeout=300)
     assert r.returncode == 0, r.stdout + r.stderr
-    img = scenes.read_pfm(str(out / "output.pfm"))
+    img = ha.read_image(str(out / "output.exr"))
     info = ha.Scene(cornell_job).info()
     c, n, a = ha.Scene(cornell_job).render_pt(seed=0, first_sample=0, num_samples=info["num_samples"])
     np.testing.assert_allclose(img, c / np.float32(1e-7 + info["num_samples"]), rtol=1e-6, atol=1e-9)
+    np.testing.assert_allclose(ha.read_image(str(out / "outputAlbedo.exr")),
+                               a / np.float32(1e-7 + info["num_samples"]), rtol=1e-6, atol=1e-9)
     out2 = tmp_path / "pm"
     out2.mkdir()
     r = subprocess.run([cli, "--sppm", caustic_small_job, str(out2)], capture_output=True, text=True, timeout=300)
     assert r.returncode == 0, r.stdout + r.stderr
-    pm = scenes.read_pfm(str(out2 / "output.pfm"))
+    pm = ha.read_image(str(out2 / "output.exr"))
     job = ha.PMRenderJob()
     job.loadJSON(caustic_small_job)
     st = job.renderSettings
