@@ -437,8 +437,14 @@ def bench_guide(args):
 
     plain = ha.Scene(job_path)
     info = plain.info()
-    ref, tref = render(plain, 101, args.guide_ref_spp, chunk=1024)
-    log("reference: %d spp in %.1f s" % (args.guide_ref_spp, tref))
+    if args.guide_ref_file and os.path.exists(args.guide_ref_file):
+        ref = np.load(args.guide_ref_file)
+        log("reference: %s" % args.guide_ref_file)
+    else:
+        ref, tref = render(plain, 101, args.guide_ref_spp, chunk=1024)
+        log("reference: %d spp in %.1f s" % (args.guide_ref_spp, tref))
+        if args.guide_ref_file:
+            np.save(args.guide_ref_file, ref)
     n = args.guide_spp
     chunk = 1024 if n > 1024 else 0  # progress lines for long renders
     pc, tp = render(plain, 5, n, chunk=chunk)
@@ -450,10 +456,12 @@ def bench_guide(args):
     f = ha.DeviceFilms(info["width"], info["height"])
     hip.sync()
     t = time.perf_counter()
-    gsc.set_guide_training(minibatch=args.guide_minibatch)
-    gsc.set_guide(g, fraction=0.5, train_lr=args.guide_lr)  # online training
+    gsc.set_guide_training(minibatch=args.guide_minibatch, weight_cap=args.guide_weight_cap)
+    gsc.set_guide_photons(args.guide_photons, args.guide_photon_share)
+    gsc.set_guide_depth(args.guide_max_depth, args.guide_max_throughput)
+    gsc.set_guide(g, fraction=args.guide_fraction, train_lr=args.guide_lr)  # online training
     gsc.render_pt_device(5, 0, ntrain, *f.ptrs)
-    gsc.set_guide(g, fraction=0.5, train_lr=0.0)  # trained guide for the remaining samples
+    gsc.set_guide(g, fraction=args.guide_fraction, train_lr=0.0)  # trained guide for the remaining samples
     step = chunk if chunk > 0 else n
     for first in range(ntrain, n, step):
         gsc.render_pt_device(5, first, min(step, n - first), *f.ptrs)
@@ -465,6 +473,9 @@ def bench_guide(args):
     gc = f.download()[0] / n
     f.close()
     eg = relmse(gc, ref)
+    if args.guide_save:
+        np.save(args.guide_save + "_plain.npy", pc)
+        np.save(args.guide_save + "_guided.npy", gc)
     loss, ns = gsc.guide_stats()
     log("guided (%d spp online training + %d spp): %.3f s, relMSE %.5f, last loss %.4f on %d samples" % (
         ntrain, n - ntrain, tg, eg, loss, ns))
@@ -485,8 +496,10 @@ def bench_guide(args):
         "config": {"workload": "metal_ring_box_guided" if args.guide_scene == "ring" else "indirect_room_guided",
                    "width": info["width"], "height": info["height"],
                    "max_depth": info["max_depth"], "triangles": info["num_triangles"], "spp": n,
-                   "reference_spp": args.guide_ref_spp, "train_spp": ntrain, "guide_fraction": 0.5,
-                   "minibatch": args.guide_minibatch,
+                   "reference_spp": args.guide_ref_spp, "train_spp": ntrain, "guide_fraction": args.guide_fraction,
+                   "minibatch": args.guide_minibatch, "weight_cap": args.guide_weight_cap,
+                   "photons_per_step": args.guide_photons, "photon_share": args.guide_photon_share,
+                   "guided_depths": args.guide_max_depth, "guided_max_throughput": args.guide_max_throughput,
                    "adam_lr": args.guide_lr,
                    "parallelism": "single-gpu"},
         "plain": {"seconds": tp, "relmse": ep, "relmse_without_worst_0.1pct_pixels": relmse_trimmed(pc, ref)},
@@ -522,7 +535,19 @@ def main():
     ap.add_argument("--guide-scene", choices=("ring", "room"), default="ring",
                     help="ring: the metal-ring caustic box; room: a Cornell room lit from behind its light")
     ap.add_argument("--guide-lr", type=float, default=1e-3, help="Adam learning rate of the online training")
-    ap.add_argument("--guide-minibatch", type=int, default=16384, help="training samples per Adam step")
+    ap.add_argument("--guide-minibatch", type=int, default=65536, help="training samples per Adam step")
+    ap.add_argument("--guide-weight-cap", type=float, default=100.0, help="cap of the normalised training weights")
+    ap.add_argument("--guide-photons", type=int, default=4194304,
+                    help="caustic light paths per training step as extra training samples (0: none)")
+    ap.add_argument("--guide-photon-share", type=float, default=0.5,
+                    help="photon samples' weight sum relative to the camera samples'")
+    ap.add_argument("--guide-max-depth", type=int, default=255, help="guide only vertices of depth below this")
+    ap.add_argument("--guide-fraction", type=float, default=0.5, help="probability of sampling the guide at a vertex")
+    ap.add_argument("--guide-max-throughput", type=float, default=3.0e38,
+                    help="guide a vertex only while the path throughput is at most this")
+    ap.add_argument("--guide-save", default="", help="save the plain and guided images as PREFIX_plain.npy / _guided.npy")
+    ap.add_argument("--guide-ref-file", default="",
+                    help="reference image (.npy): loaded when it exists, else rendered and saved there")
     ap.add_argument("--guide-ref-spp", type=int, default=None,
                     help="reference samples (default: 65536 ring, 16384 room)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",

@@ -72,6 +72,8 @@ EXPORTS = (
     "hasty_guide_train_step",
     "hasty_scene_set_guide",
     "hasty_scene_set_guide_training",
+    "hasty_scene_set_guide_photons",
+    "hasty_scene_set_guide_depth",
     "hasty_scene_guide_stats",
     "hasty_sppm_create",
     "hasty_sppm_destroy",
@@ -307,6 +309,8 @@ def lib():
     L.hasty_guide_train_step.argtypes = [P, fp, C.c_uint32, C.c_float, fp]
     L.hasty_scene_set_guide.argtypes = [P, P, C.c_float, C.c_float]
     L.hasty_scene_set_guide_training.argtypes = [P, C.c_uint32, C.c_float]
+    L.hasty_scene_set_guide_photons.argtypes = [P, C.c_uint32, C.c_float]
+    L.hasty_scene_set_guide_depth.argtypes = [P, C.c_uint32, C.c_float]
     L.hasty_scene_guide_stats.argtypes = [P, fp, C.POINTER(C.c_uint32)]
     L.hasty_sppm_create.argtypes = [P, C.c_uint32, C.POINTER(SppmParamsC), C.POINTER(P)]
     L.hasty_sppm_destroy.argtypes = [P]
@@ -506,6 +510,16 @@ class Scene:
 
     def set_guide_training(self, minibatch=16384, weight_cap=20.0):
         _err(lib().hasty_scene_set_guide_training(self._h, minibatch, weight_cap))
+
+    def set_guide_photons(self, photons=0, share=0.5):
+        """Caustic light paths per training step as extra guide training
+        samples (0 turns them off), weighted share x the camera samples."""
+        _err(lib().hasty_scene_set_guide_photons(self._h, photons, float(share)))
+
+    def set_guide_depth(self, max_depth=255, max_throughput=3.0e38):
+        """Guide only vertices of depth below max_depth while the path
+        throughput (every channel) is at most max_throughput."""
+        _err(lib().hasty_scene_set_guide_depth(self._h, int(max_depth), float(max_throughput)))
 
     def guide_stats(self):
         loss = np.zeros(1, np.float32)

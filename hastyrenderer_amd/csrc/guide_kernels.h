@@ -38,6 +38,9 @@ class Guide {
   // normalised over all n first.  Returns the mean loss of the last step.
   float trainEpoch(float4* dSamples, uint32_t n, float lr, float weightCap, uint32_t minibatch,
                    uint32_t minSteps = 1);
+  // Sum of n samples' weights; and F <- F * factor for n samples.
+  float weightSum(const float4* dSamples, uint32_t n);
+  void scaleWeights(float4* dSamples, uint32_t n, float factor);
   void getGrad(float* host);
   void zeroGrad();
   uint32_t steps() const { return step_; }

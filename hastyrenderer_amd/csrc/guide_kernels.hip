@@ -430,13 +430,25 @@ void Guide::adamStep(float lr) {
   hipOk(hipGetLastError(), "kAdam");
 }
 
-// Scales the weights to mean 1 and caps them; false if they sum to zero.
-float Guide::normaliseWeights(float4* dSamples, uint32_t n, float weightCap) {
+float Guide::weightSum(const float4* dSamples, uint32_t n) {
+  if (n == 0) return 0.0f;
   kZero<<<1, 64, 0, stream_>>>(dLoss_, 1);
   kWeightSum<<<256, 256, 0, stream_>>>(dSamples, n, dLoss_);
   float sum = 0.0f;
   hipOk(hipMemcpyAsync(&sum, dLoss_, sizeof(float), hipMemcpyDeviceToHost, stream_), "copy");
   hipOk(hipStreamSynchronize(stream_), "sync");
+  return sum;
+}
+
+void Guide::scaleWeights(float4* dSamples, uint32_t n, float factor) {
+  if (n == 0) return;
+  kWeightScale<<<256, 256, 0, stream_>>>(dSamples, n, factor, 3.0e38f);
+  hipOk(hipGetLastError(), "kWeightScale");
+}
+
+// Scales the weights to mean 1 and caps them; false if they sum to zero.
+float Guide::normaliseWeights(float4* dSamples, uint32_t n, float weightCap) {
+  const float sum = weightSum(dSamples, n);
   if (!(sum > 0.0f)) return 0.0f;
   kWeightScale<<<256, 256, 0, stream_>>>(dSamples, n, (float)n / sum, weightCap);
   return sum;

@@ -210,6 +210,12 @@ struct HastyScene {
   float guideLr = 0.0f;
   uint32_t guideMinibatch = 16384;
   float guideWeightCap = 20.0f;
+  uint32_t guideMaxDepth = 255;    // vertices of depth below this are guided
+  float guideMaxThroughput = 3.0e38f;  // ... while the path throughput is at most this
+  uint32_t guidePhotons = 0;       // caustic light paths traced per training step
+  float guidePhotonShare = 0.5f;   // their samples' weight sum / the camera samples'
+  uint32_t guidePhotonPass = 0;    // light-path pass counter (random streams)
+  uint32_t lastGuidePhotonSamples = 0;
   float4* guideRaw = nullptr;
   float4* guideRec = nullptr;
   float4* guideSamples = nullptr;
@@ -550,6 +556,8 @@ static void renderGuided(HastyScene* s, const BatchItem& it) {
   const bool training = s->guideLr > 0.0f;
   gr.sampleCap = training ? HastyScene::kGuideSampleCap : 0u;
   gr.fraction = s->guideFraction;
+  gr.maxDepth = s->guideMaxDepth;
+  gr.maxThroughput = s->guideMaxThroughput;
   s->launch.countStats = s->counting;
   if (s->profiling && !s->timer) s->timer = std::make_unique<EventTimer>();
   hasty::KernelTimer* timer = s->profiling ? s->timer.get() : nullptr;
@@ -578,6 +586,32 @@ static void renderGuided(HastyScene* s, const BatchItem& it) {
       uint32_t n = 0;
       HIPCHECK(hipMemcpy(&n, s->guideCount, sizeof(uint32_t), hipMemcpyDeviceToHost));
       n = std::min(n, HastyScene::kGuideSampleCap);
+      if (s->guidePhotons > 0 && n > 0 && n < HastyScene::kGuideSampleCap && s->dev.hasSurfaceLight) {
+        // caustic light paths append their samples after the camera samples,
+        // scaled so that they carry guidePhotonShare of the camera weight
+        const float camSum = s->guide->g->weightSum(s->guideSamples, n);
+        HIPCHECK(hipStreamSynchronize(s->stream));
+        HIPCHECK(hipMemcpy(s->guideCount, &n, sizeof(uint32_t), hipMemcpyHostToDevice));
+        hdev::DevScene ds = s->dev;
+        ds.seed = it.seed;
+        hasty::launchGuidePhotons(ds, s->guidePhotonPass++, s->guidePhotons, s->pb.stackOverflow,
+                                  s->pb.traceThreads, s->guideSamples, s->guideCount, HastyScene::kGuideSampleCap,
+                                  s->stream);
+        HIPCHECK(hipGetLastError());
+        uint32_t all = 0;
+        HIPCHECK(hipStreamSynchronize(s->stream));
+        HIPCHECK(hipMemcpy(&all, s->guideCount, sizeof(uint32_t), hipMemcpyDeviceToHost));
+        all = std::min(all, HastyScene::kGuideSampleCap);
+        float4* ph = s->guideSamples + 4 * (size_t)n;
+        const float phSum = s->guide->g->weightSum(ph, all - n);
+        if (phSum > 0.0f && camSum > 0.0f) {
+          s->guide->g->scaleWeights(ph, all - n, s->guidePhotonShare * camSum / phSum);
+          s->lastGuidePhotonSamples = all - n;
+          n = all;
+        } else {
+          s->lastGuidePhotonSamples = 0;
+        }
+      }
       s->lastGuideSamples = n;
       if (n > 0)
         s->lastGuideLoss =
@@ -1855,6 +1889,34 @@ int hasty_scene_set_guide_training(HastyScene* s, uint32_t minibatch, float weig
   }
   s->guideMinibatch = minibatch;
   s->guideWeightCap = weightCap;
+  return 0;
+}
+
+int hasty_scene_set_guide_depth(HastyScene* s, uint32_t maxDepth, float maxThroughput) {
+  if (!s) {
+    g_lastError = "null argument";
+    return 1;
+  }
+  if (!(maxThroughput > 0.0f)) {
+    g_lastError = "the throughput limit must be positive";
+    return 1;
+  }
+  s->guideMaxDepth = maxDepth;
+  s->guideMaxThroughput = maxThroughput;
+  return 0;
+}
+
+int hasty_scene_set_guide_photons(HastyScene* s, uint32_t photons, float share) {
+  if (!s) {
+    g_lastError = "null argument";
+    return 1;
+  }
+  if (photons > (1u << 26) || !(share > 0.0f && share < 1e6f)) {
+    g_lastError = "at most 2^26 light paths per training step and a positive share";
+    return 1;
+  }
+  s->guidePhotons = photons;
+  s->guidePhotonShare = share;
   return 0;
 }
 

@@ -88,6 +88,8 @@ struct GuideRun {
   uint32_t* sampleCount;
   uint32_t sampleCap;
   float fraction;   // probability of drawing the direction from the guide
+  uint32_t maxDepth;  // only vertices of depth below this are guided
+  float maxThroughput;  // ... and only while every channel of the path throughput is at most this
 };
 
 // The jobs of a render call.  With one job its scene and pass travel in the
@@ -192,6 +194,13 @@ static_assert(sizeof(HitRecord) == 32, "HitRecord is 32 bytes");
 // may use (a multiple of kBlock).
 void launchTraceHits(const hdev::DevScene& scene, const float* dRays, uint32_t n, HitRecord* dHits, uint2* dOverflow,
                      uint32_t overflowThreads, hipStream_t stream);
+
+// Caustic photons as guide training samples (4 float4 each, appended at
+// atomicAdd(sampleCount) below cap) from `count` light paths of pass `pass`;
+// `overflow` as for launchTraceHits.  Nothing without surface lights.
+void launchGuidePhotons(const hdev::DevScene& scene, uint32_t pass, uint32_t count, uint2* dOverflow,
+                        uint32_t overflowThreads, float4* samples, uint32_t* sampleCount, uint32_t cap,
+                        hipStream_t stream);
 
 // Closest hit for n rays (6 floats each): tri, t, u, v.
 void launchTraceClosest(const hdev::DevScene& scene, const float* dRays, uint32_t n, int32_t* dTri, float* dTuv,

@@ -433,6 +433,13 @@ constexpr float kGuideMinAlpha = 0.25f;
 __device__ inline bool guideEligible(const DevMaterial& m) {
   return !m.isLight && m.transmission == 0.0f && (m.tex[1] >= 0 || m.alpha >= kGuideMinAlpha);
 }
+// A path whose throughput has grown past g.maxThroughput samples its BSDF
+// alone from then on.  Which technique samples a vertex may depend on the
+// path so far without bias; this keeps the guide's 1 / (1 - fraction)
+// weights from compounding along long chains of guided vertices.
+__device__ inline bool guideThroughputOk(const GuideRun& g, V3 beta) {
+  return fmaxf(beta.x, fmaxf(beta.y, beta.z)) <= g.maxThroughput;
+}
 
 template <bool MULTI, bool GUIDED>
 __global__ void __launch_bounds__(kBlock) HASTY_SHADE_ATTR kShade(JobSet js, PathBuffers pb) {
@@ -518,7 +525,8 @@ __global__ void __launch_bounds__(kBlock) HASTY_SHADE_ATTR kShade(JobSet js, Pat
           // guided vertex: rough reflection (no delta lobe, and not so glossy
           // that broad vMF lobes could only waste samples); decided on the
           // material, before its textures, as kGuideInfer decides it
-          const bool guided = GUIDED && guideEligible(m);
+          const bool guided = GUIDED && guideEligible(m) && (uint32_t)depth < js.g.maxDepth &&
+                              (depth == 0 || guideThroughputOk(js.g, beta));
           if (depth == 0) pb.auxN[slot] = f4(it.ns, 0.0f);  // the default shading normal
           uint32_t texel = 0u;
           if (m.textured) applyTextures(s, m, it, 1.0f - h4.y - h4.z, h4.y, h4.z, &texel);
@@ -863,7 +871,9 @@ __global__ void __launch_bounds__(kGuideInferThreads) HASTY_WAVES_ATTR_(HASTY_GU
       if (tri >= 0) {
         // only hits on guided materials read the outputs
         const DevMaterial m = ldgStruct(s.mats + ldg(s.triMat + tri));
-        want = guideEligible(m);
+        const uint32_t depth = pb.state[slot].y & 0xFFu;
+        want = guideEligible(m) && depth < js.g.maxDepth;
+        if (want && depth > 0) want = guideThroughputOk(js.g, f4xyz(pb.beta[slot]));
         if (want) {
           hitPointNormal(s, tri, h4.y, h4.z, &x, &ng);
           wo = -f4xyz(pb.rayD[slot]);
@@ -969,6 +979,99 @@ __global__ void __launch_bounds__(kBlock) kTraceHits(DevScene s, const float* ra
     float4* out = reinterpret_cast<float4*>(hits + q);
     out[0] = make_float4(__uint_as_float(rec.geomID), __uint_as_float(rec.primID), rec.t, rec.u);
     out[1] = make_float4(rec.v, rec.ng[0], rec.ng[1], rec.ng[2]);
+  }
+}
+
+// ------------------------------------------------------ guide photons ----
+// Caustic light paths as training samples for the guide, the photon-traced
+// training of parametric guiding mixtures (Vorba et al. 2014, "On-line
+// learning of parametric mixture models for light transport simulation").
+// Light paths leave the surface lights as Scene::sampleLightRay emits them
+// (Scene.cpp:527-537) and continue through concentrated lobes only, with the
+// roulette of tracePhoton (PhotonMapping.cpp:191-265).  Where one lands on a
+// guided receiver with a diffuse lobe after at least one such bounce it adds
+// a sample in the camera samples' layout: (x, F = luminance of the flux),
+// the normal facing the photon, an outgoing direction drawn around that
+// normal (a diffuse receiver's incident light does not depend on it), and
+// the direction the photon came from.  Photon t of pass p uses the stream
+// (seed, t, p) of its own domain.
+__global__ void __launch_bounds__(kBlock) kGuidePhotons(DevScene s, uint32_t pass, uint32_t count, uint2* overflow,
+                                                       float4* samples, uint32_t* sampleCount, uint32_t cap) {
+  __shared__ uint2 stack[kStackLds * kBlock];
+  const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
+  uint2* ovf = overflow + (size_t)gtid * kStackOvf;
+  const float minQ = 1.0f - hm::powf_pos(0.1f, 1.0f / (float)s.maxDepth);
+  for (uint32_t t = gtid; t < count; t += gridDim.x * blockDim.x) {
+    hrng::Stream rng = hrng::make_stream(s.seed, t, pass, hrng::kDomainGuidePhoton);
+    const uint32_t li = lightCdfSearch(s, rng.canonical_double());
+    const int32_t ltri = (int32_t)ldg(s.lightTris + li);
+    const float* pp = s.triPos + (size_t)ltri * 9;
+    const V3 p0 = load3g(pp), p1 = load3g(pp + 3), p2 = load3g(pp + 6);
+    const V3 ngL = normalize(cross(p1 - p0, p2 - p0));
+    const float su = rng.uniform01f();
+    const float tt = hm::fsqrt(rng.uniform01f());
+    const float b0 = 1.0f - tt, b2 = tt * su, b1 = fmaxf(0.0f, 1.0f - b0 - b2);
+    const Interaction itL = makeInteractionB(s, ltri, b0, b1, b2, ngL);
+    float pDir;
+    V3 rd = sampleCosHemisphere(rng, itL.ng, &pDir);
+    const DevMaterial lm = ldgStruct(s.mats + itL.mat);
+    const V3 Le = v3(lm.emission[0], lm.emission[1], lm.emission[2]) * (float)kInvPi;
+    V3 flux = Le * (__builtin_fabsf(dot(itL.ng, rd)) * s.totalLightArea / pDir);
+    V3 ro = offsetOrigin(itL, true);
+    Hit h = traceClosest<kStackLds, kStackOvf>(s, ro, rd, stack + threadIdx.x, kBlock, ovf, nullptr, nullptr);
+    if (h.tri < 0) continue;
+    Media md;
+    md.n = 0;
+    md.overflow = false;
+    md.tx0 = 0u;
+    int wavelength = -1;
+    for (int depth = 0; depth < s.maxDepth; ++depth) {
+      if (!(__builtin_isfinite(flux.x) && __builtin_isfinite(flux.y) && __builtin_isfinite(flux.z))) break;
+      Interaction it = makeInteraction(s, h.tri, h.u, h.v);
+      DevMaterial m = ldgStruct(s.mats + it.mat);
+      if (m.isLight) break;
+      uint32_t texel = 0u;
+      if (m.textured) applyTextures(s, m, it, 1.0f - h.u - h.v, h.u, h.v, &texel);
+      const V3 wo = -rd;
+      if (depth > 0 && m.hasDiffuseLobe && guideEligible(m)) {
+        const V3 nf = dot(it.ng, wo) >= 0.0f ? it.ng : -it.ng;
+        float pd;
+        const V3 wos = sampleCosHemisphere(rng, nf, &pd);
+        const float F = 0.2126f * flux.x + 0.7152f * flux.y + 0.0722f * flux.z;
+        if (F > 0.0f && F < 3.0e38f) {
+          const uint32_t k = atomicAdd(sampleCount, 1u);
+          if (k < cap) {
+            samples[4 * (size_t)k] = f4(it.x, F);
+            samples[4 * (size_t)k + 1] = f4(nf, 0.0f);
+            samples[4 * (size_t)k + 2] = f4(wos, 0.0f);
+            samples[4 * (size_t)k + 3] = f4(wo, 0.0f);
+          }
+        }
+      }
+      float sp = 1.0f;
+      if (wavelength == -1 && m.varyIOR) {
+        const float q0 = 1.0f / 3.0f;
+        const float xi = rng.uniform01f();
+        wavelength = xi < q0 ? 0 : (xi < q0 + q0 ? 1 : 2);
+        sp = q0;
+      }
+      const float outsideIOR = mediaOutsideIOR(s, md, it, wo, wavelength);
+      BsdfSample bs = bsdfSample(m, rng, it, wo, wavelength, outsideIOR, true, 2);
+      V3 thr = (bs.tD + bs.tC) / sp;
+      if (isZero(thr)) break;
+      const V3 o2 = offsetOrigin(it, bs.outside);
+      const Hit h2 = traceClosest<kStackLds, kStackOvf>(s, o2, bs.dir, stack + threadIdx.x, kBlock, ovf, nullptr,
+                                                        nullptr);
+      if (h2.tri < 0 || h2.tri == it.tri) break;
+      mediaUpdate(md, it, wo, bs.dir, texel, nullptr);
+      const Interaction it2 = makeInteraction(s, h2.tri, h2.u, h2.v);
+      thr = mul(thr, beersLaw(mediaTransmittance(s, md), norm(it2.x - o2)));
+      const float rq = fmaxf(minQ, 1.0f - norm(thr));
+      if (rng.uniform01f() <= rq) break;
+      flux = mul(flux, thr) / (1.0f - rq);
+      h = h2;
+      rd = bs.dir;
+    }
   }
 }
 
@@ -1192,6 +1295,16 @@ void launchTraceHits(const DevScene& s, const float* dRays, uint32_t n, HitRecor
   if (blocks > maxBlocks) blocks = maxBlocks;
   if (blocks == 0) return;
   kTraceHits<<<blocks, kBlock, 0, stream>>>(s, dRays, n, dHits, dOverflow);
+}
+
+void launchGuidePhotons(const DevScene& s, uint32_t pass, uint32_t count, uint2* dOverflow, uint32_t overflowThreads,
+                        float4* samples, uint32_t* sampleCount, uint32_t cap, hipStream_t stream) {
+  if (count == 0 || !s.hasSurfaceLight || s.numLights == 0) return;
+  uint32_t blocks = (count + kBlock - 1) / kBlock;
+  const uint32_t maxBlocks = overflowThreads / kBlock;
+  if (blocks > maxBlocks) blocks = maxBlocks;
+  if (blocks == 0) return;
+  kGuidePhotons<<<blocks, kBlock, 0, stream>>>(s, pass, count, dOverflow, samples, sampleCount, cap);
 }
 
 void launchTraceClosest(const DevScene& s, const float* dRays, uint32_t n, int32_t* dTri, float* dTuv, uint2* dOverflow,

@@ -38,6 +38,7 @@ enum Domain : uint32_t {
   kDomainPath = 1,     // camera + path-tracing vertices
   kDomainPhoton = 2,   // one emitted photon
   kDomainRadius = 3,   // SPPM initial radius probe
+  kDomainGuidePhoton = 4,  // a light path training the path guide
 };
 
 struct U32x4 {

@@ -335,6 +335,19 @@ int hasty_scene_set_guide(HastyScene* scene, HastyGuide* guide, float fraction, 
 /* Online-training minibatch (default 16384, in [16, 2^21]) and weight cap
  * (default 20).  No reference counterpart. */
 int hasty_scene_set_guide_training(HastyScene* scene, uint32_t minibatch, float weight_cap);
+/* Photon-traced training samples (no counterpart in the reference): after
+ * each training step's camera paths, `photons` light paths leave the surface
+ * lights through concentrated lobes (as tracePhoton, PhotonMapping.cpp:
+ * 191-265); each landing on a guided diffuse receiver after such a bounce
+ * adds a sample (its position, the direction the light came from, weight =
+ * flux luminance), the photon samples scaled to `share` x the camera
+ * samples' weight sum.  photons = 0 (the default) turns them off. */
+int hasty_scene_set_guide_photons(HastyScene* scene, uint32_t photons, float share);
+/* Only path vertices of depth below max_depth are guided (default 255: all),
+ * and only while every channel of the path throughput is at most
+ * max_throughput (default 3e38: no limit); other vertices sample the BSDF
+ * alone and skip the network. */
+int hasty_scene_set_guide_depth(HastyScene* scene, uint32_t max_depth, float max_throughput);
 /* loss and sample count of the last online training step */
 int hasty_scene_guide_stats(const HastyScene* scene, float* last_loss, uint32_t* last_samples);
 

@@ -64,3 +64,70 @@ def test_trained_guide_lowers_error(room_job, room_reference):
     print("relMSE at %d spp: plain %.5f, guided %.5f (x%.2f)" % (spp, e_plain, e_guided, e_plain / e_guided),
           flush=True)
     assert e_guided < e_plain
+
+
+def _trimmed(img, ref, keep=0.999):
+    e = np.sort(np.mean((img - ref) ** 2 / (ref ** 2 + 1e-2), axis=-1).ravel())
+    return float(np.mean(e[: int(e.size * keep)]))
+
+
+@pytest.fixture(scope="module")
+def ring_job(tmp_path_factory):
+    """The metal-ring box (dispersive prism, GGX gold torus, 1 % ceiling
+    light) at 128^2, depth 32, with a reduced torus."""
+    return scenes.write_caustic_scene(str(tmp_path_factory.mktemp("ring")), width=128, height=128, passes=64,
+                                      n_major=192, n_minor=48, max_depth=32)
+
+
+def _photon_guided(job, train_spp=256):
+    sc = ha.Scene(job)
+    g = ha.Guide(sc, seed=0)
+    sc.set_guide_training(minibatch=16384, weight_cap=100.0)
+    sc.set_guide_photons(1 << 21, 0.5)
+    sc.set_guide(g, fraction=0.5, train_lr=1e-3)
+    sc.render_pt(seed=77, first_sample=0, num_samples=train_spp)  # online training with caustic photons
+    loss, ns = sc.guide_stats()
+    assert ns > 0 and np.isfinite(loss)
+    sc.set_guide(g, fraction=0.5, train_lr=0.0)
+    return sc, g
+
+
+def test_photon_trained_guide_unbiased_and_gains(ring_job):
+    """Caustic light paths as training samples: the guided image agrees with
+    the plain tracer's in image mean and 4 x 4 region means (batches give
+    the Monte Carlo error), and at equal samples its error outside the worst
+    0.1 % of pixels is lower than the plain tracer's and than a guide
+    trained without the photons."""
+    plain = ha.Scene(ring_job)
+    ref, _, _ = plain.render_pt(seed=101, first_sample=0, num_samples=16384)
+    ref /= 16384
+    sc, g = _photon_guided(ring_job)
+    batches, spp = 8, 256
+    pm, gm = [], []
+    for b in range(batches):
+        pm.append(plain.render_pt(seed=21, first_sample=b * spp, num_samples=spp)[0] / spp)
+        gm.append(sc.render_pt(seed=22, first_sample=b * spp, num_samples=spp)[0] / spp)
+    reg = lambda x: np.array([[x[i * 32:(i + 1) * 32, j * 32:(j + 1) * 32].mean() for j in range(4)]
+                              for i in range(4)])
+    P, G = np.array([reg(x) for x in pm]), np.array([reg(x) for x in gm])
+    se = np.sqrt(P.var(0, ddof=1) / batches + G.var(0, ddof=1) / batches)
+    z = np.abs(G.mean(0) - P.mean(0)) / np.maximum(se, 1e-12)
+    pimg, gimg = np.mean([x.mean() for x in pm]), np.mean([x.mean() for x in gm])
+    se_img = np.sqrt(np.var([x.mean() for x in pm], ddof=1) / batches + np.var([x.mean() for x in gm], ddof=1) / batches)
+    print("photon-trained guide: image mean plain %.5f guided %.5f (z %.2f), region z max %.2f" % (
+        pimg, gimg, abs(gimg - pimg) / se_img, z.max()), flush=True)
+    assert abs(gimg - pimg) < 4 * se_img and z.max() < 5.0
+    # equal samples: trimmed relMSE, guided with photons < plain and < guided without photons
+    pe = _trimmed(np.mean(pm, 0), ref)
+    ge = _trimmed(np.mean(gm, 0), ref)
+    sc2 = ha.Scene(ring_job)
+    g2 = ha.Guide(sc2, seed=0)
+    sc2.set_guide_training(minibatch=16384, weight_cap=100.0)
+    sc2.set_guide(g2, fraction=0.5, train_lr=1e-3)
+    sc2.render_pt(seed=77, first_sample=0, num_samples=256)
+    sc2.set_guide(g2, fraction=0.5, train_lr=0.0)
+    ne = _trimmed(np.mean([sc2.render_pt(seed=22, first_sample=b * spp, num_samples=spp)[0] / spp
+                           for b in range(batches)], 0), ref)
+    print("trimmed relMSE at %d spp: plain %.5f, guided %.5f, guided without photons %.5f" % (
+        batches * spp, pe, ge, ne), flush=True)
+    assert ge < 0.8 * pe and ge < ne
