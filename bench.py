@@ -535,10 +535,13 @@ def main():
     ap.add_argument("--guide-scene", choices=("ring", "room"), default="ring",
                     help="ring: the metal-ring caustic box; room: a Cornell room lit from behind its light")
     ap.add_argument("--guide-lr", type=float, default=1e-3, help="Adam learning rate of the online training")
-    ap.add_argument("--guide-minibatch", type=int, default=65536, help="training samples per Adam step")
-    ap.add_argument("--guide-weight-cap", type=float, default=100.0, help="cap of the normalised training weights")
-    ap.add_argument("--guide-photons", type=int, default=4194304,
-                    help="caustic light paths per training step as extra training samples (0: none)")
+    ap.add_argument("--guide-minibatch", type=int, default=None,
+                    help="training samples per Adam step (default 65536 ring, 16384 room)")
+    ap.add_argument("--guide-weight-cap", type=float, default=None,
+                    help="cap of the normalised training weights (default 100 ring, 20 room)")
+    ap.add_argument("--guide-photons", type=int, default=None,
+                    help="caustic light paths per training step as extra training samples "
+                         "(default 4194304 ring, 0 room: its light faces the ceiling, no caustics)")
     ap.add_argument("--guide-photon-share", type=float, default=0.5,
                     help="photon samples' weight sum relative to the camera samples'")
     ap.add_argument("--guide-max-depth", type=int, default=255, help="guide only vertices of depth below this")
@@ -574,6 +577,12 @@ def main():
         return
     if args.workload == "guide":
         ring = args.guide_scene == "ring"
+        if args.guide_minibatch is None:
+            args.guide_minibatch = 65536 if ring else 16384
+        if args.guide_weight_cap is None:
+            args.guide_weight_cap = 100.0 if ring else 20.0
+        if args.guide_photons is None:
+            args.guide_photons = 4194304 if ring else 0
         if args.guide_spp is None:
             args.guide_spp = 4096 if ring else 256
         if args.guide_res is None:
@@ -723,6 +732,19 @@ def main():
         },
         "rays_per_s": value * rays_per_path,
         "rays_per_path": rays_per_path,
+        # traversal work per ray of each kind (the counted render): shadow
+        # rays toward the sky find no blocker and visit every node box they
+        # cross, extension rays stop shrinking their interval at the hit
+        "traversal": {
+            "extension_rays": counters["extension_rays"],
+            "shadow_rays": counters["shadow_rays"],
+            "nodes_per_extension_ray": round((counters["node_visits"] - counters["shadow_node_visits"])
+                                             / max(1, counters["extension_rays"]), 2),
+            "tris_per_extension_ray": round((counters["triangle_tests"] - counters["shadow_triangle_tests"])
+                                            / max(1, counters["extension_rays"]), 2),
+            "nodes_per_shadow_ray": round(counters["shadow_node_visits"] / max(1, counters["shadow_rays"]), 2),
+            "tris_per_shadow_ray": round(counters["shadow_triangle_tests"] / max(1, counters["shadow_rays"]), 2),
+        },
         "load_s": load_s,
         "kernel_ms": {k: round(v[1] / max(1, v[0]), 4) for k, v in kstats.items()},
         "roofline": roofline,

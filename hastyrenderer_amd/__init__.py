@@ -55,6 +55,7 @@ EXPORTS = (
     "hasty_kernel_stats",
     "hasty_reset_stats",
     "hasty_counters",
+    "hasty_counters_ex",
     "hasty_set_launch",
     "hasty_get_launch",
     "hasty_set_launch_ex",
@@ -228,6 +229,7 @@ class SppmParamsC(C.Structure):
         ("fixed_point_flux", C.c_int32),
         ("flux_frac_bits", C.c_int32),
         ("gather_share_candidates", C.c_uint32),
+        ("gather_union_factor", C.c_float),
     ]
 
 
@@ -291,6 +293,7 @@ def lib():
     L.hasty_kernel_stats.argtypes = [P, C.POINTER(KernelStatC), C.c_int]
     L.hasty_reset_stats.argtypes = [P]
     L.hasty_counters.argtypes = [P, C.POINTER(C.c_uint64)]
+    L.hasty_counters_ex.argtypes = [P, C.POINTER(C.c_uint64), C.c_uint32]
     L.hasty_set_launch.argtypes = [P, C.c_uint32, C.c_uint32, C.c_uint32]
     L.hasty_get_launch.argtypes = [P, C.POINTER(LaunchC)]
     L.hasty_set_launch_ex.argtypes = [P, C.POINTER(LaunchC)]
@@ -589,9 +592,10 @@ class Scene:
         return {arr[i].name.decode(): (int(arr[i].launches), float(arr[i].total_ms)) for i in range(n)}
 
     def counters(self) -> dict:
-        out = (C.c_uint64 * 5)()
-        _err(lib().hasty_counters(self._h, out))
-        keys = ("extension_rays", "shadow_rays", "node_visits", "triangle_tests", "media_overflow")
+        out = (C.c_uint64 * 7)()
+        _err(lib().hasty_counters_ex(self._h, out, 7))
+        keys = ("extension_rays", "shadow_rays", "node_visits", "triangle_tests", "media_overflow",
+                "shadow_node_visits", "shadow_triangle_tests")
         return dict(zip(keys, (int(x) for x in out)))
 
 
@@ -738,10 +742,11 @@ class Sppm:
 
     def __init__(self, scene: Scene, seed: int, alpha=0.4, footprint_scale=10.0, store_fraction=0.5,
                  emit_fraction=10.0, initial_radius=0.0, fixed_point_flux=False, flux_frac_bits=0,
-                 gather_share_candidates=0):
+                 gather_share_candidates=0, gather_union_factor=0.0):
         self.scene = scene  # keeps the scene alive
         p = SppmParamsC(alpha, footprint_scale, store_fraction, emit_fraction, initial_radius,
-                        1 if fixed_point_flux else 0, flux_frac_bits, gather_share_candidates)
+                        1 if fixed_point_flux else 0, flux_frac_bits, gather_share_candidates,
+                        gather_union_factor)
         h = C.c_void_p()
         _err(lib().hasty_sppm_create(scene.handle, seed, C.byref(p), C.byref(h)))
         self._h = h

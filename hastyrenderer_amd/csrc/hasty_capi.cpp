@@ -1382,15 +1382,20 @@ int hasty_reset_stats(HastyScene* s) {
   return 0;
 }
 
-int hasty_counters(HastyScene* s, uint64_t* out) {
-  if (!s || !out) return 1;
-  out[0] = s->stats[hasty::kStatExtRays];
-  out[1] = s->stats[hasty::kStatShadowRays];
-  out[2] = s->stats[hasty::kStatNodes];
-  out[3] = s->stats[hasty::kStatTris];
-  out[4] = s->stats[hasty::kStatMediaOverflow];
+int hasty_counters_ex(HastyScene* s, uint64_t* out, uint32_t n) {
+  if (!s || (n > 0 && !out)) return 1;
+  const uint64_t v[7] = {s->stats[hasty::kStatExtRays],
+                         s->stats[hasty::kStatShadowRays],
+                         s->stats[hasty::kStatNodes] + s->stats[hasty::kStatShadowNodes],
+                         s->stats[hasty::kStatTris] + s->stats[hasty::kStatShadowTris],
+                         s->stats[hasty::kStatMediaOverflow],
+                         s->stats[hasty::kStatShadowNodes],
+                         s->stats[hasty::kStatShadowTris]};
+  for (uint32_t i = 0; i < n && i < 7; ++i) out[i] = v[i];
   return 0;
 }
+
+int hasty_counters(HastyScene* s, uint64_t* out) { return hasty_counters_ex(s, out, 5); }
 
 int hasty_set_launch(HastyScene* s, uint32_t slots, uint32_t traceBlocks, uint32_t shadeBlocks) {
   if (!s) return 1;
@@ -1604,6 +1609,7 @@ int hasty_sppm_create(HastyScene* s, uint32_t seed, const HastySppmParams* param
       p.fixedPointFlux = params->fixed_point_flux != 0;
       if (params->flux_frac_bits) p.fluxFracBits = params->flux_frac_bits;
       p.gatherShareCandidates = params->gather_share_candidates;
+      p.gatherUnionFactor = params->gather_union_factor;
     }
     if (p.fluxFracBits < 0 || p.fluxFracBits > 40) throw std::runtime_error("flux_frac_bits must be in [0, 40]");
     if (!(p.alpha > 0.0f && p.alpha <= 1.0f)) throw std::runtime_error("alpha must be in (0, 1]");

@@ -114,7 +114,18 @@ constexpr int kStackOvf = 80 - kStackLds;  // 80 entries per lane in all
 constexpr int kBlock = 256;
 
 // 64-bit device statistics (filled when LaunchConfig::countStats is set).
-enum StatId { kStatExtRays = 0, kStatShadowRays, kStatNodes, kStatTris, kStatMediaOverflow, kNumStats = 8 };
+// kStatNodes / kStatTris count the extension rays' node visits and triangle
+// tests, kStatShadowNodes / kStatShadowTris the shadow rays'.
+enum StatId {
+  kStatExtRays = 0,
+  kStatShadowRays,
+  kStatNodes,
+  kStatTris,
+  kStatMediaOverflow,
+  kStatShadowNodes,
+  kStatShadowTris,
+  kNumStats = 8
+};
 
 struct LaunchConfig {
   uint32_t traceBlocks = 2048;  // persistent-style grid of the traversal kernels

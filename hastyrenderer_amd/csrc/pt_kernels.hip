@@ -766,21 +766,24 @@ __global__ void __launch_bounds__(kBlock) HASTY_CONNECT_ATTR kConnect(JobSet js,
   }
   if (COUNT) {
     waveAddStat(&pb.stats[kStatShadowRays], rays);
-    waveAddStat(&pb.stats[kStatNodes], nodes);
-    waveAddStat(&pb.stats[kStatTris], tris);
+    waveAddStat(&pb.stats[kStatShadowNodes], nodes);
+    waveAddStat(&pb.stats[kStatShadowTris], tris);
   }
 }
 
 // --------------------------------------------------------------- retire ----
 // Finished paths: results into the ring; the slot is idle from the next
 // bounce on (its mark is not renewed) and kFill gives it a new path id.
+// With `count`, paths whose media stack overflowed are counted.
 template <bool GUIDED>
-__global__ void __launch_bounds__(kBlock) kRetire(JobSet js, PathBuffers pb) {
+__global__ void __launch_bounds__(kBlock) kRetire(JobSet js, PathBuffers pb, uint32_t count) {
   uint32_t n = pb.counters[kCntR];
   uint32_t stride = gridDim.x * blockDim.x;
+  uint32_t overflowed = 0;
   for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += stride) {
     const uint32_t slot = pb.qR[q];
     uint4 st = pb.state[slot];
+    overflowed += (st.y >> 16) & 1u;
     uint32_t ri = st.w % pb.ringPaths;
     float4 Lf = pb.L[slot];
     pb.ringL[ri] = Lf;
@@ -810,6 +813,7 @@ __global__ void __launch_bounds__(kBlock) kRetire(JobSet js, PathBuffers pb) {
     pb.ringN[ri] = pb.auxN[slot];
     pb.ringA[ri] = pb.auxA[slot];
   }
+  if (count) waveAddStat(&pb.stats[kStatMediaOverflow], overflowed);
 }
 
 // ----------------------------------------------------------- accumulate ----
@@ -1133,7 +1137,7 @@ void launchChunk(const JobSet& js, const PathBuffers& pbIn, const LaunchConfig& 
     else kConnect<MULTI, false, W><<<cfg.traceBlocks, kBlock, 0, stream>>>(js, pb);
     t1(kKConnect);
     t0(kKRetire);
-    kRetire<GUIDED><<<cfg.shadeBlocks, kBlock, 0, stream>>>(js, pb);
+    kRetire<GUIDED><<<cfg.shadeBlocks, kBlock, 0, stream>>>(js, pb, countStats ? 1u : 0u);
     t1(kKRetire);
     t0(kKMisc);
     kAdvance<<<1, 64, 0, stream>>>(pb.counters);

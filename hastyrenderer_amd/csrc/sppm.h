@@ -20,6 +20,7 @@ struct SppmParams {
   bool fixedPointFlux = false;   // gathers sum photon terms in int64 units of 2^-fluxFracBits
   int fluxFracBits = 24;
   uint32_t gatherShareCandidates = 0;  // 0: default threshold for a wave's shared gather
+  float gatherUnionFactor = 0.0f;      // > 0: union gathers (see kSppmGather); 0: off
 };
 
 struct SppmPassInfo {

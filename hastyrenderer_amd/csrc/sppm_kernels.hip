@@ -107,6 +107,7 @@ struct SppmDev {
   float fluxScale;
   double fluxInvScale;
   uint32_t coopCandidates;
+  float unionFactor;  // union gather when the wave's union box has at most this x the mean cells of its queries
   // gathers of the pass: queries, candidate photons in the cells visited,
   // photons found within the radius
   unsigned long long* gatherStats;
@@ -598,9 +599,68 @@ __device__ inline void photonRun(const SppmDev& g, uint32_t first, uint32_t end,
 // 512) gather on their own (a shared gather costs every lane each owner's
 // cell lookups).
 constexpr uint32_t kCoopCandidatesDefault = 512;
+// Union gather: the wave visits the cells of the union of its queries'
+// boxes in x -> y -> z order, stages each run of photon positions in LDS 64
+// at a time, and every lane tests them against its own query.  A photon
+// within a query's radius lies in that query's own box, and the union's
+// order restricted to the box is the box's order, so each lane sums exactly
+// its photons in the reference's order (the per-lane gather's result), while
+// each photon is read once per wave instead of once per query.
+template <class Sum>
+__device__ inline V3 unionGather(const SppmDev& g, const DevScene& s, bool has, V3 x, V3 ns, V3 ng, int mat,
+                                 float r, int3 lo, int3 hi, int3 ulo, int3 uhi, int* M, uint2* runs,
+                                 float4* staged) {
+  const uint32_t lane = __lane_id();
+  Sum est;
+  int found = 0;
+  DevMaterial m;
+  if (has) m = ldgStruct(s.mats + mat);
+  const float r2 = r * r;
+  const int ny = uhi.y - ulo.y + 1, nz = uhi.z - ulo.z + 1;
+  const int nyz = ny * nz, nc = (uhi.x - ulo.x + 1) * nyz;
+  for (int base = 0; base < nc; base += 64) {
+    const int ci = base + (int)lane;
+    uint2 run = make_uint2(0u, 0u);
+    if (ci < nc) {
+      const int ix = ci / nyz, rem = ci - ix * nyz, iy = rem / nz, iz = rem - iy * nz;
+      run = cellLookup(g, packKey(make_int3(ulo.x + ix, ulo.y + iy, ulo.z + iz)));
+    }
+    runs[lane] = run;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    const int nb = min(64, nc - base);
+    for (int t = 0; t < nb; ++t) {
+      const uint2 cell = runs[t];
+      if (cell.y == 0u) continue;
+      // the cell's coordinates, to skip it on lanes whose box does not hold it
+      const int cidx = base + t;
+      const int cx = ulo.x + cidx / nyz, crem = cidx % nyz, cy = ulo.y + crem / nz, cz = ulo.z + crem % nz;
+      const bool mine = has && cx >= lo.x && cx <= hi.x && cy >= lo.y && cy <= hi.y && cz >= lo.z && cz <= hi.z;
+      if (__ballot(mine) == 0ull) continue;
+      const uint32_t end = cell.x + cell.y;
+      for (uint32_t q0 = cell.x; q0 < end; q0 += 64u) {
+        const uint32_t k = q0 + lane;
+        staged[lane] = k < end ? g.sX[k] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        const uint32_t n = min(64u, end - q0);
+        if (mine)
+          for (uint32_t j = 0; j < n; ++j) photonTerm(g, q0 + j, staged[j], x, r2, ns, ng, m, &est, &found);
+        __builtin_amdgcn_wave_barrier();  // staged is rewritten for the next photons
+      }
+    }
+    __builtin_amdgcn_wave_barrier();  // runs are rewritten for the next cells
+  }
+  if (has && est.saturated()) atomicAdd(&g.counters[kCtrFluxOverflow], 1u);
+  *M = found;
+  return est.value(g);
+}
+
 template <class Sum>
 __device__ inline V3 gatherQuery(const SppmDev& g, const DevScene& s, bool has, V3 x, V3 ns, V3 ng, int mat,
-                                 float r, int* M, uint2* runs, uint32_t* candidates) {
+                                 float r, int* M, uint2* runs, float4* staged, uint32_t* candidates) {
   const float cs = *g.cellSize;
   {
     // candidates of this lane's own query; if no lane of the wave has many,
@@ -631,6 +691,32 @@ __device__ inline V3 gatherQuery(const SppmDev& g, const DevScene& s, bool has, 
       }
       *M = found;
       return est.value(g);
+    }
+    // many candidates: the union gather when the queries' boxes overlap
+    // enough (the union box holds at most unionFactor x their mean cells)
+    if (g.unionFactor > 0.0f) {
+      int3 ulo = make_int3(has ? lo.x : 0x7FFFFFFF, has ? lo.y : 0x7FFFFFFF, has ? lo.z : 0x7FFFFFFF);
+      int3 uhi = make_int3(has ? hi.x : -0x7FFFFFFF, has ? hi.y : -0x7FFFFFFF, has ? hi.z : -0x7FFFFFFF);
+      float own = has ? (float)max(hi.x - lo.x + 1, 0) * (float)max(hi.y - lo.y + 1, 0) *
+                            (float)max(hi.z - lo.z + 1, 0)
+                      : 0.0f;
+      int nHas = has ? 1 : 0;
+      for (int off = 32; off > 0; off >>= 1) {
+        ulo.x = min(ulo.x, __shfl_xor(ulo.x, off));
+        ulo.y = min(ulo.y, __shfl_xor(ulo.y, off));
+        ulo.z = min(ulo.z, __shfl_xor(ulo.z, off));
+        uhi.x = max(uhi.x, __shfl_xor(uhi.x, off));
+        uhi.y = max(uhi.y, __shfl_xor(uhi.y, off));
+        uhi.z = max(uhi.z, __shfl_xor(uhi.z, off));
+        own += __shfl_xor(own, off);
+        nHas += __shfl_xor(nHas, off);
+      }
+      const float unionCells = nHas > 0 ? (float)(uhi.x - ulo.x + 1) * (float)(uhi.y - ulo.y + 1) *
+                                              (float)(uhi.z - ulo.z + 1)
+                                        : 0.0f;
+      const bool nonEmpty = uhi.x >= ulo.x && uhi.y >= ulo.y && uhi.z >= ulo.z;
+      if (nHas > 0 && nonEmpty && unionCells <= g.unionFactor * own / (float)nHas && unionCells < 1.0e6f)
+        return unionGather<Sum>(g, s, has, x, ns, ng, mat, r, lo, hi, ulo, uhi, M, runs, staged);
     }
   }
   // shared gather: every lane of the wave is a rank (the caller keeps the
@@ -908,6 +994,7 @@ template <class Sum>
 __global__ void __launch_bounds__(kB) HASTY_SPPM_GATHER_ATTR kSppmGather(DevScene s, SppmDev g, float alpha,
                                                                         int emittedInt, float* color) {
   __shared__ uint2 gatherRuns[kB];
+  __shared__ float4 gatherStaged[kB];
   const uint32_t P = s.width * s.height;
   unsigned long long nQueries = 0, nCand = 0, nFound = 0;
   for (uint32_t base = blockIdx.x * kB; base < P; base += gridDim.x * kB) {
@@ -935,7 +1022,8 @@ __global__ void __launch_bounds__(kB) HASTY_SPPM_GATHER_ATTR kSppmGather(DevScen
       }
       int M = 0;
       const V3 est = gatherQuery<Sum>(g, s, has, f4xyz(q0), f4xyz(q1), f4xyz(q2), __float_as_int(q0.w), radius,
-                                      &M, gatherRuns + (threadIdx.x & ~63u), &cand);
+                                      &M, gatherRuns + (threadIdx.x & ~63u), gatherStaged + (threadIdx.x & ~63u),
+                                      &cand);
       if (has) {
         nQueries += 1;
         nCand += cand;
@@ -1194,6 +1282,7 @@ void SppmRenderer::pass(uint32_t passIndex, float offX, float offY, float* dColo
   g.fluxScale = ldexpf(1.0f, p_.fluxFracBits);
   g.fluxInvScale = ldexp(1.0, -p_.fluxFracBits);
   g.coopCandidates = p_.gatherShareCandidates ? p_.gatherShareCandidates : kCoopCandidatesDefault;
+  g.unionFactor = p_.gatherUnionFactor;
   g.gatherStats = dGatherStats_;
 
   // 1. cell size

@@ -247,6 +247,9 @@ int hasty_reset_stats(HastyScene* scene);
  * out[0] extension rays, out[1] shadow rays, out[2] BVH node visits,
  * out[3] triangle tests, out[4] paths whose media stack overflowed. */
 int hasty_counters(HastyScene* scene, uint64_t* out5);
+/* The first n (at most 7) of: the five counters above, then the shadow
+ * rays' share of the node visits (out[5]) and triangle tests (out[6]). */
+int hasty_counters_ex(HastyScene* scene, uint64_t* out, uint32_t n);
 
 /* Launch-shape knobs: path slots in flight (paths are regenerated into
  * freed slots; default 2^23, at least 2^16 unless the film is smaller) and
@@ -380,6 +383,11 @@ typedef struct HastySppmParams {
   uint32_t gather_share_candidates;  /* 0: 512; a wave gathers its queries together when one of them has
                                         more candidate photons than this (no effect on the result with
                                         fixed-point flux) */
+  float gather_union_factor;         /* > 0: such a wave reads the union of its queries' cells once, each
+                                        lane summing its own photons in the reference's order (FP32 results
+                                        of the per-lane gather), when the union box holds at most this x the
+                                        queries' mean cells; 0 (default): the shared gather, faster on dense
+                                        maps (config-4 gathers 87 against 102 ms per pass) */
 } HastySppmParams;
 
 typedef struct HastySppmInfo {

@@ -253,3 +253,55 @@ def test_headless_cli_matches_library(cornell_job, caustic_small_job, tmp_path):
     # reference error convention: load failure -> exit code 2
     r = subprocess.run([cli, str(tmp_path / "missing.json"), str(out)], capture_output=True, text=True, timeout=60)
     assert r.returncode == 2
+
+
+def _nested_glass_job(folder, shells):
+    """`shells` concentric smooth glass cubes (IOR 1.5, slightly tinted)
+    around the camera's view axis under a white sky: a camera ray crosses
+    every shell twice, entering `shells` nested media on the way in."""
+    import json
+
+    os.makedirs(folder, exist_ok=True)
+    with open(os.path.join(folder, "m.mtl"), "w") as f:
+        f.write("newmtl glass\nKd 0.9 0.95 0.9\nKs 0 0 0\nNs 1000\nNi 1.5\nd 0\n")
+    lines = ["mtllib m.mtl"]
+    nv = 0
+    for k in range(shells):
+        h = 0.4 + 0.12 * k
+        v, tris = scenes._box((0.0, 0.0, 0.0), (2 * h, 2 * h, 2 * h), 0.0)
+        lines += ["o shell%d" % k, "usemtl glass"]
+        lines += ["v %r %r %r" % tuple(map(float, p)) for p in v]
+        lines += ["f %d %d %d" % (a + 1 + nv, b + 1 + nv, c + 1 + nv) for a, b, c in tris]
+        nv += len(v)
+    with open(os.path.join(folder, "m.obj"), "w") as f:
+        f.write("\n".join(lines) + "\n")
+    json.dump({"camera": {"position": [0.05, 0.1, 4.0], "look_at": [0, 0, 0]},
+               "world": {"background_color": [1.0, 1.0, 1.0]}, "mesh": "m.obj"},
+              open(os.path.join(folder, "scene.json"), "w"))
+    json.dump({"settings": {"width": 32, "height": 32, "num_samples": 4, "max_depth": 48,
+                            "roulette_start_depth": 40}, "scene": "scene.json"},
+              open(os.path.join(folder, "job.json"), "w"))
+    return os.path.join(folder, "job.json")
+
+
+def test_media_stack_within_and_beyond_eight_entries(tmp_path):
+    """Seven nested glass media: no overflow, films match the oracle (whose
+    stack is unbounded like the reference's).  Ten: the stack's ninth entry
+    on is counted as an overflow (the pixels through the centre), and the
+    render stays finite."""
+    job7 = _nested_glass_job(str(tmp_path / "n7"), 7)
+    sc = ha.Scene(job7)
+    sc.set_profiling(events=False, counters=True)
+    c, _, _ = sc.render_pt(seed=3, first_sample=0, num_samples=4)
+    assert sc.counters()["media_overflow"] == 0
+    (rc, _, _), _ = po.OracleJob(job7, seed=3).render(0, 4, threads=8)
+    frac = rel_ok(c, rc).mean()
+    print("7 nested media: %.5f of pixels within 1e-3 of the oracle" % frac, flush=True)
+    assert frac >= 0.999
+    job10 = _nested_glass_job(str(tmp_path / "n10"), 10)
+    sc10 = ha.Scene(job10)
+    sc10.set_profiling(events=False, counters=True)
+    c10, _, _ = sc10.render_pt(seed=3, first_sample=0, num_samples=4)
+    ov = sc10.counters()["media_overflow"]
+    print("10 nested media: %d paths overflowed the 8-entry stack" % ov, flush=True)
+    assert ov > 0 and np.isfinite(c10).all()

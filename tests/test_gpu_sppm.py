@@ -204,27 +204,36 @@ def test_fixed_point_flux_matches_oracle(caustic_small_job):
 
 
 def test_fixed_point_flux_does_not_depend_on_gather_order(caustic_small_job):
-    """With fixed-point flux the films and every pixel statistic are the same
-    bits whether each lane gathers its own photons in the reference's order
-    or the 64 lanes of a wave share every gather (partial sums added by a
-    butterfly): integer sums do not depend on the order of the terms."""
+    """Three gather forms: each lane alone in the reference's order; the 64
+    lanes of a wave sharing every gather (partial sums added by a
+    butterfly); the wave reading the union of its queries' cells once, each
+    lane summing its own photons.  With fixed-point flux the films and every
+    pixel statistic are the same bits for all three (integer sums do not
+    depend on the order of the terms); in FP32 the union gather equals the
+    lane-alone gather bit for bit (same photons, same order)."""
+    forms = {"alone": dict(gather_share_candidates=1 << 30),
+             "shared": dict(gather_share_candidates=1),
+             "union": dict(gather_share_candidates=1, gather_union_factor=1.0e9)}
     outs = {}
     for fixed in (True, False):
-        for share in (1, 1 << 30):  # 1: every wave shares its gathers; 2^30: never
-            sc, g = _gpu_sppm(caustic_small_job, seed=12, fixed_point_flux=fixed, gather_share_candidates=share,
-                              **_FIXED_KW)
+        for name, kw in forms.items():
+            sc, g = _gpu_sppm(caustic_small_job, seed=12, fixed_point_flux=fixed, **kw, **_FIXED_KW)
             films = [g.pass_(k)[0].copy() for k in range(3)]
             px = g.pixels()
-            outs[(fixed, share)] = (films, px["flux"].copy(), px["radius"].copy(), px["N"].copy())
-    a, b = outs[(True, 1)], outs[(True, 1 << 30)]
-    for x, y in zip(a[0], b[0]):
-        assert np.array_equal(x.view(np.uint32), y.view(np.uint32))
-    for x, y in zip(a[1:], b[1:]):
-        assert np.array_equal(x.view(np.uint32), y.view(np.uint32))
-    fa, fb = outs[(False, 1)][1], outs[(False, 1 << 30)][1]
-    print("FP32 flux planes differ between gather orders on %.4f of pixels" % (fa != fb).any(-1).mean(), flush=True)
+            outs[(fixed, name)] = (films, px["flux"].copy(), px["radius"].copy(), px["N"].copy())
+    same = lambda x, y: np.array_equal(x.view(np.uint32), y.view(np.uint32))
+    for other in ("shared", "union"):
+        a, b = outs[(True, "alone")], outs[(True, other)]
+        assert all(same(x, y) for x, y in zip(a[0], b[0])), other
+        assert all(same(x, y) for x, y in zip(a[1:], b[1:])), other
+    a, b = outs[(False, "alone")], outs[(False, "union")]
+    assert all(same(x, y) for x, y in zip(a[0], b[0]))
+    assert all(same(x, y) for x, y in zip(a[1:], b[1:]))
+    fa, fb = outs[(False, "alone")][1], outs[(False, "shared")][1]
+    print("FP32 flux planes differ between the alone and shared gathers on %.4f of pixels" % (fa != fb).any(-1).mean(),
+          flush=True)
     # fixed point agrees with FP32 to float rounding
-    assert rel_ok(a[0][-1], outs[(False, 1 << 30)][0][-1]).mean() >= 0.999
+    assert rel_ok(outs[(True, "alone")][0][-1], outs[(False, "alone")][0][-1]).mean() >= 0.999
 
 
 def test_fixed_point_guard_counts_saturated_gathers(tmp_path):
