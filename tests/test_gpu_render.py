@@ -278,8 +278,10 @@ def _nested_glass_job(folder, shells):
     json.dump({"camera": {"position": [0.05, 0.1, 4.0], "look_at": [0, 0, 0]},
                "world": {"background_color": [1.0, 1.0, 1.0]}, "mesh": "m.obj"},
               open(os.path.join(folder, "scene.json"), "w"))
+    # a background colour without light triangles: BSDF sampling only (NEE
+    # would need a light triangle, Scene.cpp:508-512)
     json.dump({"settings": {"width": 32, "height": 32, "num_samples": 4, "max_depth": 48,
-                            "roulette_start_depth": 40}, "scene": "scene.json"},
+                            "roulette_start_depth": 40, "use_mis": False}, "scene": "scene.json"},
               open(os.path.join(folder, "job.json"), "w"))
     return os.path.join(folder, "job.json")
 
