@@ -452,6 +452,8 @@ def bench_guide(args):
     log("plain: %d spp in %.3f s, relMSE %.5f" % (n, tp, ep))
     gsc = ha.Scene(job_path)
     g = ha.Guide(gsc, seed=0)
+    if args.guide_selection:
+        g.set_selection(*args.guide_selection)
     ntrain = min(args.guide_train_spp, n)
     f = ha.DeviceFilms(info["width"], info["height"])
     hip.sync()
@@ -496,7 +498,9 @@ def bench_guide(args):
         "config": {"workload": "metal_ring_box_guided" if args.guide_scene == "ring" else "indirect_room_guided",
                    "width": info["width"], "height": info["height"],
                    "max_depth": info["max_depth"], "triangles": info["num_triangles"], "spp": n,
-                   "reference_spp": args.guide_ref_spp, "train_spp": ntrain, "guide_fraction": args.guide_fraction,
+                   "reference_spp": args.guide_ref_spp, "train_spp": ntrain,
+                   "guide_fraction": (("learned in [%g, %g]" % tuple(args.guide_selection)) if args.guide_selection
+                                      else args.guide_fraction),
                    "minibatch": args.guide_minibatch, "weight_cap": args.guide_weight_cap,
                    "photons_per_step": args.guide_photons, "photon_share": args.guide_photon_share,
                    "guided_depths": args.guide_max_depth, "guided_max_throughput": args.guide_max_throughput,
@@ -546,6 +550,10 @@ def main():
                     help="photon samples' weight sum relative to the camera samples'")
     ap.add_argument("--guide-max-depth", type=int, default=255, help="guide only vertices of depth below this")
     ap.add_argument("--guide-fraction", type=float, default=0.5, help="probability of sampling the guide at a vertex")
+    ap.add_argument("--guide-selection", type=lambda v: tuple(float(x) for x in v.split(",") if float(x) > 0.0),
+                    default=None, metavar="LO,HI",
+                    help="learned guide/BSDF selection probability in [LO, HI], replacing --guide-fraction "
+                         "(default 0.05,0.95 ring; 0: the fixed fraction, the room's default)")
     ap.add_argument("--guide-max-throughput", type=float, default=3.0e38,
                     help="guide a vertex only while the path throughput is at most this")
     ap.add_argument("--guide-save", default="", help="save the plain and guided images as PREFIX_plain.npy / _guided.npy")
@@ -583,6 +591,8 @@ def main():
             args.guide_weight_cap = 100.0 if ring else 20.0
         if args.guide_photons is None:
             args.guide_photons = 4194304 if ring else 0
+        if args.guide_selection is None:
+            args.guide_selection = (0.05, 0.95) if ring else ()
         if args.guide_spp is None:
             args.guide_spp = 4096 if ring else 256
         if args.guide_res is None:

@@ -71,6 +71,8 @@ EXPORTS = (
     "hasty_guide_infer",
     "hasty_guide_gradient",
     "hasty_guide_train_step",
+    "hasty_guide_set_selection",
+    "hasty_guide_gradient_ex",
     "hasty_scene_set_guide",
     "hasty_scene_set_guide_training",
     "hasty_scene_set_guide_photons",
@@ -310,6 +312,8 @@ def lib():
     L.hasty_guide_infer.argtypes = [P, fp, C.c_uint32, fp]
     L.hasty_guide_gradient.argtypes = [P, fp, C.c_uint32, fp, fp]
     L.hasty_guide_train_step.argtypes = [P, fp, C.c_uint32, C.c_float, fp]
+    L.hasty_guide_set_selection.argtypes = [P, C.c_float, C.c_float]
+    L.hasty_guide_gradient_ex.argtypes = [P, fp, C.c_uint32, C.c_uint32, fp, fp]
     L.hasty_scene_set_guide.argtypes = [P, P, C.c_float, C.c_float]
     L.hasty_scene_set_guide_training.argtypes = [P, C.c_uint32, C.c_float]
     L.hasty_scene_set_guide_photons.argtypes = [P, C.c_uint32, C.c_float]
@@ -722,11 +726,20 @@ class Guide:
         _err(lib().hasty_guide_infer(self._h, _fptr(q), q.shape[0], _fptr(raw)))
         return raw
 
+    def set_selection(self, lo=0.0, hi=0.0):
+        """Learned guide/BSDF selection probability in [lo, hi] (0 < lo < hi
+        < 1); lo = hi = 0 returns to the scene's fixed fraction."""
+        _err(lib().hasty_guide_set_selection(self._h, lo, hi))
+
     def gradient(self, samples):
-        sm = np.ascontiguousarray(samples, np.float32).reshape(-1, 13)
+        """Samples (n, 13): position, normal, wo, wi, F; or (n, 15) with the
+        BSDF density of wi and the density wi was drawn with last."""
+        sm = np.ascontiguousarray(samples, np.float32)
+        stride = sm.shape[1] if sm.ndim == 2 else 13
+        sm = sm.reshape(-1, stride)
         g = np.zeros(self.n_params, np.float32)
         loss = np.zeros(1, np.float32)
-        _err(lib().hasty_guide_gradient(self._h, _fptr(sm), sm.shape[0], _fptr(g), _fptr(loss)))
+        _err(lib().hasty_guide_gradient_ex(self._h, _fptr(sm), sm.shape[0], stride, _fptr(g), _fptr(loss)))
         return g, float(loss[0])
 
     def train_step(self, samples, lr=1e-3):

@@ -5,8 +5,10 @@
 //
 // Network: 80 inputs (12-frequency sin/cos encoding of the position
 // normalised to the scene box = 72, normal 3, outgoing direction 3, 2 zero
-// pads) -> 64 -> 64 -> 64 -> 64 (ReLU) -> 48 raw outputs (40 used: per lobe
-// mean direction 3, log concentration 1, logit 1).  Weights are kept as FP32
+// pads) -> 64 -> 64 -> 64 -> 64 (ReLU) -> 48 raw outputs (41 used: per lobe
+// mean direction 3, log concentration 1, logit 1; then the logit z of the
+// probability c = lo + (hi - lo) sigmoid(z) of drawing a vertex's direction
+// from the guide rather than from its BSDF).  Weights are kept as FP32
 // master copies, as BF16 W^T [out][in] for the forward pass and BF16 W
 // [in][out] for the backward pass.
 #pragma once
@@ -22,7 +24,9 @@ constexpr int kFreq = 12;
 constexpr int kInPad = 80;
 constexpr int kHidden = 64;
 constexpr int kLobes = 8;
-constexpr int kOut = 40;
+constexpr int kOut = 40;      // mixture outputs
+constexpr int kSelOut = 40;   // raw output of the selection logit
+constexpr int kOutUsed = 41;  // outputs with weights (the rest are padding)
 constexpr int kOutPad = 48;
 constexpr int kLayers = 5;
 constexpr float kCLo = -4.0f, kCHi = 9.0f;  // log-concentration clamp
@@ -50,7 +54,14 @@ struct Net {
   const short* w;      // BF16 W (transposed copies), kWeights
   const float* param;  // FP32 master parameters (biases read from here)
   float boxLo[3], boxInv[3];  // position normalisation: (x - lo) * inv
+  // learned selection probability c in [selLo, selHi]; selLo == selHi
+  // (both 0 by default) leaves the selection logit untrained
+  float selLo, selHi;
 };
+
+__device__ inline float selectionOf(const Net& n, float z) {
+  return n.selLo + (n.selHi - n.selLo) / (1.0f + __expf(-z));
+}
 
 // ------------------------------------------------------------ BF16 ------
 __host__ __device__ inline short toBf16(float f) {
@@ -309,7 +320,8 @@ struct LdsNet {
 // Output row of the last layer held at LDS row p when the rows are grouped by
 // lobe: lane group q of a 16-query tile holds rows 16 mt + 4 q + v (mt < 3,
 // v < 4) of the output; with this order its 12 values are raw outputs
-// 10 q .. 10 q + 9 (lobes 2q and 2q + 1) then two pads.
+// 10 q .. 10 q + 9 (lobes 2q and 2q + 1) then outputs 40 + 2q and 41 + 2q
+// (for q = 0 the selection logit, in the third value of tile 2).
 __device__ inline int lobeMajorRow(int p) {
   const int mt = p >> 4, q = (p >> 2) & 3, v = p & 3, sl = 4 * mt + v;
   return sl < 10 ? 10 * q + sl : kOut + 2 * q + (sl - 10);

@@ -21,12 +21,17 @@ class Guide {
   void setParams(const float* host);  // also resets Adam
   void getParams(float* host);
   void setBox(const float lo[3], const float hi[3]);
+  // Range of the learned selection probability (lo < hi), or lo == hi for
+  // an untrained selection output (the renderer's fixed fraction applies).
+  void setSelection(float lo, float hi);
+  bool learnedSelection() const { return selHi_ > selLo_; }
   Net net() const;
 
   // n queries (pos, normal, wo as 3 float4 each) -> 48 raw outputs each.
   void inferDevice(const float4* dIn, uint32_t n, float4* dRaw);
-  // Adds d(mean loss)/d(params) of n samples (pos + F, normal, wo, wi) to
-  // the gradient buffer; returns the mean loss.
+  // Adds d(mean loss)/d(params) of n samples (pos + F, normal, wo + the
+  // density wi was drawn with or 0, wi + BSDF density of wi or 0) to the
+  // gradient buffer; returns the mean loss.
   float gradientDevice(const float4* dSamples, uint32_t n);
   void adamStep(float lr);  // consumes and clears the gradient
   // One Adam step on n samples whose weights are first normalised to mean 1
@@ -68,6 +73,7 @@ class Guide {
   float4* dShuffled_ = nullptr;
   uint32_t shuffledCap_ = 0;
   float boxLo_[3], boxInv_[3];
+  float selLo_ = 0.0f, selHi_ = 0.0f;
   uint32_t step_ = 0;
 };
 

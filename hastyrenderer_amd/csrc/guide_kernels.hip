@@ -54,8 +54,16 @@ __global__ void __launch_bounds__(256) kInfer(Net net, const float4* in, uint32_
 }
 
 // -------------------------------------------------------- loss gradient ----
-// d loss / d raw of one sample (lane-local, 40 outputs), loss = -F log q.
-__device__ inline float lossGrad(const float* raw, V3 d, float F, float scale, float* g) {
+// d loss / d raw of one sample (lane-local, 41 outputs), loss = -F log q
+// and, for a sample that carries the BSDF's density pB of wi and the density
+// pS it was drawn with (both > 0) while the selection is learned (selLo <
+// selHi), also F^2 pS / (c q + (1 - c) pB) with q held fixed: the second
+// moment of the vertex's one-sample estimate (F = f L / pS), whose gradient
+// moves only the selection logit.  Unlike the KL divergence, it weighs the
+// rare samples that only the BSDF finds by their squared contribution, so
+// the selection falls where the guide misses part of the integrand.
+__device__ inline float lossGrad(const Net& net, const float* raw, V3 d, float F, float pB, float pS, float scale,
+                                 float* g) {
   Mixture m = mixtureFromRaw(raw);
   float v[kLobes];
   float qsum = 0.0f;
@@ -84,6 +92,16 @@ __device__ inline float lossGrad(const float* raw, V3 d, float F, float scale, f
     g[5 * k + 2] = sc * f * proj.z;
     g[5 * k + 3] = sc * gc;
     g[5 * k + 4] = sc * (s - m.w[k]);
+  }
+  g[kSelOut] = 0.0f;
+  if (net.selHi > net.selLo && pB > 0.0f && pS > 0.0f) {
+    const float z = raw[kSelOut];
+    const float sg = 1.0f / (1.0f + __expf(-z));
+    const float c = net.selLo + (net.selHi - net.selLo) * sg;
+    const float pm = fmaxf(c * qsum + (1.0f - c) * pB, 1e-30f);
+    const float m2 = F * F * pS / pm;
+    loss += m2;
+    g[kSelOut] = -scale * m2 * (qsum - pB) / pm * (net.selHi - net.selLo) * sg * (1.0f - sg);
   }
   return loss;
 }
@@ -124,7 +142,8 @@ __device__ inline bf16x4 maskPack(f32x4 g, f32x4 z) {
 }
 
 // ----------------------------------------------------------- train pass ----
-// Samples: 4 float4 each (pos + F, normal, wo, wi).  Writes every layer's
+// Samples: 4 float4 each (pos + F, normal, wo + the density wi was drawn
+// with or 0, wi + the BSDF density of wi or 0).  Writes every layer's
 // input operands H_l and output gradients G_l (BF16, feature-major) for the
 // weight-gradient GEMM, and accumulates the loss.  batch = padded count (a
 // multiple of 16); samples >= n have zero weight.
@@ -166,15 +185,15 @@ __global__ void __launch_bounds__(256) kTrainFwdBwd(Net net, const float4* smp, 
     }
     __syncthreads();
     if (q == 0) {
-      float raw[kOut], g[kOut];
+      float raw[kOutUsed], g[kOutUsed];
 #pragma unroll
-      for (int i = 0; i < kOut; ++i) raw[i] = ldsRaw[wv][r][i];
-      float l = lossGrad(raw, f4xyz(d4), F, scale, g);
+      for (int i = 0; i < kOutUsed; ++i) raw[i] = ldsRaw[wv][r][i];
+      float l = lossGrad(net, raw, f4xyz(d4), F, d4.w, w4.w, scale, g);
       if (valid) lossAcc += l;
 #pragma unroll
-      for (int i = 0; i < kOut; ++i) ldsRaw[wv][r][i] = valid ? g[i] : 0.0f;
+      for (int i = 0; i < kOutUsed; ++i) ldsRaw[wv][r][i] = valid ? g[i] : 0.0f;
 #pragma unroll
-      for (int i = kOut; i < kOutPad; ++i) ldsRaw[wv][r][i] = 0.0f;
+      for (int i = kOutUsed; i < kOutPad; ++i) ldsRaw[wv][r][i] = 0.0f;
     }
     __syncthreads();
     bf16x4 g4[4];
@@ -184,8 +203,11 @@ __global__ void __launch_bounds__(256) kTrainFwdBwd(Net net, const float4* smp, 
       g4[mt] = pack4(src[0], src[1], src[2], src[3]);
     }
     __syncthreads();
-    // output layer gradient G_4, then back through the hidden layers
+    // output layer gradient G_4, then back through the hidden layers; the
+    // selection output (row 40: tile 2, lane group 2, first value) trains its
+    // own weights only, not the shared hidden layers
     if (tileValid) storeOperands<3>(G + (size_t)gOffset(4) * batch, batch, b0, g4, true);
+    if (q == 2) g4[2].x = 0;
     f32x4 dh[4];
     layerBwd<3, 4>(net.w + wqOffset(4), kOutPad, g4, dh);
 #pragma unroll
@@ -339,7 +361,7 @@ Guide::~Guide() {
 void Guide::initParams(uint32_t seed, float* p) {
   std::mt19937 rng(seed);
   for (int l = 0; l < kLayers; ++l) {
-    int nIn = l == 0 ? 78 : kHidden, nOut = l == kLayers - 1 ? kOut : kHidden;
+    int nIn = l == 0 ? 78 : kHidden, nOut = l == kLayers - 1 ? kOutUsed : kHidden;
     float lim = l < kLayers - 1 ? std::sqrt(6.0f / nIn) : 0.1f * std::sqrt(3.0f / nIn);
     std::uniform_real_distribution<float> U(-lim, lim);
     for (int o = 0; o < layerOut(l); ++o)
@@ -378,7 +400,14 @@ Net Guide::net() const {
   n.w = dW_;
   n.param = dParam_;
   for (int c = 0; c < 3; ++c) n.boxLo[c] = boxLo_[c], n.boxInv[c] = boxInv_[c];
+  n.selLo = selLo_;
+  n.selHi = selHi_;
   return n;
+}
+
+void Guide::setSelection(float lo, float hi) {
+  selLo_ = lo;
+  selHi_ = hi;
 }
 
 void Guide::inferDevice(const float4* dIn, uint32_t n, float4* dRaw) {

@@ -218,6 +218,7 @@ struct HastyScene {
   uint32_t lastGuidePhotonSamples = 0;
   float4* guideRaw = nullptr;
   float4* guideRec = nullptr;
+  float* guideSel = nullptr;  // per slot: learned selection probability
   float4* guideSamples = nullptr;
   uint32_t* guideCount = nullptr;
   uint32_t guideSlots = 0;
@@ -262,7 +263,7 @@ struct HastyScene {
     }
     for (void* p : sceneAllocs) (void)hipFree(p);
     if (dTileOrder) (void)hipFree(dTileOrder);
-    for (void* p : {(void*)guideRaw, (void*)guideRec, (void*)guideSamples, (void*)guideCount})
+    for (void* p : {(void*)guideRaw, (void*)guideRec, (void*)guideSel, (void*)guideSamples, (void*)guideCount})
       if (p) (void)hipFree(p);
     if (dFilm) (void)hipFree(dFilm);
     if (traceOverflow) (void)hipFree(traceOverflow);
@@ -536,11 +537,15 @@ static void renderGuided(HastyScene* s, const BatchItem& it) {
   s->ensureTileOrder();
   uint32_t slots = s->pb.capacity;
   if (s->guideSlots != slots) {
-    for (void* p : {(void*)s->guideRaw, (void*)s->guideRec})
+    for (void* p : {(void*)s->guideRaw, (void*)s->guideRec, (void*)s->guideSel})
       if (p) (void)hipFree(p);
+    s->guideSel = nullptr;
     HIPCHECK(hipMalloc(&s->guideRaw, sizeof(float4) * 10 * (size_t)slots));
     HIPCHECK(hipMalloc(&s->guideRec, sizeof(float4) * 12 * (size_t)slots));
     s->guideSlots = slots;
+  }
+  if (s->guide->g->learnedSelection() && !s->guideSel) {
+    HIPCHECK(hipMalloc(&s->guideSel, sizeof(float) * (size_t)slots));
   }
   if (!s->guideSamples) {
     HIPCHECK(hipMalloc(&s->guideSamples, sizeof(float4) * 4 * (size_t)HastyScene::kGuideSampleCap));
@@ -556,6 +561,7 @@ static void renderGuided(HastyScene* s, const BatchItem& it) {
   const bool training = s->guideLr > 0.0f;
   gr.sampleCap = training ? HastyScene::kGuideSampleCap : 0u;
   gr.fraction = s->guideFraction;
+  gr.sel = s->guide->g->learnedSelection() ? s->guideSel : nullptr;
   gr.maxDepth = s->guideMaxDepth;
   gr.maxThroughput = s->guideMaxThroughput;
   s->launch.countStats = s->counting;
@@ -1831,14 +1837,17 @@ int hasty_guide_infer(HastyGuide* g, const float* queries, uint32_t n, float* ra
 // gradient of the mean loss -F log q(wi) to the guide's gradient buffer (the
 // buffer is cleared first when `clear`), optionally copies it to grad_out,
 // and returns the mean loss in *loss.
-static void guideGradient(HastyGuide* g, const float* smp, uint32_t n, bool clear, float* gradOut, float* loss) {
+// Samples of `stride` floats (13: position, normal, wo, wi, F; 15: then the
+// BSDF density of wi and the density wi was drawn with).
+static void guideGradient(HastyGuide* g, const float* smp, uint32_t n, bool clear, float* gradOut, float* loss,
+                          uint32_t stride = 13) {
   std::vector<float4> in((size_t)n * 4);
   for (uint32_t i = 0; i < n; ++i) {
-    const float* s = smp + 13 * (size_t)i;
+    const float* s = smp + stride * (size_t)i;
     in[4 * i] = make_float4(s[0], s[1], s[2], s[12]);
     in[4 * i + 1] = make_float4(s[3], s[4], s[5], 0.0f);
-    in[4 * i + 2] = make_float4(s[6], s[7], s[8], 0.0f);
-    in[4 * i + 3] = make_float4(s[9], s[10], s[11], 0.0f);
+    in[4 * i + 2] = make_float4(s[6], s[7], s[8], stride > 13 ? s[14] : 0.0f);
+    in[4 * i + 3] = make_float4(s[9], s[10], s[11], stride > 13 ? s[13] : 0.0f);
   }
   float4* d = g->buffer((size_t)n * 16);
   hipStream_t st = g->g->stream();
@@ -1863,6 +1872,41 @@ int hasty_guide_gradient(HastyGuide* g, const float* samples, uint32_t n, float*
   } catch (const std::exception& e) {
     return fail(e, 3);
   }
+}
+
+int hasty_guide_gradient_ex(HastyGuide* g, const float* samples, uint32_t n, uint32_t stride, float* grad,
+                            float* loss) {
+  if (!g || !samples || n == 0) {
+    g_lastError = "null argument";
+    return 1;
+  }
+  if (stride != 13 && stride != 15) {
+    g_lastError = "sample stride must be 13 or 15 floats";
+    return 1;
+  }
+  try {
+    HIPCHECK(hipSetDevice(g->scene->device));
+    guideGradient(g, samples, n, true, grad, loss, stride);
+    g->g->zeroGrad();
+    HIPCHECK(hipStreamSynchronize(g->g->stream()));
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e, 3);
+  }
+}
+
+int hasty_guide_set_selection(HastyGuide* g, float lo, float hi) {
+  if (!g) {
+    g_lastError = "null argument";
+    return 1;
+  }
+  const bool off = lo == 0.0f && hi == 0.0f;
+  if (!off && !(lo > 0.0f && lo < hi && hi < 1.0f)) {
+    g_lastError = "selection range must satisfy 0 < lo < hi < 1 (or lo = hi = 0 for the fixed fraction)";
+    return 1;
+  }
+  g->g->setSelection(lo, hi);
+  return 0;
 }
 
 int hasty_scene_set_guide(HastyScene* s, HastyGuide* g, float fraction, float trainLr) {

@@ -77,9 +77,11 @@ struct PassInfo {
 };
 
 // Guided rendering (single job): per-slot raw guide outputs, per-slot
-// training records (2 x {position, normal, wo, wi, throughput before the
-// sample, radiance before the continuation}) and the training samples
-// harvested when paths end (4 float4 each: position + weight, normal, wo, wi).
+// training records (2 x {position, normal, wo + the density wi was drawn
+// with, wi + the BSDF density of wi, throughput before the sample, radiance
+// before the continuation}) and the training samples harvested when paths
+// end (4 float4 each: position + weight, normal, wo + drawing density,
+// wi + BSDF density).
 struct GuideRun {
   guide::Net net;
   float4* raw;      // 10 float4 per slot
@@ -88,6 +90,7 @@ struct GuideRun {
   uint32_t* sampleCount;
   uint32_t sampleCap;
   float fraction;   // probability of drawing the direction from the guide
+  float* sel;       // per slot: that probability as the guide learned it (or null: `fraction`)
   uint32_t maxDepth;  // only vertices of depth below this are guided
   float maxThroughput;  // ... and only while every channel of the path throughput is at most this
 };

@@ -549,7 +549,8 @@ __global__ void __launch_bounds__(kBlock) HASTY_SHADE_ATTR kShade(JobSet js, Pat
               } else {
                 // the vertex's mixture table, written by kGuideInfer
                 const float4* gm = GUIDED ? js.g.raw + (size_t)slot * 10 : nullptr;
-                const float gf = GUIDED ? js.g.fraction : 0.0f;
+                // kGuideInfer wrote the learned probability of guided vertices
+                const float gf = GUIDED ? (js.g.sel && guided ? js.g.sel[slot] : js.g.fraction) : 0.0f;
                 if (twoStrategies) {
                   // ---- light strategy (MIS.cpp:32-54, Scene.cpp:538-586)
                   float pSurf = !s.hasEnv ? 1.0f : (s.hasSurfaceLight ? 0.5f : 0.0f);
@@ -601,6 +602,7 @@ __global__ void __launch_bounds__(kBlock) HASTY_SHADE_ATTR kShade(JobSet js, Pat
                 }
                 float outsideIOR = mediaOutsideIOR(s, md, it, wo, wavelength);
                 BsdfSample bs;
+                float pBsdf = 0.0f;  // the BSDF's density of the direction (training records)
                 if (GUIDED && guided) {
                   // one-sample MIS of guide and BSDF: p = f pG + (1 - f) pB
                   if (rng.uniform01f() < gf) {
@@ -609,6 +611,7 @@ __global__ void __launch_bounds__(kBlock) HASTY_SHADE_ATTR kShade(JobSet js, Pat
                     bs.outside = dot(it.ng, wo) >= 0.0f;
                     BsdfEval ev = bsdfEvaluate(m, it, wo, bs.dir);
                     float pB = bsdfPdf(m, it, wo, bs.dir, outsideIOR);
+                    pBsdf = pB;
                     bs.pdf = gf * guide::mixturePdfTable(gm, bs.dir) + (1.0f - gf) * pB;
                     float mm = bs.pdf > 0.0f ? __builtin_fabsf(dot(bs.dir, it.ng)) / bs.pdf : 0.0f;
                     bs.tD = ev.fD * mm;
@@ -616,6 +619,7 @@ __global__ void __launch_bounds__(kBlock) HASTY_SHADE_ATTR kShade(JobSet js, Pat
                     if (!(bs.pdf > 0.0f)) bs.pdf = 0.0f;
                   } else {
                     bs = bsdfSample(m, rng, it, wo, wavelength, outsideIOR, false, 3);
+                    pBsdf = bs.pdf;
                     if (bs.pdf != 0.0f) {
                       float pMix = gf * guide::mixturePdfTable(gm, bs.dir) + (1.0f - gf) * bs.pdf;
                       float sc = bs.pdf / pMix;
@@ -649,8 +653,8 @@ __global__ void __launch_bounds__(kBlock) HASTY_SHADE_ATTR kShade(JobSet js, Pat
                       V3 nf = dot(it.ng, wo) >= 0.0f ? it.ng : -it.ng;
                       rr[0] = f4(it.x, 0.0f);
                       rr[1] = f4(nf, 0.0f);
-                      rr[2] = f4(wo, 0.0f);
-                      rr[3] = f4(bs.dir, 0.0f);
+                      rr[2] = f4(wo, bs.pdf);
+                      rr[3] = f4(bs.dir, pBsdf);
                       rr[4] = f4(beta, 0.0f);
                       nrec += 1u;
                       recBits = (nrec << kRecShift) | (1u << (kRecShift + 2));
@@ -916,6 +920,8 @@ __global__ void __launch_bounds__(kGuideInferThreads) HASTY_WAVES_ATTR_(HASTY_GU
         out[2 * q] = guide::lobeEntry(r10, w0);
         out[2 * q + 1] = guide::lobeEntry(r10 + 5, w1);
         reinterpret_cast<float2*>(out + guide::kLobes)[q] = make_float2(w0, w1);
+        // lane group 0 holds the selection logit (raw output 40)
+        if (js.g.sel && q == 0) js.g.sel[sq] = guide::selectionOf(js.g.net, o[2].z);
       }
     }
   }
@@ -1045,10 +1051,11 @@ __global__ void __launch_bounds__(kBlock) kGuidePhotons(DevScene s, uint32_t pas
         if (F > 0.0f && F < 3.0e38f) {
           const uint32_t k = atomicAdd(sampleCount, 1u);
           if (k < cap) {
+            const float pB = bsdfPdf(m, it, wos, wo, mediaOutsideIOR(s, md, it, wos, wavelength));
             samples[4 * (size_t)k] = f4(it.x, F);
             samples[4 * (size_t)k + 1] = f4(nf, 0.0f);
             samples[4 * (size_t)k + 2] = f4(wos, 0.0f);
-            samples[4 * (size_t)k + 3] = f4(wo, 0.0f);
+            samples[4 * (size_t)k + 3] = f4(wo, pB);
           }
         }
       }

@@ -304,7 +304,8 @@ int hasty_autotune(HastyScene* scene, uint32_t seed, uint32_t trial_samples, Has
  * A small MLP (BF16 weights on MFMA, FP32 accumulation) maps (position
  * normalised to the scene box, normal, outgoing direction) to a mixture of 8
  * von Mises-Fisher lobes (raw outputs per lobe: mean direction 3, log
- * concentration 1, logit 1; 48 floats with padding).  Trained with Adam on
+ * concentration 1, logit 1; then a selection logit; 48 floats with
+ * padding).  Trained with Adam on
  * the loss -F log q(w_i).  The guide lives on its scene's device and stream. */
 typedef struct HastyGuide HastyGuide;
 uint32_t hasty_guide_num_params(void);
@@ -320,6 +321,21 @@ int hasty_guide_infer(HastyGuide* guide, const float* queries, uint32_t n, float
 int hasty_guide_gradient(HastyGuide* guide, const float* samples, uint32_t n, float* grad, float* loss);
 /* one Adam step (beta 0.9 / 0.999, eps 1e-8) on the mean loss of the samples */
 int hasty_guide_train_step(HastyGuide* guide, const float* samples, uint32_t n, float lr, float* loss);
+/* Learned guide/BSDF selection (no counterpart in the reference): raw output
+ * 40 is the logit z of the probability c = lo + (hi - lo) sigmoid(z) of
+ * drawing a guided vertex's direction from the guide.  With 0 < lo < hi < 1
+ * guided renders use c instead of the scene's fixed fraction, and samples
+ * that carry the BSDF density pB of wi and the density pS wi was drawn with
+ * add the second moment F^2 pS / (c q + (1 - c) pB) of the vertex estimate
+ * to the loss, with q held fixed and trained through the last layer's row
+ * 40 only (online training records pB and pS).  lo = hi = 0 (the default)
+ * switches it off. */
+int hasty_guide_set_selection(HastyGuide* guide, float lo, float hi);
+/* hasty_guide_gradient with samples of `stride` floats: 13 as there, or 15
+ * with the BSDF density of wi and the density wi was drawn with last (0:
+ * none) */
+int hasty_guide_gradient_ex(HastyGuide* guide, const float* samples, uint32_t n, uint32_t stride, float* grad,
+                            float* loss);
 
 /* Guided rendering of a scene: hasty_render_pt[_device] then draws the
  * continuation direction at glossy and diffuse vertices from the guide with

@@ -4,14 +4,21 @@ path guide, the checker for the GPU kernels of hastyrenderer_amd/csrc/guide*.
 The guide has no counterpart in the reference renderer.  Network: input
 78 = 12-frequency sin/cos encoding of the position normalised to the scene
 box (72) + shading normal (3) + outgoing direction (3), padded to 80; four
-hidden layers of 64 with ReLU; 40 outputs = 8 von Mises-Fisher lobes x
-(mean direction 3, log concentration 1, logit 1), padded to 48.  Weights are
-stored transposed (W^T[out][in], padded), biases separately.
+hidden layers of 64 with ReLU; 41 outputs = 8 von Mises-Fisher lobes x
+(mean direction 3, log concentration 1, logit 1) and a selection logit z,
+padded to 48.  Weights are stored transposed (W^T[out][in], padded), biases
+separately.
 
   mu_k = r_k / |r_k|,  kappa_k = exp(clamp(c_k, C_LO, C_HI)),  w = softmax(a)
   vMF(w; mu, kappa) = kappa / (2 pi (1 - exp(-2 kappa))) exp(kappa (mu.w - 1))
   q(w) = sum_k w_k vMF_k(w)
   loss(sample) = -F log q(w_i)        (F = the sample's weight)
+  with a learned selection in [lo, hi], the BSDF density pB > 0 of w_i and
+  the density pS > 0 w_i was drawn with:
+    c = lo + (hi - lo) sigmoid(z),  loss += F^2 pS / (c q + (1 - c) pB)
+    (the second moment of the one-sample estimate; q held fixed in this
+    term, and its gradient trains the output row of z only, not the hidden
+    layers)
 
 `bf16=True` rounds weights and every layer input to bfloat16 (round to
 nearest even) exactly where the GPU's MFMA operands are rounded, so the GPU
@@ -30,7 +37,9 @@ N_IN = 78
 IN_PAD = 80
 HIDDEN = 64
 N_LOBES = 8
-N_OUT = 40
+N_OUT = 40       # mixture outputs
+SEL_OUT = 40     # the selection logit
+N_OUT_USED = 41
 OUT_PAD = 48
 DIMS_IN = (IN_PAD, HIDDEN, HIDDEN, HIDDEN, HIDDEN)
 DIMS_OUT = (HIDDEN, HIDDEN, HIDDEN, HIDDEN, OUT_PAD)
@@ -70,7 +79,7 @@ def init_params(seed=0):
     p = np.zeros(N_PARAMS, np.float32)
     for li, (w, b, din, dout) in enumerate(LAYOUT):
         n_in = N_IN if li == 0 else HIDDEN
-        n_out = N_OUT if li == len(LAYOUT) - 1 else HIDDEN
+        n_out = N_OUT_USED if li == len(LAYOUT) - 1 else HIDDEN
         lim = math.sqrt(6.0 / n_in) if li < len(LAYOUT) - 1 else 0.1 * math.sqrt(3.0 / n_in)
         W = np.zeros((dout, din), np.float32)
         W[:n_out, :n_in] = rng.uniform(-lim, lim, (n_out, n_in))
@@ -139,14 +148,31 @@ def pdf(raw, d):
     return np.sum(w * vmf_pdf(mu, kappa, d), axis=1)
 
 
-def loss_and_grad_raw(raw, d, F):
-    """Mean loss -F log q and its gradient with respect to the raw outputs
-    (B, 48) (already divided by the batch size)."""
+def selection(raw, lo, hi):
+    """Probability of drawing from the guide: lo + (hi - lo) sigmoid(z)."""
+    return lo + (hi - lo) / (1.0 + np.exp(-raw[:, SEL_OUT]))
+
+
+def loss_and_grad_raw(raw, d, F, pB=None, sel=None, pS=None):
+    """Mean loss and its gradient with respect to the raw outputs (B, 48)
+    (already divided by the batch size): -F log q, plus the selection term
+    for samples with pB > 0 and pS > 0 when sel = (lo, hi) with lo < hi."""
     B = raw.shape[0]
     mu, kappa, w, nr, c = mixture(raw)
     v = vmf_pdf(mu, kappa, d)
     q = np.maximum(np.sum(w * v, axis=1), 1e-30)  # all lobes pointing away: keep log finite
-    loss = -np.mean(F * np.log(q))
+    per = -F * np.log(q)
+    g_sel = np.zeros(B)
+    if sel is not None and pB is not None and pS is not None and sel[1] > sel[0]:
+        lo, hi = sel
+        use = (pB > 0.0) & (pS > 0.0)
+        sg = 1.0 / (1.0 + np.exp(-raw[:, SEL_OUT]))
+        cs = lo + (hi - lo) * sg
+        pm = np.maximum(cs * q + (1.0 - cs) * pB, 1e-30)
+        m2 = F * F * pS / pm
+        per = per + np.where(use, m2, 0.0)
+        g_sel = np.where(use, -m2 * (q - pB) / pm * (hi - lo) * sg * (1.0 - sg), 0.0)  # d m2 / dz
+    loss = np.mean(per)
     s = (w * v) / q[:, None]                    # posterior of each lobe
     g_a = s - w                                  # d log q / d a
     e = np.exp(-2.0 * kappa)
@@ -161,7 +187,9 @@ def loss_and_grad_raw(raw, d, F):
     gl[:, :, 3] = g_c
     gl[:, :, 4] = g_a
     g[:, :N_OUT] = gl.reshape(B, N_OUT)
-    return loss, -(F[:, None] * g) / B
+    out = -(F[:, None] * g) / B
+    out[:, SEL_OUT] = g_sel / B
+    return loss, out
 
 
 def backward(params, cache, g_raw, bf16_ops=False):
@@ -178,6 +206,9 @@ def backward(params, cache, g_raw, bf16_ops=False):
         grad[b:b + dout] = gq.sum(axis=0)
         Wt = params[w:w + din * dout].reshape(dout, din).astype(np.float64)
         Wq = bf16(Wt) if bf16_ops else Wt
+        if li == len(LAYOUT) - 1:
+            gq = gq.copy()
+            gq[:, SEL_OUT] = 0.0  # the selection output does not train the hidden layers
         g = gq @ Wq
     return grad
 

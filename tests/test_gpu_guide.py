@@ -55,8 +55,8 @@ def test_infer_matches_reference(cornell_job):
     scale = np.sqrt(np.mean(ref[:, :40] ** 2))
     print("infer: median |err| / rms %.2e, max %.2e" % (np.median(err) / scale, err.max() / scale), flush=True)
     assert np.median(err) <= 1e-5 * scale
-    assert np.mean(err[:, :40] <= 1e-3 * scale) >= 0.99
-    assert np.all(raw[:, 40:] == 0.0)
+    assert np.mean(err[:, :41] <= 1e-3 * scale) >= 0.99
+    assert np.all(raw[:, 41:] == 0.0)
 
 
 def _samples(rng, B, target=None):
@@ -90,6 +90,43 @@ def test_gradient_matches_reference(cornell_job):
     # padding stays untouched
     w0, b0, din0, dout0 = gr.LAYOUT[0]
     assert np.all(grad[w0:w0 + din0 * dout0].reshape(dout0, din0)[:, 78:] == 0.0)
+
+
+def test_selection_gradient_matches_reference(cornell_job):
+    """Samples carrying the BSDF density of wi and the density it was drawn
+    with (15 floats) under a learned selection range: the selection logit's
+    gradient (trained through its own output row only) joins the mixture's,
+    as the CPU model computes it; the same samples without a range leave
+    that row untouched."""
+    sc, g, p = _guide(cornell_job)
+    rng = np.random.default_rng(6)
+    x, n, wo, wi, F = _samples(rng, 1000)
+    raw, cache = gr.forward(p, gr.encode(x, n, wo), bf16_ops=True)
+    q = gr.pdf(raw, wi.astype(np.float64))
+    pB = (q * rng.uniform(0.1, 4.0, 1000)).astype(np.float32)
+    pB[::7] = 0.0
+    pS = (0.5 * q + 0.5 * pB).astype(np.float32)
+    sm = np.concatenate([x, n, wo, wi, F[:, None], pB[:, None], pS[:, None]], 1)
+    w4, b4, din, dout = gr.LAYOUT[4]
+    grad0, _ = g.gradient(sm)
+    assert not grad0[w4 + gr.SEL_OUT * din:w4 + (gr.SEL_OUT + 1) * din].any() and grad0[b4 + gr.SEL_OUT] == 0.0
+    g.set_selection(0.1, 0.9)
+    grad, loss = g.gradient(sm)
+    lref, g_raw = gr.loss_and_grad_raw(raw, wi.astype(np.float64), F.astype(np.float64),
+                                       pB=pB.astype(np.float64), sel=(0.1, 0.9), pS=pS.astype(np.float64))
+    gref = gr.backward(p, cache, g_raw, bf16_ops=True)
+    # the hidden layers see the mixture's gradient only
+    np.testing.assert_allclose(grad[:w4], grad0[:w4], rtol=1e-5, atol=1e-7)
+    assert loss == pytest.approx(lref, rel=1e-3)
+    rel = np.linalg.norm(grad - gref) / np.linalg.norm(gref)
+    row, rrow = grad[w4 + gr.SEL_OUT * din:w4 + (gr.SEL_OUT + 1) * din], gref[w4 + gr.SEL_OUT * din:w4 + (gr.SEL_OUT + 1) * din]
+    cos = float(row @ rrow / (np.linalg.norm(row) * np.linalg.norm(rrow) + 1e-30))
+    print("selection gradient: loss %.6f (ref %.6f), relative L2 error %.2e, row cos %.6f" % (loss, lref, rel, cos),
+          flush=True)
+    assert rel < 1e-2 and cos > 0.999
+    assert grad[b4 + gr.SEL_OUT] == pytest.approx(gref[b4 + gr.SEL_OUT], rel=1e-2)
+    with pytest.raises(ha.HastyError):
+        g.set_selection(0.5, 0.5)
 
 
 def test_adam_step_matches_reference(cornell_job):

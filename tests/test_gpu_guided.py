@@ -49,6 +49,37 @@ def test_guided_render_unbiased(room_job, room_reference):
     np.testing.assert_allclose(gn, pn, rtol=1e-6, atol=1e-6)
 
 
+def test_learned_selection_unbiased_and_trained(room_job, room_reference):
+    """The guide/BSDF selection probability learned per vertex: still
+    unbiased, every probability inside its range, and training moves it
+    away from where it starts (0.5, the middle of the range)."""
+    sc = ha.Scene(room_job)
+    g = ha.Guide(sc, seed=0)
+    g.set_selection(0.1, 0.9)
+    sc.set_guide(g, fraction=0.5, train_lr=1e-3)
+    n = 1024
+    c, _, _ = sc.render_pt(seed=3, first_sample=0, num_samples=n)
+    img = c / n
+    assert np.all(np.isfinite(img))
+    ratio = img.mean() / room_reference.mean()
+    print("learned selection: guided / plain image mean %.4f" % ratio, flush=True)
+    assert abs(ratio - 1.0) < 0.02
+    # the selection the trained guide assigns to hit points seen from the camera
+    rng = np.random.default_rng(0)
+    q = np.zeros((2048, 9), np.float32)
+    q[:, 0:3] = rng.uniform(0.05, 0.95, (2048, 3))
+    q[:, 3:6] = (0.0, 1.0, 0.0)  # floor-facing normal
+    d = rng.normal(size=(2048, 3))
+    d[:, 1] = np.abs(d[:, 1])
+    q[:, 6:9] = d / np.linalg.norm(d, axis=1, keepdims=True)
+    raw = g.infer(q)
+    sel = 0.1 + 0.8 / (1.0 + np.exp(-raw[:, 40].astype(np.float64)))
+    print("learned selection on the floor: mean %.3f, range [%.3f, %.3f]" % (sel.mean(), sel.min(), sel.max()),
+          flush=True)
+    assert np.all((sel >= 0.1) & (sel <= 0.9))
+    assert abs(sel.mean() - 0.5) > 0.02 or sel.std() > 0.02
+
+
 def test_trained_guide_lowers_error(room_job, room_reference):
     spp = 64
     plain = ha.Scene(room_job)
@@ -79,9 +110,11 @@ def ring_job(tmp_path_factory):
                                       n_major=192, n_minor=48, max_depth=32)
 
 
-def _photon_guided(job, train_spp=256):
+def _photon_guided(job, train_spp=256, selection=None):
     sc = ha.Scene(job)
     g = ha.Guide(sc, seed=0)
+    if selection:
+        g.set_selection(*selection)
     sc.set_guide_training(minibatch=16384, weight_cap=100.0)
     sc.set_guide_photons(1 << 21, 0.5)
     sc.set_guide(g, fraction=0.5, train_lr=1e-3)
@@ -92,16 +125,24 @@ def _photon_guided(job, train_spp=256):
     return sc, g
 
 
-def test_photon_trained_guide_unbiased_and_gains(ring_job):
-    """Caustic light paths as training samples: the guided image agrees with
-    the plain tracer's in image mean and 4 x 4 region means (batches give
-    the Monte Carlo error), and at equal samples its error outside the worst
+@pytest.fixture(scope="module")
+def ring_reference(ring_job):
+    plain = ha.Scene(ring_job)
+    ref, _, _ = plain.render_pt(seed=101, first_sample=0, num_samples=16384)
+    return ref / 16384
+
+
+@pytest.mark.parametrize("selection", [None, (0.05, 0.95)], ids=["fixed", "learned"])
+def test_photon_trained_guide_unbiased_and_gains(ring_job, ring_reference, selection):
+    """Caustic light paths as training samples, with the fixed guide
+    fraction 0.5 or the learned selection: the guided image agrees with the
+    plain tracer's in image mean and 4 x 4 region means (batches give the
+    Monte Carlo error), and at equal samples its error outside the worst
     0.1 % of pixels is lower than the plain tracer's and than a guide
     trained without the photons."""
     plain = ha.Scene(ring_job)
-    ref, _, _ = plain.render_pt(seed=101, first_sample=0, num_samples=16384)
-    ref /= 16384
-    sc, g = _photon_guided(ring_job)
+    ref = ring_reference
+    sc, g = _photon_guided(ring_job, selection=selection)
     batches, spp = 8, 256
     pm, gm = [], []
     for b in range(batches):
@@ -114,8 +155,8 @@ def test_photon_trained_guide_unbiased_and_gains(ring_job):
     z = np.abs(G.mean(0) - P.mean(0)) / np.maximum(se, 1e-12)
     pimg, gimg = np.mean([x.mean() for x in pm]), np.mean([x.mean() for x in gm])
     se_img = np.sqrt(np.var([x.mean() for x in pm], ddof=1) / batches + np.var([x.mean() for x in gm], ddof=1) / batches)
-    print("photon-trained guide: image mean plain %.5f guided %.5f (z %.2f), region z max %.2f" % (
-        pimg, gimg, abs(gimg - pimg) / se_img, z.max()), flush=True)
+    print("photon-trained guide (%s): image mean plain %.5f guided %.5f (z %.2f), region z max %.2f" % (
+        selection, pimg, gimg, abs(gimg - pimg) / se_img, z.max()), flush=True)
     assert abs(gimg - pimg) < 4 * se_img and z.max() < 5.0
     # equal samples: trimmed relMSE, guided with photons < plain and < guided without photons
     pe = _trimmed(np.mean(pm, 0), ref)

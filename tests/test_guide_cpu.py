@@ -75,6 +75,97 @@ def test_raw_gradient_matches_finite_differences():
             assert abs(fd - g[b, i]) <= 1e-5 * max(1.0, abs(fd)), (b, i)
 
 
+def test_selection_gradient():
+    """The learned selection c = lo + (hi - lo) sigmoid(z) and its loss, the
+    second moment F^2 pS / (c q + (1 - c) pB) of the one-sample estimate:
+    the gradient reaches z only (q held fixed: the mixture's outputs get the
+    same gradient as without the term), matches central differences in z,
+    vanishes where q = pB or pB = 0, and points towards more guide samples
+    where the guide's density exceeds the BSDF's."""
+    rng = np.random.default_rng(9)
+    B = 6
+    raw = rng.normal(size=(B, 48))
+    raw[:, 3:40:5] = rng.uniform(-1, 4, (B, 8))
+    _, _, _, wi, F = _batch(rng, B)
+    q = gr.pdf(raw, wi)
+    pB = q * np.array([0.2, 5.0, 1.0, 0.0, 0.7, 3.0])  # sample 2: pB = q; sample 3: no BSDF density
+    pS = 0.5 * q + 0.5 * pB + 0.1
+    sel = (0.1, 0.9)
+    _, g0 = gr.loss_and_grad_raw(raw, wi, F)
+    _, g = gr.loss_and_grad_raw(raw, wi, F, pB=pB, sel=sel, pS=pS)
+    np.testing.assert_array_equal(g[:, :gr.N_OUT], g0[:, :gr.N_OUT])
+    assert not g0[:, gr.SEL_OUT].any()
+    h = 1e-6
+    for b in range(B):
+        r = raw.copy()
+        r[b, gr.SEL_OUT] += h
+        lp, _ = gr.loss_and_grad_raw(r, wi, F, pB=pB, sel=sel, pS=pS)
+        r[b, gr.SEL_OUT] -= 2 * h
+        lm, _ = gr.loss_and_grad_raw(r, wi, F, pB=pB, sel=sel, pS=pS)
+        fd = (lp - lm) / (2 * h)
+        assert abs(fd - g[b, gr.SEL_OUT]) <= 1e-7 * max(1.0, abs(fd)), (b, fd, g[b, gr.SEL_OUT])
+    assert abs(g[2, gr.SEL_OUT]) < 1e-12 and g[3, gr.SEL_OUT] == 0.0
+    assert g[0, gr.SEL_OUT] < 0.0 and g[4, gr.SEL_OUT] < 0.0  # q > pB: raising z lowers the loss
+    assert g[1, gr.SEL_OUT] > 0.0 and g[5, gr.SEL_OUT] > 0.0  # q < pB
+    # the selection stays inside [lo, hi] whatever the logit
+    c = gr.selection(np.array([[0.0] * 40 + [-80.0] + [0.0] * 7, [0.0] * 40 + [80.0] + [0.0] * 7]), *sel)
+    assert c[0] == pytest.approx(0.1) and c[1] == pytest.approx(0.9)
+
+
+def test_selection_minimises_estimator_variance():
+    """One vertex whose integrand g = a q + (1 - a) h mixes the guide's lobe
+    q with light from the lower hemisphere h that only the BSDF (uniform
+    here) reaches: descending the model's selection gradient on samples drawn
+    at c = 0.5 reaches the c at which the variance of the one-sample estimate
+    g / (c q + (1 - c) pB), computed by quadrature, is smallest, and not the
+    KL-optimal c (which trusts the guide more)."""
+    rng = np.random.default_rng(11)
+    a, N, kappa = 0.7, 20000, 20.0
+    # guide: one lobe about +z; BSDF: uniform over the sphere
+    raw = np.zeros((1, 48))
+    raw[0, 0:3] = (0.0, 0.0, 1.0)
+    raw[0, 3] = math.log(kappa)
+    raw[0, 4] = 10.0  # lobe 0 takes all the weight
+    for k in range(1, 8):
+        raw[0, 5 * k + 4] = -30.0
+    pB0 = 1.0 / (4.0 * math.pi)
+    from_guide = rng.uniform(size=N) < 0.5
+    d = np.where(from_guide[:, None],
+                 gr.sample_vmf(np.tile(raw[0, 0:3], (N, 1)), np.full(N, kappa), rng.uniform(size=N),
+                               rng.uniform(size=N)),
+                 _uniform_sphere(rng, N))
+    rawN = np.repeat(raw, N, 0)
+    q = gr.pdf(rawN, d)
+    pB = np.full(N, pB0)
+    pS = 0.5 * q + 0.5 * pB
+    F = (a * q + (1 - a) * np.where(d[:, 2] < 0.0, 1.0 / (2.0 * math.pi), 0.0)) / pS
+    sel = (0.02, 0.98)
+    z = 0.0
+    for _ in range(40):
+        rawN[:, gr.SEL_OUT] = z
+        _, g = gr.loss_and_grad_raw(rawN, d, F, pB=pB, sel=sel, pS=pS)
+        z -= g[:, gr.SEL_OUT].sum()
+    rawN[:, gr.SEL_OUT] = z
+    c_learned = gr.selection(rawN[:1], *sel)[0]
+    # quadrature over cos(theta): everything is symmetric about +z
+    t = np.linspace(-1.0, 1.0, 200001)
+    qt = gr.vmf_pdf(np.array([[[0.0, 0.0, 1.0]]] * len(t)), np.full((len(t), 1), kappa),
+                    np.stack([np.sqrt(1 - t * t), np.zeros_like(t), t], -1))[:, 0]
+    gt = a * qt + (1 - a) * np.where(t < 0.0, 1.0 / (2.0 * math.pi), 0.0)
+    cs = np.linspace(0.02, 0.98, 961)
+    m2 = [np.trapezoid(gt * gt / (c * qt + (1 - c) * pB0), t) for c in cs]
+    kl = [-np.trapezoid(gt * np.log(c * qt + (1 - c) * pB0), t) for c in cs]
+    c_var, c_kl = cs[int(np.argmin(m2))], cs[int(np.argmin(kl))]
+    print("learned c %.3f, variance-optimal %.3f, KL-optimal %.3f" % (c_learned, c_var, c_kl))
+    assert abs(c_learned - c_var) < 0.015
+    assert abs(c_learned - c_kl) > 0.04
+
+
+def _uniform_sphere(rng, n):
+    v = rng.normal(size=(n, 3))
+    return v / np.linalg.norm(v, axis=1, keepdims=True)
+
+
 @pytest.mark.parametrize("kappa", [0.05, 1.0, 20.0, 300.0])
 def test_vmf_normalised_and_mean(kappa):
     # integrate in (cos theta, phi) about mu = z: the density depends on cos only
