@@ -23,6 +23,13 @@ std::vector<uint8_t> readAll(const std::string& path) {
   throw std::runtime_error("could not read " + path + ": " + what);
 }
 
+// Image sizes the readers accept: at most 2^20 pixels a side and 2^30 in
+// all, so that no size product below can overflow.
+void checkDims(const std::string& path, uint64_t W, uint64_t H) {
+  if (W == 0 || H == 0 || W > (1u << 20) || H > (1u << 20) || W * H > (1ull << 30))
+    bad(path, "image size " + std::to_string(W) + "x" + std::to_string(H) + " out of range");
+}
+
 // ------------------------------------------------------------- inflate ----
 // Deflate decoder written from RFC 1951.  Codes are canonical Huffman codes:
 // for each code length the number of codes and the symbols in code order are
@@ -96,8 +103,10 @@ const uint16_t kDistBase[30] = {1, 2, 3, 4, 5, 7, 9, 13, 17, 25, 33, 49, 65, 97,
                                 193, 257, 385, 513, 769, 1025, 1537, 2049, 3073, 4097, 6145, 8193, 12289, 16385, 24577};
 const uint8_t kDistExtra[30] = {0, 0, 0, 0, 1, 1, 2, 2, 3, 3, 4, 4, 5, 5, 6, 6, 7, 7, 8, 8, 9, 9, 10, 10, 11, 11, 12, 12, 13, 13};
 
-void inflateBlock(BitReader& br, std::vector<uint8_t>& out, const Huffman& lit, const Huffman& dist) {
+void inflateBlock(BitReader& br, std::vector<uint8_t>& out, const Huffman& lit, const Huffman& dist,
+                  size_t maxOut) {
   for (;;) {
+    if (out.size() > maxOut) throw std::runtime_error("deflate stream longer than expected");
     int sym = lit.decode(br);
     if (sym < 256) {
       out.push_back((uint8_t)sym);
@@ -119,7 +128,7 @@ void inflateBlock(BitReader& br, std::vector<uint8_t>& out, const Huffman& lit, 
 
 }  // namespace
 
-std::vector<uint8_t> zlibInflate(const uint8_t* data, size_t size) {
+std::vector<uint8_t> zlibInflate(const uint8_t* data, size_t size, size_t maxOut) {
   if (size < 2) throw std::runtime_error("zlib stream too short");
   const uint8_t cmf = data[0], flg = data[1];
   if ((cmf & 0x0F) != 8 || ((cmf << 8) | flg) % 31 != 0) throw std::runtime_error("not a zlib deflate stream");
@@ -150,10 +159,11 @@ std::vector<uint8_t> zlibInflate(const uint8_t* data, size_t size) {
       br.pos += 4;
       if ((len ^ 0xFFFFu) != nlen) throw std::runtime_error("corrupt stored deflate block");
       if (br.pos + len > br.n) throw std::runtime_error("deflate stream ends early");
+      if (out.size() + len > maxOut) throw std::runtime_error("deflate stream longer than expected");
       out.insert(out.end(), br.p + br.pos, br.p + br.pos + len);
       br.pos += len;
     } else if (type == 1) {
-      inflateBlock(br, out, fixedLit, fixedDist);
+      inflateBlock(br, out, fixedLit, fixedDist, maxOut);
     } else if (type == 2) {
       const int nlen = (int)br.bits(5) + 257, ndist = (int)br.bits(5) + 1, ncode = (int)br.bits(4) + 4;
       if (nlen > 286 || ndist > 30) throw std::runtime_error("bad deflate code counts");
@@ -187,7 +197,7 @@ std::vector<uint8_t> zlibInflate(const uint8_t* data, size_t size) {
       Huffman lit, dist;
       if (!lit.build(lengths, nlen) || !dist.build(lengths + nlen, ndist))
         throw std::runtime_error("bad deflate literal or distance code");
-      inflateBlock(br, out, lit, dist);
+      inflateBlock(br, out, lit, dist, maxOut);
     } else {
       throw std::runtime_error("invalid deflate block type");
     }
@@ -228,6 +238,7 @@ FloatImage readPNG(const std::string& path, const std::vector<uint8_t>& f) {
     pos += 12 + (size_t)len;
   }
   if (W == 0 || H == 0) bad(path, "PNG without image header");
+  checkDims(path, W, H);
   if (interlace) bad(path, "interlaced PNG images are not supported");
   int samples;
   switch (ctype) {
@@ -246,7 +257,7 @@ FloatImage readPNG(const std::string& path, const std::vector<uint8_t>& f) {
   const size_t bitsPerPixel = (size_t)samples * depth;
   const size_t stride = (W * bitsPerPixel + 7) / 8;
   const size_t bpp = bitsPerPixel >= 8 ? bitsPerPixel / 8 : 1;
-  std::vector<uint8_t> raw = zlibInflate(idat.data(), idat.size());
+  std::vector<uint8_t> raw = zlibInflate(idat.data(), idat.size(), (stride + 1) * (size_t)H);
   if (raw.size() < (stride + 1) * (size_t)H) bad(path, "PNG image data too short");
   // undo the row filters in place
   std::vector<uint8_t> img(stride * H);
@@ -330,11 +341,15 @@ FloatImage readPNM(const std::string& path, const std::vector<uint8_t>& f) {
     }
     if (pos >= f.size() || !isdigit(f[pos])) bad(path, "malformed PNM header");
     long v = 0;
-    while (pos < f.size() && isdigit(f[pos])) v = v * 10 + (f[pos++] - '0');
+    while (pos < f.size() && isdigit(f[pos])) {
+      v = v * 10 + (f[pos++] - '0');
+      if (v > (1l << 30)) bad(path, "PNM number out of range");
+    }
     return v;
   };
   const long W = token(), H = token(), maxval = token();
   if (W <= 0 || H <= 0 || maxval <= 0 || maxval > 65535) bad(path, "malformed PNM header");
+  checkDims(path, (uint64_t)W, (uint64_t)H);
   FloatImage out;
   out.width = (uint32_t)W;
   out.height = (uint32_t)H;
@@ -369,6 +384,7 @@ FloatImage readPFMAny(const std::string& path, const std::vector<uint8_t>& f) {
   const double scale = atof(word().c_str());
   ++pos;
   if (W <= 0 || H <= 0 || scale == 0.0) bad(path, "malformed PFM header");
+  checkDims(path, (uint64_t)W, (uint64_t)H);
   const size_t n = (size_t)W * H * ch;
   if (pos + n * 4 > f.size()) bad(path, "truncated PFM data");
   FloatImage out;
@@ -498,7 +514,8 @@ FloatImage readEXR(const std::string& path, const std::vector<uint8_t>& f) {
   }
   if (chans.empty()) bad(path, "EXR file without channels");
   const int64_t W = (int64_t)box[2] - box[0] + 1, H = (int64_t)box[3] - box[1] + 1;
-  if (W <= 0 || H <= 0 || W * H > (1ll << 31)) bad(path, "bad EXR data window");
+  if (W <= 0 || H <= 0) bad(path, "bad EXR data window");
+  checkDims(path, (uint64_t)W, (uint64_t)H);
   int linesPerChunk;
   switch (compression) {
     case 0: case 1: case 2: linesPerChunk = 1; break;  // NONE, RLE, ZIPS
@@ -531,10 +548,10 @@ FloatImage readEXR(const std::string& path, const std::vector<uint8_t>& f) {
   for (size_t k = 0; k < nChunks; ++k) {
     uint64_t off;
     memcpy(&off, &f[pos + 8 * k], 8);
-    if (off + 8 > f.size()) bad(path, "EXR chunk offset out of range");
+    if (off > f.size() - 8) bad(path, "EXR chunk offset out of range");  // f.size() >= 8 here
     const int32_t y0 = (int32_t)le32(&f[off]) - box[1];
     const uint32_t dsize = le32(&f[off + 4]);
-    if (off + 8 + dsize > f.size()) bad(path, "truncated EXR chunk");
+    if (dsize > f.size() - off - 8) bad(path, "truncated EXR chunk");
     if (y0 < 0 || y0 >= H || y0 % linesPerChunk) bad(path, "bad EXR chunk line");
     const int lines = (int)std::min<int64_t>(linesPerChunk, H - y0);
     const size_t want = lineBytes * lines;
@@ -544,7 +561,7 @@ FloatImage readEXR(const std::string& path, const std::vector<uint8_t>& f) {
       if (dsize != want) bad(path, "EXR chunk size mismatch");
       raw.assign(d, d + dsize);
     } else {
-      raw = compression == 1 ? exrRleDecode(path, d, dsize, want) : zlibInflate(d, dsize);
+      raw = compression == 1 ? exrRleDecode(path, d, dsize, want) : zlibInflate(d, dsize, want);
       if (raw.size() != want) bad(path, "EXR chunk decodes to the wrong size");
       exrUnpredict(raw);
     }
@@ -638,21 +655,20 @@ void writeEXR(const std::string& path, uint32_t W, uint32_t H, const float* rgb)
   }
   FILE* f = fopen(path.c_str(), "wb");
   if (!f) throw std::runtime_error("could not open file " + path + " for writing");
-  fwrite(h.data(), 1, h.size(), f);
+  bool ok = fwrite(h.data(), 1, h.size(), f) == h.size();
   std::vector<float> line((size_t)W * 3);
   for (uint32_t y = 0; y < H; ++y) {
     const int32_t yy = (int32_t)y, size = (int32_t)(W * 3 * 4);
-    fwrite(&yy, 4, 1, f);
-    fwrite(&size, 4, 1, f);
+    ok = ok && fwrite(&yy, 4, 1, f) == 1 && fwrite(&size, 4, 1, f) == 1;
     const float* row = rgb + (size_t)y * W * 3;
     for (uint32_t x = 0; x < W; ++x) {  // channels in B, G, R order, each a run of W floats
       line[x] = row[3 * x + 2];
       line[W + x] = row[3 * x + 1];
       line[2 * W + x] = row[3 * x];
     }
-    fwrite(line.data(), 4, line.size(), f);
+    ok = ok && fwrite(line.data(), 4, line.size(), f) == line.size();
   }
-  fclose(f);
+  if (fclose(f) != 0 || !ok) throw std::runtime_error("could not write file " + path);
 }
 
 }  // namespace hasty

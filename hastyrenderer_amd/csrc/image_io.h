@@ -30,8 +30,9 @@ struct FloatImage {
 // std::runtime_error naming the file on a malformed or unsupported file.
 FloatImage readImage(const std::string& path);
 
-// Inflates a zlib stream (RFC 1950 wrapper, RFC 1951 deflate blocks).
-std::vector<uint8_t> zlibInflate(const uint8_t* data, size_t size);
+// Inflates a zlib stream (RFC 1950 wrapper, RFC 1951 deflate blocks); throws
+// once the output grows past maxOut bytes (by at most one match).
+std::vector<uint8_t> zlibInflate(const uint8_t* data, size_t size, size_t maxOut = SIZE_MAX);
 
 // OpenEXR scanline file, no compression, three FLOAT channels B, G, R
 // (alphabetical channel order, as the format requires), row 0 = top.

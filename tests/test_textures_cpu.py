@@ -141,6 +141,49 @@ def test_reader_refusals(tmp_path):
         ha.read_image(str(trunc))
 
 
+def test_reader_refuses_hostile_headers(tmp_path):
+    """Sizes whose products would overflow, an EXR chunk offset near 2^64 and
+    a zlib stream that inflates past the image are refused before any read
+    or write out of bounds."""
+    import struct
+    import zlib
+    big = tmp_path / "big.png"
+    scenes.write_png(str(big), np.zeros((2, 2, 3), np.uint8))
+    b = bytearray(big.read_bytes())
+    b[16:24] = struct.pack(">II", 0xFFFFFFF0, 0xFFFFFFF0)  # IHDR width, height (CRC is not checked)
+    big.write_bytes(bytes(b))
+    with pytest.raises(ha.HastyError, match="out of range"):
+        ha.read_image(str(big))
+    (tmp_path / "big.ppm").write_bytes(b"P6 99999999999999999999 2 255\n" + bytes(12))
+    with pytest.raises(ha.HastyError, match="out of range"):
+        ha.read_image(str(tmp_path / "big.ppm"))
+    (tmp_path / "big.pfm").write_bytes(b"PF\n3000000 3000000\n-1.0\n" + bytes(12))
+    with pytest.raises(ha.HastyError, match="out of range"):
+        ha.read_image(str(tmp_path / "big.pfm"))
+    # EXR (2 x 3, uncompressed FLOAT RGB): the two scanline chunks of 8 + 36
+    # bytes end the file and the offset table of 2 entries precedes them; its
+    # first entry becomes 2^64 - 4
+    ex = tmp_path / "a.exr"
+    scenes.write_exr(str(ex), np.ones((2, 3, 3), np.float32), compression="none", pixel_type="float")
+    e = bytearray(ex.read_bytes())
+    first = len(e) - 2 * (8 + 3 * 3 * 4)
+    assert struct.unpack_from("<Q", e, first - 16)[0] == first
+    e[first - 16:first - 8] = struct.pack("<Q", 2 ** 64 - 4)
+    ex.write_bytes(bytes(e))
+    with pytest.raises(ha.HastyError, match="offset out of range"):
+        ha.read_image(str(ex))
+    # PNG whose image data inflates far past W x H
+    row = b"\x00" + bytes(6)
+    raw = zlib.compress(row * 2 + bytes(1 << 20))
+    def chunk(t, d):
+        return struct.pack(">I", len(d)) + t + d + struct.pack(">I", zlib.crc32(t + d) & 0xFFFFFFFF)
+    png = b"\x89PNG\r\n\x1a\n" + chunk(b"IHDR", struct.pack(">IIBBBBB", 2, 2, 8, 2, 0, 0, 0)) + \
+        chunk(b"IDAT", raw) + chunk(b"IEND", b"")
+    (tmp_path / "bomb.png").write_bytes(png)
+    with pytest.raises(ha.HastyError, match="longer than expected"):
+        ha.read_image(str(tmp_path / "bomb.png"))
+
+
 # ------------------------------------------------------------------ scenes --
 QUAD = ["v -1 -1 0", "v 1 -1 0", "v 1 1 0", "v -1 1 0"]  # z = 0, facing +z
 
