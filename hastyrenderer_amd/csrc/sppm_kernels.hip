@@ -564,24 +564,31 @@ struct FixedSum {
   __device__ bool saturated() const { return (flags & 1) != 0; }
 };
 
+// photonMapFluxEstimate's term of photon q (position x4 already loaded) when
+// it lies within the radius.
 template <class Sum>
-__device__ inline void photonTerm(const SppmDev& g, uint32_t q, float4 x4, V3 c, float r2, V3 ns, V3 ng,
-                                  const DevMaterial& m, Sum* acc, int* found) {
-  const V3 dlt = v3(x4.x, x4.y, x4.z) - c;
-  if (!(dot(dlt, dlt) <= r2)) return;
-  const V3 wi = f4xyz(g.sWi[q]);
-  const V3 fl = f4xyz(g.sF[q]);
+__device__ inline void photonAdd(const SppmDev& g, float4 x4, float4 wi4, float4 fl4, V3 ns, V3 ng,
+                                 const DevMaterial& m, Sum* acc, int* found) {
+  const V3 wi = f4xyz(wi4);
+  const V3 fl = f4xyz(fl4);
   const V3 fD = evalDiffuse(dot(ns, wi), m, normalScale(wi, ns, ng));
   const float fc = __builtin_fabsf(dot(wi, ng)) / (1e-7f + __builtin_fabsf(x4.w));
   acc->add(mul(fl * fc, fD + v3(0.0f, 0.0f, 0.0f)), g);
   *found += 1;
 }
-// Photons first, first + stride, ... below end, in that order, with
-// kPhotonLoads position loads in flight at a time.
+template <class Sum>
+__device__ inline void photonTerm(const SppmDev& g, uint32_t q, float4 x4, V3 c, float r2, V3 ns, V3 ng,
+                                  const DevMaterial& m, Sum* acc, int* found) {
+  const V3 dlt = v3(x4.x, x4.y, x4.z) - c;
+  if (!(dot(dlt, dlt) <= r2)) return;
+  photonAdd(g, x4, g.sWi[q], g.sF[q], ns, ng, m, acc, found);
+}
 #ifndef HASTY_PHOTON_LOADS
 #define HASTY_PHOTON_LOADS 4
 #endif
 constexpr uint32_t kPhotonLoads = HASTY_PHOTON_LOADS;
+// Photons first, first + stride, ... below end, in that order, with
+// kPhotonLoads position loads in flight at a time.
 template <class Sum>
 __device__ inline void photonRun(const SppmDev& g, uint32_t first, uint32_t end, uint32_t stride, V3 c, float r2,
                                  V3 ns, V3 ng, const DevMaterial& m, Sum* acc, int* found) {
@@ -658,9 +665,25 @@ __device__ inline V3 unionGather(const SppmDev& g, const DevScene& s, bool has, 
   return est.value(g);
 }
 
+// The terms of the n <= 64 queued photons fq[head ..] (mod 128), lane l
+// taking the l-th; every lane of the wave calls this.
+template <class Sum>
+__device__ inline void foundTerms(const SppmDev& g, const uint32_t* fq, uint32_t head, uint32_t n, V3 ns, V3 ng,
+                                  const DevMaterial& m, Sum* acc, int* found) {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  const uint32_t lane = __lane_id();
+  if (lane < n) {
+    const uint32_t q = fq[(head + lane) & 127u];
+    photonAdd(g, g.sX[q], g.sWi[q], g.sF[q], ns, ng, m, acc, found);
+  }
+  __builtin_amdgcn_wave_barrier();  // the slots are refilled afterwards
+}
+
 template <class Sum>
 __device__ inline V3 gatherQuery(const SppmDev& g, const DevScene& s, bool has, V3 x, V3 ns, V3 ng, int mat,
-                                 float r, int* M, uint2* runs, float4* staged, uint32_t* candidates) {
+                                 float r, int* M, uint2* runs, float4* staged, uint32_t* fq, uint32_t* candidates) {
   const float cs = *g.cellSize;
   {
     // candidates of this lane's own query; if no lane of the wave has many,
@@ -719,11 +742,15 @@ __device__ inline V3 gatherQuery(const SppmDev& g, const DevScene& s, bool has, 
         return unionGather<Sum>(g, s, has, x, ns, ng, mat, r, lo, hi, ulo, uhi, M, runs, staged);
     }
   }
-  // shared gather: every lane of the wave is a rank (the caller keeps the
-  // whole wave here), so a query's partial sums do not depend on which
-  // other queries share its wave
+  // shared gather, one owner query at a time: the 64 lanes test the owner's
+  // candidate photons, those within the radius are queued in LDS and their
+  // terms taken 64 at a time, and a butterfly adds the lanes' partial sums.
+  // Every lane of the wave takes part (the caller keeps the whole wave
+  // here), so a query's sums do not depend on which other queries share its
+  // wave.
   const uint32_t lane = __lane_id();
   const int rank = (int)lane, nAct = 64;
+  const unsigned long long ltMask = (1ull << lane) - 1ull;
   V3 myEst = v3(0.0f, 0.0f, 0.0f);
   int myM = 0;
   uint64_t todo = __ballot(has);
@@ -741,8 +768,11 @@ __device__ inline V3 gatherQuery(const SppmDev& g, const DevScene& s, bool has, 
     hi = make_int3(min(hi.x, g.bbox[3]), min(hi.y, g.bbox[4]), min(hi.z, g.bbox[5]));
     Sum acc;
     int found = 0;
+    uint32_t head = 0u, tail = 0u;  // the queue fq[head .. tail) (mod 128) of photons found
     // the cells of the box in x -> y -> z order, 64 at a time: lane rank
-    // looks up cell base + rank (all probes in flight together)
+    // looks up cell base + rank (all probes in flight together).  Their
+    // photons are then dealt to the lanes as one sequence (cell after cell),
+    // so every lane has work whatever the cells' sizes.
     const int ny = max(hi.y - lo.y + 1, 0), nz = max(hi.z - lo.z + 1, 0);
     const int nyz = ny * nz, nc = max(hi.x - lo.x + 1, 0) * nyz;
     for (int base = 0; base < nc; base += nAct) {
@@ -752,17 +782,56 @@ __device__ inline V3 gatherQuery(const SppmDev& g, const DevScene& s, bool has, 
         const int ix = ci / nyz, rem = ci - ix * nyz, iy = rem / nz, iz = rem - iy * nz;
         run = cellLookup(g, packKey(make_int3(lo.x + ix, lo.y + iy, lo.z + iz)));
       }
-      runs[rank] = run;
+      // inclusive prefix of the photon counts over the 64 cells
+      uint32_t incl = run.y;
+      for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t t = __shfl_up(incl, off);
+        if (rank >= off) incl += t;
+      }
+      const uint32_t total = __shfl(incl, 63);
+      // sequence index i of this cell's photons maps to photon i + .x
+      // (unsigned arithmetic wraps); .y is where the cell ends
+      runs[rank] = make_uint2(run.x - (incl - run.y), incl);
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
       __builtin_amdgcn_wave_barrier();
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-      const int nb = min(nAct, nc - base);
-      for (int t = 0; t < nb; ++t) {
-        const uint2 cell = runs[t];
-        photonRun(g, cell.x + (uint32_t)rank, cell.x + cell.y, (uint32_t)nAct, c, r2, nso, ngo, m, &acc, &found);
+      int cur = 0;  // the lane's cell; its sequence indices only grow
+      // the trip count is the same on every lane: foundTerms needs the
+      // whole wave
+      for (uint32_t b0 = 0; b0 < total; b0 += kPhotonLoads * (uint32_t)nAct) {
+        uint32_t q[kPhotonLoads];
+        float4 xp[kPhotonLoads];
+#pragma unroll
+        for (uint32_t u = 0; u < kPhotonLoads; ++u) {
+          const uint32_t i = b0 + u * (uint32_t)nAct + (uint32_t)rank;
+          q[u] = ~0u;
+          if (i < total) {
+            while (runs[cur].y <= i) ++cur;
+            q[u] = runs[cur].x + i;
+          }
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < kPhotonLoads; ++u)
+          xp[u] = q[u] != ~0u ? g.sX[q[u]] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        // photons within the radius join the wave's queue; its terms are
+        // taken 64 at a time, one per lane, so the term's arithmetic runs
+        // on full waves instead of on the few lanes that found a photon
+#pragma unroll
+        for (uint32_t u = 0; u < kPhotonLoads; ++u) {
+          const V3 dlt = v3(xp[u].x, xp[u].y, xp[u].z) - c;
+          const bool in = q[u] != ~0u && dot(dlt, dlt) <= r2;
+          const unsigned long long mk = __ballot(in);
+          if (in) fq[(tail + (uint32_t)__popcll(mk & ltMask)) & 127u] = q[u];
+          tail += (uint32_t)__popcll(mk);
+          if (tail - head >= 64u) {
+            foundTerms(g, fq, head, 64u, nso, ngo, m, &acc, &found);
+            head += 64u;
+          }
+        }
       }
       __builtin_amdgcn_wave_barrier();  // runs are rewritten for the next cells
     }
+    if (tail != head) foundTerms(g, fq, head, tail - head, nso, ngo, m, &acc, &found);
     // the 64 partial sums by a butterfly: each step adds the same two
     // operands on both lanes of a pair (in either order), so every lane ends
     // with the same total
@@ -974,9 +1043,11 @@ __global__ void __launch_bounds__(kB) HASTY_SPPM_CAMERA_ATTR kSppmCamera(DevScen
 }
 
 // Occupancy target of the gather kernel (waves per SIMD; 0: the compiler's
-// choice).
+// choice, 146 VGPRs and 3 waves).  At 4 (128 VGPRs, 40 B of spills) the
+// 4M-photon gathers take 67 ms against 69 and the sparse ones 2.7 ms against
+// 3.2 on the 1920x1080 caustic box; 5 loses on the dense maps.
 #ifndef HASTY_SPPM_GATHER_WAVES
-#define HASTY_SPPM_GATHER_WAVES 0
+#define HASTY_SPPM_GATHER_WAVES 4
 #endif
 #if HASTY_SPPM_GATHER_WAVES > 0
 #define HASTY_SPPM_GATHER_ATTR __attribute__((amdgpu_waves_per_eu(HASTY_SPPM_GATHER_WAVES)))
@@ -995,6 +1066,7 @@ __global__ void __launch_bounds__(kB) HASTY_SPPM_GATHER_ATTR kSppmGather(DevScen
                                                                         int emittedInt, float* color) {
   __shared__ uint2 gatherRuns[kB];
   __shared__ float4 gatherStaged[kB];
+  __shared__ uint32_t gatherFound[2 * kB];
   const uint32_t P = s.width * s.height;
   unsigned long long nQueries = 0, nCand = 0, nFound = 0;
   for (uint32_t base = blockIdx.x * kB; base < P; base += gridDim.x * kB) {
@@ -1023,7 +1095,7 @@ __global__ void __launch_bounds__(kB) HASTY_SPPM_GATHER_ATTR kSppmGather(DevScen
       int M = 0;
       const V3 est = gatherQuery<Sum>(g, s, has, f4xyz(q0), f4xyz(q1), f4xyz(q2), __float_as_int(q0.w), radius,
                                       &M, gatherRuns + (threadIdx.x & ~63u), gatherStaged + (threadIdx.x & ~63u),
-                                      &cand);
+                                      gatherFound + 2u * (threadIdx.x & ~63u), &cand);
       if (has) {
         nQueries += 1;
         nCand += cand;
