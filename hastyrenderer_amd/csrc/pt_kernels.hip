@@ -345,6 +345,7 @@ template <bool MULTI>
 __global__ void __launch_bounds__(kBlock) kFill(JobSet js, PathBuffers pb) {
   __shared__ uint32_t ldsWave[kBlock / 64];
   __shared__ uint32_t ldsBase[3];
+  __shared__ uint32_t ldsIdle[2 * kBlock];  // idle slots waiting for a path, in slot order
   const uint32_t avail = fillAvail(pb.counters);
   if (avail == 0u) return;  // the same for every thread
   const uint32_t next = pb.counters[kCntNext], epoch = pb.counters[kCntEpoch];
@@ -362,22 +363,38 @@ __global__ void __launch_bounds__(kBlock) kFill(JobSet js, PathBuffers pb) {
   }
   __syncthreads();
   const uint32_t b = ldsBase[0], live = ldsBase[1], qa = ldsBase[2];
-  uint32_t rankBase = 0;
+  // The r-th idle slot of the range starts path next + b + r.  The idle
+  // slots are queued in LDS and their paths started a block at a time, one
+  // per thread, so that the camera-ray code runs on full waves.
+  auto start = [&](uint32_t r, uint32_t slot) {
+    const uint32_t pid = next + b + r;
+    const uint32_t j = findJob<MULTI>(js, pid);
+    const uint32_t flags = MULTI ? jobBits(js, j) : 0u;
+    pb.mark[slot] = epoch;
+    forJob<MULTI>(js, j, [&](const DevScene& sc, const PassInfo& pi) { startPath(sc, pb, pi, slot, pid, flags); });
+    pb.qA[qa + r] = slot;
+  };
+  uint32_t rankBase = 0, done = 0;  // idle slots seen / started (uniform)
   for (uint32_t base = lo; base < hi && rankBase < live; base += kBlock) {  // uniform bounds
     const uint32_t slot = base + threadIdx.x;
     const bool idle = slot < hi && pb.mark[slot] != epoch;
     uint32_t count;
     const uint32_t r = rankBase + blockPrefix(idle, ldsWave, &count);
-    if (idle && r < live) {
-      const uint32_t pid = next + b + r;
-      const uint32_t j = findJob<MULTI>(js, pid);
-      const uint32_t flags = MULTI ? jobBits(js, j) : 0u;
-      pb.mark[slot] = epoch;
-      forJob<MULTI>(js, j, [&](const DevScene& sc, const PassInfo& pi) { startPath(sc, pb, pi, slot, pid, flags); });
-      pb.qA[qa + r] = slot;
-    }
+    if (idle && r < live) ldsIdle[r - done] = slot;  // fewer than kBlock were queued before
     rankBase += count;
+    const uint32_t queued = min(rankBase, live) - done;
+    if (queued >= kBlock) {
+      __syncthreads();
+      start(done + threadIdx.x, ldsIdle[threadIdx.x]);
+      const uint32_t rest = threadIdx.x < queued - kBlock ? ldsIdle[kBlock + threadIdx.x] : 0u;
+      __syncthreads();
+      if (threadIdx.x < queued - kBlock) ldsIdle[threadIdx.x] = rest;
+      done += kBlock;
+    }
   }
+  __syncthreads();
+  const uint32_t queued = min(rankBase, live) - done;
+  if (threadIdx.x < queued) start(done + threadIdx.x, ldsIdle[threadIdx.x]);
 }
 __global__ void kFillCommit(uint32_t* c) {
   if (threadIdx.x == 0) {
