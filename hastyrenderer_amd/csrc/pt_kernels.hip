@@ -541,9 +541,11 @@ __global__ void __launch_bounds__(kBlock) HASTY_SHADE_ATTR kShade(JobSet js, Pat
           DevMaterial m = ldgStruct(s.mats + it.mat);
           // guided vertex: rough reflection (no delta lobe, and not so glossy
           // that broad vMF lobes could only waste samples); decided on the
-          // material, before its textures, as kGuideInfer decides it
+          // material before its textures and on the stored throughput
+          // before this vertex's MIS weight, exactly as kGuideInfer decides
+          // it, so that every guided vertex has its mixture table
           const bool guided = GUIDED && guideEligible(m) && (uint32_t)depth < js.g.maxDepth &&
-                              (depth == 0 || guideThroughputOk(js.g, beta));
+                              (depth == 0 || guideThroughputOk(js.g, f4xyz(b4)));
           if (depth == 0) pb.auxN[slot] = f4(it.ns, 0.0f);  // the default shading normal
           uint32_t texel = 0u;
           if (m.textured) applyTextures(s, m, it, 1.0f - h4.y - h4.z, h4.y, h4.z, &texel);

@@ -49,6 +49,25 @@ def test_guided_render_unbiased(room_job, room_reference):
     np.testing.assert_allclose(gn, pn, rtol=1e-6, atol=1e-6)
 
 
+@pytest.mark.parametrize("max_depth,max_throughput", [(2, 3.0e38), (255, 0.5)])
+def test_guided_depth_and_throughput_limits_unbiased(room_job, room_reference, max_depth, max_throughput):
+    """Guiding limited to shallow vertices or to paths of small throughput:
+    the inference kernel and the shading kernel pick the same vertices (the
+    stored throughput before the vertex's MIS weight), so every guided vertex
+    has its mixture, and the image mean stays the plain tracer's."""
+    sc = ha.Scene(room_job)
+    g = ha.Guide(sc, seed=0)
+    sc.set_guide_depth(max_depth, max_throughput)
+    sc.set_guide(g, fraction=0.5, train_lr=1e-3)
+    n = 1024
+    c, _, _ = sc.render_pt(seed=3, first_sample=0, num_samples=n)
+    img = c / n
+    assert np.all(np.isfinite(img))
+    ratio = img.mean() / room_reference.mean()
+    print("limits (%d, %g): guided / plain image mean %.4f" % (max_depth, max_throughput, ratio), flush=True)
+    assert abs(ratio - 1.0) < 0.02
+
+
 def test_learned_selection_unbiased_and_trained(room_job, room_reference):
     """The guide/BSDF selection probability learned per vertex: still
     unbiased, every probability inside its range, and training moves it
