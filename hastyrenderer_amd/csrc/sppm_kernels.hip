@@ -564,8 +564,8 @@ struct FixedSum {
   __device__ bool saturated() const { return (flags & 1) != 0; }
 };
 
-// photonMapFluxEstimate's term of photon q (position x4 already loaded) when
-// it lies within the radius.
+// photonMapFluxEstimate's term of a photon within the radius, from its
+// position, direction and flux records.
 template <class Sum>
 __device__ inline void photonAdd(const SppmDev& g, float4 x4, float4 wi4, float4 fl4, V3 ns, V3 ng,
                                  const DevMaterial& m, Sum* acc, int* found) {
