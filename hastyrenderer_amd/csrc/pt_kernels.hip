@@ -596,18 +596,24 @@ __global__ void __launch_bounds__(kBlock) HASTY_SHADE_ATTR kShade(JobSet js, Pat
                     so = offsetOrigin(it, dot(it.ng, sd) >= 0.0f);
                   }
                   BsdfEval ev = bsdfEvaluate(m, it, wo, sd);
-                  float outsideIOR = mediaOutsideIOR(s, md, it, wo, wavelength);
-                  float pB = bsdfPdf(m, it, wo, sd, outsideIOR);
-                  if (GUIDED && guided) pB = gf * guide::mixturePdfTable(gm, sd) + (1.0f - gf) * pB;
-                  V3 Tc = mediaTransmittanceAfter(s, md, it, wo, sd, texel, txs);
-                  pb.shO[slot] = f4(so, pB);
-                  pb.shD[slot] = f4(sd, __int_as_float(target));
-                  pb.shFD[slot] = f4(ev.fD, __builtin_fabsf(dot(sd, it.ng)));
-                  pb.shFC[slot] = f4(ev.fC, __int_as_float(tri));
-                  pb.shThr[slot] = f4(thr, 0.0f);
-                  pb.shBeta[slot] = f4(beta, 0.0f);
-                  pb.shT[slot] = f4(Tc, 0.0f);
-                  pushShadow = true;
+                  // A light sample the BSDF gives nothing (e.g. below the
+                  // surface) adds exactly zero whatever its shadow ray meets
+                  // (kConnect's t2 = thr (fD + fC) |cos| / pL), so no ray is
+                  // cast; about a quarter of the samples on the GGX grid.
+                  if (!isZero(ev.fD + ev.fC)) {
+                    float outsideIOR = mediaOutsideIOR(s, md, it, wo, wavelength);
+                    float pB = bsdfPdf(m, it, wo, sd, outsideIOR);
+                    if (GUIDED && guided) pB = gf * guide::mixturePdfTable(gm, sd) + (1.0f - gf) * pB;
+                    V3 Tc = mediaTransmittanceAfter(s, md, it, wo, sd, texel, txs);
+                    pb.shO[slot] = f4(so, pB);
+                    pb.shD[slot] = f4(sd, __int_as_float(target));
+                    pb.shFD[slot] = f4(ev.fD, __builtin_fabsf(dot(sd, it.ng)));
+                    pb.shFC[slot] = f4(ev.fC, __int_as_float(tri));
+                    pb.shThr[slot] = f4(thr, 0.0f);
+                    pb.shBeta[slot] = f4(beta, 0.0f);
+                    pb.shT[slot] = f4(Tc, 0.0f);
+                    pushShadow = true;
+                  }
                 }
                 // ---- BSDF strategy (MIS.cpp:6-26, Scene.cpp:328-338)
                 float sampleProbability = 1.0f;
