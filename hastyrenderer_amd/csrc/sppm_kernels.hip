@@ -960,14 +960,20 @@ __global__ void __launch_bounds__(kB) HASTY_SPPM_CAMERA_ATTR kSppmCamera(DevScen
           sd = envSample(s, rng, &dens);
           so = offsetOrigin(it, dot(it.ng, sd) >= 0.0f);
         }
-        Hit hl = traceClosest<kStackLds, kStackOvf>(s, so, sd, stack + threadIdx.x, kB, ovf, nullptr, nullptr);
-        bool visible = target >= 0 ? hl.tri == target : (hl.tri < 0 || s.mats[s.triMat[hl.tri]].isLight);
+        // a sample the BSDF gives nothing adds exactly zero whatever the
+        // shadow ray meets: no ray is cast
+        const BsdfEval ev = bsdfEvaluate(m, it, wo, sd);
+        Hit hl;
+        hl.tri = -2;  // not traced
+        if (!isZero(ev.fD + ev.fC))
+          hl = traceClosest<kStackLds, kStackOvf>(s, so, sd, stack + threadIdx.x, kB, ovf, nullptr, nullptr);
+        bool visible = hl.tri != -2 && (target >= 0 ? hl.tri == target
+                                                     : (hl.tri < 0 || s.mats[s.triMat[hl.tri]].isLight));
         if (visible) {
           V3 hx = v3(0.0f, 0.0f, 0.0f), hn = v3(0.0f, 0.0f, 0.0f);
           if (hl.tri >= 0) hitPointNormal(s, hl.tri, hl.u, hl.v, &hx, &hn);
           float pL = lightDensity(s, so, sd, hl.tri, hx, hn);
           if (pL != 0.0f) {
-            BsdfEval ev = bsdfEvaluate(m, it, wo, sd);
             float mm = __builtin_fabsf(dot(sd, it.ng)) / pL;
             V3 sD = ev.fD * mm, sC = ev.fC * mm;
             if (!isZero(sD + sC) && hl.tri != it.tri) {
