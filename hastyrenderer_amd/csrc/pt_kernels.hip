@@ -576,6 +576,7 @@ __global__ void __launch_bounds__(kBlock) HASTY_SHADE_ATTR kShade(JobSet js, Pat
                   float xi = rng.uniform01f();
                   V3 so, sd;
                   int32_t target = -1;
+                  bool lit = true;  // the sampled light point emits toward so
                   if (xi < pSurf) {
                     uint32_t li = lightCdfSearch(s, rng.canonical_double());
                     target = (int32_t)ldg(s.lightTris + li);
@@ -590,6 +591,9 @@ __global__ void __launch_bounds__(kBlock) HASTY_SHADE_ATTR kShade(JobSet js, Pat
                     V3 diff = x2 - it.x;
                     so = offsetOrigin(it, dot(it.ng, diff) >= 0.0f);
                     sd = normalize(x2 - so);
+                    // kConnect's emission test on the target (hitPointNormal's
+                    // normal): a one-sided light seen from behind gives zero
+                    lit = !(dot(normalize(cross(p1 - p0, p2 - p0)), -sd) <= 0.0f);
                   } else {
                     float dens;
                     sd = envSample(s, rng, &dens);
@@ -597,10 +601,11 @@ __global__ void __launch_bounds__(kBlock) HASTY_SHADE_ATTR kShade(JobSet js, Pat
                   }
                   BsdfEval ev = bsdfEvaluate(m, it, wo, sd);
                   // A light sample the BSDF gives nothing (e.g. below the
-                  // surface) adds exactly zero whatever its shadow ray meets
-                  // (kConnect's t2 = thr (fD + fC) |cos| / pL), so no ray is
-                  // cast; about a quarter of the samples on the GGX grid.
-                  if (!isZero(ev.fD + ev.fC)) {
+                  // surface; kConnect's t2 = thr (fD + fC) |cos| / pL) or
+                  // that sees a light from behind adds exactly zero whatever
+                  // its shadow ray meets, so no ray is cast; about a quarter
+                  // of the samples on the GGX grid.
+                  if (lit && !isZero(ev.fD + ev.fC)) {
                     float outsideIOR = mediaOutsideIOR(s, md, it, wo, wavelength);
                     float pB = bsdfPdf(m, it, wo, sd, outsideIOR);
                     if (GUIDED && guided) pB = gf * guide::mixturePdfTable(gm, sd) + (1.0f - gf) * pB;

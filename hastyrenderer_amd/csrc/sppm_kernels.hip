@@ -942,6 +942,7 @@ __global__ void __launch_bounds__(kB) HASTY_SPPM_CAMERA_ATTR kSppmCamera(DevScen
         float xi = rng.uniform01f();
         V3 so, sd;
         int32_t target = -1;
+        bool lit = true;  // the sampled light point emits toward so
         if (xi < pSurf) {
           uint32_t li = lightCdfSearch(s, rng.canonical_double());
           target = (int32_t)s.lightTris[li];
@@ -955,17 +956,19 @@ __global__ void __launch_bounds__(kB) HASTY_SPPM_CAMERA_ATTR kSppmCamera(DevScen
           V3 x2 = p0 * b1 + p1 * b2 + p2 * b3;
           so = offsetOrigin(it, dot(it.ng, x2 - it.x) >= 0.0f);
           sd = normalize(x2 - so);
+          // the emission test below on the target: seen from behind, zero
+          lit = !(dot(normalize(cross(p1 - p0, p2 - p0)), -sd) <= 0.0f);
         } else {
           float dens;
           sd = envSample(s, rng, &dens);
           so = offsetOrigin(it, dot(it.ng, sd) >= 0.0f);
         }
-        // a sample the BSDF gives nothing adds exactly zero whatever the
-        // shadow ray meets: no ray is cast
+        // a sample the BSDF gives nothing, or that sees a light from behind,
+        // adds exactly zero whatever the shadow ray meets: no ray is cast
         const BsdfEval ev = bsdfEvaluate(m, it, wo, sd);
         Hit hl;
         hl.tri = -2;  // not traced
-        if (!isZero(ev.fD + ev.fC))
+        if (lit && !isZero(ev.fD + ev.fC))
           hl = traceClosest<kStackLds, kStackOvf>(s, so, sd, stack + threadIdx.x, kB, ovf, nullptr, nullptr);
         bool visible = hl.tri != -2 && (target >= 0 ? hl.tri == target
                                                      : (hl.tri < 0 || s.mats[s.triMat[hl.tri]].isLight));
