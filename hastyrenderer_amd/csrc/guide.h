@@ -33,20 +33,35 @@ constexpr float kCLo = -4.0f, kCHi = 9.0f;  // log-concentration clamp
 
 __host__ __device__ constexpr int layerIn(int l) { return l == 0 ? kInPad : kHidden; }
 __host__ __device__ constexpr int layerOut(int l) { return l == kLayers - 1 ? kOutPad : kHidden; }
+// Offsets are loops, not recursions: a recursive call with a run-time layer
+// index stays a real call on the device and gives the kernel a dynamic stack
+// (scratch memory for every wave).
 // flat parameter layout: per layer W^T (out x in) then b (out)
 __host__ __device__ constexpr int wOffset(int l) {
-  return l == 0 ? 0 : wOffset(l - 1) + layerIn(l - 1) * layerOut(l - 1) + layerOut(l - 1);
+  int off = 0;
+  for (int k = 0; k < l; ++k) off += layerIn(k) * layerOut(k) + layerOut(k);
+  return off;
 }
 __host__ __device__ constexpr int bOffset(int l) { return wOffset(l) + layerIn(l) * layerOut(l); }
 constexpr int kParams = bOffset(kLayers - 1) + layerOut(kLayers - 1);
 // offsets of the BF16 weight matrices (biases excluded)
 __host__ __device__ constexpr int wqOffset(int l) {
-  return l == 0 ? 0 : wqOffset(l - 1) + layerIn(l - 1) * layerOut(l - 1);
+  int off = 0;
+  for (int k = 0; k < l; ++k) off += layerIn(k) * layerOut(k);
+  return off;
 }
 constexpr int kWeights = wqOffset(kLayers - 1) + layerIn(kLayers - 1) * layerOut(kLayers - 1);
 // rows of the training activations: layer inputs (H) and output gradients (G)
-__host__ __device__ constexpr int hOffset(int l) { return l == 0 ? 0 : hOffset(l - 1) + layerIn(l - 1); }
-__host__ __device__ constexpr int gOffset(int l) { return l == 0 ? 0 : gOffset(l - 1) + layerOut(l - 1); }
+__host__ __device__ constexpr int hOffset(int l) {
+  int off = 0;
+  for (int k = 0; k < l; ++k) off += layerIn(k);
+  return off;
+}
+__host__ __device__ constexpr int gOffset(int l) {
+  int off = 0;
+  for (int k = 0; k < l; ++k) off += layerOut(k);
+  return off;
+}
 
 // Device view of the network (read by inference and training kernels).
 struct Net {
