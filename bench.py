@@ -539,7 +539,8 @@ def main():
     ap.add_argument("--guide-spp", type=int, default=None,
                     help="samples of both renders (default: 4096 on the ring, 256 on the room)")
     ap.add_argument("--guide-res", type=int, default=None, help="image side (default: 1024 ring, 512 room)")
-    ap.add_argument("--guide-train-spp", type=int, default=16)
+    ap.add_argument("--guide-train-spp", type=int, default=None,
+                    help="sample indices rendered with online training (default: 48 ring, 16 room)")
     ap.add_argument("--guide-scene", choices=("ring", "room"), default="ring",
                     help="ring: the metal-ring caustic box; room: a Cornell room lit from behind its light")
     ap.add_argument("--guide-lr", type=float, default=1e-3, help="Adam learning rate of the online training")
@@ -599,6 +600,8 @@ def main():
             args.guide_selection = (0.05, 0.95) if ring else ()
         if args.guide_spp is None:
             args.guide_spp = 4096 if ring else 256
+        if args.guide_train_spp is None:
+            args.guide_train_spp = 48 if ring else 16
         if args.guide_res is None:
             args.guide_res = 1024 if ring else 512
         if args.guide_ref_spp is None:
