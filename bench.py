@@ -558,7 +558,7 @@ def main():
     ap.add_argument("--guide-selection", type=lambda v: tuple(float(x) for x in v.split(",") if float(x) > 0.0),
                     default=None, metavar="LO,HI",
                     help="learned guide/BSDF selection probability in [LO, HI], replacing --guide-fraction "
-                         "(default 0.05,0.95 ring; 0: the fixed fraction, the room's default)")
+                         "(default 0.05,0.9 ring; 0: the fixed fraction, the room's default)")
     ap.add_argument("--guide-max-throughput", type=float, default=3.0e38,
                     help="guide a vertex only while the path throughput is at most this")
     ap.add_argument("--guide-save", default="", help="save the plain and guided images as PREFIX_plain.npy / _guided.npy")
@@ -597,7 +597,7 @@ def main():
         if args.guide_photons is None:
             args.guide_photons = 4194304 if ring else 0
         if args.guide_selection is None:
-            args.guide_selection = (0.05, 0.95) if ring else ()
+            args.guide_selection = (0.05, 0.9) if ring else ()
         if args.guide_spp is None:
             args.guide_spp = 4096 if ring else 256
         if args.guide_train_spp is None:
