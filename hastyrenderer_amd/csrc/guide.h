@@ -398,6 +398,34 @@ __device__ inline void forwardLds(const LdsNet* L, const bf16x4* h0, f32x4* raw)
   layerFwdLds<4, 3>(L, kLayers - 1, h, raw);
 }
 
+// Outputs of a 16-query tile for mixture tables (every lane of the wave
+// calls this; lane l supplies query l % 16, position normalised; the LDS copy
+// staged in lobeMajorRow order).  Lane group q = l / 16 receives its query's
+// raw outputs 10 q .. 10 q + 9 (lobes 2q and 2q + 1) in r10, those lobes'
+// mixture weights (the softmax over the 8 logits runs across the query's
+// four lanes) in w0, w1, and in *logit raw output 40 (the selection logit)
+// on lane group 0.
+__device__ inline void tileOutputs(const LdsNet* L, hdev::V3 x, hdev::V3 n, hdev::V3 wo, float* r10, float* w0,
+                                   float* w1, float* logit) {
+  bf16x4 h0[5];
+  inputOperands(x, n, wo, h0);
+  f32x4 o[3];
+  forwardLds(L, h0, o);
+  r10[0] = o[0].x, r10[1] = o[0].y, r10[2] = o[0].z, r10[3] = o[0].w;
+  r10[4] = o[1].x, r10[5] = o[1].y, r10[6] = o[1].z, r10[7] = o[1].w;
+  r10[8] = o[2].x, r10[9] = o[2].y;
+  float amax = fmaxf(r10[4], r10[9]);
+  amax = fmaxf(amax, __shfl_xor(amax, 16));
+  amax = fmaxf(amax, __shfl_xor(amax, 32));
+  const float e0 = __expf(r10[4] - amax), e1 = __expf(r10[9] - amax);
+  float sum = e0 + e1;
+  sum += __shfl_xor(sum, 16);
+  sum += __shfl_xor(sum, 32);
+  *w0 = e0 / sum;
+  *w1 = e1 / sum;
+  *logit = o[2].z;
+}
+
 __device__ inline hdev::V3 normPos(const Net& net, hdev::V3 p) {
   return hdev::v3((p.x - net.boxLo[0]) * net.boxInv[0], (p.y - net.boxLo[1]) * net.boxInv[1],
             (p.z - net.boxLo[2]) * net.boxInv[2]);

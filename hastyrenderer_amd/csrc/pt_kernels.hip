@@ -930,28 +930,15 @@ __global__ void __launch_bounds__(kGuideInferThreads) HASTY_WAVES_ATTR_(HASTY_GU
       const V3 nq = v3(__shfl(ng.x, src), __shfl(ng.y, src), __shfl(ng.z, src));
       const V3 wq = v3(__shfl(wo.x, src), __shfl(wo.y, src), __shfl(wo.z, src));
       const uint32_t sq = __shfl(slot, src);
-      guide::bf16x4 h0[5];
-      guide::inputOperands(xq, nq, wq, h0);
-      guide::f32x4 o[3];
-      guide::forwardLds(&ldsNet, h0, o);
-      // lane group q holds lobes 2q and 2q + 1 (raw outputs 10q .. 10q + 9);
-      // the softmax over the 8 logits runs across the path's four lanes
-      const float r10[10] = {o[0].x, o[0].y, o[0].z, o[0].w, o[1].x, o[1].y, o[1].z, o[1].w, o[2].x, o[2].y};
-      float amax = fmaxf(r10[4], r10[9]);
-      amax = fmaxf(amax, __shfl_xor(amax, 16));
-      amax = fmaxf(amax, __shfl_xor(amax, 32));
-      const float e0 = __expf(r10[4] - amax), e1 = __expf(r10[9] - amax);
-      float sum = e0 + e1;
-      sum += __shfl_xor(sum, 16);
-      sum += __shfl_xor(sum, 32);
-      const float w0 = e0 / sum, w1 = e1 / sum;
+      // lane group q holds lobes 2q and 2q + 1 of the path's mixture
+      float r10[10], w0, w1, logit;
+      guide::tileOutputs(&ldsNet, xq, nq, wq, r10, &w0, &w1, &logit);
       if ((wantMask >> src) & 1ull) {
         float4* out = js.g.raw + (size_t)sq * 10;
         out[2 * q] = guide::lobeEntry(r10, w0);
         out[2 * q + 1] = guide::lobeEntry(r10 + 5, w1);
         reinterpret_cast<float2*>(out + guide::kLobes)[q] = make_float2(w0, w1);
-        // lane group 0 holds the selection logit (raw output 40)
-        if (js.g.sel && q == 0) js.g.sel[sq] = guide::selectionOf(js.g.net, o[2].z);
+        if (js.g.sel && q == 0) js.g.sel[sq] = guide::selectionOf(js.g.net, logit);
       }
     }
   }

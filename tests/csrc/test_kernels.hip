@@ -6,8 +6,10 @@
 #include <hip/hip_runtime.h>
 #include <string.h>
 
+#include <algorithm>
 #include <vector>
 
+#include "../../hastyrenderer_amd/csrc/guide.h"
 #include "../../hastyrenderer_amd/csrc/pt_device.h"
 
 using namespace hdev;
@@ -191,6 +193,40 @@ __global__ void kMfma(const float* A, const float* B, float* D) {
   D[(4 * q + 3) * 16 + r] = acc.w;
 }
 
+// Mixture tables of n guide queries (x, n, wo: 9 floats each) by the path
+// tracer's inference code: the network staged in LDS in lobe-major order,
+// guide::tileOutputs on 16-query tiles, guide::lobeEntry.  Per query 10
+// float4 (lobes 0..7 as (kappa mu, log(w norm) - kappa), then the 8 weights)
+// and the selection logit.  A wave takes tiles t = wave, wave + waves, ...
+// (uniform over the wave); reads and writes only queries e < n.
+__global__ void __launch_bounds__(256) kGuideTables(guide::Net net, const float* q9, uint32_t n, float4* tables,
+                                                    float* logits) {
+  __shared__ guide::LdsNet ldsNet;
+  guide::stageNet(net, &ldsNet, true);
+  const int lane = threadIdx.x & 63, r = lane & 15, q = lane >> 4;
+  const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const uint32_t waves = (gridDim.x * blockDim.x) >> 6;
+  for (uint32_t t = wave; t * 16u < n; t += waves) {
+    const uint32_t e = t * 16u + (uint32_t)r;
+    V3 x = v3(0.0f, 0.0f, 0.0f), nn = v3(0.0f, 0.0f, 1.0f), wo = v3(0.0f, 0.0f, 1.0f);
+    if (e < n) {
+      const float* p = q9 + (size_t)e * 9;
+      x = guide::normPos(net, v3(p[0], p[1], p[2]));
+      nn = v3(p[3], p[4], p[5]);
+      wo = v3(p[6], p[7], p[8]);
+    }
+    float r10[10], w0, w1, logit;
+    guide::tileOutputs(&ldsNet, x, nn, wo, r10, &w0, &w1, &logit);
+    if (e < n) {
+      float4* out = tables + (size_t)e * 10;
+      out[2 * q] = guide::lobeEntry(r10, w0);
+      out[2 * q + 1] = guide::lobeEntry(r10 + 5, w1);
+      reinterpret_cast<float2*>(out + guide::kLobes)[q] = make_float2(w0, w1);
+      if (q == 0) logits[e] = logit;
+    }
+  }
+}
+
 extern "C" {
 
 int tk_mfma(const float* A, const float* B, float* D) {
@@ -262,6 +298,34 @@ int tk_bsdf_sample(const float* matParams, const float* n, const float* wo, uint
   kBsdfSample<<<1, 64>>>(makeMaterial(matParams), v3(n[0], n[1], n[2]), v3(wo[0], wo[1], wo[2]), seed, count, dout);
   int rc = finish(dout, out7, (size_t)count * 7);
   (void)hipFree(dout);
+  return rc;
+}
+
+// wT: the BF16 W^T of every layer (guide::kWeights), params: the FP32
+// parameters (guide::kParams; biases read from them), lo3 / inv3: the
+// position normalisation (x - lo) * inv.
+int tk_guide_tables(const uint16_t* wT, uint32_t nWeights, const float* params, uint32_t nParams, const float* lo3,
+                    const float* inv3, const float* q9, uint32_t n, float* tables, float* logits) {
+  if (nWeights != (uint32_t)guide::kWeights || nParams != (uint32_t)guide::kParams || n == 0) return 4;
+  short* dW = toDevice(reinterpret_cast<const short*>(wT), (size_t)nWeights);
+  float* dP = toDevice(params, (size_t)nParams);
+  float* dQ = toDevice(q9, (size_t)n * 9);
+  float4* dT = devAlloc<float4>((size_t)n * 10);
+  float* dL = devAlloc<float>(n);
+  int rc = 1;
+  if (dW && dP && dQ && dT && dL) {
+    guide::Net net;
+    memset(&net, 0, sizeof(net));
+    net.wT = dW;
+    net.w = nullptr;
+    net.param = dP;
+    for (int i = 0; i < 3; ++i) net.boxLo[i] = lo3[i], net.boxInv[i] = inv3[i];
+    const uint32_t blocks = std::min<uint32_t>((n + 63) / 64, 64u);
+    kGuideTables<<<blocks, 256>>>(net, dQ, n, dT, dL);
+    rc = finish(dL, logits, n);
+    if (rc == 0 && hipMemcpy(tables, dT, sizeof(float4) * (size_t)n * 10, hipMemcpyDeviceToHost) != hipSuccess) rc = 3;
+  }
+  (void)hipFree(dW), (void)hipFree(dP), (void)hipFree(dQ), (void)hipFree(dT), (void)hipFree(dL);
   return rc;
 }
 

@@ -59,6 +59,60 @@ def test_infer_matches_reference(cornell_job):
     assert np.all(raw[:, 41:] == 0.0)
 
 
+def _tables_reference(raw):
+    """The mixture table the path tracer keeps per guided vertex, from raw
+    outputs in float64: lobe k as (kappa_k mu_k, log(w_k norm_k) - kappa_k)
+    with norm_k = kappa_k / (2 pi (1 - exp(-2 kappa_k))), then the weights."""
+    mu, kappa, w, _, _ = gr.mixture(raw)
+    norm = kappa / (2.0 * math.pi * -np.expm1(-2.0 * kappa))
+    v = mu * kappa[:, :, None]
+    b = np.log(w * norm) - kappa
+    return v, b, w
+
+
+def test_inference_tables_match_reference():
+    """The path tracer's inference code (the network staged in LDS in
+    lobe-major row order, guide::tileOutputs with the softmax across a
+    query's four lanes, guide::lobeEntry) against the CPU model: every lobe's
+    kappa mu, log-weight term and weight, and the selection logit.  A
+    permuted lobe, a swapped weight or a lane group left out of the softmax
+    gives O(1) errors; BF16 rounding flips give rare ~1e-3 ones."""
+    L = load_testkernels()
+    rng = np.random.default_rng(5)
+    p = gr.init_params(7)
+    w4, b4, din4, dout4 = gr.LAYOUT[4]
+    p[w4:w4 + din4 * dout4] *= 10.0
+    for li, (w, b, din, dout) in enumerate(gr.LAYOUT):  # biases exercise the LDS bias rows
+        used = gr.N_OUT_USED if li == len(gr.LAYOUT) - 1 else dout
+        p[b:b + used] = rng.normal(0.0, 0.1, used).astype(np.float32)
+    wT = np.concatenate([gr.bf16(p[w:w + din * dout]).astype(np.float32).view(np.uint32) >> 16
+                         for (w, b, din, dout) in gr.LAYOUT]).astype(np.uint16)
+    B = 1000  # not a multiple of 16: the last tile is ragged
+    lo = np.array([-1.0, 0.5, -2.0], np.float32)
+    inv = np.array([0.5, 0.25, 0.125], np.float32)
+    xn, n, wo = _queries(rng, B)
+    x = (xn / inv + lo).astype(np.float32)
+    q9 = np.ascontiguousarray(np.concatenate([x, n, wo], 1), np.float32)
+    tables = np.zeros((B, 10, 4), np.float32)
+    logits = np.zeros(B, np.float32)
+    assert L.tk_guide_tables(wT, wT.size, p, p.size, lo, inv, q9, B, tables, logits) == 0
+    xq = ((x - lo) * inv).astype(np.float32)  # the kernel's normalisation, in float32
+    raw, _ = gr.forward(p, gr.encode(xq, n, wo), bf16_ops=True)
+    v, b, w = _tables_reference(raw)
+    dv = np.abs(tables[:, :8, :3] - v) / (1.0 + np.abs(v))
+    db = np.abs(tables[:, :8, 3] - b) / (1.0 + np.abs(b))
+    dw = np.abs(tables[:, 8:, :].reshape(B, 8) - w)
+    e = np.maximum(np.maximum(dv.max(axis=(1, 2)), db.max(axis=1)), dw.max(axis=1))
+    scale = np.sqrt(np.mean(raw[:, :40] ** 2))
+    dl = np.abs(logits - raw[:, gr.SEL_OUT]) / scale
+    print("tables: median %.2e, 99%% %.2e, max %.2e; logit median %.2e" % (
+        np.median(e), np.quantile(e, 0.99), e.max(), np.median(dl)), flush=True)
+    assert np.median(e) <= 1e-4
+    assert np.quantile(e, 0.99) <= 1e-2
+    assert np.median(dl) <= 1e-5 and np.mean(dl <= 1e-3) >= 0.99
+    assert np.allclose(tables[:, 8:, :].reshape(B, 8).sum(axis=1), 1.0, atol=1e-5)
+
+
 def _samples(rng, B, target=None):
     x, n, wo = _queries(rng, B)
     if target is None:
