@@ -275,7 +275,12 @@ __device__ inline bf16x4 reluPack(f32x4 z) {
 // A lane's 4 consecutive features k..k+3 (k a multiple of 4): below 72 they
 // are the sine/cosine pairs of octaves f and f+1 of one axis, computed as
 // sinpi / cospi of 2^f x_a (the scaling is exact, so no angle is rounded
-// before the reduction).
+// before the reduction).  DOUBLE_ANGLE (the path tracer's inference) takes
+// octave f+1 from octave f instead: sin 2a = 2 sin a cos a and
+// cos 2a = (cos a - sin a)(cos a + sin a), a few FP32 ulp from the direct
+// values (an absolute 1e-7 near zero), below the BF16 rounding of the
+// operands; training and the library's inference keep the direct form.
+template <bool DOUBLE_ANGLE = false>
 __device__ inline void inputOperands(hdev::V3 x, hdev::V3 n, hdev::V3 wo, bf16x4* in) {
   const int q = (threadIdx.x & 63) >> 4;
 #pragma unroll
@@ -287,7 +292,12 @@ __device__ inline void inputOperands(hdev::V3 x, hdev::V3 n, hdev::V3 wo, bf16x4
       const float t = xa * (float)(1u << f);
       float s0, c0, s1, c1;
       sincospif(t, &s0, &c0);
-      sincospif(t + t, &s1, &c1);
+      if (DOUBLE_ANGLE) {
+        s1 = 2.0f * s0 * c0;
+        c1 = (c0 - s0) * (c0 + s0);
+      } else {
+        sincospif(t + t, &s1, &c1);
+      }
       in[ks] = pack4(s0, c0, s1, c1);
     } else {
       in[ks] = pack4(feature(k, x, n, wo), feature(k + 1, x, n, wo), feature(k + 2, x, n, wo),
@@ -404,11 +414,11 @@ __device__ inline void forwardLds(const LdsNet* L, const bf16x4* h0, f32x4* raw)
 // raw outputs 10 q .. 10 q + 9 (lobes 2q and 2q + 1) in r10, those lobes'
 // mixture weights (the softmax over the 8 logits runs across the query's
 // four lanes) in w0, w1, and in *logit raw output 40 (the selection logit)
-// on lane group 0.
+// on lane group 0.  The inputs take the double-angle form of inputOperands.
 __device__ inline void tileOutputs(const LdsNet* L, hdev::V3 x, hdev::V3 n, hdev::V3 wo, float* r10, float* w0,
                                    float* w1, float* logit) {
   bf16x4 h0[5];
-  inputOperands(x, n, wo, h0);
+  inputOperands<true>(x, n, wo, h0);
   f32x4 o[3];
   forwardLds(L, h0, o);
   r10[0] = o[0].x, r10[1] = o[0].y, r10[2] = o[0].z, r10[3] = o[0].w;
