@@ -539,7 +539,8 @@ static void renderGuided(HastyScene* s, const BatchItem& it) {
   if (s->guideSlots != slots) {
     for (void* p : {(void*)s->guideRaw, (void*)s->guideRec, (void*)s->guideSel})
       if (p) (void)hipFree(p);
-    s->guideSel = nullptr;
+    s->guideRaw = nullptr, s->guideRec = nullptr, s->guideSel = nullptr;
+    s->guideSlots = 0;
     HIPCHECK(hipMalloc(&s->guideRaw, sizeof(float4) * 10 * (size_t)slots));
     HIPCHECK(hipMalloc(&s->guideRec, sizeof(float4) * 12 * (size_t)slots));
     s->guideSlots = slots;
