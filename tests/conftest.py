@@ -86,6 +86,7 @@ def load_testkernels():
         L.tk_bsdf_sample.argtypes = [fp, fp, fp, C.c_uint32, C.c_uint32, fp]
         hp = np.ctypeslib.ndpointer(dtype=np.uint16, flags="C_CONTIGUOUS")
         L.tk_guide_tables.argtypes = [hp, C.c_uint32, fp, C.c_uint32, fp, fp, fp, C.c_uint32, fp, fp]
+        L.tk_guide_mixture.argtypes = [fp, C.c_uint32, C.c_uint32, fp, fp, fp, fp]
         _TK = L
     return _TK
 

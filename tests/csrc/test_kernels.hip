@@ -227,6 +227,24 @@ __global__ void __launch_bounds__(256) kGuideTables(guide::Net net, const float*
   }
 }
 
+// Per query i (its table at tables[10 i ..], as kGuideTables writes it): m
+// directions sampled by guide::mixtureSampleTable from xi[3 t ..] and
+// guide::mixturePdfTable at the directions evalDirs[3 t ..], t = i m + j.
+// One thread per (query, sample), in bounds.
+__global__ void kGuideMixture(const float4* tables, uint32_t total, uint32_t m, const float* xi, float* dirs,
+                              const float* evalDirs, float* pdfs) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= total) return;
+  const float4* g = tables + (size_t)(t / m) * 10;
+  const float* x = xi + (size_t)t * 3;
+  const V3 d = guide::mixtureSampleTable(g, x[0], x[1], x[2]);
+  dirs[(size_t)t * 3] = d.x;
+  dirs[(size_t)t * 3 + 1] = d.y;
+  dirs[(size_t)t * 3 + 2] = d.z;
+  const float* e = evalDirs + (size_t)t * 3;
+  pdfs[t] = guide::mixturePdfTable(g, v3(e[0], e[1], e[2]));
+}
+
 extern "C" {
 
 int tk_mfma(const float* A, const float* B, float* D) {
@@ -326,6 +344,28 @@ int tk_guide_tables(const uint16_t* wT, uint32_t nWeights, const float* params, 
     if (rc == 0 && hipMemcpy(tables, dT, sizeof(float4) * (size_t)n * 10, hipMemcpyDeviceToHost) != hipSuccess) rc = 3;
   }
   (void)hipFree(dW), (void)hipFree(dP), (void)hipFree(dQ), (void)hipFree(dT), (void)hipFree(dL);
+  return rc;
+}
+
+// nq tables (10 float4 each), m samples per table: xi and evalDirs hold
+// 3 nq m floats, dirs receives 3 nq m, pdfs nq m.
+int tk_guide_mixture(const float* tables, uint32_t nq, uint32_t m, const float* xi, float* dirs,
+                     const float* evalDirs, float* pdfs) {
+  const uint64_t total64 = (uint64_t)nq * m;
+  if (nq == 0 || m == 0 || total64 > (1u << 24)) return 4;
+  const uint32_t total = (uint32_t)total64;
+  float4* dT = toDevice(reinterpret_cast<const float4*>(tables), (size_t)nq * 10);
+  float* dX = toDevice(xi, (size_t)total * 3);
+  float* dE = toDevice(evalDirs, (size_t)total * 3);
+  float* dD = devAlloc<float>((size_t)total * 3);
+  float* dP = devAlloc<float>(total);
+  int rc = 1;
+  if (dT && dX && dE && dD && dP) {
+    kGuideMixture<<<(total + 255) / 256, 256>>>(dT, total, m, dX, dD, dE, dP);
+    rc = finish(dP, pdfs, total);
+    if (rc == 0 && hipMemcpy(dirs, dD, sizeof(float) * (size_t)total * 3, hipMemcpyDeviceToHost) != hipSuccess) rc = 3;
+  }
+  (void)hipFree(dT), (void)hipFree(dX), (void)hipFree(dE), (void)hipFree(dD), (void)hipFree(dP);
   return rc;
 }
 

@@ -70,24 +70,18 @@ def _tables_reference(raw):
     return v, b, w
 
 
-def test_inference_tables_match_reference():
-    """The path tracer's inference code (the network staged in LDS in
-    lobe-major row order, guide::tileOutputs with the softmax across a
-    query's four lanes, guide::lobeEntry) against the CPU model: every lobe's
-    kappa mu, log-weight term and weight, and the selection logit.  A
-    permuted lobe, a swapped weight or a lane group left out of the softmax
-    gives O(1) errors; BF16 rounding flips give rare ~1e-3 ones."""
-    L = load_testkernels()
-    rng = np.random.default_rng(5)
-    p = gr.init_params(7)
+def _inference_tables(L, B, out_scale, seed):
+    """Mixture tables of B random queries by the path tracer's inference
+    code (test kernel), and the CPU model's raw outputs for them."""
+    rng = np.random.default_rng(seed)
+    p = gr.init_params(seed + 2)
     w4, b4, din4, dout4 = gr.LAYOUT[4]
-    p[w4:w4 + din4 * dout4] *= 10.0
+    p[w4:w4 + din4 * dout4] *= out_scale
     for li, (w, b, din, dout) in enumerate(gr.LAYOUT):  # biases exercise the LDS bias rows
         used = gr.N_OUT_USED if li == len(gr.LAYOUT) - 1 else dout
         p[b:b + used] = rng.normal(0.0, 0.1, used).astype(np.float32)
     wT = np.concatenate([gr.bf16(p[w:w + din * dout]).astype(np.float32).view(np.uint32) >> 16
                          for (w, b, din, dout) in gr.LAYOUT]).astype(np.uint16)
-    B = 1000  # not a multiple of 16: the last tile is ragged
     lo = np.array([-1.0, 0.5, -2.0], np.float32)
     inv = np.array([0.5, 0.25, 0.125], np.float32)
     xn, n, wo = _queries(rng, B)
@@ -98,6 +92,20 @@ def test_inference_tables_match_reference():
     assert L.tk_guide_tables(wT, wT.size, p, p.size, lo, inv, q9, B, tables, logits) == 0
     xq = ((x - lo) * inv).astype(np.float32)  # the kernel's normalisation, in float32
     raw, _ = gr.forward(p, gr.encode(xq, n, wo), bf16_ops=True)
+    return tables, logits, raw
+
+
+def test_inference_tables_match_reference():
+    """The path tracer's inference code (the network staged in LDS in
+    lobe-major row order, guide::tileOutputs with the softmax across a
+    query's four lanes, guide::lobeEntry) against the CPU model: every lobe's
+    kappa mu, log-weight term and weight, and the selection logit.  A
+    permuted lobe, a swapped weight or a lane group left out of the softmax
+    gives O(1) errors; BF16 rounding flips give rare ~1e-3 ones.  B = 1000
+    is not a multiple of 16: the last tile is ragged."""
+    L = load_testkernels()
+    B = 1000
+    tables, logits, raw = _inference_tables(L, B, 10.0, 5)
     v, b, w = _tables_reference(raw)
     dv = np.abs(tables[:, :8, :3] - v) / (1.0 + np.abs(v))
     db = np.abs(tables[:, :8, 3] - b) / (1.0 + np.abs(b))
@@ -111,6 +119,50 @@ def test_inference_tables_match_reference():
     assert np.quantile(e, 0.99) <= 1e-2
     assert np.median(dl) <= 1e-5 and np.mean(dl <= 1e-3) >= 0.99
     assert np.allclose(tables[:, 8:, :].reshape(B, 8).sum(axis=1), 1.0, atol=1e-5)
+
+
+def test_mixture_table_sampling_and_density():
+    """guide::mixtureSampleTable and guide::mixturePdfTable (the guided
+    vertex's sampling and density) on inference tables: the density equals
+    the CPU model's mixture density, and the first and second moments of the
+    sampled directions equal the mixture's within five standard errors.  For
+    a vMF lobe with A = coth kappa - 1 / kappa, E[d] = A mu and
+    E[d d^T] = (A / kappa) I + (1 - 3 A / kappa) mu mu^T.  A wrong lobe pick,
+    a concentration off by 30 % in the sampler alone (checked on a CPU copy
+    of the sampler: z 9 against 2 when right) or a wrong normalisation fails
+    one of them."""
+    L = load_testkernels()
+    nq, m = 64, 16384
+    tables, _, raw = _inference_tables(L, nq, 4.0, 9)
+    rng = np.random.default_rng(11)
+    xi = rng.random((nq * m, 3), dtype=np.float32)
+    ev = rng.normal(size=(nq * m, 3))
+    ev /= np.linalg.norm(ev, axis=1, keepdims=True)
+    ev = ev.astype(np.float32)
+    dirs = np.zeros((nq * m, 3), np.float32)
+    pdfs = np.zeros(nq * m, np.float32)
+    assert L.tk_guide_mixture(np.ascontiguousarray(tables), nq, m, xi, dirs, ev, pdfs) == 0
+    assert np.all(np.abs(np.linalg.norm(dirs, axis=1) - 1.0) <= 1e-5)
+    ref = np.concatenate([gr.pdf(raw[i:i + 1].repeat(m, 0), ev[i * m:(i + 1) * m].astype(np.float64))
+                          for i in range(nq)])
+    ok = ref > 1e-20
+    rel = np.abs(pdfs[ok] - ref[ok]) / ref[ok]
+    print("mixture pdf: median rel %.2e, 99%% %.2e" % (np.median(rel), np.quantile(rel, 0.99)), flush=True)
+    assert np.median(rel) <= 1e-4 and np.quantile(rel, 0.99) <= 1e-2
+    mu, kappa, w, _, _ = gr.mixture(raw)
+    A = 1.0 / np.tanh(kappa) - 1.0 / kappa
+    want1 = np.einsum("bk,bkj->bj", w * A, mu)
+    want2 = (np.einsum("bk,ij->bij", w * A / kappa, np.eye(3)) +
+             np.einsum("bk,bki,bkj->bij", w * (1.0 - 3.0 * A / kappa), mu, mu))
+    zmax = 0.0
+    for i in range(nq):
+        d = dirs[i * m:(i + 1) * m].astype(np.float64)
+        z1 = np.abs(d.mean(axis=0) - want1[i]) / (d.std(axis=0) / math.sqrt(m) + 1e-4)
+        dd = np.einsum("ni,nj->nij", d, d)
+        z2 = np.abs(dd.mean(axis=0) - want2[i]) / (dd.std(axis=0) / math.sqrt(m) + 1e-4)
+        zmax = max(zmax, float(z1.max()), float(z2.max()))
+    print("sample moments: max z %.2f" % zmax, flush=True)
+    assert zmax <= 5.0
 
 
 def _samples(rng, B, target=None):
