@@ -227,3 +227,28 @@ def test_encoding_octaves():
     assert abs(s[0, 1, 1] - 1.0) < 1e-15 and abs(s[0, 2, :]).max() == 0.0
     assert np.array_equal(f[:, 72:75], n) and np.array_equal(f[:, 75:78], wo)
     assert not f[:, 78:].any()
+
+
+def test_table_form_is_the_mixture_density():
+    """The per-vertex table the path tracer keeps (lobe k as v_k = kappa_k
+    mu_k and b_k = log(w_k norm_k) - kappa_k, density sum_k exp(v_k . d +
+    b_k)) is the mixture density of the raw outputs, and it integrates to one
+    over the sphere.  The GPU tests compare the device tables against this
+    form; dropping the -kappa_k or the weight from b_k fails here."""
+    rng = np.random.default_rng(4)
+    raw = rng.normal(size=(6, gr.OUT_PAD)) * 2.0
+    mu, kappa, w, _, _ = gr.mixture(raw)
+    norm = kappa / (2.0 * math.pi * -np.expm1(-2.0 * kappa))
+    v = mu * kappa[:, :, None]
+    b = np.log(w * norm) - kappa
+    # Fibonacci sphere quadrature
+    n = 200000
+    i = np.arange(n) + 0.5
+    z = 1.0 - 2.0 * i / n
+    phi = math.pi * (1.0 + 5.0 ** 0.5) * i
+    r = np.sqrt(1.0 - z * z)
+    d = np.stack([r * np.cos(phi), r * np.sin(phi), z], axis=1)
+    for q in range(raw.shape[0]):
+        table = np.exp(d @ v[q].T + b[q]).sum(axis=1)
+        np.testing.assert_allclose(table, gr.pdf(raw[q:q + 1].repeat(n, 0), d), rtol=1e-9, atol=1e-300)
+        assert abs(table.mean() * 4.0 * math.pi - 1.0) < 2e-3
