@@ -183,9 +183,11 @@ __device__ inline hdev::V3 mixtureSample(const Mixture& m, float xi0, float xi1,
 __device__ inline float4 lobeEntry(const float* r5, float wk) {
   hdev::V3 mu = hdev::v3(r5[0], r5[1], r5[2]);
   const float nr = sqrtf(hdev::dot(mu, mu));
-  mu = nr > 1e-12f ? mu / nr : hdev::v3(0.0f, 0.0f, 1.0f);
+  // reciprocals by v_rcp_f32 (1 ulp) rather than IEEE divisions: sampling
+  // and density both read the table, so they stay consistent
+  mu = nr > 1e-12f ? mu * __builtin_amdgcn_rcpf(nr) : hdev::v3(0.0f, 0.0f, 1.0f);
   const float kappa = __expf(fminf(fmaxf(r5[3], kCLo), kCHi));
-  const float norm = kappa / (2.0f * 3.14159265358979323846f * -expm1f(-2.0f * kappa));
+  const float norm = kappa * __builtin_amdgcn_rcpf(2.0f * 3.14159265358979323846f * -expm1f(-2.0f * kappa));
   return make_float4(mu.x * kappa, mu.y * kappa, mu.z * kappa, __logf(wk * norm) - kappa);
 }
 __device__ inline float mixturePdfTable(const float4* g, hdev::V3 d) {
@@ -431,8 +433,11 @@ __device__ inline void tileOutputs(const LdsNet* L, hdev::V3 x, hdev::V3 n, hdev
   float sum = e0 + e1;
   sum += __shfl_xor(sum, 16);
   sum += __shfl_xor(sum, 32);
-  *w0 = e0 / sum;
-  *w1 = e1 / sum;
+  // v_rcp_f32 (1 ulp) instead of two IEEE divisions: the weights then sum
+  // to one within ~1e-7, and sampling and density read the same table
+  const float inv = __builtin_amdgcn_rcpf(sum);
+  *w0 = e0 * inv;
+  *w1 = e1 * inv;
   *logit = o[2].z;
 }
 
